@@ -1,0 +1,221 @@
+/*
+ * tgx.h — C ABI of the MI355X (gfx950) voxel ray-tracing hot path.
+ *
+ * This library replaces the GPU half of tg's voxel ray tracer: the
+ * Vulkan/GLSL passes behind tg_raytracer_render() and the CPU SVO build
+ * behind tg_svo_create(). The host keeps tg's object/cluster data model
+ * byte for byte; everything crosses this boundary as plain pointers and
+ * sizes. Every entry point returns a status code (tgx_status); the
+ * reference asserts instead (tg_common.h:32-47), so callers that relied on
+ * a crash now get TGX_ERR_* and a message from tgx_last_error().
+ *
+ * Threading: one context per host thread. All work of a context is
+ * ordered on the context's own HIP stream.
+ *
+ * Reference citations are file:line under torben-gross/tg.
+ */
+#ifndef TGX_H
+#define TGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Constants of the data model (tg_graphics_core.h:18-23, cluster.inc:1-6)   */
+/* ------------------------------------------------------------------------ */
+
+#define TGX_CLUSTER_SIDE            8u    /* voxels per cluster edge          */
+#define TGX_VOXELS_PER_CLUSTER      512u  /* 8^3, 9-bit voxel index 64z+8y+x  */
+#define TGX_BITS_WORDS_PER_CLUSTER  16u   /* 512 solid bits as 16 x u32       */
+#define TGX_CLUSTER_POINTER_BITS    31u   /* 31-bit cluster pointer field     */
+#define TGX_LUT_ENTRIES             256u  /* entries per material LUT         */
+#define TGX_VISIBILITY_EMPTY        0xFFFFFFFFFFFFFFFFull /* clear.comp:19   */
+
+/* Visibility word: depth24 << 40 | cluster_pointer31 << 9 | voxel9
+ * (visibility.frag:194-201). */
+#define TGX_VIS_DEPTH_SHIFT         40u
+#define TGX_VIS_CPTR_SHIFT          9u
+#define TGX_VIS_CPTR_MASK           0x7FFFFFFFu
+#define TGX_VIS_VOXEL_MASK          0x1FFu
+
+/* Iteration cap of every DDA loop (oracle and kernels alike). An 8^3 walk
+ * takes at most 22 steps; the cap only ends walks whose step sizes have
+ * overflowed to infinity (direction components near FLT_MIN). */
+#define TGX_DDA_MAX_ITERATIONS      64u
+
+/* ------------------------------------------------------------------------ */
+/* Byte layouts (std430 of the reference's SSBOs)                           */
+/* ------------------------------------------------------------------------ */
+
+/* Object record, 96 B. buffers.inc:6-13, tgvk_raytracer.c:35-42.
+ * rotation is a column-major m4 (tg_math.h:240-275): rotation[4*c + r].
+ * lut_idx occupies the reference's `pad` word (buffers.inc:11) and selects
+ * the object's 256-entry material LUT. */
+typedef struct tgx_object {
+    uint32_t n_cluster_pointers_per_dim[3];
+    uint32_t first_cluster_pointer;
+    float    translation[3];
+    uint32_t lut_idx;
+    float    rotation[16];
+} tgx_object;
+
+/* Camera record, 88 B. buffers.inc:30-39, tgvk_raytracer.c:65-74.
+ * camera.xyz = position; ray_* = normalised corner rays
+ * (tgvk_core.c:435-444); w components are unused. */
+typedef struct tgx_camera {
+    float camera[4];
+    float ray_bl[4];
+    float ray_br[4];
+    float ray_tr[4];
+    float ray_tl[4];
+    float near_plane;
+    float far_plane;
+} tgx_camera;
+
+/* Material LUT entry, 16 B: fp32 albedo and roughness. The reference packs
+ * RGBA8 (tgvk_raytracer.c:1131-1135); this path carries fp32 albedo. */
+typedef struct tgx_lut_entry {
+    float albedo[3];
+    float roughness;
+} tgx_lut_entry;
+
+/* Host-side scene description; arrays are caller-owned and copied.
+ *   cluster_pointers[cptr]          = cluster idx (cluster_pointer_ssbo)
+ *   cluster_idx_to_object[cidx]     = object idx
+ *   bits[16*cidx + w]               = solid bits, voxel v of cluster c is
+ *                                     bit (512c+v)%32 of word (512c+v)/32
+ *   materials[512*cidx + v]         = LUT index of voxel v (may be NULL)
+ * (tgvk_raytracer.c:851-978, visibility.frag:139-148, shading.frag:128-131) */
+typedef struct tgx_scene_desc {
+    const tgx_object*    objects;
+    uint32_t             n_objects;
+    const uint32_t*      cluster_pointers;
+    uint32_t             n_cluster_pointers;
+    const uint32_t*      cluster_idx_to_object;
+    uint32_t             n_clusters;
+    const uint32_t*      bits;
+    const uint8_t*       materials;
+    const tgx_lut_entry* luts;
+    uint32_t             n_luts;
+} tgx_scene_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef enum tgx_status {
+    TGX_OK                 = 0,
+    TGX_ERR_INVALID_ARG    = 1,
+    TGX_ERR_CAPACITY       = 2,
+    TGX_ERR_HIP            = 3,
+    TGX_ERR_NO_SCENE       = 4,
+    TGX_ERR_NO_DEVICE      = 5,
+    TGX_ERR_OUT_OF_MEMORY  = 6,
+    TGX_ERR_NOT_BUILT      = 7
+} tgx_status;
+
+typedef struct tgx_context tgx_context;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces tg_raytracer_create's device half (tgvk_raytracer.c:662-789):
+ * selects HIP device `device`, creates the context's stream. */
+int         tgx_create(int device, tgx_context** out_ctx);
+/* Replaces tg_raytracer_destroy: frees every device buffer of the context. */
+int         tgx_destroy(tgx_context* ctx);
+/* Message of the last failing call on this context ("" if none). The
+ * reference has no error channel (tgvk_common.h:25-35 asserts). */
+const char* tgx_last_error(const tgx_context* ctx);
+/* Blocks until all work on the context's stream has finished
+ * (the fence wait of tgvk_raytracer.c:1560-1561). */
+int         tgx_synchronize(tgx_context* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Scene                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Uploads the whole scene (objects, cluster maps, solid bits, materials,
+ * LUTs) into library-owned device buffers, replacing any previous scene.
+ * Replaces the staging pushes of tg_raytracer_create_object
+ * (tgvk_raytracer.c:805-992) and tg_raytracer_color_lut_set (:1122-1142).
+ * Errors: TGX_ERR_CAPACITY if n_cluster_pointers >= 2^31 (the 31-bit
+ * field, tg_graphics_core.h:21-22); TGX_ERR_INVALID_ARG if a cluster
+ * pointer or object index is out of range. */
+int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* scene);
+
+/* ------------------------------------------------------------------------ */
+/* Primary visibility                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Clears n_pixels visibility words at device pointer vis_dev to all ones.
+ * Replaces clear.comp (clear.comp:15-20) / tg_raytracer_clear
+ * (tgvk_raytracer.c:1556-1603). */
+int tgx_visibility_clear_device(tgx_context* ctx, uint64_t* vis_dev, uint64_t n_pixels);
+
+/* Renders the visibility buffers of n_cams cameras at w x h into host
+ * memory out_host[n_cams * w * h] (row-major, y = 0 is the top row).
+ * Each word is the minimum over all cluster pointers whose 8^3 DDA hits a
+ * solid voxel with depth <= 1 of the packed word (visibility.frag:22-208),
+ * or TGX_VISIBILITY_EMPTY. Replaces the visibility draw of
+ * tg_raytracer_render (tgvk_raytracer.c:1275-1317). */
+int tgx_visibility(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams,
+                   uint32_t w, uint32_t h, uint64_t* out_host);
+
+/* As tgx_visibility, but writes into the device buffer out_dev
+ * (n_cams * w * h words) and returns without waiting for the GPU. */
+int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams,
+                          uint32_t w, uint32_t h, uint64_t* out_dev);
+
+/* Host-side decode of one visibility word, replacing
+ * tg_raytracer_get_hovered_voxel's unpack (tgvk_raytracer.c:1639-1653).
+ * The 31-bit field is returned as what the GPU wrote: a cluster POINTER
+ * (visibility.frag:199). Returns 1 if the word is a hit (depth < 1). */
+int tgx_visibility_decode(uint64_t word, float* out_depth, uint32_t* out_cluster_pointer,
+                          uint32_t* out_voxel_idx);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and profiling helpers (for callers without their own       */
+/* allocator; the reference's equivalents are tgvk_memory.c:182-240)        */
+/* ------------------------------------------------------------------------ */
+
+int tgx_device_alloc(tgx_context* ctx, size_t bytes, void** out_dev);
+int tgx_device_free(tgx_context* ctx, void* dev);
+int tgx_memcpy_d2h(tgx_context* ctx, void* dst_host, const void* src_dev, size_t bytes);
+int tgx_memcpy_h2d(tgx_context* ctx, void* dst_dev, const void* src_host, size_t bytes);
+
+/* Per-kernel timing with HIP events recorded on the context's stream.
+ * When enabled, every kernel launch of the context is bracketed by events;
+ * tgx_profile_get() returns, per kernel name, the launch count and the
+ * summed event time in milliseconds. */
+typedef struct tgx_kernel_stat {
+    char     name[48];
+    uint64_t launches;
+    double   total_ms;
+} tgx_kernel_stat;
+
+int tgx_profile_enable(tgx_context* ctx, int enable);
+int tgx_profile_reset(tgx_context* ctx);
+int tgx_profile_get(tgx_context* ctx, tgx_kernel_stat* out, uint32_t capacity, uint32_t* out_n);
+
+/* Statistics of the last visibility call (work items after culling). */
+typedef struct tgx_visibility_stats {
+    uint64_t clusters_total;     /* cluster pointers x cameras              */
+    uint64_t clusters_binned;    /* (cluster, camera) pairs surviving cull  */
+    uint64_t tile_entries;       /* (tile, cluster) pairs after binning     */
+    uint32_t tiles;              /* tiles per camera                        */
+    uint32_t tile_size;          /* tile edge in pixels                     */
+} tgx_visibility_stats;
+
+int tgx_visibility_last_stats(tgx_context* ctx, tgx_visibility_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGX_H */

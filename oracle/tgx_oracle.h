@@ -1,0 +1,75 @@
+/*
+ * tgx_oracle.h — CPU oracle of the voxel ray-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a scalar C restatement of tg's
+ * algorithms, built with strict IEEE fp32 (-ffp-contract=off
+ * -fno-fast-math). Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline may load it, and only as the checker; the product library never
+ * links it.
+ *
+ * Byte layouts are those of include/tgx.h (the data model).
+ */
+#ifndef TGX_ORACLE_H
+#define TGX_ORACLE_H
+
+#include "../include/tgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scene view used by every oracle function (arrays are not copied). */
+typedef struct tgxo_scene {
+    const tgx_object*    objects;
+    uint32_t             n_objects;
+    const uint32_t*      cluster_pointers;
+    uint32_t             n_cluster_pointers;
+    const uint32_t*      cluster_idx_to_object;
+    uint32_t             n_clusters;
+    const uint32_t*      bits;
+    const uint8_t*       materials;
+    const tgx_lut_entry* luts;
+    uint32_t             n_luts;
+} tgxo_scene;
+
+/* Candidate word of one (pixel, cluster pointer) pair, or all ones
+ * (visibility.frag:22-208). */
+uint64_t tgxo_vis_pair(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h,
+                       uint32_t px, uint32_t py, uint32_t cptr);
+
+/* Visibility buffer by exhaustive min over all (pixel, cluster pointer). */
+void tgxo_visibility_brute(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h,
+                           uint64_t* out);
+
+/* Same result through a conservative per-cluster screen rectangle. Rows
+ * with y % row_step != 0 are left untouched (row_step = 1: all rows).
+ * n_threads <= 0 uses the OpenMP default. Returns the number of
+ * (pixel, cluster) pairs evaluated. */
+uint64_t tgxo_visibility_culled(const tgxo_scene* s, const tgx_camera* cam, uint32_t w,
+                                uint32_t h, uint32_t row_step, int n_threads, uint64_t* out);
+
+/* Conservative pixel rectangle of one cluster pointer: returns 0 if no
+ * pixel ray can hit it, else 1 with the inclusive rectangle. */
+int tgxo_cluster_rect(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h,
+                      uint32_t cptr, uint32_t rect[4]);
+
+/* Algorithmic bytes of a visibility frame: 64 x unique 64-B lines of the
+ * input arrays that the culled oracle reads for pairs whose slab test
+ * passes, plus 8 bytes per output pixel. */
+uint64_t tgxo_visibility_alg_bytes(const tgxo_scene* s, const tgx_camera* cam, uint32_t w,
+                                   uint32_t h);
+
+/* Slab test (collide.inc:3-24, tg_physics.c:394-406). */
+int tgxo_intersect_ray_aabb(const float o[3], const float d[3], const float mn[3],
+                            const float mx[3], float* enter, float* exit_);
+
+/* The reference's m4 helpers, exported for the closed-form tests. */
+void tgxo_m4_mul(const float a[16], const float b[16], float out[16]);
+void tgxo_m4_inverse(const float a[16], float out[16]);
+void tgxo_world_to_cluster(const tgx_object* o, uint32_t rel_cptr, float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,147 @@
+"""GPU parity of primary visibility: libtgx.so (gfx950) against the CPU
+oracle, bit-exact on every pixel, through the C ABI."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tg_amd as T
+from tests import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(f"[gpu] {msg}", file=sys.stderr, flush=True)
+
+
+def assert_same(gpu: np.ndarray, ref: np.ndarray, what: str) -> None:
+    bad = np.argwhere(gpu != ref)
+    assert bad.size == 0, f"{what}: {len(bad)} pixels differ, first {bad[:5].tolist()} " \
+                          f"gpu={[hex(int(gpu[tuple(b)])) for b in bad[:3]]} ref={[hex(int(ref[tuple(b)])) for b in bad[:3]]}"
+
+
+def test_single_object_256_bit_exact(ctx):
+    p = T.single_object_params(1)
+    sc = T.generate_scene(p)
+    ctx.upload(sc)
+    os_ = O.OracleScene(sc)
+    for ci in range(3):
+        say(f"single object 256x256 camera {ci}")
+        cam = T.generate_camera(p, ci, 256, 256)
+        gpu = ctx.visibility(cam, 256, 256)[0]
+        ref = O.visibility_brute(os_, cam, 256, 256)
+        assert_same(gpu, ref, f"camera {ci}")
+        assert (ref != T.VISIBILITY_EMPTY).mean() > 0.1
+
+
+@pytest.fixture(scope="module")
+def noise_scene():
+    p = T.noise_objects_params(seed=1)
+    return p, T.generate_scene(p)
+
+
+def test_noise_objects_1080p_bit_exact(ctx, noise_scene):
+    p, sc = noise_scene
+    ctx.upload(sc)
+    os_ = O.OracleScene(sc)
+    for ci in range(2):
+        say(f"512 noise objects 1920x1080 camera {ci}")
+        cam = T.generate_camera(p, ci, 1920, 1080)
+        gpu = ctx.visibility(cam, 1920, 1080)[0]
+        ref, _ = O.visibility_culled(os_, cam, 1920, 1080)
+        assert_same(gpu, ref, f"camera {ci}")
+        assert (ref != T.VISIBILITY_EMPTY).mean() > 0.3
+
+
+def test_multi_camera_batch_equals_single_cameras(ctx, noise_scene):
+    p, sc = noise_scene
+    ctx.upload(sc)
+    cams = np.concatenate([T.generate_camera(p, i, 640, 360) for i in range(5)])
+    say("5-camera batch 640x360")
+    batch = ctx.visibility(cams, 640, 360)
+    os_ = O.OracleScene(sc)
+    for i in range(5):
+        ref, _ = O.visibility_culled(os_, cams[i:i + 1], 640, 360)
+        assert_same(batch[i], ref, f"camera {i}")
+
+
+def test_odd_resolution_and_permuted_clusters(ctx, noise_scene):
+    p, sc = noise_scene
+    perm = S.permute_clusters(sc, 5)
+    cam = T.generate_camera(p, 3, 333, 197)
+    ref, _ = O.visibility_culled(O.OracleScene(sc), cam, 333, 197)
+    say("333x197, identity cluster map")
+    ctx.upload(sc)
+    assert_same(ctx.visibility(cam, 333, 197)[0], ref, "identity map")
+    say("333x197, permuted cluster map")
+    ctx.upload(perm)
+    assert_same(ctx.visibility(cam, 333, 197)[0], ref, "permuted map")
+
+
+def test_camera_inside_and_near_clusters(ctx):
+    rng = np.random.default_rng(11)
+    objs = [{"n": (3, 2, 2), "translation": (0.0, 0.0, 0.0), "rotation": S.random_rotation(rng)},
+            {"n": (1, 1, 1), "translation": (0.0, 0.0, 4.0)},
+            {"n": (2, 2, 1), "translation": (6.0, -1.0, 0.5), "rotation": S.random_rotation(rng)}]
+    sc = S.make_scene(objs, seed=4)
+    ctx.upload(sc)
+    os_ = O.OracleScene(sc)
+    for i, pos in enumerate([(0.1, 0.2, 0.3), (0.0, 0.0, 8.0), (4.0, 4.0, 4.0), (-4.0, 0.0, 0.0)]):
+        say(f"near-camera case {i} at {pos}")
+        cam = T.make_camera(pos, pitch=0.3 * i - 0.5, yaw=1.1 * i, roll=0.0, fov_y=1.2, aspect=1.25)
+        assert_same(ctx.visibility(cam, 40, 32)[0], O.visibility_brute(os_, cam, 40, 32), f"case {i}")
+
+
+def test_zero_direction_components(ctx):
+    objs = [{"n": (2, 2, 2), "translation": (float(x), float(y), -40.0)}
+            for x, y in ((-20, 0), (0, 0), (21, 5), (0, -18), (-8, 9))]
+    sc = S.make_scene(objs, seed=8)
+    ctx.upload(sc)
+    cam = T.make_camera((0.0, 0.0, 0.0), pitch=0.0, yaw=0.0, roll=0.0, fov_y=1.0, aspect=1.0)
+    say("axis-aligned camera 33x33")
+    assert_same(ctx.visibility(cam, 33, 33)[0], O.visibility_brute(O.OracleScene(sc), cam, 33, 33), "axis camera")
+
+
+def test_device_output_and_clear(ctx):
+    p = T.single_object_params(4)
+    sc = T.generate_scene(p)
+    ctx.upload(sc)
+    cam = T.generate_camera(p, 0, 64, 48)
+    dev = ctx.alloc(64 * 48 * 8)
+    try:
+        say("device-pointer output 64x48")
+        ctx.visibility_device(cam, 64, 48, dev)
+        ctx.synchronize()
+        out = np.empty((48, 64), np.uint64)
+        ctx.d2h(out, dev)
+        assert_same(out, O.visibility_brute(O.OracleScene(sc), cam, 64, 48), "device output")
+        rc = ctx.lib.tgx_visibility_clear_device(ctx.h, dev, 64 * 48)
+        assert rc == 0
+        ctx.d2h(out, dev)
+        assert np.all(out == T.VISIBILITY_EMPTY)
+    finally:
+        ctx.free(dev)
+
+
+def test_error_paths(ctx):
+    sc = T.generate_scene(T.single_object_params(1))
+    bad = T.Scene(sc.objects, sc.cluster_pointers.copy(), sc.cluster_idx_to_object, sc.bits, sc.materials, sc.luts)
+    bad.cluster_pointers[3] = 64  # out of range
+    with pytest.raises(T.TgxError) as e:
+        ctx.upload(bad)
+    assert e.value.code == 1
+    cam = T.generate_camera(T.single_object_params(1), 0, 8, 8)
+    with pytest.raises(T.TgxError) as e:
+        ctx.visibility(cam, 0, 8)
+    assert e.value.code == 1
+    fresh = T.Context(0)
+    try:
+        with pytest.raises(T.TgxError) as e:
+            fresh.visibility(cam, 8, 8)
+        assert e.value.code == 4
+    finally:
+        fresh.close()

@@ -1,0 +1,325 @@
+"""tg_amd — MI355X-native voxel ray-tracing hot path of tg.
+
+Python view of the C ABI in include/tgx.h (libtgx.so, built in-tree by
+``python -m tg_amd.build``). The GPU work happens in libtgx.so; this module
+only moves numpy arrays across the boundary. There is no CPU fallback: if
+libtgx.so is missing, importing the bindings raises.
+
+Byte layouts (numpy dtypes below) are tg's std430 records
+(buffers.inc:6-13 objects, buffers.inc:30-39 camera).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libtgx.so"
+
+OBJECT_DTYPE = np.dtype([
+    ("n_cluster_pointers_per_dim", "<u4", (3,)),
+    ("first_cluster_pointer", "<u4"),
+    ("translation", "<f4", (3,)),
+    ("lut_idx", "<u4"),
+    ("rotation", "<f4", (16,)),
+])
+CAMERA_DTYPE = np.dtype([
+    ("camera", "<f4", (4,)),
+    ("ray_bl", "<f4", (4,)),
+    ("ray_br", "<f4", (4,)),
+    ("ray_tr", "<f4", (4,)),
+    ("ray_tl", "<f4", (4,)),
+    ("near_plane", "<f4"),
+    ("far_plane", "<f4"),
+])
+LUT_DTYPE = np.dtype([("albedo", "<f4", (3,)), ("roughness", "<f4")])
+assert OBJECT_DTYPE.itemsize == 96 and CAMERA_DTYPE.itemsize == 88 and LUT_DTYPE.itemsize == 16
+
+VISIBILITY_EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+GEN_SINGLE_BERNOULLI = 0
+GEN_NOISE_OBJECTS = 1
+
+STATUS = {0: "TGX_OK", 1: "TGX_ERR_INVALID_ARG", 2: "TGX_ERR_CAPACITY", 3: "TGX_ERR_HIP",
+          4: "TGX_ERR_NO_SCENE", 5: "TGX_ERR_NO_DEVICE", 6: "TGX_ERR_OUT_OF_MEMORY",
+          7: "TGX_ERR_NOT_BUILT"}
+
+
+class TgxError(RuntimeError):
+    def __init__(self, code: int, message: str = ""):
+        self.code = code
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+
+
+class _SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("objects", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
+        ("cluster_pointers", ctypes.c_void_p), ("n_cluster_pointers", ctypes.c_uint32),
+        ("cluster_idx_to_object", ctypes.c_void_p), ("n_clusters", ctypes.c_uint32),
+        ("bits", ctypes.c_void_p), ("materials", ctypes.c_void_p),
+        ("luts", ctypes.c_void_p), ("n_luts", ctypes.c_uint32),
+    ]
+
+
+class _GenParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("n_objects", ctypes.c_uint32),
+                ("max_extent", ctypes.c_uint32), ("world", ctypes.c_float), ("n_luts", ctypes.c_uint32)]
+
+
+class _GenSizes(ctypes.Structure):
+    _fields_ = [("n_objects", ctypes.c_uint32), ("n_cluster_pointers", ctypes.c_uint32),
+                ("n_clusters", ctypes.c_uint32), ("n_luts", ctypes.c_uint32)]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
+
+
+class VisibilityStats(ctypes.Structure):
+    _fields_ = [("clusters_total", ctypes.c_uint64), ("clusters_binned", ctypes.c_uint64),
+                ("tile_entries", ctypes.c_uint64), ("tiles", ctypes.c_uint32), ("tile_size", ctypes.c_uint32)]
+
+
+_LIB: ctypes.CDLL | None = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Loads libtgx.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m tg_amd.build` (there is no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sigs = {
+        "tgx_create": (i32, [i32, ctypes.POINTER(vp)]),
+        "tgx_destroy": (i32, [vp]),
+        "tgx_last_error": (ctypes.c_char_p, [vp]),
+        "tgx_synchronize": (i32, [vp]),
+        "tgx_scene_upload": (i32, [vp, ctypes.POINTER(_SceneDesc)]),
+        "tgx_visibility": (i32, [vp, vp, u32, u32, u32, vp]),
+        "tgx_visibility_device": (i32, [vp, vp, u32, u32, u32, vp]),
+        "tgx_visibility_clear_device": (i32, [vp, vp, u64]),
+        "tgx_visibility_decode": (i32, [u64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "tgx_visibility_last_stats": (i32, [vp, ctypes.POINTER(VisibilityStats)]),
+        "tgx_device_alloc": (i32, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
+        "tgx_device_free": (i32, [vp, vp]),
+        "tgx_memcpy_d2h": (i32, [vp, vp, vp, ctypes.c_size_t]),
+        "tgx_memcpy_h2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
+        "tgx_profile_enable": (i32, [vp, i32]),
+        "tgx_profile_reset": (i32, [vp]),
+        "tgx_profile_get": (i32, [vp, ctypes.POINTER(KernelStat), u32, ctypes.POINTER(u32)]),
+        "tgx_gen_scene_sizes": (i32, [ctypes.POINTER(_GenParams), ctypes.POINTER(_GenSizes)]),
+        "tgx_gen_scene_fill": (i32, [ctypes.POINTER(_GenParams), vp, vp, vp, vp, vp, vp]),
+        "tgx_gen_camera": (i32, [ctypes.POINTER(_GenParams), u32, u32, u32, vp]),
+        "tgx_camera_make": (i32, [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
+        "tgx_gen_stream_seed": (u64, [u64, u64]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C-ABI functions the bindings declare."""
+    return [n for n in dir(load_library()) if n.startswith("tgx_")]
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays crossing the C ABI must be C-contiguous"
+    return a.ctypes.data
+
+
+# --------------------------------------------------------------------------
+# Scenes
+# --------------------------------------------------------------------------
+
+@dataclass
+class Scene:
+    """tg's object/cluster data model as numpy arrays (include/tgx.h)."""
+    objects: np.ndarray                 # OBJECT_DTYPE[n_objects]
+    cluster_pointers: np.ndarray        # u32[n_cluster_pointers]: cptr -> cidx
+    cluster_idx_to_object: np.ndarray   # u32[n_clusters]
+    bits: np.ndarray                    # u32[16 * n_clusters]
+    materials: np.ndarray               # u8[512 * n_clusters]
+    luts: np.ndarray                    # LUT_DTYPE[256 * n_luts]
+
+    @property
+    def n_clusters(self) -> int:
+        return int(self.cluster_idx_to_object.size)
+
+    @property
+    def n_cluster_pointers(self) -> int:
+        return int(self.cluster_pointers.size)
+
+    def desc(self) -> _SceneDesc:
+        return _SceneDesc(
+            _ptr(self.objects), self.objects.size,
+            _ptr(self.cluster_pointers), self.cluster_pointers.size,
+            _ptr(self.cluster_idx_to_object), self.cluster_idx_to_object.size,
+            _ptr(self.bits), _ptr(self.materials), _ptr(self.luts), self.luts.size // 256,
+        )
+
+    def solid_fraction(self) -> float:
+        return float(np.unpackbits(self.bits.view(np.uint8)).mean())
+
+
+def gen_params(kind: int, seed: int, n_objects: int = 0, max_extent: int = 0, world: float = 0.0,
+               n_luts: int = 1) -> _GenParams:
+    return _GenParams(kind, seed, n_objects, max_extent, world, n_luts)
+
+
+def generate_scene(params: _GenParams) -> Scene:
+    lib = load_library()
+    sizes = _GenSizes()
+    rc = lib.tgx_gen_scene_sizes(ctypes.byref(params), ctypes.byref(sizes))
+    if rc:
+        raise TgxError(rc, "tgx_gen_scene_sizes")
+    s = Scene(
+        objects=np.zeros(sizes.n_objects, OBJECT_DTYPE),
+        cluster_pointers=np.zeros(sizes.n_cluster_pointers, np.uint32),
+        cluster_idx_to_object=np.zeros(sizes.n_clusters, np.uint32),
+        bits=np.zeros(16 * sizes.n_clusters, np.uint32),
+        materials=np.zeros(512 * sizes.n_clusters, np.uint8),
+        luts=np.zeros(256 * sizes.n_luts, LUT_DTYPE),
+    )
+    rc = lib.tgx_gen_scene_fill(ctypes.byref(params), _ptr(s.objects), _ptr(s.cluster_pointers),
+                                _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials), _ptr(s.luts))
+    if rc:
+        raise TgxError(rc, "tgx_gen_scene_fill")
+    return s
+
+
+def generate_camera(params: _GenParams, index: int, w: int, h: int) -> np.ndarray:
+    cam = np.zeros(1, CAMERA_DTYPE)
+    rc = load_library().tgx_gen_camera(ctypes.byref(params), index, w, h, _ptr(cam))
+    if rc:
+        raise TgxError(rc, "tgx_gen_camera")
+    return cam
+
+
+def make_camera(position, pitch: float, yaw: float, roll: float, fov_y: float, aspect: float,
+                near: float = 0.1, far: float = 1000.0) -> np.ndarray:
+    cam = np.zeros(1, CAMERA_DTYPE)
+    pos = np.ascontiguousarray(position, dtype=np.float32)
+    rc = load_library().tgx_camera_make(_ptr(pos), pitch, yaw, roll, fov_y, aspect, near, far, _ptr(cam))
+    if rc:
+        raise TgxError(rc, "tgx_camera_make")
+    return cam
+
+
+# Workloads of the benchmark and the parity tests.
+def single_object_params(seed: int = 1) -> _GenParams:
+    """One 4x4x4-cluster object, Bernoulli(0.3) solid bits, one LUT."""
+    return gen_params(GEN_SINGLE_BERNOULLI, seed, n_luts=1)
+
+
+def noise_objects_params(seed: int = 1, n_objects: int = 512, max_extent: int = 8, world: float = 512.0,
+                         n_luts: int = 1) -> _GenParams:
+    """n_objects noise-filled objects of 1..max_extent clusters per axis."""
+    return gen_params(GEN_NOISE_OBJECTS, seed, n_objects, max_extent, world, n_luts)
+
+
+# --------------------------------------------------------------------------
+# GPU context
+# --------------------------------------------------------------------------
+
+def decode(word: int) -> tuple[bool, float, int, int]:
+    """tgx_visibility_decode: (hit, depth, cluster pointer, voxel idx)."""
+    d, c, v = ctypes.c_float(), ctypes.c_uint32(), ctypes.c_uint32()
+    hit = load_library().tgx_visibility_decode(int(word), ctypes.byref(d), ctypes.byref(c), ctypes.byref(v))
+    return bool(hit), d.value, c.value, v.value
+
+
+class Context:
+    """Owns one tgx_context (one HIP device, one stream)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.tgx_create(device, ctypes.byref(h))
+        if rc:
+            raise TgxError(rc, f"tgx_create(device={device})")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.tgx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc:
+            msg = self.lib.tgx_last_error(self.h)
+            raise TgxError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def upload(self, scene: Scene) -> None:
+        self._scene_ref = scene  # keep arrays alive while uploading
+        desc = scene.desc()
+        self._check(self.lib.tgx_scene_upload(self.h, ctypes.byref(desc)), "tgx_scene_upload")
+
+    def visibility(self, cams: np.ndarray, w: int, h: int) -> np.ndarray:
+        cams = np.ascontiguousarray(cams, dtype=CAMERA_DTYPE)
+        out = np.empty((cams.size, h, w), np.uint64)
+        self._check(self.lib.tgx_visibility(self.h, _ptr(cams), cams.size, w, h, _ptr(out)), "tgx_visibility")
+        return out
+
+    def visibility_device(self, cams: np.ndarray, w: int, h: int, out_dev: int) -> None:
+        cams = np.ascontiguousarray(cams, dtype=CAMERA_DTYPE)
+        self._check(self.lib.tgx_visibility_device(self.h, _ptr(cams), cams.size, w, h, out_dev),
+                    "tgx_visibility_device")
+
+    def last_stats(self) -> VisibilityStats:
+        st = VisibilityStats()
+        self._check(self.lib.tgx_visibility_last_stats(self.h, ctypes.byref(st)), "tgx_visibility_last_stats")
+        return st
+
+    def alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        self._check(self.lib.tgx_device_alloc(self.h, nbytes, ctypes.byref(p)), "tgx_device_alloc")
+        return p.value
+
+    def free(self, ptr: int) -> None:
+        self._check(self.lib.tgx_device_free(self.h, ptr), "tgx_device_free")
+
+    def d2h(self, dst: np.ndarray, src_dev: int) -> None:
+        self._check(self.lib.tgx_memcpy_d2h(self.h, _ptr(dst), src_dev, dst.nbytes), "tgx_memcpy_d2h")
+
+    def h2d(self, dst_dev: int, src: np.ndarray) -> None:
+        self._check(self.lib.tgx_memcpy_h2d(self.h, dst_dev, _ptr(src), src.nbytes), "tgx_memcpy_h2d")
+
+    def synchronize(self) -> None:
+        self._check(self.lib.tgx_synchronize(self.h), "tgx_synchronize")
+
+    def profile(self, enable: bool = True) -> None:
+        self._check(self.lib.tgx_profile_enable(self.h, int(enable)), "tgx_profile_enable")
+
+    def profile_reset(self) -> None:
+        self._check(self.lib.tgx_profile_reset(self.h), "tgx_profile_reset")
+
+    def profile_stats(self) -> dict[str, tuple[int, float]]:
+        n = ctypes.c_uint32()
+        buf = (KernelStat * 64)()
+        self._check(self.lib.tgx_profile_get(self.h, buf, 64, ctypes.byref(n)), "tgx_profile_get")
+        return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(min(n.value, 64))}

@@ -1,0 +1,90 @@
+/*
+ * tgx_internal.h — context state shared by the library's translation units.
+ */
+#ifndef TGX_INTERNAL_H
+#define TGX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tgx.h"
+
+/* Device buffer that only grows. */
+struct tgx_buffer {
+    void*  ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct tgx_pending_event {
+    int         stat_idx;
+    hipEvent_t  start;
+    hipEvent_t  stop;
+};
+
+struct tgx_context {
+    int          device = 0;
+    hipStream_t  stream = nullptr;
+    std::string  last_error;
+
+    /* Scene (device copies of tgx_scene_desc arrays). */
+    bool         has_scene = false;
+    uint32_t     n_objects = 0;
+    uint32_t     n_cluster_pointers = 0;
+    uint32_t     n_clusters = 0;
+    uint32_t     n_luts = 0;
+    tgx_buffer   objects, cluster_pointers, cluster_idx_to_object, bits, materials, luts;
+
+    /* Visibility work buffers. */
+    tgx_buffer   vis_clusters;   /* compacted per-(camera, cluster) records */
+    tgx_buffer   vis_counters;   /* per-camera record counts + scalars      */
+    tgx_buffer   vis_tile_count; /* per-tile entry counts                   */
+    tgx_buffer   vis_tile_off;   /* per-tile list offsets (exclusive scan)  */
+    tgx_buffer   vis_tile_list;  /* record indices per tile                 */
+    tgx_buffer   vis_cams;       /* camera constants                        */
+    tgx_buffer   vis_out;        /* staging output for tgx_visibility       */
+    void*        host_pinned = nullptr; /* small pinned readback area       */
+    tgx_visibility_stats vis_stats = {};
+
+    /* Profiling. */
+    bool                           profiling = false;
+    std::vector<tgx_kernel_stat>   stats;
+    std::vector<tgx_pending_event> pending;
+};
+
+int  tgx_fail(tgx_context* ctx, int code, const char* fmt, ...);
+int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
+void tgx_buffer_release(tgx_buffer* b);
+
+/* Kernel launch bracket: records HIP events on the context stream around a
+ * launch when profiling is enabled. */
+struct tgx_launch_scope {
+    tgx_context* ctx;
+    int          stat_idx = -1;
+    hipEvent_t   start = nullptr, stop = nullptr;
+    tgx_launch_scope(tgx_context* c, const char* name);
+    ~tgx_launch_scope();
+};
+
+#define TGX_HIP_CHECK(ctx, call)                                                          \
+    do {                                                                                  \
+        hipError_t _e = (call);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return tgx_fail((ctx), TGX_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                            hipGetErrorString(_e), __FILE__, __LINE__);                   \
+    } while (0)
+
+#define TGX_LAUNCH_CHECK(ctx, name)                                                       \
+    do {                                                                                  \
+        hipError_t _e = hipGetLastError();                                                \
+        if (_e != hipSuccess)                                                             \
+            return tgx_fail((ctx), TGX_ERR_HIP, "launch of %s failed: %s", (name),        \
+                            hipGetErrorString(_e));                                       \
+    } while (0)
+
+#endif

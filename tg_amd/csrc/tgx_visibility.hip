@@ -1,0 +1,618 @@
+/*
+ * tgx_visibility.hip — primary visibility on gfx950.
+ *
+ * Replaces tg's instanced raster of cluster boxes (visibility.vert:19-26)
+ * and the per-fragment DDA (visibility.frag:22-208) with three compute
+ * passes per frame:
+ *
+ *   1. vis_setup     one thread per (camera, cluster pointer): drops empty
+ *                    clusters and clusters no pixel ray can reach, builds
+ *                    the cluster's world->cluster frame (exact fp32 order of
+ *                    visibility.frag:57-66), a conservative pixel rectangle
+ *                    and a lower bound of its depth; counts 16x16 tiles.
+ *   2. vis_scan +    tile lists: exclusive scan of the tile counts and a
+ *      vis_scatter   scatter of record indices into each covered tile.
+ *   3. vis_traverse  one 256-thread workgroup per tile, one pixel per lane,
+ *                    each wave an 8x8 pixel block. For each listed cluster
+ *                    the wave skips it unless some lane's pixel lies in the
+ *                    rectangle and that lane's current best depth is not
+ *                    below the cluster's depth bound; the cluster's 64 B of
+ *                    solid bits are staged in LDS and every active lane runs
+ *                    the slab test and the 8^3 Amanatides-Woo walk, keeping
+ *                    the minimum packed word. A tile is owned by one
+ *                    workgroup, so the min needs no atomics and the buffer
+ *                    needs no clear.
+ *
+ * Bit-exactness: all code here is compiled with -ffp-contract=off, IEEE
+ * fp32 division and square root, denormals preserved; the per-pair
+ * arithmetic follows visibility.frag operation by operation.
+ */
+#include "tgx_internal.h"
+#include "tgx_m4.h"
+
+#include <math.h>
+
+#define VIS_TILE 16u
+#define VIS_F32_MAX 3.402823466e+38f
+
+/* Conservative-rectangle constants: the cluster box is grown by GAMMA
+ * voxels, clipped at the camera-plane distance EPS; a box reaching within
+ * EPS + DELTA of the camera in every basis coordinate covers the screen. */
+#define VIS_RECT_GAMMA 0.03125f
+#define VIS_RECT_EPS   0.25f
+#define VIS_RECT_DELTA 0.01f
+
+struct vis_cam {
+    float    pos[3];
+    float    far_plane;
+    float    bl[3], br[3], tr[3], tl[3];
+    float    inv[9];      /* [bl | br-bl | tl-bl]^-1, row-major            */
+    uint32_t full_screen; /* corner quad not planar: no rectangle cull    */
+    uint32_t rec_base;    /* first record slot of this camera             */
+    uint32_t tile_base;   /* first tile of this camera                    */
+    uint32_t pad;
+};
+
+/* Per-(camera, cluster) record, 80 B. */
+struct vis_cluster {
+    float    m[12];       /* world->cluster rows 0..2, columns 0..3        */
+    float    o[3];        /* camera position in cluster space              */
+    uint32_t cptr;
+    uint32_t cidx;
+    uint32_t rect_lo;     /* x0 | y0 << 16                                 */
+    uint32_t rect_hi;     /* x1 | y1 << 16                                 */
+    uint32_t depth_lb24;  /* no hit of this cluster has a smaller depth24  */
+};
+static_assert(sizeof(vis_cluster) == 80, "vis_cluster layout");
+
+/* ------------------------------------------------------------------------ */
+/* Pair evaluation (visibility.frag:65-206)                                 */
+/* ------------------------------------------------------------------------ */
+
+__device__ static inline float glsl_max(float x, float y) { return x < y ? y : x; }
+__device__ static inline float glsl_clamp(float x, float lo, float hi)
+{
+    const float t = x < lo ? lo : x;
+    return hi < t ? hi : t;
+}
+__device__ static inline float min2(float a, float b) { return a < b ? a : b; }
+__device__ static inline float max2(float a, float b) { return a > b ? a : b; }
+
+/* collide.inc:3-24 */
+__device__ static inline bool slab(float ox, float oy, float oz, float dx, float dy, float dz,
+                                   float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                   float& enter, float& exit_)
+{
+    const float v0x = (dx == 0.0f) ? -VIS_F32_MAX : ((mnx - ox) / dx);
+    const float v0y = (dy == 0.0f) ? -VIS_F32_MAX : ((mny - oy) / dy);
+    const float v0z = (dz == 0.0f) ? -VIS_F32_MAX : ((mnz - oz) / dz);
+    const float v1x = (dx == 0.0f) ? VIS_F32_MAX : ((mxx - ox) / dx);
+    const float v1y = (dy == 0.0f) ? VIS_F32_MAX : ((mxy - oy) / dy);
+    const float v1z = (dz == 0.0f) ? VIS_F32_MAX : ((mxz - oz) / dz);
+    const float nx = min2(v0x, v1x), ny = min2(v0y, v1y), nz = min2(v0z, v1z);
+    const float fx = max2(v0x, v1x), fy = max2(v0y, v1y), fz = max2(v0z, v1z);
+    enter = max2(max2(nx, ny), nz);
+    exit_ = min2(min2(fx, fy), fz);
+    return exit_ > 0.0f && enter <= exit_;
+}
+
+/* Candidate word of one (pixel, cluster) pair; lds_bits holds the 16 solid
+ * words of the cluster. */
+__device__ static inline uint64_t pair_word(const vis_cluster& c, const uint32_t* lds_bits,
+                                            float dwx, float dwy, float dwz, float far_plane)
+{
+    /* normalize((ws2ms * v4(dir, 0)).xyz) */
+    const float rx = dwx * c.m[0] + dwy * c.m[1] + dwz * c.m[2] + 0.0f * c.m[3];
+    const float ry = dwx * c.m[4] + dwy * c.m[5] + dwz * c.m[6] + 0.0f * c.m[7];
+    const float rz = dwx * c.m[8] + dwy * c.m[9] + dwz * c.m[10] + 0.0f * c.m[11];
+    const float mag = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float dx = rx / mag, dy = ry / mag, dz = rz / mag;
+    const float ox = c.o[0], oy = c.o[1], oz = c.o[2];
+
+    /* A ray parallel to a slab but outside it is a miss (the slab test
+     * alone leaves a zero-direction axis unconstrained, collide.inc:5-11). */
+    if ((dx == 0.0f && (ox < 0.0f || ox > 8.0f)) || (dy == 0.0f && (oy < 0.0f || oy > 8.0f)) ||
+        (dz == 0.0f && (oz < 0.0f || oz > 8.0f)))
+        return TGX_VISIBILITY_EMPTY;
+    float enter, exit_;
+    if (!slab(ox, oy, oz, dx, dy, dz, 0.0f, 0.0f, 0.0f, 8.0f, 8.0f, 8.0f, enter, exit_))
+        return TGX_VISIBILITY_EMPTY;
+
+    float hx = ox, hy = oy, hz = oz;
+    if (enter > 0.0f) {
+        hx = ox + enter * dx;
+        hy = oy + enter * dy;
+        hz = oz + enter * dz;
+    }
+    int x = (int)glsl_clamp(floorf(hx), 0.0f, 7.0f);
+    int y = (int)glsl_clamp(floorf(hy), 0.0f, 7.0f);
+    int z = (int)glsl_clamp(floorf(hz), 0.0f, 7.0f);
+
+    int sx = 0, sy = 0, sz = 0;
+    float tmx = VIS_F32_MAX, tmy = VIS_F32_MAX, tmz = VIS_F32_MAX;
+    float tdx = VIS_F32_MAX, tdy = VIS_F32_MAX, tdz = VIS_F32_MAX;
+    if (dx > 0.0f) { sx = 1; tmx = enter + ((float)(x + 1) - hx) / dx; tdx = 1.0f / dx; }
+    else if (dx < 0.0f) { sx = -1; tmx = enter + (hx - (float)x) / -dx; tdx = 1.0f / -dx; }
+    if (dy > 0.0f) { sy = 1; tmy = enter + ((float)(y + 1) - hy) / dy; tdy = 1.0f / dy; }
+    else if (dy < 0.0f) { sy = -1; tmy = enter + (hy - (float)y) / -dy; tdy = 1.0f / -dy; }
+    if (dz > 0.0f) { sz = 1; tmz = enter + ((float)(z + 1) - hz) / dz; tdz = 1.0f / dz; }
+    else if (dz < 0.0f) { sz = -1; tmz = enter + (hz - (float)z) / -dz; tdz = 1.0f / -dz; }
+
+    for (uint32_t it = 0; it < TGX_DDA_MAX_ITERATIONS; it++) {
+        /* cidx*512 is a multiple of 32: the word within the cluster is
+         * rel/32 and the bit rel%32 (visibility.frag:143-148). */
+        const uint32_t rel = 64u * (uint32_t)z + 8u * (uint32_t)y + (uint32_t)x;
+        if (lds_bits[rel >> 5] & (1u << (rel & 31u))) {
+            float ve, vx;
+            (void)slab(ox, oy, oz, dx, dy, dz, 0.0f + (float)x, 0.0f + (float)y, 0.0f + (float)z,
+                       0.0f + (float)(x + 1), 0.0f + (float)(y + 1), 0.0f + (float)(z + 1), ve, vx);
+            const float d = glsl_max(0.0f, ve / far_plane);
+            if (d <= 1.0f)
+                return ((uint64_t)(d * 16777215.0f) << 40) | ((uint64_t)c.cptr << 9) | (uint64_t)rel;
+            return TGX_VISIBILITY_EMPTY;
+        }
+        if (tmx < tmy) {
+            if (tmx < tmz) { tmx += tdx; x += sx; if (x < 0 || x >= 8) break; }
+            else           { tmz += tdz; z += sz; if (z < 0 || z >= 8) break; }
+        } else {
+            if (tmy < tmz) { tmy += tdy; y += sy; if (y < 0 || y >= 8) break; }
+            else           { tmz += tdz; z += sz; if (z < 0 || z >= 8) break; }
+        }
+    }
+    return TGX_VISIBILITY_EMPTY;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pass 1: per-cluster setup and conservative rectangle                     */
+/* ------------------------------------------------------------------------ */
+
+struct rect_acc { float fxmin, fxmax, fymin, fymax; int n; };
+
+__device__ static inline void rect_add(rect_acc& r, float w0, float w1, float w2)
+{
+    const float fx = w1 / w0, fy = w2 / w0;
+    r.fxmin = fminf(r.fxmin, fx); r.fxmax = fmaxf(r.fxmax, fx);
+    r.fymin = fminf(r.fymin, fy); r.fymax = fmaxf(r.fymax, fy);
+    r.n++;
+}
+
+/* Returns false if no pixel ray can hit the grown box. */
+__device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, uint32_t rx,
+                                    uint32_t ry, uint32_t rz, uint32_t W, uint32_t H,
+                                    uint32_t& x0, uint32_t& y0, uint32_t& x1, uint32_t& y1)
+{
+    if (cam.full_screen) { x0 = 0; y0 = 0; x1 = W - 1; y1 = H - 1; return true; }
+    const float bx = -4.0f * (float)ob.n_cluster_pointers_per_dim[0] + 8.0f * (float)rx;
+    const float by = -4.0f * (float)ob.n_cluster_pointers_per_dim[1] + 8.0f * (float)ry;
+    const float bz = -4.0f * (float)ob.n_cluster_pointers_per_dim[2] + 8.0f * (float)rz;
+    const float tcx = ob.translation[0] - cam.pos[0];
+    const float tcy = ob.translation[1] - cam.pos[1];
+    const float tcz = ob.translation[2] - cam.pos[2];
+    float w[8][3];
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float lx = bx + ((k & 1) ? 8.0f + VIS_RECT_GAMMA : -VIS_RECT_GAMMA);
+        const float ly = by + ((k & 2) ? 8.0f + VIS_RECT_GAMMA : -VIS_RECT_GAMMA);
+        const float lz = bz + ((k & 4) ? 8.0f + VIS_RECT_GAMMA : -VIS_RECT_GAMMA);
+        const float px = ob.rotation[0] * lx + ob.rotation[4] * ly + ob.rotation[8] * lz + tcx;
+        const float py = ob.rotation[1] * lx + ob.rotation[5] * ly + ob.rotation[9] * lz + tcy;
+        const float pz = ob.rotation[2] * lx + ob.rotation[6] * ly + ob.rotation[10] * lz + tcz;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float v = cam.inv[3 * i + 0] * px + cam.inv[3 * i + 1] * py + cam.inv[3 * i + 2] * pz;
+            w[k][i] = v;
+            lo[i] = fminf(lo[i], v);
+            hi[i] = fmaxf(hi[i], v);
+        }
+    }
+    bool near_cube = true;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if (hi[i] < -VIS_RECT_DELTA || lo[i] > VIS_RECT_EPS + VIS_RECT_DELTA) near_cube = false;
+    if (near_cube) { x0 = 0; y0 = 0; x1 = W - 1; y1 = H - 1; return true; }
+    if (hi[0] < VIS_RECT_EPS) return false;
+
+    rect_acc r = {3e38f, -3e38f, 3e38f, -3e38f, 0};
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (w[k][0] >= VIS_RECT_EPS) rect_add(r, w[k][0], w[k][1], w[k][2]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+#pragma unroll
+        for (int bit = 1; bit < 8; bit <<= 1) {
+            if (k & bit) continue;
+            const int j = k | bit;
+            const float a0 = w[k][0], a1 = w[j][0];
+            if ((a0 < VIS_RECT_EPS) == (a1 < VIS_RECT_EPS)) continue;
+            const float t = (VIS_RECT_EPS - a0) / (a1 - a0);
+            rect_add(r, VIS_RECT_EPS, w[k][1] + t * (w[j][1] - w[k][1]), w[k][2] + t * (w[j][2] - w[k][2]));
+        }
+    }
+    if (r.n == 0) return false;
+    const float fw = (float)W, fh = (float)H;
+    const float fx0 = floorf(r.fxmin * fw - 0.5f) - 1.0f, fx1 = ceilf(r.fxmax * fw - 0.5f) + 1.0f;
+    const float fy0 = floorf((1.0f - r.fymax) * fh - 0.5f) - 1.0f, fy1 = ceilf((1.0f - r.fymin) * fh - 0.5f) + 1.0f;
+    if (fx1 < 0.0f || fy1 < 0.0f || fx0 > fw - 1.0f || fy0 > fh - 1.0f) return false;
+    x0 = fx0 < 0.0f ? 0u : (uint32_t)fx0;
+    y0 = fy0 < 0.0f ? 0u : (uint32_t)fy0;
+    x1 = fx1 > fw - 1.0f ? W - 1 : (uint32_t)fx1;
+    y1 = fy1 > fh - 1.0f ? H - 1 : (uint32_t)fy1;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
+                                                 const uint32_t* __restrict__ cptrs,
+                                                 const uint32_t* __restrict__ c2o,
+                                                 const uint32_t* __restrict__ bits, uint32_t n_cptr,
+                                                 const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
+                                                 uint32_t tiles_x, vis_cluster* __restrict__ recs,
+                                                 uint32_t* __restrict__ rec_count,
+                                                 uint32_t* __restrict__ tile_count)
+{
+    const uint32_t cptr = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ci = blockIdx.y;
+    if (cptr >= n_cptr) return;
+    const uint32_t cidx = cptrs[cptr];
+    const uint4* b4 = reinterpret_cast<const uint4*>(bits + (size_t)cidx * 16u);
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { const uint4 q = b4[i]; any |= q.x | q.y | q.z | q.w; }
+    if (!any) return; /* no solid voxel: the walk can never hit */
+
+    const vis_cam cam = cams[ci];
+    const tgx_object ob = objects[c2o[cidx]];
+    const uint32_t rel = cptr - ob.first_cluster_pointer;
+    const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
+    const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
+    uint32_t x0, y0, x1, y1;
+    if (!cluster_rect(cam, ob, rx, ry, rz, W, H, x0, y0, x1, y1)) return;
+
+    const tgx_m4 m = tgx_world_to_cluster(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_v4 o = tgx_m4_mulv4(m, cam.pos[0], cam.pos[1], cam.pos[2], 1.0f);
+
+    /* Lower bound of the slab entry of any ray from o: its distance to the
+     * box [0,8]^3 (unit direction), shrunk for fp32 rounding. */
+    const float ddx = o.x < 0.0f ? -o.x : (o.x > 8.0f ? o.x - 8.0f : 0.0f);
+    const float ddy = o.y < 0.0f ? -o.y : (o.y > 8.0f ? o.y - 8.0f : 0.0f);
+    const float ddz = o.z < 0.0f ? -o.z : (o.z > 8.0f ? o.z - 8.0f : 0.0f);
+    float lb = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) * 0.9999f - 1e-3f;
+    lb = lb > 0.0f ? lb : 0.0f;
+    const float dlb = lb / cam.far_plane;
+    if (dlb > 1.0f) return; /* every hit would have depth > 1 */
+    uint32_t depth_lb24 = (uint32_t)(dlb * 16777215.0f);
+    depth_lb24 = depth_lb24 > 0u ? depth_lb24 - 1u : 0u;
+
+    vis_cluster rec;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) rec.m[4 * r + c] = TGX_M(m, r, c);
+    rec.o[0] = o.x; rec.o[1] = o.y; rec.o[2] = o.z;
+    rec.cptr = cptr;
+    rec.cidx = cidx;
+    rec.rect_lo = x0 | (y0 << 16);
+    rec.rect_hi = x1 | (y1 << 16);
+    rec.depth_lb24 = depth_lb24;
+    const uint32_t k = atomicAdd(&rec_count[ci], 1u);
+    recs[cam.rec_base + k] = rec;
+
+    const uint32_t tx0 = x0 / VIS_TILE, tx1 = x1 / VIS_TILE, ty0 = y0 / VIS_TILE, ty1 = y1 / VIS_TILE;
+    for (uint32_t ty = ty0; ty <= ty1; ty++)
+        for (uint32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&tile_count[cam.tile_base + ty * tiles_x + tx], 1u);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pass 2: tile lists                                                       */
+/* ------------------------------------------------------------------------ */
+
+__global__ void vis_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t value)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+/* Single-workgroup exclusive scan: off[i] = sum(count[0..i)), and
+ * count[i] becomes the scatter cursor (= off[i]); total in *total. */
+__global__ __launch_bounds__(1024) void vis_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ off,
+                                                 uint32_t n, uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = t * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += count[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t c = count[i];
+        off[i] = run;
+        count[i] = run;
+        run += c;
+    }
+    if (t == 1023u) *total = part[1023];
+}
+
+__global__ __launch_bounds__(256) void vis_scatter(const vis_cluster* __restrict__ recs,
+                                                   const uint32_t* __restrict__ rec_count,
+                                                   const vis_cam* __restrict__ cams, uint32_t per_cam_slots,
+                                                   uint32_t tiles_x, uint32_t* __restrict__ cursor,
+                                                   uint32_t* __restrict__ list)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ci = blockIdx.y;
+    if (k >= rec_count[ci]) return;
+    const uint32_t slot = cams[ci].rec_base + k;
+    const vis_cluster& r = recs[slot];
+    const uint32_t tb = cams[ci].tile_base;
+    const uint32_t x0 = r.rect_lo & 0xFFFFu, y0 = r.rect_lo >> 16, x1 = r.rect_hi & 0xFFFFu, y1 = r.rect_hi >> 16;
+    for (uint32_t ty = y0 / VIS_TILE; ty <= y1 / VIS_TILE; ty++)
+        for (uint32_t tx = x0 / VIS_TILE; tx <= x1 / VIS_TILE; tx++) {
+            const uint32_t t = tb + ty * tiles_x + tx;
+            list[atomicAdd(&cursor[t], 1u)] = slot;
+        }
+    (void)per_cam_slots;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pass 3: per-tile traversal                                               */
+/* ------------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(256) void vis_traverse(const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
+                                                    uint32_t tiles_x, uint32_t tiles_per_cam,
+                                                    const uint32_t* __restrict__ tile_off,
+                                                    const uint32_t* __restrict__ tile_end,
+                                                    const uint32_t* __restrict__ list,
+                                                    const vis_cluster* __restrict__ recs,
+                                                    const uint32_t* __restrict__ bits,
+                                                    uint64_t* __restrict__ out)
+{
+    __shared__ uint32_t lds_bits[4][16];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t ci = tile / tiles_per_cam;
+    const uint32_t t = tile - ci * tiles_per_cam;
+    const uint32_t tx = t % tiles_x, ty = t / tiles_x;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t px = tx * VIS_TILE + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t py = ty * VIS_TILE + (wave >> 1) * 8u + (lane >> 3);
+    const bool valid = px < W && py < H;
+
+    const vis_cam& cam = cams[ci];
+    /* tg_lerp_corner_ray_directions_nn: mix(mix(bl, tl, fy), mix(br, tr, fy), fx) */
+    const float fx = ((float)px + 0.5f) / (float)W;
+    const float fy = 1.0f - ((float)py + 0.5f) / (float)H;
+    const float omy = 1.0f - fy, omx = 1.0f - fx;
+    const float lx = cam.bl[0] * omy + cam.tl[0] * fy, ly = cam.bl[1] * omy + cam.tl[1] * fy, lz = cam.bl[2] * omy + cam.tl[2] * fy;
+    const float rx = cam.br[0] * omy + cam.tr[0] * fy, ry = cam.br[1] * omy + cam.tr[1] * fy, rz = cam.br[2] * omy + cam.tr[2] * fy;
+    const float dwx = lx * omx + rx * fx, dwy = ly * omx + ry * fx, dwz = lz * omx + rz * fx;
+    const float far_plane = cam.far_plane;
+
+    uint64_t best = TGX_VISIBILITY_EMPTY;
+    const uint32_t b = tile_off[tile], e = tile_end[tile];
+    uint32_t* my_bits = lds_bits[wave];
+    for (uint32_t i = b; i < e; i++) {
+        const vis_cluster& c = recs[list[i]];
+        const uint32_t lo = c.rect_lo, hi = c.rect_hi;
+        const bool act = valid && px >= (lo & 0xFFFFu) && px <= (hi & 0xFFFFu) && py >= (lo >> 16) &&
+                         py <= (hi >> 16) && (uint32_t)(best >> 40) >= c.depth_lb24;
+        if (!__any(act)) continue;
+        if (lane < 16u) my_bits[lane] = bits[(size_t)c.cidx * 16u + lane];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (act) {
+            const uint64_t word = pair_word(c, my_bits, dwx, dwy, dwz, far_plane);
+            best = word < best ? word : best;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (valid) out[(size_t)ci * W * H + (size_t)py * W + px] = best;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Host side                                                                */
+/* ------------------------------------------------------------------------ */
+
+static bool make_cam(const tgx_camera& c, vis_cam& v)
+{
+    memset(&v, 0, sizeof(v));
+    for (int i = 0; i < 3; i++) {
+        v.pos[i] = c.camera[i];
+        v.bl[i] = c.ray_bl[i]; v.br[i] = c.ray_br[i]; v.tr[i] = c.ray_tr[i]; v.tl[i] = c.ray_tl[i];
+    }
+    v.far_plane = c.far_plane;
+    double m[3][3], quad = 0.0;
+    for (int i = 0; i < 3; i++) {
+        m[i][0] = c.ray_bl[i];
+        m[i][1] = (double)c.ray_br[i] - (double)c.ray_bl[i];
+        m[i][2] = (double)c.ray_tl[i] - (double)c.ray_bl[i];
+        const double q = (double)c.ray_tr[i] - (double)c.ray_br[i] - (double)c.ray_tl[i] + (double)c.ray_bl[i];
+        quad += q * q;
+    }
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    if (!(sqrt(quad) <= 1e-5) || !(fabs(det) > 1e-12)) {
+        v.full_screen = 1;
+        return true;
+    }
+    const double inv[9] = {
+        (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det, (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det,
+        (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det, (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det,
+        (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det, (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det,
+        (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det, (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det,
+        (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det};
+    for (int i = 0; i < 9; i++) v.inv[i] = (float)inv[i];
+    return true;
+}
+
+static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t W,
+                            uint32_t H, uint64_t* out_dev)
+{
+    const uint32_t tiles_x = (W + VIS_TILE - 1) / VIS_TILE, tiles_y = (H + VIS_TILE - 1) / VIS_TILE;
+    const uint32_t tiles_per_cam = tiles_x * tiles_y;
+    const uint64_t n_tiles64 = (uint64_t)tiles_per_cam * n_cams;
+    const uint32_t n_cptr = ctx->n_cluster_pointers;
+    const uint64_t n_slots = (uint64_t)n_cptr * n_cams;
+    if (n_tiles64 >= (1ull << 31) || n_slots >= (1ull << 32))
+        return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
+    const uint32_t n_tiles = (uint32_t)n_tiles64;
+
+    std::vector<vis_cam> vc(n_cams);
+    for (uint32_t i = 0; i < n_cams; i++) {
+        make_cam(cams[i], vc[i]);
+        vc[i].rec_base = i * n_cptr;
+        vc[i].tile_base = i * tiles_per_cam;
+    }
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_cams, sizeof(vis_cam) * n_cams, "vis cams"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_clusters, sizeof(vis_cluster) * n_slots, "vis records"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (n_cams + 8), "vis counters"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_count, 4ull * n_tiles, "tile counts"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
+
+    vis_cam* d_cams = (vis_cam*)ctx->vis_cams.ptr;
+    uint32_t* d_counters = (uint32_t*)ctx->vis_counters.ptr; /* [0..n_cams): records; [n_cams]: total */
+    uint32_t* d_count = (uint32_t*)ctx->vis_tile_count.ptr;
+    uint32_t* d_off = (uint32_t*)ctx->vis_tile_off.ptr;
+    vis_cluster* d_recs = (vis_cluster*)ctx->vis_clusters.ptr;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_cams, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice, ctx->stream));
+
+    {
+        tgx_launch_scope s(ctx, "vis_fill_u32");
+        hipLaunchKernelGGL(vis_fill_u32, dim3((n_tiles + 255) / 256), dim3(256), 0, ctx->stream, d_count, n_tiles, 0u);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters, n_cams + 8, 0u);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    {
+        tgx_launch_scope s(ctx, "vis_setup");
+        hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint32_t*)ctx->bits.ptr,
+                           n_cptr, d_cams, W, H, tiles_x, d_recs, d_counters, d_count);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_setup");
+    {
+        tgx_launch_scope s(ctx, "vis_scan");
+        hipLaunchKernelGGL(vis_scan, dim3(1), dim3(1024), 0, ctx->stream, d_count, d_off, n_tiles, d_counters + n_cams);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_scan");
+
+    /* Read back the list length and the record counts (one small copy). */
+    uint32_t* hp = (uint32_t*)ctx->host_pinned;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(hp, d_counters, 4ull * (n_cams + 1), hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t total = hp[n_cams];
+    uint64_t binned = 0;
+    for (uint32_t i = 0; i < n_cams; i++) binned += hp[i];
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_list, 4ull * (total ? total : 1), "tile list"))) return rc;
+    uint32_t* d_list = (uint32_t*)ctx->vis_tile_list.ptr;
+    {
+        tgx_launch_scope s(ctx, "vis_scatter");
+        hipLaunchKernelGGL(vis_scatter, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream, d_recs,
+                           d_counters, d_cams, n_cptr, tiles_x, d_count, d_list);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_scatter");
+    {
+        /* After the scatter, d_count[t] is the end of tile t's list. */
+        tgx_launch_scope s(ctx, "vis_traverse");
+        hipLaunchKernelGGL(vis_traverse, dim3(n_tiles), dim3(256), 0, ctx->stream, d_cams, W, H, tiles_x,
+                           tiles_per_cam, d_off, d_count, d_list, d_recs, (const uint32_t*)ctx->bits.ptr, out_dev);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_traverse");
+
+    ctx->vis_stats.clusters_total += n_slots;
+    ctx->vis_stats.clusters_binned += binned;
+    ctx->vis_stats.tile_entries += total;
+    ctx->vis_stats.tiles = tiles_per_cam;
+    ctx->vis_stats.tile_size = VIS_TILE;
+    return TGX_OK;
+}
+
+static int check_args(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h)
+{
+    if (!ctx) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
+    if (!cams || n_cams == 0 || w == 0 || h == 0 || w > 65535 || h > 65535)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "invalid cameras or resolution (%u cams, %ux%u)", n_cams, w, h);
+    for (uint32_t i = 0; i < n_cams; i++)
+        if (!(cams[i].far_plane > 0.0f))
+            return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "camera %u: far plane must be > 0", i);
+    return TGX_OK;
+}
+
+/* Cameras per batch: bounded so that per-camera record slots stay modest. */
+static uint32_t batch_cams(const tgx_context* ctx, uint32_t n_cams)
+{
+    const uint64_t budget = 1ull << 28; /* 256 MiB of records */
+    uint64_t per = (uint64_t)ctx->n_cluster_pointers * sizeof(vis_cluster);
+    uint64_t b = per ? budget / per : n_cams;
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024; /* grid.y of the per-camera passes */
+    return (uint32_t)(b < n_cams ? b : n_cams);
+}
+
+extern "C" {
+
+int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w,
+                          uint32_t h, uint64_t* out_dev)
+{
+    int rc = check_args(ctx, cams, n_cams, w, h);
+    if (rc) return rc;
+    if (!out_dev) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "out_dev is NULL");
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    memset(&ctx->vis_stats, 0, sizeof(ctx->vis_stats));
+    const uint32_t per = batch_cams(ctx, n_cams);
+    for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
+        const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
+        rc = visibility_batch(ctx, cams + c0, n, w, h, out_dev + (size_t)c0 * w * h);
+        if (rc) return rc;
+    }
+    return TGX_OK;
+}
+
+int tgx_visibility(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                   uint64_t* out_host)
+{
+    int rc = check_args(ctx, cams, n_cams, w, h);
+    if (rc) return rc;
+    if (!out_host) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "out_host is NULL");
+    const size_t bytes = (size_t)n_cams * w * h * sizeof(uint64_t);
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_out, bytes, "visibility output"))) return rc;
+    if ((rc = tgx_visibility_device(ctx, cams, n_cams, w, h, (uint64_t*)ctx->vis_out.ptr))) return rc;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, ctx->vis_out.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
+int tgx_visibility_clear_device(tgx_context* ctx, uint64_t* vis_dev, uint64_t n_pixels)
+{
+    if (!ctx || !vis_dev) return TGX_ERR_INVALID_ARG;
+    if (n_pixels == 0) return TGX_OK;
+    if (2 * n_pixels >= (1ull << 32)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "clear of %llu pixels", (unsigned long long)n_pixels);
+    const uint32_t n = (uint32_t)(2 * n_pixels);
+    {
+        tgx_launch_scope s(ctx, "vis_fill_u32");
+        hipLaunchKernelGGL(vis_fill_u32, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, (uint32_t*)vis_dev, n, 0xFFFFFFFFu);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    return TGX_OK;
+}
+
+int tgx_visibility_last_stats(tgx_context* ctx, tgx_visibility_stats* out)
+{
+    if (!ctx || !out) return TGX_ERR_INVALID_ARG;
+    *out = ctx->vis_stats;
+    return TGX_OK;
+}
+
+} /* extern "C" */
