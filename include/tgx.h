@@ -212,7 +212,15 @@ typedef struct tgx_visibility_stats {
     uint32_t tile_size;          /* tile edge in pixels                     */
 } tgx_visibility_stats;
 
+/* Synchronises the context stream and returns the statistics of the last
+ * tgx_visibility / tgx_visibility_device call. */
 int tgx_visibility_last_stats(tgx_context* ctx, tgx_visibility_stats* out);
+
+/* Tile-list capacity in entries; 0 (the default) sizes it automatically
+ * from the largest list total seen. A frame whose lists exceed the
+ * capacity is still exact: its overflowing tiles test every record. No
+ * reference counterpart (tg rasterises instead of binning). */
+int tgx_visibility_set_list_capacity(tgx_context* ctx, uint32_t entries);
 
 #ifdef __cplusplus
 }

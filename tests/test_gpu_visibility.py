@@ -82,6 +82,22 @@ def test_odd_resolution_and_permuted_clusters(ctx, noise_scene):
     assert_same(ctx.visibility(cam, 333, 197)[0], ref, "permuted map")
 
 
+def test_tile_list_overflow_stays_exact(ctx, noise_scene):
+    p, sc = noise_scene
+    ctx.upload(sc)
+    cam = T.generate_camera(p, 1, 480, 270)
+    ref, _ = O.visibility_culled(O.OracleScene(sc), cam, 480, 270)
+    try:
+        for cap in (1, 3000):
+            say(f"480x270 with tile-list capacity {cap}")
+            ctx.set_list_capacity(cap)
+            assert_same(ctx.visibility(cam, 480, 270)[0], ref, f"capacity {cap}")
+            assert ctx.last_stats().tile_entries > cap
+    finally:
+        ctx.set_list_capacity(0)
+    assert_same(ctx.visibility(cam, 480, 270)[0], ref, "automatic capacity")
+
+
 def test_camera_inside_and_near_clusters(ctx):
     rng = np.random.default_rng(11)
     objs = [{"n": (3, 2, 2), "translation": (0.0, 0.0, 0.0), "rotation": S.random_rotation(rng)},

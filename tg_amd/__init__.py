@@ -106,6 +106,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_visibility_clear_device": (i32, [vp, vp, u64]),
         "tgx_visibility_decode": (i32, [u64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "tgx_visibility_last_stats": (i32, [vp, ctypes.POINTER(VisibilityStats)]),
+        "tgx_visibility_set_list_capacity": (i32, [vp, u32]),
         "tgx_device_alloc": (i32, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
         "tgx_device_free": (i32, [vp, vp]),
         "tgx_memcpy_d2h": (i32, [vp, vp, vp, ctypes.c_size_t]),
@@ -294,6 +295,9 @@ class Context:
         st = VisibilityStats()
         self._check(self.lib.tgx_visibility_last_stats(self.h, ctypes.byref(st)), "tgx_visibility_last_stats")
         return st
+
+    def set_list_capacity(self, entries: int) -> None:
+        self._check(self.lib.tgx_visibility_set_list_capacity(self.h, entries), "tgx_visibility_set_list_capacity")
 
     def alloc(self, nbytes: int) -> int:
         p = ctypes.c_void_p()

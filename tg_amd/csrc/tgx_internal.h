@@ -42,12 +42,19 @@ struct tgx_context {
 
     /* Visibility work buffers. */
     tgx_buffer   vis_clusters;   /* compacted per-(camera, cluster) records */
+    tgx_buffer   vis_rec_flags;  /* 1 if a record is in the tile lists      */
+    tgx_buffer   vis_global_keys;/* records with screen-sized rectangles    */
     tgx_buffer   vis_counters;   /* per-camera record counts + scalars      */
     tgx_buffer   vis_tile_count; /* per-tile entry counts                   */
     tgx_buffer   vis_tile_off;   /* per-tile list offsets (exclusive scan)  */
     tgx_buffer   vis_tile_list;  /* record indices per tile                 */
     tgx_buffer   vis_cams;       /* camera constants                        */
     tgx_buffer   vis_out;        /* staging output for tgx_visibility       */
+    tgx_buffer   vis_stats_dev;  /* frame statistics (records, entries, max)*/
+    uint32_t     vis_list_capacity = 0;   /* tile-list entries allocated    */
+    uint32_t     vis_list_capacity_fixed = 0; /* caller override, 0 = auto */
+    hipEvent_t   vis_stats_event = nullptr;
+    bool         vis_stats_pending = false;
     void*        host_pinned = nullptr; /* small pinned readback area       */
     tgx_visibility_stats vis_stats = {};
 

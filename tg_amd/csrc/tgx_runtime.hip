@@ -110,10 +110,12 @@ int tgx_destroy(tgx_context* ctx)
     }
     tgx_buffer* bufs[] = {&ctx->objects, &ctx->cluster_pointers, &ctx->cluster_idx_to_object,
                           &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters,
+                          &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
-                          &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out};
+                          &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+    if (ctx->vis_stats_event) (void)hipEventDestroy(ctx->vis_stats_event);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TGX_OK;

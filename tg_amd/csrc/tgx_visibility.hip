@@ -33,6 +33,7 @@
 #include <math.h>
 
 #define VIS_TILE 16u
+#define VIS_GLOBAL_TILES 256u
 #define VIS_F32_MAX 3.402823466e+38f
 
 /* Conservative-rectangle constants: the cluster box is grown by GAMMA
@@ -241,32 +242,25 @@ __device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, ui
     return true;
 }
 
-__global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
-                                                 const uint32_t* __restrict__ cptrs,
-                                                 const uint32_t* __restrict__ c2o,
-                                                 const uint32_t* __restrict__ bits, uint32_t n_cptr,
-                                                 const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
-                                                 uint32_t tiles_x, vis_cluster* __restrict__ recs,
-                                                 uint32_t* __restrict__ rec_count,
-                                                 uint32_t* __restrict__ tile_count)
+/* Builds the record of one (camera, cluster pointer); false if the cluster
+ * is empty, out of every pixel ray's reach or beyond the far plane. */
+__device__ static bool setup_one(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
+                                 const uint32_t* __restrict__ c2o, const uint32_t* __restrict__ bits,
+                                 uint32_t cptr, const vis_cam& cam, uint32_t W, uint32_t H, vis_cluster& rec)
 {
-    const uint32_t cptr = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t ci = blockIdx.y;
-    if (cptr >= n_cptr) return;
     const uint32_t cidx = cptrs[cptr];
     const uint4* b4 = reinterpret_cast<const uint4*>(bits + (size_t)cidx * 16u);
     uint32_t any = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) { const uint4 q = b4[i]; any |= q.x | q.y | q.z | q.w; }
-    if (!any) return; /* no solid voxel: the walk can never hit */
+    if (!any) return false; /* no solid voxel: the walk can never hit */
 
-    const vis_cam cam = cams[ci];
     const tgx_object ob = objects[c2o[cidx]];
     const uint32_t rel = cptr - ob.first_cluster_pointer;
     const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
     const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
     uint32_t x0, y0, x1, y1;
-    if (!cluster_rect(cam, ob, rx, ry, rz, W, H, x0, y0, x1, y1)) return;
+    if (!cluster_rect(cam, ob, rx, ry, rz, W, H, x0, y0, x1, y1)) return false;
 
     const tgx_m4 m = tgx_world_to_cluster(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
     const tgx_v4 o = tgx_m4_mulv4(m, cam.pos[0], cam.pos[1], cam.pos[2], 1.0f);
@@ -279,11 +273,10 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
     float lb = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) * 0.9999f - 1e-3f;
     lb = lb > 0.0f ? lb : 0.0f;
     const float dlb = lb / cam.far_plane;
-    if (dlb > 1.0f) return; /* every hit would have depth > 1 */
+    if (dlb > 1.0f) return false; /* every hit would have depth > 1 */
     uint32_t depth_lb24 = (uint32_t)(dlb * 16777215.0f);
     depth_lb24 = depth_lb24 > 0u ? depth_lb24 - 1u : 0u;
 
-    vis_cluster rec;
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
@@ -294,12 +287,67 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
     rec.rect_lo = x0 | (y0 << 16);
     rec.rect_hi = x1 | (y1 << 16);
     rec.depth_lb24 = depth_lb24;
-    const uint32_t k = atomicAdd(&rec_count[ci], 1u);
-    recs[cam.rec_base + k] = rec;
+    return true;
+}
 
-    const uint32_t tx0 = x0 / VIS_TILE, tx1 = x1 / VIS_TILE, ty0 = y0 / VIS_TILE, ty1 = y1 / VIS_TILE;
-    for (uint32_t ty = ty0; ty <= ty1; ty++)
-        for (uint32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&tile_count[cam.tile_base + ty * tiles_x + tx], 1u);
+/* The wave walks the tiles of each lane's rectangle together, 64 tiles per
+ * step, so one screen-sized rectangle does not serialise on one lane. */
+template <typename F>
+__device__ static inline void for_each_rect_tile(bool has, uint32_t rect_lo, uint32_t rect_hi,
+                                                 uint64_t payload, F&& f)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    unsigned long long mask = __ballot(has);
+    while (mask) { /* uniform: every lane takes part in the broadcasts */
+        const int l = __ffsll(mask) - 1;
+        mask &= mask - 1ull;
+        const uint32_t lo = __shfl(rect_lo, l), hi = __shfl(rect_hi, l);
+        const uint64_t pl = __shfl(payload, l);
+        const uint32_t tx0 = (lo & 0xFFFFu) / VIS_TILE, ty0 = (lo >> 16) / VIS_TILE;
+        const uint32_t tx1 = (hi & 0xFFFFu) / VIS_TILE, ty1 = (hi >> 16) / VIS_TILE;
+        const uint32_t ntx = tx1 - tx0 + 1u, nt = ntx * (ty1 - ty0 + 1u);
+        for (uint32_t j = lane; j < nt; j += 64u) f(pl, tx0 + j % ntx, ty0 + j / ntx);
+    }
+}
+
+__global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
+                                                 const uint32_t* __restrict__ cptrs,
+                                                 const uint32_t* __restrict__ c2o,
+                                                 const uint32_t* __restrict__ bits, uint32_t n_cptr,
+                                                 const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
+                                                 uint32_t tiles_x, vis_cluster* __restrict__ recs,
+                                                 uint32_t* __restrict__ rec_flags,
+                                                 uint32_t* __restrict__ rec_count,
+                                                 uint32_t* __restrict__ global_count,
+                                                 uint64_t* __restrict__ global_keys,
+                                                 uint32_t* __restrict__ tile_count)
+{
+    const uint32_t cptr = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ci = blockIdx.y;
+    const vis_cam& cam = cams[ci];
+    vis_cluster rec;
+    bool has = false;
+    if (cptr < n_cptr) has = setup_one(objects, cptrs, c2o, bits, cptr, cam, W, H, rec);
+    bool binned = false;
+    if (has) {
+        const uint32_t k = atomicAdd(&rec_count[ci], 1u);
+        recs[cam.rec_base + k] = rec;
+        /* Rectangles of more than VIS_GLOBAL_TILES tiles skip the tile
+         * lists: every tile of the camera merges them from the global list. */
+        const uint32_t ntx = (rec.rect_hi & 0xFFFFu) / VIS_TILE - (rec.rect_lo & 0xFFFFu) / VIS_TILE + 1u;
+        const uint32_t nty = (rec.rect_hi >> 16) / VIS_TILE - (rec.rect_lo >> 16) / VIS_TILE + 1u;
+        if (ntx * nty > VIS_GLOBAL_TILES) {
+            const uint32_t g = atomicAdd(&global_count[ci], 1u);
+            global_keys[cam.rec_base + g] = ((uint64_t)rec.depth_lb24 << 32) | (cam.rec_base + k);
+        } else {
+            binned = true;
+            rec_flags[cam.rec_base + k] = 1u;
+        }
+        if (!binned) rec_flags[cam.rec_base + k] = 0u;
+    }
+    const uint32_t tb = cam.tile_base;
+    for_each_rect_tile(binned, binned ? rec.rect_lo : 0u, binned ? rec.rect_hi : 0u, 0ull,
+                       [&](uint64_t, uint32_t tx, uint32_t ty) { atomicAdd(&tile_count[tb + ty * tiles_x + tx], 1u); });
 }
 
 /* ------------------------------------------------------------------------ */
@@ -313,68 +361,120 @@ __global__ void vis_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t valu
 }
 
 /* Single-workgroup exclusive scan: off[i] = sum(count[0..i)), and
- * count[i] becomes the scatter cursor (= off[i]); total in *total. */
+ * count[i] becomes the scatter cursor (= off[i]); the total goes to
+ * *total. The last thread also accumulates the frame statistics:
+ * stats[0] += records, stats[1] += list entries, stats[2] = max entries. */
 __global__ __launch_bounds__(1024) void vis_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ off,
-                                                 uint32_t n, uint32_t* __restrict__ total)
+                                                 uint32_t n, uint32_t* __restrict__ total,
+                                                 const uint32_t* __restrict__ rec_count, uint32_t n_cams,
+                                                 unsigned long long* __restrict__ stats)
 {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = t * per, e = min(n, b + per);
+    const uint32_t b = min(n, t * per), e = min(n, b + per);
     uint32_t s = 0;
     for (uint32_t i = b; i < e; i++) s += count[i];
-    part[t] = s;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    uint32_t x = s; /* inclusive scan within the wave */
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
     }
-    uint32_t run = part[t] - s;
+    if (lane == 63u) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t v = lane < 16u ? wsum[lane] : 0u;
+        for (uint32_t d = 1; d < 16u; d <<= 1) {
+            const uint32_t y = __shfl_up(v, d);
+            if (lane >= d) v += y;
+        }
+        if (lane < 16u) wsum[lane] = v;
+    }
+    __syncthreads();
+    uint32_t run = x - s + (w ? wsum[w - 1] : 0u);
     for (uint32_t i = b; i < e; i++) {
         const uint32_t c = count[i];
         off[i] = run;
         count[i] = run;
         run += c;
     }
-    if (t == 1023u) *total = part[1023];
+    if (t == 1023u) {
+        *total = run;
+        unsigned long long recs = 0;
+        for (uint32_t c = 0; c < n_cams; c++) recs += rec_count[c];
+        stats[0] += recs;
+        stats[1] += run;
+        if (run > stats[2]) stats[2] = run;
+    }
 }
 
+/* Each list entry is the sort key (depth_lb24 << 32) | record slot. Eight
+ * lanes share one record and stride over its tiles, so the latency of the
+ * returning atomics is spread over many waves. Entries beyond `capacity`
+ * are dropped; vis_traverse detects such tiles and walks every record. */
+#define VIS_SCATTER_LANES 8u
 __global__ __launch_bounds__(256) void vis_scatter(const vis_cluster* __restrict__ recs,
+                                                   const uint32_t* __restrict__ rec_flags,
                                                    const uint32_t* __restrict__ rec_count,
-                                                   const vis_cam* __restrict__ cams, uint32_t per_cam_slots,
-                                                   uint32_t tiles_x, uint32_t* __restrict__ cursor,
-                                                   uint32_t* __restrict__ list)
+                                                   const vis_cam* __restrict__ cams, uint32_t tiles_x,
+                                                   uint32_t* __restrict__ cursor,
+                                                   uint64_t* __restrict__ list, uint32_t capacity)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = gid / VIS_SCATTER_LANES, sub = gid % VIS_SCATTER_LANES;
     const uint32_t ci = blockIdx.y;
     if (k >= rec_count[ci]) return;
     const uint32_t slot = cams[ci].rec_base + k;
-    const vis_cluster& r = recs[slot];
+    if (rec_flags[slot] == 0u) return;
+    const uint32_t lo = recs[slot].rect_lo, hi = recs[slot].rect_hi;
+    const uint64_t key = ((uint64_t)recs[slot].depth_lb24 << 32) | slot;
+    const uint32_t tx0 = (lo & 0xFFFFu) / VIS_TILE, ty0 = (lo >> 16) / VIS_TILE;
+    const uint32_t tx1 = (hi & 0xFFFFu) / VIS_TILE, ty1 = (hi >> 16) / VIS_TILE;
+    const uint32_t ntx = tx1 - tx0 + 1u, nt = ntx * (ty1 - ty0 + 1u);
     const uint32_t tb = cams[ci].tile_base;
-    const uint32_t x0 = r.rect_lo & 0xFFFFu, y0 = r.rect_lo >> 16, x1 = r.rect_hi & 0xFFFFu, y1 = r.rect_hi >> 16;
-    for (uint32_t ty = y0 / VIS_TILE; ty <= y1 / VIS_TILE; ty++)
-        for (uint32_t tx = x0 / VIS_TILE; tx <= x1 / VIS_TILE; tx++) {
-            const uint32_t t = tb + ty * tiles_x + tx;
-            list[atomicAdd(&cursor[t], 1u)] = slot;
-        }
-    (void)per_cam_slots;
+    for (uint32_t j = sub; j < nt; j += VIS_SCATTER_LANES) {
+        const uint32_t pos = atomicAdd(&cursor[tb + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx], 1u);
+        if (pos < capacity) list[pos] = key;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
 /* Pass 3: per-tile traversal                                               */
 /* ------------------------------------------------------------------------ */
 
+#define VIS_SORT_CAP 1024u
+
+/* Sorts keys[0..n2) ascending (n2 a power of two) with the workgroup. */
+__device__ static inline void lds_bitonic_sort(uint64_t* keys, uint32_t n2)
+{
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const uint64_t a = keys[i], b = keys[p];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { keys[i] = b; keys[p] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void vis_traverse(const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                     uint32_t tiles_x, uint32_t tiles_per_cam,
                                                     const uint32_t* __restrict__ tile_off,
                                                     const uint32_t* __restrict__ tile_end,
-                                                    const uint32_t* __restrict__ list,
+                                                    const uint64_t* __restrict__ list, uint32_t capacity,
+                                                    const uint32_t* __restrict__ rec_count,
+                                                    const uint32_t* __restrict__ global_count,
+                                                    const uint64_t* __restrict__ global_keys,
                                                     const vis_cluster* __restrict__ recs,
                                                     const uint32_t* __restrict__ bits,
                                                     uint64_t* __restrict__ out)
 {
+    __shared__ uint64_t keys[VIS_SORT_CAP];
     __shared__ uint32_t lds_bits[4][16];
     const uint32_t tile = blockIdx.x;
     const uint32_t ci = tile / tiles_per_cam;
@@ -396,23 +496,53 @@ __global__ __launch_bounds__(256) void vis_traverse(const vis_cam* __restrict__ 
     const float far_plane = cam.far_plane;
 
     uint64_t best = TGX_VISIBILITY_EMPTY;
-    const uint32_t b = tile_off[tile], e = tile_end[tile];
+    /* The tile's own list followed by the camera's global list; if the
+     * frame's lists overflowed the list buffer, every record instead. */
+    const bool overflow = tile_end[tile] > capacity;
+    const uint32_t b = tile_off[tile];
+    const uint32_t n_own = overflow ? rec_count[ci] : tile_end[tile] - b;
+    const uint32_t n_all = n_own + (overflow ? 0u : global_count[ci]);
+    const uint64_t* gk = global_keys + cam.rec_base;
     uint32_t* my_bits = lds_bits[wave];
-    for (uint32_t i = b; i < e; i++) {
-        const vis_cluster& c = recs[list[i]];
-        const uint32_t lo = c.rect_lo, hi = c.rect_hi;
-        const bool act = valid && px >= (lo & 0xFFFFu) && px <= (hi & 0xFFFFu) && py >= (lo >> 16) &&
-                         py <= (hi >> 16) && (uint32_t)(best >> 40) >= c.depth_lb24;
-        if (!__any(act)) continue;
-        if (lane < 16u) my_bits[lane] = bits[(size_t)c.cidx * 16u + lane];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (act) {
-            const uint64_t word = pair_word(c, my_bits, dwx, dwy, dwz, far_plane);
-            best = word < best ? word : best;
+    for (uint32_t cb = 0; cb < n_all; cb += VIS_SORT_CAP) {
+        /* Front to back: sort this chunk of the tile's list by depth bound. */
+        const uint32_t n = min(VIS_SORT_CAP, n_all - cb);
+        uint32_t n2 = 1;
+        while (n2 < n) n2 <<= 1;
+        for (uint32_t j = threadIdx.x; j < n2; j += blockDim.x) {
+            const uint32_t v = cb + j;
+            uint64_t kv = ~0ull;
+            if (j < n) {
+                if (overflow)
+                    kv = ((uint64_t)recs[cam.rec_base + v].depth_lb24 << 32) | (cam.rec_base + v);
+                else
+                    kv = v < n_own ? list[b + v] : gk[v - n_own];
+            }
+            keys[j] = kv;
         }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
+        lds_bitonic_sort(keys, n2);
+        for (uint32_t j = 0; j < n; j++) {
+            const uint64_t key = keys[j];
+            const uint32_t lb = (uint32_t)(key >> 32);
+            /* Sorted bounds: once no lane can improve, none of the rest can. */
+            if (!__any(valid && (uint32_t)(best >> 40) >= lb)) break;
+            const vis_cluster& c = recs[(uint32_t)key];
+            const uint32_t lo = c.rect_lo, hi = c.rect_hi;
+            const bool act = valid && px >= (lo & 0xFFFFu) && px <= (hi & 0xFFFFu) && py >= (lo >> 16) &&
+                             py <= (hi >> 16) && (uint32_t)(best >> 40) >= lb;
+            if (!__any(act)) continue;
+            if (lane < 16u) my_bits[lane] = bits[(size_t)c.cidx * 16u + lane];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (act) {
+                const uint64_t word = pair_word(c, my_bits, dwx, dwy, dwz, far_plane);
+                best = word < best ? word : best;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads(); /* keys are rewritten by the next chunk */
     }
     if (valid) out[(size_t)ci * W * H + (size_t)py * W + px] = best;
 }
@@ -475,12 +605,18 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_cams, sizeof(vis_cam) * n_cams, "vis cams"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_clusters, sizeof(vis_cluster) * n_slots, "vis records"))) return rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (n_cams + 8), "vis counters"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_rec_flags, 4ull * n_slots, "vis record flags"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_global_keys, 8ull * n_slots, "vis global keys"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (2 * n_cams + 8), "vis counters"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_count, 4ull * n_tiles, "tile counts"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
 
     vis_cam* d_cams = (vis_cam*)ctx->vis_cams.ptr;
-    uint32_t* d_counters = (uint32_t*)ctx->vis_counters.ptr; /* [0..n_cams): records; [n_cams]: total */
+    /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts */
+    uint32_t* d_counters = (uint32_t*)ctx->vis_counters.ptr;
+    uint32_t* d_global_count = d_counters + n_cams + 8;
+    uint32_t* d_flags = (uint32_t*)ctx->vis_rec_flags.ptr;
+    uint64_t* d_gkeys = (uint64_t*)ctx->vis_global_keys.ptr;
     uint32_t* d_count = (uint32_t*)ctx->vis_tile_count.ptr;
     uint32_t* d_off = (uint32_t*)ctx->vis_tile_off.ptr;
     vis_cluster* d_recs = (vis_cluster*)ctx->vis_clusters.ptr;
@@ -489,7 +625,7 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     {
         tgx_launch_scope s(ctx, "vis_fill_u32");
         hipLaunchKernelGGL(vis_fill_u32, dim3((n_tiles + 255) / 256), dim3(256), 0, ctx->stream, d_count, n_tiles, 0u);
-        hipLaunchKernelGGL(vis_fill_u32, dim3((n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters, n_cams + 8, 0u);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((2 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters, 2 * n_cams + 8, 0u);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
     {
@@ -497,41 +633,37 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
         hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
                            (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
                            (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint32_t*)ctx->bits.ptr,
-                           n_cptr, d_cams, W, H, tiles_x, d_recs, d_counters, d_count);
+                           n_cptr, d_cams, W, H, tiles_x, d_recs, d_flags, d_counters, d_global_count, d_gkeys, d_count);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_setup");
     {
         tgx_launch_scope s(ctx, "vis_scan");
-        hipLaunchKernelGGL(vis_scan, dim3(1), dim3(1024), 0, ctx->stream, d_count, d_off, n_tiles, d_counters + n_cams);
+        hipLaunchKernelGGL(vis_scan, dim3(1), dim3(1024), 0, ctx->stream, d_count, d_off, n_tiles, d_counters + n_cams,
+                           d_counters, n_cams, (unsigned long long*)ctx->vis_stats_dev.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_scan");
 
-    /* Read back the list length and the record counts (one small copy). */
-    uint32_t* hp = (uint32_t*)ctx->host_pinned;
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(hp, d_counters, 4ull * (n_cams + 1), hipMemcpyDeviceToHost, ctx->stream));
-    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const uint32_t total = hp[n_cams];
-    uint64_t binned = 0;
-    for (uint32_t i = 0; i < n_cams; i++) binned += hp[i];
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_list, 4ull * (total ? total : 1), "tile list"))) return rc;
-    uint32_t* d_list = (uint32_t*)ctx->vis_tile_list.ptr;
+    /* The tile lists go into a buffer of fixed capacity (grown between
+     * frames from the largest total seen); no host round trip mid-frame. */
+    const uint32_t capacity = ctx->vis_list_capacity;
+    uint64_t* d_list = (uint64_t*)ctx->vis_tile_list.ptr;
     {
         tgx_launch_scope s(ctx, "vis_scatter");
-        hipLaunchKernelGGL(vis_scatter, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream, d_recs,
-                           d_counters, d_cams, n_cptr, tiles_x, d_count, d_list);
+        const uint64_t threads = (uint64_t)n_cptr * VIS_SCATTER_LANES;
+        hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)((threads + 255) / 256), n_cams), dim3(256), 0, ctx->stream,
+                           d_recs, d_flags, d_counters, d_cams, tiles_x, d_count, d_list, capacity);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_scatter");
     {
         /* After the scatter, d_count[t] is the end of tile t's list. */
         tgx_launch_scope s(ctx, "vis_traverse");
         hipLaunchKernelGGL(vis_traverse, dim3(n_tiles), dim3(256), 0, ctx->stream, d_cams, W, H, tiles_x,
-                           tiles_per_cam, d_off, d_count, d_list, d_recs, (const uint32_t*)ctx->bits.ptr, out_dev);
+                           tiles_per_cam, d_off, d_count, d_list, capacity, d_counters, d_global_count, d_gkeys,
+                           d_recs, (const uint32_t*)ctx->bits.ptr, out_dev);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_traverse");
 
     ctx->vis_stats.clusters_total += n_slots;
-    ctx->vis_stats.clusters_binned += binned;
-    ctx->vis_stats.tile_entries += total;
     ctx->vis_stats.tiles = tiles_per_cam;
     ctx->vis_stats.tile_size = VIS_TILE;
     return TGX_OK;
@@ -547,6 +679,29 @@ static int check_args(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams,
     for (uint32_t i = 0; i < n_cams; i++)
         if (!(cams[i].far_plane > 0.0f))
             return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "camera %u: far plane must be > 0", i);
+    return TGX_OK;
+}
+
+/* Sizes the tile-list buffer. A frame whose lists exceed the capacity is
+ * still exact (its overflowing tiles walk every record); the largest total
+ * of the previous call, once its readback has landed, grows the buffer. */
+static int list_capacity_update(tgx_context* ctx, uint32_t w, uint32_t h)
+{
+    int rc;
+    if (!ctx->vis_stats_event) TGX_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->vis_stats_event, hipEventDisableTiming));
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_stats_dev, 64, "vis stats"))) return rc;
+    uint64_t want = ctx->vis_list_capacity;
+    const uint64_t tiles = (uint64_t)((w + VIS_TILE - 1) / VIS_TILE) * ((h + VIS_TILE - 1) / VIS_TILE);
+    if (want == 0) want = tiles * 64u > (1u << 22) ? tiles * 64u : (1u << 22);
+    if (ctx->vis_list_capacity_fixed) want = ctx->vis_list_capacity_fixed;
+    else if (ctx->vis_stats_pending && hipEventQuery(ctx->vis_stats_event) == hipSuccess) {
+        ctx->vis_stats_pending = false;
+        const uint64_t max_total = ((const unsigned long long*)ctx->host_pinned)[2];
+        if (max_total > want) want = max_total + max_total / 4u + 1024u;
+    }
+    if (want > 0xFFFFFFFFull) want = 0xFFFFFFFFull;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_list, 8ull * want, "tile list"))) return rc;
+    ctx->vis_list_capacity = (uint32_t)want;
     return TGX_OK;
 }
 
@@ -571,12 +726,23 @@ int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_c
     if (!out_dev) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "out_dev is NULL");
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     memset(&ctx->vis_stats, 0, sizeof(ctx->vis_stats));
+    if ((rc = list_capacity_update(ctx, w, h))) return rc;
+    {
+        tgx_launch_scope s(ctx, "vis_fill_u32");
+        hipLaunchKernelGGL(vis_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->vis_stats_dev.ptr, 8u, 0u);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
     const uint32_t per = batch_cams(ctx, n_cams);
     for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
         const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
         rc = visibility_batch(ctx, cams + c0, n, w, h, out_dev + (size_t)c0 * w * h);
         if (rc) return rc;
     }
+    /* Frame statistics travel back asynchronously; the next call (or
+     * tgx_visibility_last_stats) reads them. */
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->host_pinned, ctx->vis_stats_dev.ptr, 32, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipEventRecord(ctx->vis_stats_event, ctx->stream));
+    ctx->vis_stats_pending = true;
     return TGX_OK;
 }
 
@@ -608,10 +774,22 @@ int tgx_visibility_clear_device(tgx_context* ctx, uint64_t* vis_dev, uint64_t n_
     return TGX_OK;
 }
 
+int tgx_visibility_set_list_capacity(tgx_context* ctx, uint32_t entries)
+{
+    if (!ctx) return TGX_ERR_INVALID_ARG;
+    ctx->vis_list_capacity_fixed = entries;
+    if (entries == 0) ctx->vis_list_capacity = 0;
+    return TGX_OK;
+}
+
 int tgx_visibility_last_stats(tgx_context* ctx, tgx_visibility_stats* out)
 {
     if (!ctx || !out) return TGX_ERR_INVALID_ARG;
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned long long* hs = (const unsigned long long*)ctx->host_pinned;
     *out = ctx->vis_stats;
+    out->clusters_binned = hs[0];
+    out->tile_entries = hs[1];
     return TGX_OK;
 }
 
