@@ -180,6 +180,87 @@ int tgx_visibility_decode(uint64_t word, float* out_depth, uint32_t* out_cluster
                           uint32_t* out_voxel_idx);
 
 /* ------------------------------------------------------------------------ */
+/* Sparse voxel octree (tg_sparse_voxel_octree.{c,h}, svo.inc)              */
+/* ------------------------------------------------------------------------ */
+
+#define TGX_SVO_BLOCK_SIDE          32u     /* leaf block edge (svo.inc:2)       */
+#define TGX_SVO_BLOCK_WORDS         1024u   /* 32^3 bits as u32                  */
+#define TGX_SVO_BLOCK_VOXELS        32768u
+#define TGX_SVO_MIN_DEPTH           6u      /* root side 2^depth voxels          */
+#define TGX_SVO_MAX_DEPTH           13u
+/* Iteration cap of the traversal loop: the advance by exit + FLT_EPSILON
+ * (tg_sparse_voxel_octree.c:709-712) can stop moving once |coords| >= 1. */
+#define TGX_SVO_MAX_STEPS           4096u
+/* Reproduce tg's leaf-bounds quirk: the cluster AABB of a leaf is taken
+ * over corners 0,1,2,2,4,5,6,7 (tg_sparse_voxel_octree.c:183,191). Off by
+ * default: the build uses all eight corners. */
+#define TGX_SVO_FLAG_REFERENCE_CORNER_QUIRK 1u
+
+typedef struct tgx_svo_info {
+    float    min[3];          /* world position of the root's min corner     */
+    uint32_t depth;           /* root side = 2^depth voxels, leaves 32^3     */
+    uint32_t n_nodes;         /* inner and leaf nodes, root included         */
+    uint32_t n_leaves;
+    uint32_t n_list_entries;  /* total length of the leaf cluster lists      */
+    uint32_t flags;
+} tgx_svo_info;
+
+/* Canonical node record. Nodes are ordered by (level, Morton code of
+ * coord); the children of a node are contiguous in child-index order
+ * (bit 1 = +x, 2 = +y, 4 = +z; tg_sparse_voxel_octree.c:356-361). The
+ * reference allocates the same nodes in depth-first order with u16
+ * relative child pointers (tg_sparse_voxel_octree.h:6-11); here pointers are
+ * absolute u32 so depth 11-13 trees fit. */
+typedef struct tgx_svo_node {
+    uint32_t level;           /* 0 = root                                     */
+    uint32_t coord[3];        /* node min = svo min + coord * 2^(depth-level) */
+    uint32_t valid_mask;      /* children present                            */
+    uint32_t leaf_mask;       /* children that are 32^3 leaf blocks           */
+    uint32_t first_child;     /* node index of the first child, or ~0         */
+    uint32_t leaf_idx;        /* leaves: index into the leaf arrays, else ~0  */
+} tgx_svo_node;
+
+/* Result of one SVO ray (tg_svo_traverse, tg_sparse_voxel_octree.c:558-740).
+ * The hit key (block, voxel) names the voxel independently of node layout. */
+typedef struct tgx_svo_hit {
+    uint32_t hit;             /* 1 if a solid voxel was reached within range  */
+    float    distance;        /* origin to the voxel entry                    */
+    uint32_t block[3];        /* leaf block coord, in units of 32 voxels      */
+    uint32_t voxel;           /* 1024 z + 32 y + x within the block           */
+    int32_t  normal[3];       /* entry face of the voxel, axis-aligned        */
+    uint32_t steps;           /* traversal loop iterations                    */
+} tgx_svo_hit;
+
+/* Builds the SVO of the uploaded scene: root [min, min + 2^depth)^3, 32^3
+ * leaf blocks, every non-empty cluster (tg_svo_create,
+ * tg_sparse_voxel_octree.c:466-542). Inner levels keep a cluster in a
+ * child's list iff tg_intersect_aabb_obb_ignore_contact passes
+ * (tg_physics.c:226-392); leaves point-sample block-voxel centres into
+ * their clusters (tg_sparse_voxel_octree.c:102-317). Each leaf also
+ * stores, per solid voxel, (lut_idx << 8 | material) of the first cluster
+ * in its list that sampled it. min must be integral; depth in
+ * [TGX_SVO_MIN_DEPTH, TGX_SVO_MAX_DEPTH]. */
+int tgx_svo_build(tgx_context* ctx, const float min[3], uint32_t depth, uint32_t flags);
+int tgx_svo_get_info(tgx_context* ctx, tgx_svo_info* out);
+
+/* Copies the SVO out in canonical form: nodes[n_nodes]; per leaf bits
+ * (1024 u32), materials (32768 u16), n (clusters that sampled a solid
+ * voxel), list offsets (n_leaves + 1) and cluster-index list. Any output
+ * pointer may be NULL. */
+int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes, uint32_t* leaf_bits,
+                   uint16_t* leaf_materials, uint32_t* leaf_n, uint32_t* list_offsets,
+                   uint32_t* list);
+
+/* Traces n rays (world-space origins and directions, 3 floats each)
+ * through the SVO; a hit farther than max_distance is reported as a miss.
+ * Follows tg_svo_traverse: the origin is taken relative to the root
+ * centre, the descent picks the child containing the current position,
+ * leaves run tg_amanatides_woo (tg_amanatides_woo.c:3-114) and misses
+ * advance to the child border plus FLT_EPSILON and pop finished nodes. */
+int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n,
+                  float max_distance, tgx_svo_hit* out_host);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and profiling helpers (for callers without their own       */
 /* allocator; the reference's equivalents are tgvk_memory.c:182-240)        */
 /* ------------------------------------------------------------------------ */

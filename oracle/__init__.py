@@ -49,6 +49,14 @@ def load() -> ctypes.CDLL:
         "tgxo_m4_mul": (None, [vp, vp, vp]),
         "tgxo_m4_inverse": (None, [vp, vp]),
         "tgxo_world_to_cluster": (None, [vp, u32, vp]),
+        "tgxo_svo_build": (vp, [vp, vp, u32, u32]),
+        "tgxo_svo_free": (None, [vp]),
+        "tgxo_svo_get_info": (None, [vp, vp]),
+        "tgxo_svo_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "tgxo_svo_traverse": (None, [vp, vp, vp, ctypes.c_float, vp]),
+        "tgxo_svo_trace": (None, [vp, vp, vp, u32, ctypes.c_float, vp]),
+        "tgxo_aabb_obb_ignore_contact": (i32, [vp, vp, vp]),
+        "tgxo_svo_node_key": (u64, [u32, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -133,3 +141,78 @@ def world_to_cluster(obj: np.ndarray, rel_cptr: int) -> np.ndarray:
     out = np.zeros(16, np.float32)
     load().tgxo_world_to_cluster(_p(obj), rel_cptr, _p(out))
     return out
+
+
+# --------------------------------------------------------------------------
+# Sparse voxel octree
+# --------------------------------------------------------------------------
+
+SVO_INFO_DTYPE = np.dtype([("min", "<f4", (3,)), ("depth", "<u4"), ("n_nodes", "<u4"), ("n_leaves", "<u4"),
+                           ("n_list_entries", "<u4"), ("flags", "<u4")])
+SVO_NODE_DTYPE = np.dtype([("level", "<u4"), ("coord", "<u4", (3,)), ("valid_mask", "<u4"), ("leaf_mask", "<u4"),
+                           ("first_child", "<u4"), ("leaf_idx", "<u4")])
+SVO_HIT_DTYPE = np.dtype([("hit", "<u4"), ("distance", "<f4"), ("block", "<u4", (3,)), ("voxel", "<u4"),
+                          ("normal", "<i4", (3,)), ("steps", "<u4")])
+SVO_FLAG_REFERENCE_CORNER_QUIRK = 1
+
+
+class SvoExport:
+    """Canonical SVO arrays (same layout from the oracle and from the GPU)."""
+
+    def __init__(self, info, nodes, bits, materials, leaf_n, list_offsets, lst):
+        self.info, self.nodes, self.bits, self.materials = info, nodes, bits, materials
+        self.leaf_n, self.list_offsets, self.list = leaf_n, list_offsets, lst
+
+    @staticmethod
+    def alloc(info) -> "SvoExport":
+        nl = int(info["n_leaves"])
+        return SvoExport(info, np.zeros(int(info["n_nodes"]), SVO_NODE_DTYPE), np.zeros((nl, 1024), np.uint32),
+                         np.zeros((nl, 32768), np.uint16), np.zeros(nl, np.uint32), np.zeros(nl + 1, np.uint32),
+                         np.zeros(max(1, int(info["n_list_entries"])), np.uint32))
+
+    def leaf_coords(self) -> np.ndarray:
+        leaves = self.nodes[self.nodes["leaf_idx"] != 0xFFFFFFFF]
+        order = np.argsort(leaves["leaf_idx"])
+        return leaves["coord"][order]
+
+
+class OracleSvo:
+    def __init__(self, oscene: OracleScene, mn, depth: int, flags: int = 0):
+        self.oscene = oscene  # keep the scene arrays alive
+        m = np.ascontiguousarray(mn, np.float32)
+        self.h = load().tgxo_svo_build(oscene.ref, _p(m), depth, flags)
+        if not self.h:
+            raise RuntimeError("tgxo_svo_build failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().tgxo_svo_free(self.h)
+            self.h = None
+
+    def info(self):
+        i = np.zeros(1, SVO_INFO_DTYPE)
+        load().tgxo_svo_get_info(self.h, _p(i))
+        return i[0]
+
+    def export(self) -> SvoExport:
+        e = SvoExport.alloc(self.info())
+        load().tgxo_svo_export(self.h, _p(e.nodes), _p(e.bits), _p(e.materials), _p(e.leaf_n),
+                               _p(e.list_offsets), _p(e.list))
+        return e
+
+    def trace(self, origins: np.ndarray, dirs: np.ndarray, max_distance: float = 3.0e38) -> np.ndarray:
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(o.shape[0], SVO_HIT_DTYPE)
+        load().tgxo_svo_trace(self.h, _p(o), _p(d), o.shape[0], max_distance, _p(out))
+        return out
+
+
+def aabb_obb_ignore_contact(mn, mx, corners) -> bool:
+    a = [np.ascontiguousarray(v, np.float32) for v in (mn, mx, corners)]
+    return bool(load().tgxo_aabb_obb_ignore_contact(_p(a[0]), _p(a[1]), _p(a[2])))
+
+
+def svo_node_key(level: int, coord) -> int:
+    c = np.ascontiguousarray(coord, np.uint32)
+    return int(load().tgxo_svo_node_key(level, _p(c)))

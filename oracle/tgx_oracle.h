@@ -68,6 +68,27 @@ void tgxo_m4_mul(const float a[16], const float b[16], float out[16]);
 void tgxo_m4_inverse(const float a[16], float out[16]);
 void tgxo_world_to_cluster(const tgx_object* o, uint32_t rel_cptr, float out[16]);
 
+/* ------------------------------------------------------------------------ */
+/* Sparse voxel octree (tgx_oracle_svo.c)                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct tgxo_svo tgxo_svo;
+
+/* Depth-first build as tg_svo_create; NULL on bad depth or allocation. */
+tgxo_svo* tgxo_svo_build(const tgxo_scene* s, const float min[3], uint32_t depth, uint32_t flags);
+void      tgxo_svo_free(tgxo_svo* svo);
+void      tgxo_svo_get_info(const tgxo_svo* svo, tgx_svo_info* out);
+/* Canonical export (same layout as tgx_svo_export). */
+int       tgxo_svo_export(const tgxo_svo* svo, tgx_svo_node* nodes, uint32_t* leaf_bits,
+                          uint16_t* leaf_materials, uint32_t* leaf_n, uint32_t* list_offsets,
+                          uint32_t* list);
+void      tgxo_svo_traverse(const tgxo_svo* svo, const float origin[3], const float dir[3],
+                            float max_distance, tgx_svo_hit* out);
+void      tgxo_svo_trace(const tgxo_svo* svo, const float* origins, const float* dirs, uint32_t n,
+                         float max_distance, tgx_svo_hit* out);
+int       tgxo_aabb_obb_ignore_contact(const float mn[3], const float mx[3], const float corners[24]);
+uint64_t  tgxo_svo_node_key(uint32_t level, const uint32_t coord[3]);
+
 #ifdef __cplusplus
 }
 #endif

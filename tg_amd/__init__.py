@@ -38,6 +38,14 @@ CAMERA_DTYPE = np.dtype([
 LUT_DTYPE = np.dtype([("albedo", "<f4", (3,)), ("roughness", "<f4")])
 assert OBJECT_DTYPE.itemsize == 96 and CAMERA_DTYPE.itemsize == 88 and LUT_DTYPE.itemsize == 16
 
+SVO_INFO_DTYPE = np.dtype([("min", "<f4", (3,)), ("depth", "<u4"), ("n_nodes", "<u4"), ("n_leaves", "<u4"),
+                           ("n_list_entries", "<u4"), ("flags", "<u4")])
+SVO_NODE_DTYPE = np.dtype([("level", "<u4"), ("coord", "<u4", (3,)), ("valid_mask", "<u4"), ("leaf_mask", "<u4"),
+                           ("first_child", "<u4"), ("leaf_idx", "<u4")])
+SVO_HIT_DTYPE = np.dtype([("hit", "<u4"), ("distance", "<f4"), ("block", "<u4", (3,)), ("voxel", "<u4"),
+                          ("normal", "<i4", (3,)), ("steps", "<u4")])
+SVO_FLAG_REFERENCE_CORNER_QUIRK = 1
+
 VISIBILITY_EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 GEN_SINGLE_BERNOULLI = 0
@@ -120,6 +128,10 @@ def load_library() -> ctypes.CDLL:
         "tgx_camera_make": (i32, [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                   ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
         "tgx_gen_stream_seed": (u64, [u64, u64]),
+        "tgx_svo_build": (i32, [vp, vp, u32, u32]),
+        "tgx_svo_get_info": (i32, [vp, vp]),
+        "tgx_svo_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "tgx_svo_trace": (i32, [vp, vp, vp, u32, ctypes.c_float, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -298,6 +310,42 @@ class Context:
 
     def set_list_capacity(self, entries: int) -> None:
         self._check(self.lib.tgx_visibility_set_list_capacity(self.h, entries), "tgx_visibility_set_list_capacity")
+
+    # -- sparse voxel octree ------------------------------------------------
+
+    def svo_build(self, mn, depth: int, flags: int = 0) -> None:
+        m = np.ascontiguousarray(mn, np.float32)
+        self._check(self.lib.tgx_svo_build(self.h, _ptr(m), depth, flags), "tgx_svo_build")
+
+    def svo_info(self):
+        i = np.zeros(1, SVO_INFO_DTYPE)
+        self._check(self.lib.tgx_svo_get_info(self.h, _ptr(i)), "tgx_svo_get_info")
+        return i[0]
+
+    def svo_export(self) -> dict:
+        info = self.svo_info()
+        nl = int(info["n_leaves"])
+        out = {
+            "info": info,
+            "nodes": np.zeros(int(info["n_nodes"]), SVO_NODE_DTYPE),
+            "bits": np.zeros((nl, 1024), np.uint32),
+            "materials": np.zeros((nl, 32768), np.uint16),
+            "leaf_n": np.zeros(nl, np.uint32),
+            "list_offsets": np.zeros(nl + 1, np.uint32),
+            "list": np.zeros(max(1, int(info["n_list_entries"])), np.uint32),
+        }
+        self._check(self.lib.tgx_svo_export(self.h, _ptr(out["nodes"]), _ptr(out["bits"]), _ptr(out["materials"]),
+                                            _ptr(out["leaf_n"]), _ptr(out["list_offsets"]), _ptr(out["list"])),
+                    "tgx_svo_export")
+        return out
+
+    def svo_trace(self, origins: np.ndarray, dirs: np.ndarray, max_distance: float = 3.0e38) -> np.ndarray:
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(o.shape[0], SVO_HIT_DTYPE)
+        self._check(self.lib.tgx_svo_trace(self.h, _ptr(o), _ptr(d), o.shape[0], max_distance, _ptr(out)),
+                    "tgx_svo_trace")
+        return out
 
     def alloc(self, nbytes: int) -> int:
         p = ctypes.c_void_p()

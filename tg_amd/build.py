@@ -14,6 +14,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -26,7 +27,7 @@ ORACLE_DIR = ROOT / "oracle"
 
 ARCH = os.environ.get("TGX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["tgx_runtime.hip", "tgx_visibility.hip"]
+HIP_SOURCES = ["tgx_runtime.hip", "tgx_visibility.hip", "tgx_svo.hip", "tgx_prims.hip"]
 C_SOURCES = ["tgx_gen.c"]
 
 HIPCC_FLAGS = [
@@ -54,16 +55,17 @@ def _hipcc() -> str:
 
 def build_lib() -> Path:
     BUILD.mkdir(exist_ok=True)
-    objs = []
     hipcc = _hipcc()
+    jobs = []
     for src in HIP_SOURCES:
         obj = BUILD / (Path(src).stem + ".o")
-        _run([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(str(obj))
+        jobs.append(([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], str(obj)))
     for src in C_SOURCES:
         obj = BUILD / (Path(src).stem + ".c.o")
-        _run(["gcc", *CC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(str(obj))
+        jobs.append((["gcc", *CC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], str(obj)))
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
+        list(pool.map(lambda j: _run(j[0]), jobs))
+    objs = [o for _, o in jobs]
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", str(tmp), "-lm"])
     os.replace(tmp, LIB)

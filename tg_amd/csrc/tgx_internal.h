@@ -58,6 +58,20 @@ struct tgx_context {
     void*        host_pinned = nullptr; /* small pinned readback area       */
     tgx_visibility_stats vis_stats = {};
 
+    /* Sparse voxel octree (built by tgx_svo_build). */
+    bool         svo_built = false;
+    tgx_svo_info svo_info = {};
+    tgx_buffer   svo_nodes;      /* svo_gnode[n_nodes], canonical order     */
+    tgx_buffer   svo_leaf_bits;  /* 1024 u32 per leaf                       */
+    tgx_buffer   svo_leaf_mats;  /* 32768 u16 per leaf                      */
+    tgx_buffer   svo_leaf_n;     /* clusters per leaf that sampled a voxel  */
+    tgx_buffer   svo_list_off;   /* n_leaves + 1                            */
+    tgx_buffer   svo_list;       /* cluster idx lists                       */
+    tgx_buffer   svo_frames;     /* per-cptr cluster transforms (build)     */
+    tgx_buffer   svo_rays;       /* staging for tgx_svo_trace               */
+    tgx_buffer   svo_hits;
+    tgx_buffer   prim_tmp;       /* rocPRIM scratch                         */
+
     /* Profiling. */
     bool                           profiling = false;
     std::vector<tgx_kernel_stat>   stats;
@@ -65,6 +79,10 @@ struct tgx_context {
 };
 
 int  tgx_fail(tgx_context* ctx, int code, const char* fmt, ...);
+int  tgx_prim_exclusive_scan_u32(tgx_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n);
+int  tgx_prim_sort_pairs_u32(tgx_context* ctx, const uint32_t* keys_in, uint32_t* keys_out,
+                             const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, unsigned end_bit);
+void tgx_svo_release(tgx_context* ctx);
 int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
 void tgx_buffer_release(tgx_buffer* b);
 

@@ -137,4 +137,19 @@ TGX_HD tgx_m4 tgx_world_to_cluster(const float rotation[16], const float transla
     return tgx_m4_mul(tgx_m4_mul(tgx_m4_mul(m3, m2), m1), m0);
 }
 
+/* Cluster -> world space: T(t) * R * T(-half) * T(+offset), associated
+ * left to right (tg_sparse_voxel_octree.c:92-97). */
+TGX_HD tgx_m4 tgx_cluster_to_world(const float rotation[16], const float translation[3],
+                                   const unsigned n[3], unsigned rx, unsigned ry, unsigned rz)
+{
+    tgx_m4 rot;
+    for (int i = 0; i < 16; i++) rot.m[i] = rotation[i];
+    const float offx = (float)(rx * 8u), offy = (float)(ry * 8u), offz = (float)(rz * 8u);
+    const float hx = (float)n[0] * 4.0f, hy = (float)n[1] * 4.0f, hz = (float)n[2] * 4.0f;
+    const tgx_m4 m1 = tgx_m4_translate(offx, offy, offz);
+    const tgx_m4 m2 = tgx_m4_translate(-hx, -hy, -hz);
+    const tgx_m4 m4_ = tgx_m4_translate(translation[0], translation[1], translation[2]);
+    return tgx_m4_mul(tgx_m4_mul(tgx_m4_mul(m4_, rot), m2), m1);
+}
+
 #endif /* TGX_M4_H */

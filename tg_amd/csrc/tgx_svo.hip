@@ -1,0 +1,940 @@
+/*
+ * tgx_svo.hip — sparse voxel octree build and traversal on gfx950.
+ *
+ * Build (replaces tg_svo_create, tg_sparse_voxel_octree.c:466-542, a
+ * single-threaded depth-first recursion on the host):
+ *   1. svo_frames_k      per cluster pointer: emptiness flag and the three
+ *                        cluster transforms of the reference (world->cluster
+ *                        :23-60, cluster->world :62-100 and its inverse).
+ *   2. per level         (node, cluster pointer) pairs; svo_children_k runs
+ *                        tg_intersect_aabb_obb_ignore_contact for the 8
+ *                        children of each pair, a scan sizes the next level
+ *                        and svo_emit_k writes it. Pairs stay ordered by the
+ *                        parent pair, so every node's clusters stay in
+ *                        ascending pointer order (the reference's list order)
+ *                        without sorting between levels.
+ *   3. per level         stable radix sort of the pairs by Morton code; each
+ *                        distinct code is a node. Nodes are numbered by
+ *                        (level, Morton code), children of a node are then
+ *                        contiguous in child-index order; svo_nodes_k links
+ *                        each node to its parent (binary search one level up).
+ *   4. svo_leaf_k        one workgroup per 32^3 leaf: for each listed
+ *                        cluster in order, the trimmed block box is point
+ *                        sampled in the reference's exact fp32 order
+ *                        (:242-315); solid bits collect in LDS, the first
+ *                        cluster to set a voxel writes its material.
+ * Traversal (tg_svo_traverse, :558-740): one lane per ray; the reference's
+ * node stack is the path from the root to the current node, so the lane
+ * keeps only the current node and walks parent links to pop; node boxes are
+ * recomputed exactly from (level, coord).
+ */
+#include "tgx_internal.h"
+#include "tgx_m4.h"
+
+#include <algorithm>
+
+#define SVO_F32_MAX 3.402823466e+38f
+#define SVO_EPSILON 1.192092896e-07f
+#define SVO_NONE    0xFFFFFFFFu
+
+/* Node layout on the device (32 B): canonical order, parent links. */
+struct svo_gnode {
+    uint32_t first_child;
+    uint32_t parent;
+    uint32_t leaf_idx;
+    uint32_t masks;       /* valid | leaf << 8 | level << 16 */
+    uint32_t cx, cy, cz;
+    uint32_t leaf_n;
+};
+static_assert(sizeof(svo_gnode) == 32, "svo_gnode layout");
+
+/* Per-cluster-pointer transforms (rows 0..2 of each m4). */
+struct svo_frame {
+    float    ws2ms[12];
+    float    cs2ws[12];
+    float    ws2cs[12];
+    uint32_t cidx;
+    uint32_t lut;         /* (lut_idx & 255) << 8 */
+    uint32_t pad[2];
+};
+static_assert(sizeof(svo_frame) == 160, "svo_frame layout");
+
+__device__ static inline float fmin2(float a, float b) { return a < b ? a : b; }
+__device__ static inline float fmax2(float a, float b) { return a > b ? a : b; }
+
+struct f3 { float x, y, z; };
+
+/* rows 0..2 of m times (x, y, z, w), tg_math.c:2428-2438 order */
+__device__ static inline f3 mulv(const float* m, float x, float y, float z, float w)
+{
+    f3 r;
+    r.x = x * m[0] + y * m[1] + z * m[2] + w * m[3];
+    r.y = x * m[4] + y * m[5] + z * m[6] + w * m[7];
+    r.z = x * m[8] + y * m[9] + z * m[10] + w * m[11];
+    return r;
+}
+
+__device__ static inline uint32_t spread3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ static inline uint32_t compact3(uint32_t v)
+{
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030C30C3u;
+    v = (v | (v >> 4)) & 0x0300F00Fu;
+    v = (v | (v >> 8)) & 0x030000FFu;
+    v = (v | (v >> 16)) & 0x000003FFu;
+    return v;
+}
+__device__ static inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* tg_intersect_aabb_obb_ignore_contact against the cluster box [0,8]^3     */
+/* (tg_physics.c:226-392), corners c[8] in cluster space                    */
+/* ------------------------------------------------------------------------ */
+
+__device__ static inline f3 f3_sub(f3 a, f3 b) { f3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+__device__ static inline float f3_dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ static inline float f3_mag(f3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ static inline f3 f3_normalized(f3 a)
+{
+    const float m = f3_mag(a);
+    f3 r = {a.x / m, a.y / m, a.z / m};
+    return r;
+}
+
+__device__ static bool obb_overlaps_cluster(const f3* c)
+{
+    bool sep;
+    sep = true; for (int i = 0; i < 8; i++) sep = sep && c[i].x <= 0.0f; if (sep) return false;
+    sep = true; for (int i = 0; i < 8; i++) sep = sep && c[i].x >= 8.0f; if (sep) return false;
+    sep = true; for (int i = 0; i < 8; i++) sep = sep && c[i].y <= 0.0f; if (sep) return false;
+    sep = true; for (int i = 0; i < 8; i++) sep = sep && c[i].y >= 8.0f; if (sep) return false;
+    sep = true; for (int i = 0; i < 8; i++) sep = sep && c[i].z <= 0.0f; if (sep) return false;
+    sep = true; for (int i = 0; i < 8; i++) sep = sep && c[i].z >= 8.0f; if (sep) return false;
+
+    const f3 nxp = f3_normalized(f3_sub(c[1], c[0]));
+    const f3 nyp = f3_normalized(f3_sub(c[2], c[0]));
+    const f3 nzp = f3_normalized(f3_sub(c[4], c[0]));
+    f3 nrm[6];
+    nrm[0] = {-nxp.x, -nxp.y, -nxp.z}; nrm[1] = nxp;
+    nrm[2] = {-nyp.x, -nyp.y, -nyp.z}; nrm[3] = nyp;
+    nrm[4] = {-nzp.x, -nzp.y, -nzp.z}; nrm[5] = nzp;
+    const float l0 = f3_mag(c[0]);
+    const float l1 = f3_mag(c[7]);
+    f3 n0 = {0.0f, 0.0f, 0.0f}, n1 = {0.0f, 0.0f, 0.0f};
+    if (l0 != 0.0f) { n0.x = c[0].x / l0; n0.y = c[0].y / l0; n0.z = c[0].z / l0; }
+    if (l1 != 0.0f) { n1.x = c[7].x / l1; n1.y = c[7].y / l1; n1.z = c[7].z / l1; }
+    float dist[6];
+    dist[0] = l0 * f3_dot(n0, nrm[0]);
+    dist[1] = l1 * f3_dot(n1, nrm[1]);
+    dist[2] = l0 * f3_dot(n0, nrm[2]);
+    dist[3] = l1 * f3_dot(n1, nrm[3]);
+    dist[4] = l0 * f3_dot(n0, nrm[4]);
+    dist[5] = l1 * f3_dot(n1, nrm[5]);
+    for (int p = 0; p < 6; p++) {
+        sep = true;
+        for (int i = 0; i < 8 && sep; i++) {
+            const f3 pt = {(i & 1) ? 8.0f : 0.0f, (i & 2) ? 8.0f : 0.0f, (i & 4) ? 8.0f : 0.0f};
+            sep = (f3_dot(nrm[p], pt) - dist[p]) >= 0.0f;
+        }
+        if (sep) return false;
+    }
+    return true;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Build kernels                                                            */
+/* ------------------------------------------------------------------------ */
+
+__global__ void svo_fill_k(uint32_t* p, uint64_t n, uint32_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void svo_frames_k(const tgx_object* __restrict__ objects,
+                                                    const uint32_t* __restrict__ cptrs,
+                                                    const uint32_t* __restrict__ c2o,
+                                                    const uint32_t* __restrict__ bits, uint32_t n_cptr,
+                                                    svo_frame* __restrict__ frames, uint32_t* __restrict__ nonempty)
+{
+    const uint32_t cp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cp >= n_cptr) return;
+    const uint32_t cidx = cptrs[cp];
+    uint32_t any = 0;
+    for (int i = 0; i < 16; i++) any |= bits[(size_t)cidx * 16u + i];
+    nonempty[cp] = any ? 1u : 0u;
+    const tgx_object ob = objects[c2o[cidx]];
+    const uint32_t rel = cp - ob.first_cluster_pointer;
+    const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
+    const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
+    const tgx_m4 w2c = tgx_world_to_cluster(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_m4 c2w = tgx_cluster_to_world(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_m4 w2c2 = tgx_m4_inverse(c2w);
+    svo_frame f;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++) {
+            f.ws2ms[4 * r + c] = TGX_M(w2c, r, c);
+            f.cs2ws[4 * r + c] = TGX_M(c2w, r, c);
+            f.ws2cs[4 * r + c] = TGX_M(w2c2, r, c);
+        }
+    f.cidx = cidx;
+    f.lut = (ob.lut_idx & 0xFFu) << 8;
+    f.pad[0] = f.pad[1] = 0u;
+    frames[cp] = f;
+}
+
+/* Level 0: the root's pairs are the non-empty cluster pointers in order. */
+__global__ void svo_compact_k(const uint32_t* __restrict__ nonempty, const uint32_t* __restrict__ pos, uint32_t n,
+                              uint32_t* __restrict__ out_coord, uint32_t* __restrict__ out_cptr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !nonempty[i]) return;
+    out_coord[pos[i]] = 0u;
+    out_cptr[pos[i]] = i;
+}
+
+/* Packed node coordinates: x | y << 10 | z << 20. */
+__device__ static inline void unpack(uint32_t p, uint32_t& x, uint32_t& y, uint32_t& z)
+{
+    x = p & 0x3FFu; y = (p >> 10) & 0x3FFu; z = p >> 20;
+}
+
+__global__ __launch_bounds__(256) void svo_children_k(const uint32_t* __restrict__ coord, const uint32_t* __restrict__ cptr,
+                                                      uint32_t n, const svo_frame* __restrict__ frames, float mnx,
+                                                      float mny, float mnz, float side,
+                                                      uint32_t* __restrict__ masks, uint32_t* __restrict__ counts)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z;
+    unpack(coord[i], x, y, z);
+    /* Parent box in world space: exact integers (tg_sparse_voxel_octree.c:344,359-365). */
+    const float pminx = mnx + (float)x * side, pminy = mny + (float)y * side, pminz = mnz + (float)z * side;
+    const float pmaxx = pminx + side, pmaxy = pminy + side, pmaxz = pminz + side;
+    const float cex = (pmaxx - pminx) * 0.5f, cey = (pmaxy - pminy) * 0.5f, cez = (pmaxz - pminz) * 0.5f;
+    const float* m = frames[cptr[i]].ws2ms;
+    uint32_t mask = 0;
+    for (uint32_t c = 0; c < 8; c++) {
+        const float cminx = pminx + (float)(c % 2) * cex;
+        const float cminy = pminy + (float)((c / 2) % 2) * cey;
+        const float cminz = pminz + (float)((c / 4) % 2) * cez;
+        const float cmaxx = cminx + cex, cmaxy = cminy + cey, cmaxz = cminz + cez;
+        f3 k[8];
+        for (int j = 0; j < 8; j++)
+            k[j] = mulv(m, (j & 1) ? cmaxx : cminx, (j & 2) ? cmaxy : cminy, (j & 4) ? cmaxz : cminz, 1.0f);
+        if (obb_overlaps_cluster(k)) mask |= 1u << c;
+    }
+    masks[i] = mask;
+    counts[i] = (uint32_t)__popc(mask);
+}
+
+__global__ void svo_emit_k(const uint32_t* __restrict__ coord, const uint32_t* __restrict__ cptr, uint32_t n,
+                           const uint32_t* __restrict__ masks, const uint32_t* __restrict__ off,
+                           uint32_t* __restrict__ out_coord, uint32_t* __restrict__ out_cptr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z;
+    unpack(coord[i], x, y, z);
+    const uint32_t mask = masks[i];
+    uint32_t o = off[i];
+    for (uint32_t c = 0; c < 8; c++) {
+        if (!(mask & (1u << c))) continue;
+        const uint32_t cx = 2u * x + (c & 1u), cy = 2u * y + ((c >> 1) & 1u), cz = 2u * z + ((c >> 2) & 1u);
+        out_coord[o] = cx | (cy << 10) | (cz << 20);
+        out_cptr[o] = cptr[i];
+        o++;
+    }
+}
+
+__global__ void svo_morton_k(const uint32_t* __restrict__ coord, uint32_t n, uint32_t* __restrict__ key)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z;
+    unpack(coord[i], x, y, z);
+    key[i] = morton3(x, y, z);
+}
+
+__global__ void svo_unique_flags_k(const uint32_t* __restrict__ key, uint32_t n, uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+}
+
+/* One thread per pair starting a node: creates the node (canonical index
+ * base + rank) and links it to its parent one level up. */
+__global__ void svo_nodes_k(const uint32_t* __restrict__ key, const uint32_t* __restrict__ flag,
+                            const uint32_t* __restrict__ rank, uint32_t n, uint32_t level, uint32_t base,
+                            uint32_t is_leaf_level, const uint32_t* __restrict__ parent_keys, uint32_t n_parent,
+                            uint32_t parent_base, svo_gnode* __restrict__ nodes, uint32_t* __restrict__ first_pair)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const uint32_t r = rank[i];
+    const uint32_t idx = base + r;
+    const uint32_t k = key[i];
+    first_pair[r] = i;
+    svo_gnode nd;
+    nd.first_child = SVO_NONE;
+    nd.leaf_idx = is_leaf_level ? r : SVO_NONE;
+    nd.masks = level << 16;
+    nd.cx = compact3(k);
+    nd.cy = compact3(k >> 1);
+    nd.cz = compact3(k >> 2);
+    nd.leaf_n = 0;
+    /* parent: Morton code >> 3 among the sorted keys of the level above */
+    const uint32_t pk = k >> 3;
+    uint32_t lo = 0, hi = n_parent;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (parent_keys[mid] < pk) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t parent = parent_base + lo;
+    nd.parent = parent;
+    nodes[idx] = nd;
+    const uint32_t bit = 1u << (k & 7u);
+    atomicOr(&nodes[parent].masks, is_leaf_level ? (bit | (bit << 8)) : bit);
+    atomicMin(&nodes[parent].first_child, idx);
+}
+
+/* Keys of the distinct nodes of a level, in order. */
+__global__ void svo_node_keys_k(const uint32_t* __restrict__ key, const uint32_t* __restrict__ flag,
+                                const uint32_t* __restrict__ rank, uint32_t n, uint32_t* __restrict__ node_keys)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) node_keys[rank[i]] = key[i];
+}
+
+/* Trilinear interpolation of the block corners (tg_sparse_voxel_octree.c:242-291). */
+__device__ static inline void sample_center(const f3* bc, uint32_t x, uint32_t y, uint32_t z, float& cx, float& cy,
+                                            float& cz)
+{
+    const float tz = ((float)z + 0.5f) / 32.0f;
+    const float omtz = 1.0f - tz;
+    const float pz0x = omtz * bc[0].x + tz * bc[4].x, pz0y = omtz * bc[0].y + tz * bc[4].y, pz0z = omtz * bc[0].z + tz * bc[4].z;
+    const float pz1x = omtz * bc[1].x + tz * bc[5].x, pz1y = omtz * bc[1].y + tz * bc[5].y, pz1z = omtz * bc[1].z + tz * bc[5].z;
+    const float pz2x = omtz * bc[2].x + tz * bc[6].x, pz2y = omtz * bc[2].y + tz * bc[6].y, pz2z = omtz * bc[2].z + tz * bc[6].z;
+    const float pz3x = omtz * bc[3].x + tz * bc[7].x, pz3y = omtz * bc[3].y + tz * bc[7].y, pz3z = omtz * bc[3].z + tz * bc[7].z;
+    const float ty = ((float)y + 0.5f) / 32.0f;
+    const float omty = 1.0f - ty;
+    const float py0x = omty * pz0x + ty * pz2x, py0y = omty * pz0y + ty * pz2y, py0z = omty * pz0z + ty * pz2z;
+    const float py1x = omty * pz1x + ty * pz3x, py1y = omty * pz1y + ty * pz3y, py1z = omty * pz1z + ty * pz3z;
+    const float tx = ((float)x + 0.5f) / 32.0f;
+    const float omtx = 1.0f - tx;
+    cx = omtx * py0x + tx * py1x;
+    cy = omtx * py0y + tx * py1y;
+    cz = omtx * py0z + tx * py1z;
+}
+
+__global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ nodes, uint32_t leaf_base,
+                                                  const uint32_t* __restrict__ pair_cptr,
+                                                  const uint32_t* __restrict__ leaf_first_pair, uint32_t n_pairs,
+                                                  uint32_t n_leaves, const svo_frame* __restrict__ frames,
+                                                  const uint32_t* __restrict__ bits,
+                                                  const uint8_t* __restrict__ materials, float mnx, float mny,
+                                                  float mnz, uint32_t quirk, uint32_t* __restrict__ leaf_bits,
+                                                  uint16_t* __restrict__ leaf_mats, uint32_t* __restrict__ leaf_n,
+                                                  uint32_t* __restrict__ list_tmp, svo_gnode* __restrict__ nodes_rw)
+{
+    __shared__ uint32_t lbits[TGX_SVO_BLOCK_WORDS];
+    __shared__ int any_flag;
+    __shared__ uint32_t n_contrib, last_cidx;
+    const uint32_t leaf = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const svo_gnode nd = nodes[leaf_base + leaf];
+    const float pminx = mnx + (float)nd.cx * 32.0f, pminy = mny + (float)nd.cy * 32.0f, pminz = mnz + (float)nd.cz * 32.0f;
+    const float pmaxx = pminx + 32.0f, pmaxy = pminy + 32.0f, pmaxz = pminz + 32.0f;
+    for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS; w += blockDim.x) lbits[w] = 0u;
+    if (tid == 0) { n_contrib = 0u; last_cidx = SVO_NONE; }
+    uint16_t* mats = leaf_mats + (size_t)leaf * TGX_SVO_BLOCK_VOXELS;
+    const uint32_t b = leaf_first_pair[leaf];
+    const uint32_t e = leaf + 1 < n_leaves ? leaf_first_pair[leaf + 1] : n_pairs;
+    __syncthreads();
+    for (uint32_t k = b; k < e; k++) {
+        const svo_frame& f = frames[pair_cptr[k]];
+        /* cluster corners in world space and their AABB (:134-197) */
+        f3 cw[8];
+        for (int i = 0; i < 8; i++)
+            cw[i] = mulv(f.cs2ws, (i & 1) ? 8.0f : 0.0f, (i & 2) ? 8.0f : 0.0f, (i & 4) ? 8.0f : 0.0f, 1.0f);
+        const int c3 = quirk ? 2 : 3;
+        const float minx = fmin2(fmin2(fmin2(cw[0].x, cw[1].x), fmin2(cw[2].x, cw[c3].x)), fmin2(fmin2(cw[4].x, cw[6].x), fmin2(cw[5].x, cw[7].x)));
+        const float miny = fmin2(fmin2(fmin2(cw[0].y, cw[1].y), fmin2(cw[2].y, cw[c3].y)), fmin2(fmin2(cw[4].y, cw[6].y), fmin2(cw[5].y, cw[7].y)));
+        const float minz = fmin2(fmin2(fmin2(cw[0].z, cw[1].z), fmin2(cw[2].z, cw[c3].z)), fmin2(fmin2(cw[4].z, cw[6].z), fmin2(cw[5].z, cw[7].z)));
+        const float maxx = fmax2(fmax2(fmax2(cw[0].x, cw[1].x), fmax2(cw[2].x, cw[c3].x)), fmax2(fmax2(cw[4].x, cw[6].x), fmax2(cw[5].x, cw[7].x)));
+        const float maxy = fmax2(fmax2(fmax2(cw[0].y, cw[1].y), fmax2(cw[2].y, cw[c3].y)), fmax2(fmax2(cw[4].y, cw[6].y), fmax2(cw[5].y, cw[7].y)));
+        const float maxz = fmax2(fmax2(fmax2(cw[0].z, cw[1].z), fmax2(cw[2].z, cw[c3].z)), fmax2(fmax2(cw[4].z, cw[6].z), fmax2(cw[5].z, cw[7].z)));
+        /* block corners in cluster space (:199-211) */
+        f3 bc[8];
+        for (int i = 0; i < 8; i++)
+            bc[i] = mulv(f.ws2cs, (i & 1) ? pmaxx : pminx, (i & 2) ? pmaxy : pminy, (i & 4) ? pmaxz : pminz, 1.0f);
+        /* trimmed box (:218-233), clamped into [0, 32] */
+        const float tlo[3] = {fmax2(pminx, floorf(minx)) - pminx, fmax2(pminy, floorf(miny)) - pminy, fmax2(pminz, floorf(minz)) - pminz};
+        const float thi[3] = {fmin2(pmaxx, ceilf(maxx)) - pminx, fmin2(pmaxy, ceilf(maxy)) - pminy, fmin2(pmaxz, ceilf(maxz)) - pminz};
+        uint32_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (uint32_t)fmin2(32.0f, fmax2(0.0f, tlo[a]));
+            hi[a] = (uint32_t)fmin2(32.0f, fmax2(0.0f, thi[a]));
+        }
+        const uint32_t nx = hi[0] > lo[0] ? hi[0] - lo[0] : 0u;
+        const uint32_t ny = hi[1] > lo[1] ? hi[1] - lo[1] : 0u;
+        const uint32_t nz = hi[2] > lo[2] ? hi[2] - lo[2] : 0u;
+        const uint32_t nv = nx * ny * nz;
+        const uint64_t coff = (uint64_t)f.cidx * 512u;
+        if (tid == 0) any_flag = 0;
+        __syncthreads();
+        bool any = false;
+        for (uint32_t j = tid; j < nv; j += blockDim.x) {
+            const uint32_t x = lo[0] + j % nx, y = lo[1] + (j / nx) % ny, z = lo[2] + j / (nx * ny);
+            float cx, cy, cz;
+            sample_center(bc, x, y, z, cx, cy, cz);
+            if (cx < 0.0f || cx >= 8.0f || cy < 0.0f || cy >= 8.0f || cz < 0.0f || cz >= 8.0f) continue;
+            const uint32_t v = 64u * (uint32_t)cz + 8u * (uint32_t)cy + (uint32_t)cx;
+            const uint64_t cv = coff + v;
+            if (!(bits[cv >> 5] & (1u << (uint32_t)(cv & 31u)))) continue;
+            any = true;
+            const uint32_t bv = 1024u * z + 32u * y + x;
+            const uint32_t m = 1u << (bv & 31u);
+            /* Within one cluster each block voxel is visited once; earlier
+             * clusters' bits are complete (barrier), so the first cluster in
+             * list order writes the material. */
+            if (!(lbits[bv >> 5] & m)) {
+                atomicOr(&lbits[bv >> 5], m);
+                mats[bv] = (uint16_t)(f.lut | (materials ? (uint32_t)materials[cv] : 0u));
+            }
+        }
+        if (any) any_flag = 1;
+        __syncthreads();
+        if (tid == 0 && any_flag && last_cidx != f.cidx) {
+            list_tmp[b + n_contrib] = f.cidx;
+            n_contrib++;
+            last_cidx = f.cidx;
+        }
+        __syncthreads();
+    }
+    uint32_t* out = leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS;
+    for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS; w += blockDim.x) out[w] = lbits[w];
+    if (tid == 0) {
+        leaf_n[leaf] = n_contrib;
+        nodes_rw[leaf_base + leaf].leaf_n = n_contrib;
+    }
+}
+
+__global__ void svo_list_compact_k(const uint32_t* __restrict__ leaf_first_pair, const uint32_t* __restrict__ leaf_n,
+                                   const uint32_t* __restrict__ list_off, uint32_t n_leaves,
+                                   const uint32_t* __restrict__ list_tmp, uint32_t* __restrict__ list)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n_leaves) return;
+    const uint32_t b = leaf_first_pair[l], o = list_off[l];
+    for (uint32_t i = 0; i < leaf_n[l]; i++) list[o + i] = list_tmp[b + i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Traversal                                                                */
+/* ------------------------------------------------------------------------ */
+
+struct svo_ray_result {
+    uint32_t hit;
+    float    distance;
+    uint32_t node;       /* leaf node of the hit */
+    uint32_t voxel;
+    int      ax;         /* entry face axis */
+    int      sign;       /* entry face normal sign */
+    uint32_t steps;
+};
+
+__device__ static inline bool svo_slab(f3 o, f3 d, f3 mn, f3 mx, float& enter, float& exit_)
+{
+    const float ax = (d.x == 0.0f) ? -SVO_F32_MAX : ((mn.x - o.x) / d.x);
+    const float ay = (d.y == 0.0f) ? -SVO_F32_MAX : ((mn.y - o.y) / d.y);
+    const float az = (d.z == 0.0f) ? -SVO_F32_MAX : ((mn.z - o.z) / d.z);
+    const float bx = (d.x == 0.0f) ? SVO_F32_MAX : ((mx.x - o.x) / d.x);
+    const float by = (d.y == 0.0f) ? SVO_F32_MAX : ((mx.y - o.y) / d.y);
+    const float bz = (d.z == 0.0f) ? SVO_F32_MAX : ((mx.z - o.z) / d.z);
+    const float nx = fmin2(ax, bx), ny = fmin2(ay, by), nz = fmin2(az, bz);
+    const float fx = fmax2(ax, bx), fy = fmax2(ay, by), fz = fmax2(az, bz);
+    enter = fmax2(fmax2(nx, ny), nz);
+    exit_ = fmin2(fmin2(fx, fy), fz);
+    return exit_ > 0.0f && enter <= exit_;
+}
+
+/* min over axes of max((mn - p)/d, (mx - p)/d) with the zero-direction
+ * alternatives of tgm_v3_div_zero_check (tg_sparse_voxel_octree.c:704-707). */
+__device__ static inline float svo_exit(f3 p, f3 d, f3 mn, f3 mx)
+{
+    const float ax = (d.x == 0.0f) ? -SVO_F32_MAX : ((mn.x - p.x) / d.x);
+    const float ay = (d.y == 0.0f) ? -SVO_F32_MAX : ((mn.y - p.y) / d.y);
+    const float az = (d.z == 0.0f) ? -SVO_F32_MAX : ((mn.z - p.z) / d.z);
+    const float bx = (d.x == 0.0f) ? SVO_F32_MAX : ((mx.x - p.x) / d.x);
+    const float by = (d.y == 0.0f) ? SVO_F32_MAX : ((mx.y - p.y) / d.y);
+    const float bz = (d.z == 0.0f) ? SVO_F32_MAX : ((mx.z - p.z) / d.z);
+    return fmin2(fmin2(fmax2(ax, bx), fmax2(ay, by)), fmax2(az, bz));
+}
+
+/* tg_amanatides_woo over one 32^3 block (entry voxel clamped both sides). */
+__device__ static inline bool svo_block_dda(f3 h, f3 d, const uint32_t* __restrict__ grid, int& ox, int& oy, int& oz)
+{
+    int x = (int)fmin2(fmax2(floorf(h.x), 0.0f), 31.0f);
+    int y = (int)fmin2(fmax2(floorf(h.y), 0.0f), 31.0f);
+    int z = (int)fmin2(fmax2(floorf(h.z), 0.0f), 31.0f);
+    int sx = 0, sy = 0, sz = 0;
+    float tmx = SVO_F32_MAX, tmy = SVO_F32_MAX, tmz = SVO_F32_MAX;
+    float tdx = SVO_F32_MAX, tdy = SVO_F32_MAX, tdz = SVO_F32_MAX;
+    if (d.x > 0.0f) { sx = 1; tmx = ((float)(x + 1) - h.x) / d.x; tdx = 1.0f / d.x; }
+    else if (d.x < 0.0f) { sx = -1; tmx = (h.x - (float)x) / -d.x; tdx = 1.0f / -d.x; }
+    if (d.y > 0.0f) { sy = 1; tmy = ((float)(y + 1) - h.y) / d.y; tdy = 1.0f / d.y; }
+    else if (d.y < 0.0f) { sy = -1; tmy = (h.y - (float)y) / -d.y; tdy = 1.0f / -d.y; }
+    if (d.z > 0.0f) { sz = 1; tmz = ((float)(z + 1) - h.z) / d.z; tdz = 1.0f / d.z; }
+    else if (d.z < 0.0f) { sz = -1; tmz = (h.z - (float)z) / -d.z; tdz = 1.0f / -d.z; }
+    for (uint32_t it = 0; it < 4u * TGX_DDA_MAX_ITERATIONS; it++) {
+        const uint32_t v = 1024u * (uint32_t)z + 32u * (uint32_t)y + (uint32_t)x;
+        if (grid[v >> 5] & (1u << (v & 31u))) { ox = x; oy = y; oz = z; return true; }
+        if (tmx < tmy) {
+            if (tmx < tmz) { tmx += tdx; x += sx; if (x < 0 || x >= 32) break; }
+            else           { tmz += tdz; z += sz; if (z < 0 || z >= 32) break; }
+        } else {
+            if (tmy < tmz) { tmy += tdy; y += sy; if (y < 0 || y >= 32) break; }
+            else           { tmz += tdz; z += sz; if (z < 0 || z >= 32) break; }
+        }
+    }
+    return false;
+}
+
+__device__ static inline f3 node_min(const svo_gnode& n, float half, uint32_t depth)
+{
+    const float side = (float)(1u << (depth - ((n.masks >> 16) & 0xFFu)));
+    f3 r = {-half + (float)n.cx * side, -half + (float)n.cy * side, -half + (float)n.cz * side};
+    return r;
+}
+
+/* tg_svo_traverse for one ray; origin and direction in world space. */
+__device__ svo_ray_result svo_trace_ray(const svo_gnode* __restrict__ nodes, const uint32_t* __restrict__ leaf_bits,
+                                        float mnx, float mny, float mnz, uint32_t depth, f3 origin, f3 d,
+                                        float max_distance)
+{
+    svo_ray_result res = {0u, SVO_F32_MAX, SVO_NONE, 0u, 0, 0, 0u};
+    const float side = (float)(1u << depth);
+    const float half = side * 0.5f;
+    const f3 o = {origin.x - (mnx + half), origin.y - (mny + half), origin.z - (mnz + half)};
+    const f3 rmin = {-half, -half, -half}, rmax = {half, half, half};
+    float enter, exit_;
+    if (!svo_slab(o, d, rmin, rmax, enter, exit_)) return res;
+    f3 pos = o;
+    if (enter > 0.0f) { pos.x = o.x + d.x * enter; pos.y = o.y + d.y * enter; pos.z = o.z + d.z * enter; }
+    float dist = enter > 0.0f ? enter : 0.0f;
+    uint32_t cur = 0;
+    uint32_t steps = 0;
+    while (cur != SVO_NONE) {
+        if (steps >= TGX_SVO_MAX_STEPS || dist > max_distance) break;
+        steps++;
+        const svo_gnode pn = nodes[cur];
+        const f3 pmn = node_min(pn, half, depth);
+        const float ps = (float)(1u << (depth - ((pn.masks >> 16) & 0xFFu)));
+        const f3 pmx = {pmn.x + ps, pmn.y + ps, pmn.z + ps};
+        const f3 ce = {(pmx.x - pmn.x) * 0.5f, (pmx.y - pmn.y) * 0.5f, (pmx.z - pmn.z) * 0.5f};
+        uint32_t rc = 0;
+        f3 cmin = pmn;
+        f3 cmax = {cmin.x + ce.x, cmin.y + ce.y, cmin.z + ce.z};
+        if (cmax.x < pos.x || (pos.x == cmax.x && d.x > 0.0f)) { rc += 1; cmin.x += ce.x; cmax.x += ce.x; }
+        if (cmax.y < pos.y || (pos.y == cmax.y && d.y > 0.0f)) { rc += 2; cmin.y += ce.y; cmax.y += ce.y; }
+        if (cmax.z < pos.z || (pos.z == cmax.z && d.z > 0.0f)) { rc += 4; cmin.z += ce.z; cmax.z += ce.z; }
+        bool advance = true;
+        const uint32_t valid = pn.masks & 0xFFu, leafm = (pn.masks >> 8) & 0xFFu;
+        if (valid & (1u << rc)) {
+            const uint32_t ci = pn.first_child + (uint32_t)__popc(valid & ((1u << rc) - 1u));
+            if (leafm & (1u << rc)) {
+                const svo_gnode cn = nodes[ci];
+                if (cn.leaf_n != 0u) {
+                    const f3 hg = {pos.x - cmin.x, pos.y - cmin.y, pos.z - cmin.z};
+                    int vx, vy, vz;
+                    if (svo_block_dda(hg, d, leaf_bits + (size_t)cn.leaf_idx * TGX_SVO_BLOCK_WORDS, vx, vy, vz)) {
+                        const f3 vmn = {cmin.x + (float)vx, cmin.y + (float)vy, cmin.z + (float)vz};
+                        const f3 vmx = {vmn.x + 1.0f, vmn.y + 1.0f, vmn.z + 1.0f};
+                        (void)svo_slab(pos, d, vmn, vmx, enter, exit_);
+                        res.distance = dist + enter;
+                        res.hit = res.distance <= max_distance ? 1u : 0u;
+                        res.node = ci;
+                        res.voxel = 1024u * (uint32_t)vz + 32u * (uint32_t)vy + (uint32_t)vx;
+                        /* entry face: largest near-plane distance */
+                        const float nx_ = fmin2((d.x == 0.0f) ? -SVO_F32_MAX : ((vmn.x - pos.x) / d.x), (d.x == 0.0f) ? SVO_F32_MAX : ((vmx.x - pos.x) / d.x));
+                        const float ny_ = fmin2((d.y == 0.0f) ? -SVO_F32_MAX : ((vmn.y - pos.y) / d.y), (d.y == 0.0f) ? SVO_F32_MAX : ((vmx.y - pos.y) / d.y));
+                        const float nz_ = fmin2((d.z == 0.0f) ? -SVO_F32_MAX : ((vmn.z - pos.z) / d.z), (d.z == 0.0f) ? SVO_F32_MAX : ((vmx.z - pos.z) / d.z));
+                        int ax = nx_ > ny_ ? 0 : 1;
+                        const float nax = ax == 0 ? nx_ : ny_;
+                        if (!(nax > nz_)) ax = 2;
+                        const float dc = ax == 0 ? d.x : (ax == 1 ? d.y : d.z);
+                        res.ax = ax;
+                        res.sign = dc > 0.0f ? -1 : 1;
+                        res.steps = steps;
+                        return res;
+                    }
+                }
+            } else {
+                advance = false;
+                cur = ci;
+            }
+        }
+        if (advance) {
+            exit_ = svo_exit(pos, d, cmin, cmax);
+            const float t_adv = exit_ + SVO_EPSILON;
+            pos.x = pos.x + d.x * t_adv;
+            pos.y = pos.y + d.y * t_adv;
+            pos.z = pos.z + d.z * t_adv;
+            dist += t_adv;
+            /* pop every node the position has left: walk parent links */
+            while (cur != SVO_NONE) {
+                const svo_gnode nn = nodes[cur];
+                const f3 nmn = node_min(nn, half, depth);
+                const float ns = (float)(1u << (depth - ((nn.masks >> 16) & 0xFFu)));
+                const f3 nmx = {nmn.x + ns, nmn.y + ns, nmn.z + ns};
+                if (svo_exit(pos, d, nmn, nmx) > SVO_EPSILON) break;
+                cur = nn.parent;
+            }
+        }
+    }
+    res.steps = steps;
+    return res;
+}
+
+__global__ __launch_bounds__(256) void svo_trace_k(const svo_gnode* __restrict__ nodes,
+                                                   const uint32_t* __restrict__ leaf_bits, float mnx, float mny,
+                                                   float mnz, uint32_t depth, const float* __restrict__ origins,
+                                                   const float* __restrict__ dirs, uint32_t n, float max_distance,
+                                                   tgx_svo_hit* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 o = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
+    const f3 d = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    const svo_ray_result r = svo_trace_ray(nodes, leaf_bits, mnx, mny, mnz, depth, o, d, max_distance);
+    tgx_svo_hit h;
+    h.hit = r.hit;
+    h.distance = r.distance;
+    h.block[0] = h.block[1] = h.block[2] = 0u;
+    h.voxel = r.voxel;
+    h.normal[0] = h.normal[1] = h.normal[2] = 0;
+    if (r.node != SVO_NONE) {
+        const svo_gnode cn = nodes[r.node];
+        h.block[0] = cn.cx; h.block[1] = cn.cy; h.block[2] = cn.cz;
+        h.normal[r.ax] = r.sign;
+    }
+    h.steps = r.steps;
+    out[i] = h;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Host side                                                                */
+/* ------------------------------------------------------------------------ */
+
+static inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+void tgx_svo_release(tgx_context* ctx)
+{
+    tgx_buffer* bufs[] = {&ctx->svo_nodes, &ctx->svo_leaf_bits, &ctx->svo_leaf_mats, &ctx->svo_leaf_n,
+                          &ctx->svo_list_off, &ctx->svo_list, &ctx->svo_frames, &ctx->svo_rays,
+                          &ctx->svo_hits, &ctx->prim_tmp};
+    for (tgx_buffer* b : bufs) tgx_buffer_release(b);
+    ctx->svo_built = false;
+}
+
+namespace {
+/* Temporary device arrays of one build. */
+struct dbuf {
+    void* p = nullptr;
+    ~dbuf() { if (p) (void)hipFree(p); }
+    template <typename T> T* as() { return (T*)p; }
+};
+int dalloc(tgx_context* ctx, dbuf& b, size_t bytes, const char* what)
+{
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 4);
+    if (e != hipSuccess) return tgx_fail(ctx, TGX_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) for %s", bytes, what);
+    return TGX_OK;
+}
+/* Pinned scratch of the SVO code: bytes [64, 128) of the context's pinned
+ * page (bytes [0, 64) hold the visibility statistics readback). */
+inline uint32_t* svo_pinned(tgx_context* ctx) { return (uint32_t*)((char*)ctx->host_pinned + 64); }
+int read_u32(tgx_context* ctx, const uint32_t* dev, uint32_t* out)
+{
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(svo_pinned(ctx), dev, 4, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = *svo_pinned(ctx);
+    return TGX_OK;
+}
+/* total of an exclusive scan = last offset + last count */
+int scan_total(tgx_context* ctx, const uint32_t* counts, const uint32_t* offs, uint32_t n, uint32_t* total)
+{
+    if (n == 0) { *total = 0; return TGX_OK; }
+    uint32_t a, b;
+    int rc;
+    if ((rc = read_u32(ctx, offs + n - 1, &a))) return rc;
+    if ((rc = read_u32(ctx, counts + n - 1, &b))) return rc;
+    *total = a + b;
+    return TGX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t flags)
+{
+    if (!ctx || !mn) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
+    if (depth < TGX_SVO_MIN_DEPTH || depth > TGX_SVO_MAX_DEPTH)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "depth %u outside [%u, %u]", depth, TGX_SVO_MIN_DEPTH, TGX_SVO_MAX_DEPTH);
+    for (int a = 0; a < 3; a++)
+        if (!(mn[a] == floorf(mn[a])) || fabsf(mn[a]) + (float)(1u << depth) >= 16777216.0f)
+            return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "svo min must be integral and |min| + 2^depth < 2^24");
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    tgx_svo_release(ctx);
+    int rc;
+    const uint32_t n_cptr = ctx->n_cluster_pointers;
+    const uint32_t leaf_level = depth - 5u;
+
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_frames, sizeof(svo_frame) * (size_t)n_cptr, "svo frames"))) return rc;
+    svo_frame* frames = (svo_frame*)ctx->svo_frames.ptr;
+    dbuf nonempty, pos;
+    if ((rc = dalloc(ctx, nonempty, 4ull * n_cptr, "svo flags")) || (rc = dalloc(ctx, pos, 4ull * n_cptr, "svo positions"))) return rc;
+    {
+        tgx_launch_scope s(ctx, "svo_frames_k");
+        hipLaunchKernelGGL(svo_frames_k, dim3(blocks_for(n_cptr)), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint32_t*)ctx->bits.ptr, n_cptr,
+                           frames, nonempty.as<uint32_t>());
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_frames_k");
+    if ((rc = tgx_prim_exclusive_scan_u32(ctx, nonempty.as<uint32_t>(), pos.as<uint32_t>(), n_cptr))) return rc;
+    uint32_t n_pairs = 0;
+    if ((rc = scan_total(ctx, nonempty.as<uint32_t>(), pos.as<uint32_t>(), n_cptr, &n_pairs))) return rc;
+
+    /* Per level: pairs (coords, cptr) in emission order, then sorted. */
+    std::vector<dbuf> lvl_coord(leaf_level + 1), lvl_cptr(leaf_level + 1);
+    std::vector<uint32_t> lvl_n(leaf_level + 1, 0);
+    if ((rc = dalloc(ctx, lvl_coord[0], 4ull * n_pairs, "svo pairs")) || (rc = dalloc(ctx, lvl_cptr[0], 4ull * n_pairs, "svo pairs"))) return rc;
+    lvl_n[0] = n_pairs;
+    hipLaunchKernelGGL(svo_compact_k, dim3(blocks_for(n_cptr)), dim3(256), 0, ctx->stream, nonempty.as<uint32_t>(),
+                       pos.as<uint32_t>(), n_cptr, lvl_coord[0].as<uint32_t>(), lvl_cptr[0].as<uint32_t>());
+    TGX_LAUNCH_CHECK(ctx, "svo_compact_k");
+
+    for (uint32_t l = 0; l < leaf_level; l++) {
+        const uint32_t n = lvl_n[l];
+        dbuf masks, counts, offs;
+        if ((rc = dalloc(ctx, masks, 4ull * n, "svo masks")) || (rc = dalloc(ctx, counts, 4ull * n, "svo counts")) ||
+            (rc = dalloc(ctx, offs, 4ull * n, "svo offsets")))
+            return rc;
+        const float side = (float)(1u << (depth - l));
+        if (n) {
+            tgx_launch_scope s(ctx, "svo_children_k");
+            hipLaunchKernelGGL(svo_children_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, lvl_coord[l].as<uint32_t>(),
+                               lvl_cptr[l].as<uint32_t>(), n, frames, mn[0], mn[1], mn[2], side, masks.as<uint32_t>(),
+                               counts.as<uint32_t>());
+        }
+        TGX_LAUNCH_CHECK(ctx, "svo_children_k");
+        if ((rc = tgx_prim_exclusive_scan_u32(ctx, counts.as<uint32_t>(), offs.as<uint32_t>(), n))) return rc;
+        uint32_t next = 0;
+        if ((rc = scan_total(ctx, counts.as<uint32_t>(), offs.as<uint32_t>(), n, &next))) return rc;
+        if ((rc = dalloc(ctx, lvl_coord[l + 1], 4ull * next, "svo pairs")) || (rc = dalloc(ctx, lvl_cptr[l + 1], 4ull * next, "svo pairs"))) return rc;
+        if (n)
+            hipLaunchKernelGGL(svo_emit_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, lvl_coord[l].as<uint32_t>(),
+                               lvl_cptr[l].as<uint32_t>(), n, masks.as<uint32_t>(), offs.as<uint32_t>(),
+                               lvl_coord[l + 1].as<uint32_t>(), lvl_cptr[l + 1].as<uint32_t>());
+        TGX_LAUNCH_CHECK(ctx, "svo_emit_k");
+        lvl_n[l + 1] = next;
+    }
+
+    /* Sort each level by Morton code (stable: clusters stay in pointer
+     * order within a node) and count the distinct nodes. */
+    std::vector<dbuf> lvl_keys(leaf_level + 1), lvl_scptr(leaf_level + 1), lvl_flag(leaf_level + 1),
+        lvl_rank(leaf_level + 1), lvl_node_keys(leaf_level + 1);
+    std::vector<uint32_t> lvl_nodes(leaf_level + 1, 0), lvl_base(leaf_level + 1, 0);
+    lvl_nodes[0] = 1;
+    uint32_t total_nodes = 1;
+    for (uint32_t l = 1; l <= leaf_level; l++) {
+        const uint32_t n = lvl_n[l];
+        dbuf raw;
+        if ((rc = dalloc(ctx, raw, 4ull * n, "svo keys")) || (rc = dalloc(ctx, lvl_keys[l], 4ull * n, "svo keys")) ||
+            (rc = dalloc(ctx, lvl_scptr[l], 4ull * n, "svo cptr")) || (rc = dalloc(ctx, lvl_flag[l], 4ull * n, "svo flags")) ||
+            (rc = dalloc(ctx, lvl_rank[l], 4ull * n, "svo ranks")))
+            return rc;
+        if (n) hipLaunchKernelGGL(svo_morton_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, lvl_coord[l].as<uint32_t>(), n, raw.as<uint32_t>());
+        TGX_LAUNCH_CHECK(ctx, "svo_morton_k");
+        if ((rc = tgx_prim_sort_pairs_u32(ctx, raw.as<uint32_t>(), lvl_keys[l].as<uint32_t>(), lvl_cptr[l].as<uint32_t>(),
+                                          lvl_scptr[l].as<uint32_t>(), n, 3u * l)))
+            return rc;
+        if (n) hipLaunchKernelGGL(svo_unique_flags_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, lvl_keys[l].as<uint32_t>(), n, lvl_flag[l].as<uint32_t>());
+        TGX_LAUNCH_CHECK(ctx, "svo_unique_flags_k");
+        if ((rc = tgx_prim_exclusive_scan_u32(ctx, lvl_flag[l].as<uint32_t>(), lvl_rank[l].as<uint32_t>(), n))) return rc;
+        uint32_t cnt = 0;
+        if ((rc = scan_total(ctx, lvl_flag[l].as<uint32_t>(), lvl_rank[l].as<uint32_t>(), n, &cnt))) return rc;
+        lvl_nodes[l] = cnt;
+        lvl_base[l] = total_nodes;
+        total_nodes += cnt;
+        if ((rc = dalloc(ctx, lvl_node_keys[l], 4ull * cnt, "svo node keys"))) return rc;
+        if (n) hipLaunchKernelGGL(svo_node_keys_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, lvl_keys[l].as<uint32_t>(),
+                                  lvl_flag[l].as<uint32_t>(), lvl_rank[l].as<uint32_t>(), n, lvl_node_keys[l].as<uint32_t>());
+        TGX_LAUNCH_CHECK(ctx, "svo_node_keys_k");
+    }
+    const uint32_t n_leaves = lvl_nodes[leaf_level];
+
+    /* Nodes: root, then each level linked to its parents. */
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_nodes, sizeof(svo_gnode) * (size_t)total_nodes, "svo nodes"))) return rc;
+    svo_gnode* nodes = (svo_gnode*)ctx->svo_nodes.ptr;
+    {
+        const svo_gnode root = {SVO_NONE, SVO_NONE, SVO_NONE, 0u, 0u, 0u, 0u, 0u};
+        TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy((char*)ctx->host_pinned + 256, &root, sizeof(root));
+        TGX_HIP_CHECK(ctx, hipMemcpyAsync(nodes, (char*)ctx->host_pinned + 256, sizeof(root), hipMemcpyHostToDevice, ctx->stream));
+    }
+    dbuf root_key;
+    if ((rc = dalloc(ctx, root_key, 4, "svo root key"))) return rc;
+    hipLaunchKernelGGL(svo_fill_k, dim3(1), dim3(64), 0, ctx->stream, root_key.as<uint32_t>(), 1ull, 0u);
+    std::vector<dbuf> lvl_first(leaf_level + 1);
+    for (uint32_t l = 1; l <= leaf_level; l++) {
+        const uint32_t n = lvl_n[l];
+        if ((rc = dalloc(ctx, lvl_first[l], 4ull * (lvl_nodes[l] + 1), "svo first pair"))) return rc;
+        const uint32_t* pkeys = l == 1 ? root_key.as<uint32_t>() : lvl_node_keys[l - 1].as<uint32_t>();
+        if (n) {
+            tgx_launch_scope s(ctx, "svo_nodes_k");
+            hipLaunchKernelGGL(svo_nodes_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, lvl_keys[l].as<uint32_t>(),
+                               lvl_flag[l].as<uint32_t>(), lvl_rank[l].as<uint32_t>(), n, l, lvl_base[l],
+                               l == leaf_level ? 1u : 0u, pkeys, lvl_nodes[l - 1], lvl_base[l - 1], nodes,
+                               lvl_first[l].as<uint32_t>());
+        }
+        TGX_LAUNCH_CHECK(ctx, "svo_nodes_k");
+    }
+
+    /* Leaves. */
+    const size_t nl = n_leaves ? n_leaves : 1;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_bits, 4ull * TGX_SVO_BLOCK_WORDS * nl, "svo leaf bits")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_mats, 2ull * TGX_SVO_BLOCK_VOXELS * nl, "svo leaf materials")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_n, 4ull * nl, "svo leaf n")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_list_off, 4ull * (nl + 1), "svo list offsets")))
+        return rc;
+    const uint32_t n_leaf_pairs = lvl_n[leaf_level];
+    dbuf list_tmp;
+    if ((rc = dalloc(ctx, list_tmp, 4ull * (n_leaf_pairs ? n_leaf_pairs : 1), "svo list"))) return rc;
+    {
+        const uint64_t words = (uint64_t)TGX_SVO_BLOCK_VOXELS * nl / 2u;
+        hipLaunchKernelGGL(svo_fill_k, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (uint32_t*)ctx->svo_leaf_mats.ptr, words, 0u);
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_fill_k");
+    if (n_leaves) {
+        tgx_launch_scope s(ctx, "svo_leaf_k");
+        hipLaunchKernelGGL(svo_leaf_k, dim3(n_leaves), dim3(256), 0, ctx->stream, nodes, lvl_base[leaf_level],
+                           lvl_scptr[leaf_level].as<uint32_t>(), lvl_first[leaf_level].as<uint32_t>(), n_leaf_pairs,
+                           n_leaves, frames, (const uint32_t*)ctx->bits.ptr, (const uint8_t*)ctx->materials.ptr,
+                           mn[0], mn[1], mn[2], (flags & TGX_SVO_FLAG_REFERENCE_CORNER_QUIRK) ? 1u : 0u,
+                           (uint32_t*)ctx->svo_leaf_bits.ptr, (uint16_t*)ctx->svo_leaf_mats.ptr,
+                           (uint32_t*)ctx->svo_leaf_n.ptr, list_tmp.as<uint32_t>(), nodes);
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_leaf_k");
+    if ((rc = tgx_prim_exclusive_scan_u32(ctx, (const uint32_t*)ctx->svo_leaf_n.ptr, (uint32_t*)ctx->svo_list_off.ptr, n_leaves))) return rc;
+    uint32_t n_list = 0;
+    if ((rc = scan_total(ctx, (const uint32_t*)ctx->svo_leaf_n.ptr, (const uint32_t*)ctx->svo_list_off.ptr, n_leaves, &n_list))) return rc;
+    *(uint32_t*)((char*)ctx->host_pinned + 320) = n_list;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync((uint32_t*)ctx->svo_list_off.ptr + n_leaves, (char*)ctx->host_pinned + 320, 4,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_list, 4ull * (n_list ? n_list : 1), "svo list"))) return rc;
+    if (n_leaves)
+        hipLaunchKernelGGL(svo_list_compact_k, dim3(blocks_for(n_leaves)), dim3(256), 0, ctx->stream,
+                           lvl_first[leaf_level].as<uint32_t>(), (const uint32_t*)ctx->svo_leaf_n.ptr,
+                           (const uint32_t*)ctx->svo_list_off.ptr, n_leaves, list_tmp.as<uint32_t>(),
+                           (uint32_t*)ctx->svo_list.ptr);
+    TGX_LAUNCH_CHECK(ctx, "svo_list_compact_k");
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+
+    ctx->svo_info.min[0] = mn[0]; ctx->svo_info.min[1] = mn[1]; ctx->svo_info.min[2] = mn[2];
+    ctx->svo_info.depth = depth;
+    ctx->svo_info.n_nodes = total_nodes;
+    ctx->svo_info.n_leaves = n_leaves;
+    ctx->svo_info.n_list_entries = n_list;
+    ctx->svo_info.flags = flags;
+    ctx->svo_built = true;
+    return TGX_OK;
+}
+
+int tgx_svo_get_info(tgx_context* ctx, tgx_svo_info* out)
+{
+    if (!ctx || !out) return TGX_ERR_INVALID_ARG;
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    *out = ctx->svo_info;
+    return TGX_OK;
+}
+
+int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes_out, uint32_t* leaf_bits, uint16_t* leaf_materials,
+                   uint32_t* leaf_n, uint32_t* list_offsets, uint32_t* list)
+{
+    if (!ctx) return TGX_ERR_INVALID_ARG;
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    const tgx_svo_info& in = ctx->svo_info;
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (nodes_out) {
+        std::vector<svo_gnode> g(in.n_nodes);
+        TGX_HIP_CHECK(ctx, hipMemcpy(g.data(), ctx->svo_nodes.ptr, sizeof(svo_gnode) * in.n_nodes, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < in.n_nodes; i++) {
+            tgx_svo_node& o = nodes_out[i];
+            o.level = (g[i].masks >> 16) & 0xFFu;
+            o.coord[0] = g[i].cx; o.coord[1] = g[i].cy; o.coord[2] = g[i].cz;
+            o.valid_mask = g[i].masks & 0xFFu;
+            o.leaf_mask = (g[i].masks >> 8) & 0xFFu;
+            o.first_child = g[i].first_child;
+            o.leaf_idx = g[i].leaf_idx;
+        }
+    }
+    if (leaf_bits && in.n_leaves)
+        TGX_HIP_CHECK(ctx, hipMemcpy(leaf_bits, ctx->svo_leaf_bits.ptr, 4ull * TGX_SVO_BLOCK_WORDS * in.n_leaves, hipMemcpyDeviceToHost));
+    if (leaf_materials && in.n_leaves)
+        TGX_HIP_CHECK(ctx, hipMemcpy(leaf_materials, ctx->svo_leaf_mats.ptr, 2ull * TGX_SVO_BLOCK_VOXELS * in.n_leaves, hipMemcpyDeviceToHost));
+    if (leaf_n && in.n_leaves)
+        TGX_HIP_CHECK(ctx, hipMemcpy(leaf_n, ctx->svo_leaf_n.ptr, 4ull * in.n_leaves, hipMemcpyDeviceToHost));
+    if (list_offsets)
+        TGX_HIP_CHECK(ctx, hipMemcpy(list_offsets, ctx->svo_list_off.ptr, 4ull * (in.n_leaves + 1), hipMemcpyDeviceToHost));
+    if (list && in.n_list_entries)
+        TGX_HIP_CHECK(ctx, hipMemcpy(list, ctx->svo_list.ptr, 4ull * in.n_list_entries, hipMemcpyDeviceToHost));
+    return TGX_OK;
+}
+
+int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n, float max_distance,
+                  tgx_svo_hit* out_host)
+{
+    if (!ctx || (n && (!origins || !directions || !out_host))) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (n == 0) return TGX_OK;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, 24ull * n, "svo rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_hits, sizeof(tgx_svo_hit) * (size_t)n, "svo hits")))
+        return rc;
+    float* d_o = (float*)ctx->svo_rays.ptr;
+    float* d_d = d_o + 3ull * n;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_o, origins, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_d, directions, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    {
+        tgx_launch_scope s(ctx, "svo_trace_k");
+        hipLaunchKernelGGL(svo_trace_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (const svo_gnode*)ctx->svo_nodes.ptr,
+                           (const uint32_t*)ctx->svo_leaf_bits.ptr, ctx->svo_info.min[0], ctx->svo_info.min[1],
+                           ctx->svo_info.min[2], ctx->svo_info.depth, d_o, d_d, n, max_distance,
+                           (tgx_svo_hit*)ctx->svo_hits.ptr);
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_trace_k");
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, ctx->svo_hits.ptr, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
+} /* extern "C" */
