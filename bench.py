@@ -1,15 +1,21 @@
-"""Benchmark: primary visibility at 1920x1080 over 512 noise-filled objects.
+"""Benchmark: primary visibility + one-bounce SVO GI at 1920x1080 over 512
+noise-filled objects (the metric's workload).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload frame|visibility]
 
-One step = one frame of the hot path on one GPU: cull/bin of every cluster
-pointer, tile lists, and the per-tile 8^3 DDA producing the bit-exact 64-bit
-visibility buffer (tg's visibility.frag). Inputs (scene arrays) are resident
-in HBM before the timed region; the output stays in HBM.
+One step = one frame of the hot path on one GPU:
+  - primary visibility: cull/bin of every cluster pointer, tile lists and
+    the per-tile 8^3 DDA producing the bit-exact 64-bit visibility buffer
+    (tg's visibility.frag);
+  - one-bounce GI through the depth-11 SVO built around the camera: 4
+    cosine-weighted bounce rays per visible pixel and one shadow ray per
+    sun-facing bounce hit.
+The scene arrays and the SVO are resident in HBM before the timed region
+(the SVO build is timed separately); outputs stay in HBM.
 
-Prints one JSON line with rays/s (= primary rays = pixels), the roofline of
-the dominant kernel (vis_traverse, HIP events on the library's stream) and
-the CPU oracle timed on this host as the baseline.
+Prints one JSON line: rays/s (primary + bounce + shadow rays per frame over
+the frame time), the roofline of the dominant kernel (HIP events on the
+library's stream) and the CPU oracle timed on this host as the baseline.
 """
 from __future__ import annotations
 
@@ -28,52 +34,71 @@ import numpy as np  # noqa: E402
 import tg_amd as T  # noqa: E402
 
 W, H = 1920, 1080
+SPP = 4
+DEPTH = 11
 HBM_PEAK_GBS = 8000.0
-DOMINANT = "vis_traverse"
 
 
-def algorithmic_bytes(stats: T.VisibilityStats, n_objects: int, n_pixels: int) -> int:
-    """Per frame: 8 B written per pixel; per cluster that survives the cull
-    its 64 B of solid bits plus its 4 B pointer-map and 4 B cluster->object
+def vis_alg_bytes(stats, n_objects: int, n_pixels: int) -> int:
+    """Visibility frame: 8 B written per pixel; per cluster surviving the
+    cull its 64 B of solid bits plus 4 B pointer-map and 4 B cluster->object
     entries; 96 B per object record."""
     return 8 * n_pixels + 72 * int(stats.clusters_binned) + 96 * n_objects
 
 
-def cpu_baseline(scene: T.Scene, params, budget_s: float = 12.0) -> dict:
+def gi_alg_bytes(gst, info, n_pixels: int) -> int:
+    """GI frame: 8 B of visibility read per pixel and 24 B of radiance and
+    irradiance written; per visible pixel one 64-B line each of object
+    record, cluster maps, material and LUT; the SVO's nodes (32 B) and leaf
+    bit blocks (4 KiB) once; one 64-B material line per bounce hit."""
+    return (32 * n_pixels + 256 * int(gst["visible_pixels"]) + 32 * int(info["n_nodes"]) +
+            4096 * int(info["n_leaves"]) + 64 * int(gst["bounce_hits"]))
+
+
+def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 12.0) -> dict:
     import oracle  # the CPU baseline leg is the only bench code that uses the oracle
 
     oscene = oracle.OracleScene(scene)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     threads = min(threads, 16)
-    res = {}
-    for label, n_thr, rows in (("all", threads, 1), ("one", 1, 8)):
+    if workload == "visibility":
         t0 = time.perf_counter()
-        rays = 0
-        ci = 0
-        while True:
-            cam = T.generate_camera(params, ci, W, H)
-            oracle.visibility_culled(oscene, cam, W, H, row_step=rows, n_threads=n_thr)
-            rays += W * ((H + rows - 1) // rows)
+        rays, ci = 0, 0
+        while time.perf_counter() - t0 < budget_s:
+            c = T.generate_camera(params, ci, W, H)
+            oracle.visibility_culled(oscene, c, W, H, n_threads=threads)
+            rays += W * H
             ci += 1
-            if time.perf_counter() - t0 > budget_s / 2:
-                break
-        res[label] = (rays / (time.perf_counter() - t0), ci, rows)
-    all_rate, n_frames, _ = res["all"]
-    one_rate, n1, rows1 = res["one"]
-    return {
-        "value": all_rate, "unit": "rays/s", "cores": threads, "kind": "port",
-        "sample": f"oracle culled visibility, {n_frames} full 1920x1080 frames (seeded cameras) on {threads} threads; "
-                  f"single-core rate {one_rate:.4g} rays/s over every {rows1}th row of {n1} frames",
-        "single_core_value": one_rate,
-    }
+        rate = rays / (time.perf_counter() - t0)
+        return {"value": rate, "unit": "rays/s", "cores": threads, "kind": "port",
+                "sample": f"oracle culled visibility, {ci} full 1920x1080 frames on {threads} threads"}
+    # Frame workload: visibility of the whole frame, GI on every k-th row.
+    t0 = time.perf_counter()
+    svo = oracle.OracleSvo(oscene, mn, DEPTH)
+    t_build = time.perf_counter() - t0
+    row_step = 8
+    t0 = time.perf_counter()
+    vis, _ = oracle.visibility_culled(oscene, cam, W, H, n_threads=threads)
+    t_vis = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, cnt = svo.gi(cam, W, H, vis, prm, row_step=row_step)
+    t_gi = time.perf_counter() - t0
+    gi_rays = int(cnt[1] + cnt[3])
+    rays = W * H + gi_rays * row_step
+    frame_s = t_vis + t_gi * row_step
+    return {"value": rays / frame_s, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle visibility of one full 1920x1080 frame ({t_vis:.2f} s) plus oracle GI of every "
+                      f"{row_step}th row ({gi_rays} bounce+shadow rays, {t_gi:.2f} s, scaled x{row_step}) on "
+                      f"{threads} threads; oracle SVO build {t_build:.2f} s"}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", choices=("frame", "visibility"), default="frame")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -84,12 +109,26 @@ def main() -> int:
     params = T.noise_objects_params(seed=args.seed)
     scene = T.generate_scene(params)
     cam = T.generate_camera(params, 0, W, H)
+    mn = np.floor(cam[0]["camera"][:3]) - float(1 << (DEPTH - 1))
+    prm = T.gi_params(T.seeded_sun(args.seed), spp=SPP, max_distance=1024.0)
     ctx = T.Context(0)
     ctx.upload(scene)
-    out = ctx.alloc(W * H * 8)
+    t0 = time.perf_counter()
+    ctx.svo_build(mn, DEPTH)
+    svo_build_ms = 1e3 * (time.perf_counter() - t0)
+    info = ctx.svo_info()
+    vis_d = ctx.alloc(W * H * 8)
+    rad_d = ctx.alloc(W * H * 12)
+    irr_d = ctx.alloc(W * H * 12)
+
+    def step():
+        if args.workload == "frame":
+            ctx.frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+        else:
+            ctx.visibility_device(cam, W, H, vis_d)
 
     for _ in range(args.warmup):
-        ctx.visibility_device(cam, W, H, out)
+        step()
     ctx.synchronize()
 
     ctx.profile(True)
@@ -97,23 +136,34 @@ def main() -> int:
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.visibility_device(cam, W, H, out)
+        step()
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     prof = ctx.profile_stats()
     ctx.profile(False)
-    stats = ctx.last_stats()
+    vstats = ctx.last_stats()
 
     ms_per_step = 1e3 * elapsed / args.steps
-    rays_per_s = W * H * args.steps / elapsed
-    launches, total_ms = prof[DOMINANT]
+    primary = W * H
+    kernels_ms = {k: v[1] / args.steps for k, v in prof.items()}
+    if args.workload == "frame":
+        gst = ctx.gi_stats()
+        rays_per_frame = primary + int(gst["bounce_rays"]) + int(gst["shadow_rays"])
+        dominant = max(("gi_pixel_k", "vis_traverse"), key=lambda k: kernels_ms.get(k, 0.0))
+        alg = gi_alg_bytes(gst, info, primary) if dominant == "gi_pixel_k" else vis_alg_bytes(vstats, scene.objects.size, primary)
+    else:
+        gst = None
+        rays_per_frame = primary
+        dominant = "vis_traverse"
+        alg = vis_alg_bytes(vstats, scene.objects.size, primary)
+    launches, total_ms = prof[dominant]
     avg_ms = total_ms / launches
-    b_alg = algorithmic_bytes(stats, scene.objects.size, W * H)
-    achieved = b_alg / (avg_ms * 1e-3) / 1e9
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    vis_ms = sum(v for k, v in kernels_ms.items() if k.startswith("vis_"))
 
     line = {
         "metric": "rays/s (primary DDA + SVO bounce, 1080p); neural vs plain GI at <=2% rel-MSE",
-        "value": rays_per_s,
+        "value": rays_per_frame * args.steps / elapsed,
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -125,23 +175,34 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "primary visibility, 512 noise-filled objects of 1-8 clusters per axis in a 512^3 world, "
-                        "1920x1080, camera at the world centre (seeded yaw/pitch), bit-exact 64-bit visibility",
+            "workload": ("primary visibility + one-bounce SVO GI" if args.workload == "frame" else "primary visibility")
+                        + ", 512 noise-filled objects of 1-8 clusters per axis in a 512^3 world, 1920x1080, camera at "
+                          "the world centre (seeded yaw/pitch), bit-exact 64-bit visibility"
+                        + (f", depth-{DEPTH} SVO around the camera, {SPP} spp cosine bounce rays, range 1024 voxels, "
+                           "seeded sun, one shadow ray per sun-facing bounce hit" if args.workload == "frame" else ""),
             "objects": int(scene.objects.size), "clusters": scene.n_clusters, "width": W, "height": H,
-            "cameras": 1, "seed": args.seed, "rays_counted": "primary (one per pixel)",
+            "cameras": 1, "seed": args.seed, "spp": SPP if args.workload == "frame" else 0,
+            "rays_per_frame": rays_per_frame,
+            "rays_counted": "primary (one per pixel) + bounce + shadow" if args.workload == "frame" else "primary",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": DOMINANT, "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": b_alg,
+            "kernel": dominant, "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": alg,
         },
-        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in prof.items()},
-        "cull": {"clusters_binned": int(stats.clusters_binned), "tile_entries": int(stats.tile_entries)},
+        "kernels_ms_per_step": kernels_ms,
+        "primary_visibility": {"ms": vis_ms, "rays_per_s": primary / (vis_ms * 1e-3) if vis_ms else None,
+                               "clusters_binned": int(vstats.clusters_binned),
+                               "tile_entries": int(vstats.tile_entries)},
+        "svo": {"build_ms": svo_build_ms, "nodes": int(info["n_nodes"]), "leaves": int(info["n_leaves"])},
     }
-    ctx.free(out)
+    if gst is not None:
+        line["gi"] = {k: int(gst[k]) for k in gst.dtype.names}
+    for p in (vis_d, rad_d, irr_d):
+        ctx.free(p)
     ctx.close()
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(scene, params)
+        line["cpu_baseline"] = cpu_baseline(scene, params, cam, mn, prm, args.workload)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -261,6 +261,65 @@ int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* direction
                   float max_distance, tgx_svo_hit* out_host);
 
 /* ------------------------------------------------------------------------ */
+/* One-bounce global illumination through the SVO                           */
+/* ------------------------------------------------------------------------ */
+
+/* tg has no GI estimator: shading.frag binds the SVO but never traces it
+ * (shading.frag:40-43) and the secondary-ray code is a disabled CPU debug
+ * block (tgvk_raytracer.c:1390-1434). This estimator is defined here and
+ * computed identically by the kernels and the CPU oracle:
+ *   - primary surface: decode the visibility word, re-intersect the hit
+ *     voxel in cluster space, major-axis face normal rotated to world space
+ *     and the world hit point (shading.frag:116-229);
+ *   - spp bounce rays per visible pixel, cosine-weighted about the normal:
+ *     xorshift32 (util.inc:11-25) cube rejection to a unit vector s, then
+ *     d = normalize(n + s), started sqrt(3) voxels along the normal n
+ *     (tg's debug bounce starts sqrt(3) along d, tgvk_raytracer.c:1419,
+ *     which lets grazing rays start behind an adjacent wall), traced
+ *     through the SVO up to max_distance; a hit at negative distance (the
+ *     start lies in solid) counts as occluded;
+ *   - at a bounce hit facing the sun: LUT albedo of the voxel's material
+ *     and one shadow ray towards the sun, started 1/64 voxel off the hit
+ *     face, same range;
+ *   - irradiance E = (1/spp) sum_k albedo_k * max(0, n_k . sun) * radiance / pi
+ *     over lit bounce hits; radiance L = albedo_p * E (per RGB channel).
+ * RNG state of (pixel i, sample s): murmur3_fmix32 (util.inc:47-56) of
+ * i * spp + s + 0x9E3779B9 * (frame + 1), with 0 replaced by 0x6D2B79F5. */
+typedef struct tgx_gi_params {
+    float    sun_dir[3];      /* unit vector towards the sun                */
+    float    sun_radiance;
+    uint32_t spp;             /* bounce rays per visible pixel (1..64)      */
+    float    max_distance;    /* range of bounce and shadow rays (voxels)   */
+    uint32_t frame;           /* decorrelates the random streams            */
+    uint32_t pad;
+} tgx_gi_params;
+
+/* Ray counts of the last GI call. */
+typedef struct tgx_gi_stats {
+    uint64_t visible_pixels;
+    uint64_t bounce_rays;
+    uint64_t bounce_hits;
+    uint64_t shadow_rays;
+} tgx_gi_stats;
+
+/* Computes per-pixel RGB radiance (and optionally irradiance) of one camera
+ * from its visibility buffer vis_dev (w * h words, as written by
+ * tgx_visibility_device) with the SVO of tgx_svo_build. Outputs are device
+ * buffers of 3 * w * h floats, row-major RGB; pixels without a hit get 0.
+ * Returns without waiting for the GPU. */
+int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h,
+                  const uint64_t* vis_dev, const tgx_gi_params* params,
+                  float* radiance_dev, float* irradiance_dev);
+/* Synchronises and returns the ray counts of the last tgx_gi_device call. */
+int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out);
+
+/* One frame of the hot path: primary visibility of one camera into vis_dev
+ * followed by tgx_gi_device into radiance_dev. */
+int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h,
+                     const tgx_gi_params* params, uint64_t* vis_dev, float* radiance_dev,
+                     float* irradiance_dev);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and profiling helpers (for callers without their own       */
 /* allocator; the reference's equivalents are tgvk_memory.c:182-240)        */
 /* ------------------------------------------------------------------------ */

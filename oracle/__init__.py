@@ -57,6 +57,10 @@ def load() -> ctypes.CDLL:
         "tgxo_svo_trace": (None, [vp, vp, vp, u32, ctypes.c_float, vp]),
         "tgxo_aabb_obb_ignore_contact": (i32, [vp, vp, vp]),
         "tgxo_svo_node_key": (u64, [u32, vp]),
+        "tgxo_gi": (None, [vp, vp, u32, u32, vp, vp, u32, vp, vp, vp]),
+        "tgxo_gi_surface": (i32, [vp, vp, u32, u32, u32, u32, u64, vp, vp, vp]),
+        "tgxo_gi_seed": (u32, [u32, u32, u32, u32]),
+        "tgxo_gi_direction": (None, [vp, u32, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -199,6 +203,15 @@ class OracleSvo:
         load().tgxo_svo_export(self.h, _p(e.nodes), _p(e.bits), _p(e.materials), _p(e.leaf_n),
                                _p(e.list_offsets), _p(e.list))
         return e
+
+    def gi(self, cam: np.ndarray, w: int, h: int, vis: np.ndarray, params: np.ndarray, row_step: int = 1):
+        """(radiance[h, w, 3], irradiance[h, w, 3], counts[4]) of one camera."""
+        vis = np.ascontiguousarray(vis, np.uint64)
+        rad = np.zeros((h, w, 3), np.float32)
+        irr = np.zeros((h, w, 3), np.float32)
+        cnt = np.zeros(4, np.uint64)
+        load().tgxo_gi(self.h, _p(cam), w, h, _p(vis), _p(params), row_step, _p(rad), _p(irr), _p(cnt))
+        return rad, irr, cnt
 
     def trace(self, origins: np.ndarray, dirs: np.ndarray, max_distance: float = 3.0e38) -> np.ndarray:
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)

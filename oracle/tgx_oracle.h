@@ -86,6 +86,17 @@ void      tgxo_svo_traverse(const tgxo_svo* svo, const float origin[3], const fl
                             float max_distance, tgx_svo_hit* out);
 void      tgxo_svo_trace(const tgxo_svo* svo, const float* origins, const float* dirs, uint32_t n,
                          float max_distance, tgx_svo_hit* out);
+/* One-bounce GI of one camera from its visibility buffer (estimator of
+ * tgx_gi_params); rows y % row_step != 0 are skipped. counts = visible
+ * pixels, bounce rays, bounce hits, shadow rays. */
+void      tgxo_gi(const tgxo_svo* svo, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
+                  const tgx_gi_params* params, uint32_t row_step, float* radiance, float* irradiance,
+                  uint64_t counts[4]);
+int       tgxo_gi_surface(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h, uint32_t px,
+                          uint32_t py, uint64_t word, float out_p[3], float out_n[3], float out_albedo[3]);
+uint32_t  tgxo_gi_seed(uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t frame);
+/* Cosine-weighted bounce direction about unit n from one RNG seed. */
+void      tgxo_gi_direction(const float n[3], uint32_t seed, float out[3]);
 int       tgxo_aabb_obb_ignore_contact(const float mn[3], const float mx[3], const float corners[24]);
 uint64_t  tgxo_svo_node_key(uint32_t level, const uint32_t coord[3]);
 

@@ -697,3 +697,210 @@ void tgxo_svo_trace(const tgxo_svo* s, const float* origins, const float* dirs, 
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t i = 0; i < (int64_t)n; i++) tgxo_svo_traverse(s, &origins[3 * i], &dirs[3 * i], max_distance, &out[i]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* One-bounce GI (the estimator documented at tgx_gi_params in tgx.h)        */
+/* ------------------------------------------------------------------------ */
+
+#define SQRT3 1.73205080757f
+#define SHADOW_OFFSET 0.015625f
+#define INV_PI 0.318309886f
+
+/* MurmurHash3 finaliser (util.inc:47-56). */
+static uint32_t fmix32(uint32_t v)
+{
+    v ^= v >> 16;
+    v *= 0x85ebca6bu;
+    v ^= v >> 13;
+    v *= 0xc2b2ae35u;
+    v ^= v >> 16;
+    return v;
+}
+
+/* xorshift32 (util.inc:11-19) and its [low, high] float (util.inc:21-31). */
+static uint32_t xorshift32(uint32_t* s)
+{
+    uint32_t r = *s;
+    r ^= r << 13;
+    r ^= r >> 17;
+    r ^= r << 5;
+    *s = r;
+    return r;
+}
+static float xorshift_f32_range(uint32_t* s, float lo, float hi)
+{
+    const float u = (float)xorshift32(s) / (float)0xFFFFFFFFu;
+    return u * (hi - lo) + lo;
+}
+
+static v3 cosine_direction(v3 n, uint32_t* st)
+{
+    for (int attempt = 0; attempt < 32; attempt++) {
+        const float x = xorshift_f32_range(st, -1.0f, 1.0f);
+        const float y = xorshift_f32_range(st, -1.0f, 1.0f);
+        const float z = xorshift_f32_range(st, -1.0f, 1.0f);
+        const float l2 = x * x + y * y + z * z;
+        if (l2 > 1.0f || l2 < 1e-6f) continue;
+        const float l = sqrtf(l2);
+        const float dx = n.x + x / l, dy = n.y + y / l, dz = n.z + z / l;
+        const float dl2 = dx * dx + dy * dy + dz * dz;
+        if (dl2 < 1e-6f) continue;
+        const float dl = sqrtf(dl2);
+        v3 r = {dx / dl, dy / dl, dz / dl};
+        return r;
+    }
+    return n;
+}
+
+void tgxo_gi_direction(const float n[3], uint32_t seed, float out[3])
+{
+    uint32_t st = seed;
+    const v3 nv = {n[0], n[1], n[2]};
+    const v3 d = cosine_direction(nv, &st);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+uint32_t tgxo_gi_seed(uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t frame)
+{
+    const uint32_t s = fmix32(pixel * spp + sample + 0x9E3779B9u * (frame + 1u));
+    return s ? s : 0x6D2B79F5u;
+}
+
+/* Primary surface of a visibility word: world hit point, world normal and
+ * albedo (shading.frag:116-229). Returns 0 if the pixel is not shaded. */
+int tgxo_gi_surface(const tgxo_scene* sc, const tgx_camera* cam, uint32_t w, uint32_t h, uint32_t px, uint32_t py,
+                    uint64_t word, float out_p[3], float out_n[3], float out_albedo[3])
+{
+    const float depth = (float)(word >> 40) / 16777215.0f;
+    if (!(depth < 1.0f)) return 0;
+    const uint32_t cptr = (uint32_t)(word >> 9) & 0x7FFFFFFFu;
+    const uint32_t v = (uint32_t)word & 511u;
+    const uint32_t cidx = sc->cluster_pointers[cptr];
+    const tgx_object* o = &sc->objects[sc->cluster_idx_to_object[cidx]];
+    const uint32_t rel = cptr - o->first_cluster_pointer;
+    const m4 ws2ms = ws2ms_of(o, rel);
+    const v4 cpos = {cam->camera[0], cam->camera[1], cam->camera[2], 1.0f};
+    const v3 orig = m4_mulv4_xyz(ws2ms, cpos);
+    const float fx = ((float)px + 0.5f) / (float)w;
+    const float fy = 1.0f - ((float)py + 0.5f) / (float)h;
+    const float omy = 1.0f - fy, omx = 1.0f - fx;
+    v3 dws;
+    {
+        const float lx = cam->ray_bl[0] * omy + cam->ray_tl[0] * fy, ly = cam->ray_bl[1] * omy + cam->ray_tl[1] * fy,
+                    lz = cam->ray_bl[2] * omy + cam->ray_tl[2] * fy;
+        const float rx = cam->ray_br[0] * omy + cam->ray_tr[0] * fy, ry = cam->ray_br[1] * omy + cam->ray_tr[1] * fy,
+                    rz = cam->ray_br[2] * omy + cam->ray_tr[2] * fy;
+        dws.x = lx * omx + rx * fx; dws.y = ly * omx + ry * fx; dws.z = lz * omx + rz * fx;
+    }
+    const v4 dv = {dws.x, dws.y, dws.z, 0.0f};
+    const v3 dms = v3_normalized(m4_mulv4_xyz(ws2ms, dv));
+    const v3 vmn = {(float)(v % 8u), (float)((v / 8u) % 8u), (float)(v / 64u)};
+    const v3 vmx = {vmn.x + 1.0f, vmn.y + 1.0f, vmn.z + 1.0f};
+    float enter, exit_;
+    if (!intersect_ray_aabb(orig, dms, vmn, vmx, &enter, &exit_)) return 0;
+    const v3 hit = enter > 0.0f ? v3_add(orig, v3_mulf(dms, enter)) : orig;
+    const v3 vc = {vmn.x + 0.5f, vmn.y + 0.5f, vmn.z + 0.5f};
+    v3 n = v3_sub(hit, vc);
+    const float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+#define SIGNF(a) ((a) > 0.0f ? 1.0f : ((a) < 0.0f ? -1.0f : 0.0f))
+    if (ax > ay) {
+        n.y = 0.0f;
+        if (ax > az) { n.x = SIGNF(n.x); n.z = 0.0f; } else { n.z = SIGNF(n.z); n.x = 0.0f; }
+    } else {
+        n.x = 0.0f;
+        if (ay > az) { n.y = SIGNF(n.y); n.z = 0.0f; } else { n.z = SIGNF(n.z); n.y = 0.0f; }
+    }
+#undef SIGNF
+    m4 rot;
+    memcpy(rot.m, o->rotation, sizeof(rot.m));
+    const v4 nv = {n.x, n.y, n.z, 0.0f};
+    const v3 nws = v3_normalized(m4_mulv4_xyz(rot, nv));
+    const m4 cs2ws = ms2ws_of(o, rel);
+    const v4 hv = {hit.x, hit.y, hit.z, 1.0f};
+    const v3 pws = m4_mulv4_xyz(cs2ws, hv);
+    const uint32_t mat = sc->materials ? sc->materials[(size_t)cidx * 512u + v] : 0u;
+    const tgx_lut_entry* e = &sc->luts[(size_t)o->lut_idx * TGX_LUT_ENTRIES + mat];
+    out_p[0] = pws.x; out_p[1] = pws.y; out_p[2] = pws.z;
+    out_n[0] = nws.x; out_n[1] = nws.y; out_n[2] = nws.z;
+    out_albedo[0] = e->albedo[0]; out_albedo[1] = e->albedo[1]; out_albedo[2] = e->albedo[2];
+    return 1;
+}
+
+/* Leaf index of the hit's block (binary search over the canonical order
+ * is not needed: the traversal below records it). */
+static void traverse_leaf(const tgxo_svo* s, const float o[3], const float d[3], float maxd, tgx_svo_hit* h,
+                          uint32_t* leaf)
+{
+    tgxo_svo_traverse(s, o, d, maxd, h);
+    *leaf = 0xFFFFFFFFu;
+    if (!h->hit) return;
+    /* locate the leaf node of the hit block by descending from the root */
+    uint32_t ni = 0;
+    const uint32_t leaf_level = s->info.depth - 5u;
+    for (uint32_t l = 0; l < leaf_level; l++) {
+        const uint32_t sh = leaf_level - l - 1u;
+        const uint32_t c = ((h->block[0] >> sh) & 1u) | (((h->block[1] >> sh) & 1u) << 1) | (((h->block[2] >> sh) & 1u) << 2);
+        const onode* n = &s->nodes[ni];
+        ni = n->first_child + (uint32_t)__builtin_popcount(n->valid & ((1u << c) - 1u));
+    }
+    *leaf = s->nodes[ni].leaf_idx;
+}
+
+void tgxo_gi(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
+             const tgx_gi_params* prm, uint32_t row_step, float* radiance, float* irradiance, uint64_t counts[4])
+{
+    const tgxo_scene* sc = s->scene;
+    const v3 sun = {prm->sun_dir[0], prm->sun_dir[1], prm->sun_dir[2]};
+    uint64_t n_vis = 0, n_b = 0, n_bh = 0, n_sh = 0;
+    if (row_step == 0) row_step = 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_vis, n_b, n_bh, n_sh)
+    for (int64_t yy = 0; yy < (int64_t)h; yy += row_step) {
+        const uint32_t y = (uint32_t)yy;
+        for (uint32_t x = 0; x < w; x++) {
+            const size_t pix = (size_t)y * w + x;
+            float p[3], nrm[3], alb[3];
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            radiance[3 * pix] = radiance[3 * pix + 1] = radiance[3 * pix + 2] = 0.0f;
+            if (irradiance) irradiance[3 * pix] = irradiance[3 * pix + 1] = irradiance[3 * pix + 2] = 0.0f;
+            if (!tgxo_gi_surface(sc, cam, w, h, x, y, vis[pix], p, nrm, alb)) continue;
+            n_vis++;
+            const v3 n = {nrm[0], nrm[1], nrm[2]};
+            for (uint32_t k = 0; k < prm->spp; k++) {
+                uint32_t st = tgxo_gi_seed((uint32_t)pix, k, prm->spp, prm->frame);
+                const v3 d = cosine_direction(n, &st);
+                const float ob[3] = {p[0] + SQRT3 * n.x, p[1] + SQRT3 * n.y, p[2] + SQRT3 * n.z};
+                const float dd[3] = {d.x, d.y, d.z};
+                tgx_svo_hit hb;
+                uint32_t leaf;
+                traverse_leaf(s, ob, dd, prm->max_distance, &hb, &leaf);
+                n_b++;
+                if (!hb.hit) continue;
+                n_bh++;
+                if (hb.distance < 0.0f) continue; /* started inside solid: occluded */
+                int ax = hb.normal[0] != 0 ? 0 : (hb.normal[1] != 0 ? 1 : 2);
+                const float cosv = (float)hb.normal[ax] * prm->sun_dir[ax];
+                if (!(cosv > 0.0f)) continue;
+                float os[3] = {ob[0] + d.x * hb.distance, ob[1] + d.y * hb.distance, ob[2] + d.z * hb.distance};
+                os[ax] = os[ax] + (float)hb.normal[ax] * SHADOW_OFFSET;
+                const float sd[3] = {sun.x, sun.y, sun.z};
+                tgx_svo_hit hs;
+                tgxo_svo_traverse(s, os, sd, prm->max_distance, &hs);
+                n_sh++;
+                if (hs.hit) continue;
+                const uint16_t m = s->mats[(size_t)leaf * TGX_SVO_BLOCK_VOXELS + hb.voxel];
+                const tgx_lut_entry* e = &sc->luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
+                const float wgt = cosv * prm->sun_radiance * INV_PI;
+                acc[0] = acc[0] + e->albedo[0] * wgt;
+                acc[1] = acc[1] + e->albedo[1] * wgt;
+                acc[2] = acc[2] + e->albedo[2] * wgt;
+            }
+            const float inv = 1.0f / (float)prm->spp;
+            const float ex = acc[0] * inv, ey = acc[1] * inv, ez = acc[2] * inv;
+            radiance[3 * pix] = alb[0] * ex;
+            radiance[3 * pix + 1] = alb[1] * ey;
+            radiance[3 * pix + 2] = alb[2] * ez;
+            if (irradiance) { irradiance[3 * pix] = ex; irradiance[3 * pix + 1] = ey; irradiance[3 * pix + 2] = ez; }
+        }
+    }
+    counts[0] = n_vis; counts[1] = n_b; counts[2] = n_bh; counts[3] = n_sh;
+}

@@ -45,6 +45,29 @@ SVO_NODE_DTYPE = np.dtype([("level", "<u4"), ("coord", "<u4", (3,)), ("valid_mas
 SVO_HIT_DTYPE = np.dtype([("hit", "<u4"), ("distance", "<f4"), ("block", "<u4", (3,)), ("voxel", "<u4"),
                           ("normal", "<i4", (3,)), ("steps", "<u4")])
 SVO_FLAG_REFERENCE_CORNER_QUIRK = 1
+GI_PARAMS_DTYPE = np.dtype([("sun_dir", "<f4", (3,)), ("sun_radiance", "<f4"), ("spp", "<u4"),
+                            ("max_distance", "<f4"), ("frame", "<u4"), ("pad", "<u4")])
+GI_STATS_DTYPE = np.dtype([("visible_pixels", "<u8"), ("bounce_rays", "<u8"), ("bounce_hits", "<u8"),
+                           ("shadow_rays", "<u8")])
+
+
+def gi_params(sun_dir=(0.3, 0.8, 0.2), sun_radiance: float = 1.0, spp: int = 4, max_distance: float = 1024.0,
+              frame: int = 0) -> np.ndarray:
+    p = np.zeros(1, GI_PARAMS_DTYPE)
+    d = np.asarray(sun_dir, np.float64)
+    p["sun_dir"] = (d / np.linalg.norm(d)).astype(np.float32)
+    p["sun_radiance"] = sun_radiance
+    p["spp"] = spp
+    p["max_distance"] = max_distance
+    p["frame"] = frame
+    return p
+
+
+def seeded_sun(seed: int) -> np.ndarray:
+    """Fixed seeded sun direction: azimuth U[0, 2pi), elevation U[25, 65] deg."""
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 0x5E1]))
+    az, el = rng.uniform(0, 2 * np.pi), np.radians(rng.uniform(25, 65))
+    return np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
 
 VISIBILITY_EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -132,6 +155,9 @@ def load_library() -> ctypes.CDLL:
         "tgx_svo_get_info": (i32, [vp, vp]),
         "tgx_svo_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "tgx_svo_trace": (i32, [vp, vp, vp, u32, ctypes.c_float, vp]),
+        "tgx_gi_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
+        "tgx_gi_last_stats": (i32, [vp, vp]),
+        "tgx_frame_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -346,6 +372,43 @@ class Context:
         self._check(self.lib.tgx_svo_trace(self.h, _ptr(o), _ptr(d), o.shape[0], max_distance, _ptr(out)),
                     "tgx_svo_trace")
         return out
+
+    # -- one-bounce GI -------------------------------------------------------
+
+    def gi_device(self, cam: np.ndarray, w: int, h: int, vis_dev: int, params: np.ndarray, radiance_dev: int,
+                  irradiance_dev: int | None = None) -> None:
+        cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
+        self._check(self.lib.tgx_gi_device(self.h, _ptr(cam), w, h, vis_dev, _ptr(params), radiance_dev,
+                                           irradiance_dev), "tgx_gi_device")
+
+    def frame_device(self, cam: np.ndarray, w: int, h: int, params: np.ndarray, vis_dev: int, radiance_dev: int,
+                     irradiance_dev: int | None = None) -> None:
+        cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
+        self._check(self.lib.tgx_frame_device(self.h, _ptr(cam), w, h, _ptr(params), vis_dev, radiance_dev,
+                                              irradiance_dev), "tgx_frame_device")
+
+    def gi_stats(self):
+        st = np.zeros(1, GI_STATS_DTYPE)
+        self._check(self.lib.tgx_gi_last_stats(self.h, _ptr(st)), "tgx_gi_last_stats")
+        return st[0]
+
+    def frame(self, cam: np.ndarray, w: int, h: int, params: np.ndarray):
+        """Visibility + GI of one camera: (vis[h, w], radiance[h, w, 3], irradiance[h, w, 3])."""
+        vis_d, rad_d, irr_d = self.alloc(8 * w * h), self.alloc(12 * w * h), self.alloc(12 * w * h)
+        try:
+            self.frame_device(cam, w, h, params, vis_d, rad_d, irr_d)
+            vis = np.empty((h, w), np.uint64)
+            rad = np.empty((h, w, 3), np.float32)
+            irr = np.empty((h, w, 3), np.float32)
+            self.d2h(vis, vis_d)
+            self.d2h(rad, rad_d)
+            self.d2h(irr, irr_d)
+        finally:
+            for p in (vis_d, rad_d, irr_d):
+                self.free(p)
+        return vis, rad, irr
 
     def alloc(self, nbytes: int) -> int:
         p = ctypes.c_void_p()

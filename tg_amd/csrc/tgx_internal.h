@@ -71,6 +71,7 @@ struct tgx_context {
     tgx_buffer   svo_rays;       /* staging for tgx_svo_trace               */
     tgx_buffer   svo_hits;
     tgx_buffer   prim_tmp;       /* rocPRIM scratch                         */
+    tgx_buffer   gi_counts;      /* GI ray counters (4 x u64)               */
 
     /* Profiling. */
     bool                           profiling = false;

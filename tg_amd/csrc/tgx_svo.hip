@@ -636,6 +636,181 @@ __global__ __launch_bounds__(256) void svo_trace_k(const svo_gnode* __restrict__
 }
 
 /* ------------------------------------------------------------------------ */
+/* One-bounce GI (estimator documented at tgx_gi_params in tgx.h)            */
+/* ------------------------------------------------------------------------ */
+
+#define GI_SQRT3 1.73205080757f
+#define GI_SHADOW_OFFSET 0.015625f
+#define GI_INV_PI 0.318309886f
+
+struct gi_cam {
+    float pos[3];
+    float bl[3], br[3], tr[3], tl[3];
+};
+
+/* MurmurHash3 finaliser (util.inc:47-56) */
+__device__ static inline uint32_t gi_fmix32(uint32_t v)
+{
+    v ^= v >> 16; v *= 0x85ebca6bu; v ^= v >> 13; v *= 0xc2b2ae35u; v ^= v >> 16;
+    return v;
+}
+/* xorshift32 and its [lo, hi] float (util.inc:11-31) */
+__device__ static inline float gi_rand(uint32_t& s, float lo, float hi)
+{
+    uint32_t r = s;
+    r ^= r << 13; r ^= r >> 17; r ^= r << 5;
+    s = r;
+    const float u = (float)r / (float)0xFFFFFFFFu;
+    return u * (hi - lo) + lo;
+}
+__device__ static inline f3 gi_cosine_direction(f3 n, uint32_t& st)
+{
+    for (int attempt = 0; attempt < 32; attempt++) {
+        const float x = gi_rand(st, -1.0f, 1.0f);
+        const float y = gi_rand(st, -1.0f, 1.0f);
+        const float z = gi_rand(st, -1.0f, 1.0f);
+        const float l2 = x * x + y * y + z * z;
+        if (l2 > 1.0f || l2 < 1e-6f) continue;
+        const float l = sqrtf(l2);
+        const float dx = n.x + x / l, dy = n.y + y / l, dz = n.z + z / l;
+        const float dl2 = dx * dx + dy * dy + dz * dz;
+        if (dl2 < 1e-6f) continue;
+        const float dl = sqrtf(dl2);
+        f3 r = {dx / dl, dy / dl, dz / dl};
+        return r;
+    }
+    return n;
+}
+
+__device__ static inline float gi_sign(float a) { return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f); }
+
+/* Primary surface of a visibility word (shading.frag:116-229). */
+__device__ static bool gi_surface(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
+                                  const uint32_t* __restrict__ c2o, const uint8_t* __restrict__ materials,
+                                  const tgx_lut_entry* __restrict__ luts, const gi_cam& cam, uint32_t W, uint32_t H,
+                                  uint32_t px, uint32_t py, uint64_t word, f3& p, f3& n, f3& albedo)
+{
+    const float depth = (float)(word >> 40) / 16777215.0f;
+    if (!(depth < 1.0f)) return false;
+    const uint32_t cptr = (uint32_t)(word >> 9) & 0x7FFFFFFFu;
+    const uint32_t v = (uint32_t)word & 511u;
+    const uint32_t cidx = cptrs[cptr];
+    const tgx_object ob = objects[c2o[cidx]];
+    const uint32_t rel = cptr - ob.first_cluster_pointer;
+    const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
+    const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
+    const tgx_m4 w2c = tgx_world_to_cluster(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_v4 o4 = tgx_m4_mulv4(w2c, cam.pos[0], cam.pos[1], cam.pos[2], 1.0f);
+    const float fx = ((float)px + 0.5f) / (float)W;
+    const float fy = 1.0f - ((float)py + 0.5f) / (float)H;
+    const float omy = 1.0f - fy, omx = 1.0f - fx;
+    const float lx = cam.bl[0] * omy + cam.tl[0] * fy, ly = cam.bl[1] * omy + cam.tl[1] * fy, lz = cam.bl[2] * omy + cam.tl[2] * fy;
+    const float qx = cam.br[0] * omy + cam.tr[0] * fy, qy = cam.br[1] * omy + cam.tr[1] * fy, qz = cam.br[2] * omy + cam.tr[2] * fy;
+    const float dwx = lx * omx + qx * fx, dwy = ly * omx + qy * fx, dwz = lz * omx + qz * fx;
+    const tgx_v4 d4 = tgx_m4_mulv4(w2c, dwx, dwy, dwz, 0.0f);
+    const float dm = sqrtf(d4.x * d4.x + d4.y * d4.y + d4.z * d4.z);
+    const f3 dms = {d4.x / dm, d4.y / dm, d4.z / dm};
+    const f3 orig = {o4.x, o4.y, o4.z};
+    const f3 vmn = {(float)(v % 8u), (float)((v / 8u) % 8u), (float)(v / 64u)};
+    const f3 vmx = {vmn.x + 1.0f, vmn.y + 1.0f, vmn.z + 1.0f};
+    float enter, exit_;
+    if (!svo_slab(orig, dms, vmn, vmx, enter, exit_)) return false;
+    f3 hit = orig;
+    if (enter > 0.0f) { hit.x = orig.x + dms.x * enter; hit.y = orig.y + dms.y * enter; hit.z = orig.z + dms.z * enter; }
+    f3 nn = {hit.x - (vmn.x + 0.5f), hit.y - (vmn.y + 0.5f), hit.z - (vmn.z + 0.5f)};
+    const float ax = fabsf(nn.x), ay = fabsf(nn.y), az = fabsf(nn.z);
+    if (ax > ay) {
+        nn.y = 0.0f;
+        if (ax > az) { nn.x = gi_sign(nn.x); nn.z = 0.0f; } else { nn.z = gi_sign(nn.z); nn.x = 0.0f; }
+    } else {
+        nn.x = 0.0f;
+        if (ay > az) { nn.y = gi_sign(nn.y); nn.z = 0.0f; } else { nn.z = gi_sign(nn.z); nn.y = 0.0f; }
+    }
+    tgx_m4 rot;
+    for (int i = 0; i < 16; i++) rot.m[i] = ob.rotation[i];
+    const tgx_v4 n4 = tgx_m4_mulv4(rot, nn.x, nn.y, nn.z, 0.0f);
+    const float nm = sqrtf(n4.x * n4.x + n4.y * n4.y + n4.z * n4.z);
+    n.x = n4.x / nm; n.y = n4.y / nm; n.z = n4.z / nm;
+    const tgx_m4 c2w = tgx_cluster_to_world(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_v4 p4 = tgx_m4_mulv4(c2w, hit.x, hit.y, hit.z, 1.0f);
+    p.x = p4.x; p.y = p4.y; p.z = p4.z;
+    const uint32_t mat = materials ? (uint32_t)materials[(size_t)cidx * 512u + v] : 0u;
+    const tgx_lut_entry e = luts[(size_t)ob.lut_idx * TGX_LUT_ENTRIES + mat];
+    albedo.x = e.albedo[0]; albedo.y = e.albedo[1]; albedo.z = e.albedo[2];
+    return true;
+}
+
+__device__ static inline unsigned long long gi_wave_sum(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void gi_pixel_k(const tgx_object* __restrict__ objects,
+                                                  const uint32_t* __restrict__ cptrs, const uint32_t* __restrict__ c2o,
+                                                  const uint8_t* __restrict__ materials,
+                                                  const tgx_lut_entry* __restrict__ luts,
+                                                  const svo_gnode* __restrict__ nodes,
+                                                  const uint32_t* __restrict__ leaf_bits,
+                                                  const uint16_t* __restrict__ leaf_mats, float smx, float smy,
+                                                  float smz, uint32_t depth, gi_cam cam, uint32_t W, uint32_t H,
+                                                  const uint64_t* __restrict__ vis, tgx_gi_params prm,
+                                                  float* __restrict__ radiance, float* __restrict__ irradiance,
+                                                  unsigned long long* __restrict__ counts)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n_vis = 0, n_b = 0, n_bh = 0, n_sh = 0;
+    if (pix < W * H) {
+        const uint32_t px = pix % W, py = pix / W;
+        f3 p, n, alb;
+        float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+        if (gi_surface(objects, cptrs, c2o, materials, luts, cam, W, H, px, py, vis[pix], p, n, alb)) {
+            n_vis = 1;
+            const f3 sun = {prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2]};
+            for (uint32_t k = 0; k < prm.spp; k++) {
+                uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
+                if (st == 0u) st = 0x6D2B79F5u;
+                const f3 d = gi_cosine_direction(n, st);
+                const f3 ob = {p.x + GI_SQRT3 * n.x, p.y + GI_SQRT3 * n.y, p.z + GI_SQRT3 * n.z};
+                const svo_ray_result hb = svo_trace_ray(nodes, leaf_bits, smx, smy, smz, depth, ob, d, prm.max_distance);
+                n_b++;
+                if (!hb.hit) continue;
+                n_bh++;
+                if (hb.distance < 0.0f) continue; /* started inside solid: occluded */
+                const float sc = hb.ax == 0 ? sun.x : (hb.ax == 1 ? sun.y : sun.z);
+                const float cosv = (float)hb.sign * sc;
+                if (!(cosv > 0.0f)) continue;
+                f3 os = {ob.x + d.x * hb.distance, ob.y + d.y * hb.distance, ob.z + d.z * hb.distance};
+                const float off = (float)hb.sign * GI_SHADOW_OFFSET;
+                if (hb.ax == 0) os.x = os.x + off; else if (hb.ax == 1) os.y = os.y + off; else os.z = os.z + off;
+                const svo_ray_result hs = svo_trace_ray(nodes, leaf_bits, smx, smy, smz, depth, os, sun, prm.max_distance);
+                n_sh++;
+                if (hs.hit) continue;
+                const uint16_t m = leaf_mats[(size_t)nodes[hb.node].leaf_idx * TGX_SVO_BLOCK_VOXELS + hb.voxel];
+                const tgx_lut_entry e = luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
+                const float wgt = cosv * prm.sun_radiance * GI_INV_PI;
+                acc0 = acc0 + e.albedo[0] * wgt;
+                acc1 = acc1 + e.albedo[1] * wgt;
+                acc2 = acc2 + e.albedo[2] * wgt;
+            }
+            const float inv = 1.0f / (float)prm.spp;
+            e0 = acc0 * inv; e1 = acc1 * inv; e2 = acc2 * inv;
+            r0 = alb.x * e0; r1 = alb.y * e1; r2 = alb.z * e2;
+        }
+        radiance[3 * (size_t)pix] = r0; radiance[3 * (size_t)pix + 1] = r1; radiance[3 * (size_t)pix + 2] = r2;
+        if (irradiance) { irradiance[3 * (size_t)pix] = e0; irradiance[3 * (size_t)pix + 1] = e1; irradiance[3 * (size_t)pix + 2] = e2; }
+    }
+    const unsigned long long a = gi_wave_sum(n_vis), b = gi_wave_sum(n_b), c = gi_wave_sum(n_bh), d = gi_wave_sum(n_sh);
+    if ((threadIdx.x & 63u) == 0u) {
+        if (a) atomicAdd(&counts[0], a);
+        if (b) atomicAdd(&counts[1], b);
+        if (c) atomicAdd(&counts[2], c);
+        if (d) atomicAdd(&counts[3], d);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Host side                                                                */
 /* ------------------------------------------------------------------------ */
 
@@ -645,7 +820,7 @@ void tgx_svo_release(tgx_context* ctx)
 {
     tgx_buffer* bufs[] = {&ctx->svo_nodes, &ctx->svo_leaf_bits, &ctx->svo_leaf_mats, &ctx->svo_leaf_n,
                           &ctx->svo_list_off, &ctx->svo_list, &ctx->svo_frames, &ctx->svo_rays,
-                          &ctx->svo_hits, &ctx->prim_tmp};
+                          &ctx->svo_hits, &ctx->prim_tmp, &ctx->gi_counts};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     ctx->svo_built = false;
 }
@@ -935,6 +1110,61 @@ int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* direction
     TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, ctx->svo_hits.ptr, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return TGX_OK;
+}
+
+int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
+                  const tgx_gi_params* prm, float* radiance_dev, float* irradiance_dev)
+{
+    if (!ctx || !cam || !vis_dev || !prm || !radiance_dev) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (!ctx->luts.ptr || ctx->n_luts == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "the scene has no LUTs");
+    if (w == 0 || h == 0 || (uint64_t)w * h >= (1ull << 31)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad resolution");
+    if (prm->spp < 1 || prm->spp > 64 || !(prm->max_distance > 0.0f))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "spp must be in [1, 64] and max_distance > 0");
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 64, "gi counts"))) return rc;
+    gi_cam gc;
+    for (int i = 0; i < 3; i++) {
+        gc.pos[i] = cam->camera[i];
+        gc.bl[i] = cam->ray_bl[i]; gc.br[i] = cam->ray_br[i]; gc.tr[i] = cam->ray_tr[i]; gc.tl[i] = cam->ray_tl[i];
+    }
+    hipLaunchKernelGGL(svo_fill_k, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->gi_counts.ptr, 8ull, 0u);
+    {
+        tgx_launch_scope s(ctx, "gi_pixel_k");
+        hipLaunchKernelGGL(gi_pixel_k, dim3(blocks_for((uint64_t)w * h)), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
+                           (const tgx_lut_entry*)ctx->luts.ptr, (const svo_gnode*)ctx->svo_nodes.ptr,
+                           (const uint32_t*)ctx->svo_leaf_bits.ptr, (const uint16_t*)ctx->svo_leaf_mats.ptr,
+                           ctx->svo_info.min[0], ctx->svo_info.min[1], ctx->svo_info.min[2], ctx->svo_info.depth, gc,
+                           w, h, vis_dev, *prm, radiance_dev, irradiance_dev, (unsigned long long*)ctx->gi_counts.ptr);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_pixel_k");
+    return TGX_OK;
+}
+
+int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out)
+{
+    if (!ctx || !out) return TGX_ERR_INVALID_ARG;
+    if (!ctx->gi_counts.ptr) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no GI call yet");
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long c[4];
+    TGX_HIP_CHECK(ctx, hipMemcpy(c, ctx->gi_counts.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    out->visible_pixels = c[0];
+    out->bounce_rays = c[1];
+    out->bounce_hits = c[2];
+    out->shadow_rays = c[3];
+    return TGX_OK;
+}
+
+int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* params,
+                     uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev)
+{
+    int rc = tgx_visibility_device(ctx, cam, 1, w, h, vis_dev);
+    if (rc) return rc;
+    return tgx_gi_device(ctx, cam, w, h, vis_dev, params, radiance_dev, irradiance_dev);
 }
 
 } /* extern "C" */
