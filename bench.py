@@ -46,13 +46,12 @@ def vis_alg_bytes(stats, n_objects: int, n_pixels: int) -> int:
     return 8 * n_pixels + 72 * int(stats.clusters_binned) + 96 * n_objects
 
 
-def gi_alg_bytes(gst, info, n_pixels: int) -> int:
-    """GI frame: 8 B of visibility read per pixel and 24 B of radiance and
-    irradiance written; per visible pixel one 64-B line each of object
-    record, cluster maps, material and LUT; the SVO's nodes (32 B) and leaf
-    bit blocks (4 KiB) once; one 64-B material line per bounce hit."""
-    return (32 * n_pixels + 256 * int(gst["visible_pixels"]) + 32 * int(info["n_nodes"]) +
-            4096 * int(info["n_leaves"]) + 64 * int(gst["bounce_hits"]))
+def trace_alg_bytes(gst, info) -> int:
+    """Both SVO trace launches of a GI frame: per ray its 32-B queue record
+    read and 24-B result written; per launch the SVO's nodes (32 B each) and
+    leaf bit blocks (4 KiB each) read once."""
+    rays = int(gst["bounce_rays"]) + int(gst["shadow_rays"])
+    return 56 * rays + 2 * (32 * int(info["n_nodes"]) + 4096 * int(info["n_leaves"]))
 
 
 def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 12.0) -> dict:
@@ -149,8 +148,10 @@ def main() -> int:
     if args.workload == "frame":
         gst = ctx.gi_stats()
         rays_per_frame = primary + int(gst["bounce_rays"]) + int(gst["shadow_rays"])
-        dominant = max(("gi_pixel_k", "vis_traverse"), key=lambda k: kernels_ms.get(k, 0.0))
-        alg = gi_alg_bytes(gst, info, primary) if dominant == "gi_pixel_k" else vis_alg_bytes(vstats, scene.objects.size, primary)
+        dominant = max(("gi_trace_k", "vis_traverse"), key=lambda k: kernels_ms.get(k, 0.0))
+        # per-launch bytes: the trace kernel runs twice per frame (bounce, shadow)
+        alg = (trace_alg_bytes(gst, info) // 2 if dominant == "gi_trace_k"
+               else vis_alg_bytes(vstats, scene.objects.size, primary))
     else:
         gst = None
         rays_per_frame = primary

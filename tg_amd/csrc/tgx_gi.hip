@@ -1,0 +1,895 @@
+/*
+ * tgx_gi.hip — SVO ray traversal and one-bounce GI on gfx950.
+ *
+ * Traversal (tg_svo_traverse, tg_sparse_voxel_octree.c:558-740). The
+ * reference's node stack is exactly the path from the root to the current
+ * node, so a lane keeps only the current node, its level and its integer
+ * coordinates; popping walks parent links, and every node box is recomputed
+ * from (level, coordinates) with the values the reference's stack holds
+ * (integers below 2^24, exact in fp32). Slab quotient pairs (box min, box
+ * max) are reduced to the one quotient that min/max would pick, chosen by
+ * the sign of the direction component: subtraction and IEEE division are
+ * monotone, so the value is the same and half of the divisions disappear.
+ * The pop test (exit distance > FLT_EPSILON) needs no division at all
+ * (axis_exit_gt_eps), and the DDA step lengths 1/|d| are computed once per
+ * ray.
+ *
+ * Layout read by the traversal (written by the SVO build):
+ *   inner[n_inner]   uint2 per inner node: first child, parent | valid << 24.
+ *                    Held in LDS with the brick masks when both fit.
+ *                    Leaves follow the inner nodes in node order, so leaf
+ *                    index = node - n_inner,
+ *                    and a child is a leaf exactly at the leaf level.
+ *   bricks[n_leaves] u64 per leaf: one bit per 8^3 brick holding a solid
+ *                    voxel. The 32^3 DDA tests the brick bit before it
+ *                    loads a bit word, and keeps the last word, so most DDA
+ *                    iterations read no memory. Voxels tested and their
+ *                    order are the reference's; only loads are skipped.
+ *
+ * gi_trace_k is persistent: each lane runs one ray as a state machine
+ * whose steps are either one node step of the reference's loop or up to
+ * DDA_BURST iterations of the leaf DDA, so lanes of a wave stay busy while
+ * they are in different phases. Whenever GI_REFILL_LANES lanes of a wave
+ * have finished, the wave takes that many new rays from the queue with one
+ * atomic (ballot + lane rank).
+ *
+ * GI frame (estimator documented at tgx_gi_params in tgx.h):
+ *   gi_gen_k     surface of each visible pixel, spp bounce rays appended
+ *   gi_trace_k   bounce rays
+ *   gi_shade_k   bounce hits facing the sun: material albedo and a shadow ray
+ *   gi_trace_k   shadow rays
+ *   gi_lit_k     unshadowed samples store their contribution
+ *   gi_resolve_k per pixel, contributions summed in sample order
+ * Contributions are stored per (pixel, sample) and summed in sample order,
+ * so the fp32 result equals a sequential per-pixel loop bit for bit.
+ */
+#include "tgx_svo_dev.h"
+#include "tgx_m4.h"
+
+#define GI_SQRT3 1.73205080757f
+#define GI_SHADOW_OFFSET 0.015625f
+#define GI_INV_PI 0.318309886f
+
+#define TRACE_BLOCK 1024
+#define TRACE_LDS_NODES_MAX 8000u   /* inner nodes + leaf brick masks in LDS (64000 B) */
+#define DDA_BURST 8
+#define GI_REFILL_LANES 16u
+
+/* ------------------------------------------------------------------------ */
+/* Traversal                                                                */
+/* ------------------------------------------------------------------------ */
+
+struct trace_params {
+    const uint2*              inner;
+    const unsigned long long* bricks;
+    const uint32_t*           leaf_bits;
+    uint32_t                  n_inner;
+    uint32_t                  n_leaves;
+    uint32_t                  depth;
+    uint32_t                  leaf_level;
+    float                     mnx, mny, mnz;
+};
+
+/* Ray record of the queues (32 B). */
+struct svo_ray {
+    float    ox, oy, oz;
+    float    dx, dy, dz;
+    uint32_t id;
+    uint32_t pad;
+};
+
+/* Result record (24 B): flags bit 0 hit, bit 1 reached a voxel (node and
+ * voxel valid), bits 2-3 entry axis, bit 4 normal sign is positive. */
+struct svo_res {
+    float    distance;
+    uint32_t node;
+    uint32_t voxel;
+    uint32_t flags;
+    uint32_t steps;
+    uint32_t pad;
+};
+
+/* Near / far plane distance along one axis: the numerator is selected
+ * before the one division, so a single IEEE division is issued. */
+__device__ static inline float axis_near(float o, float d, float mn, float mx)
+{
+    const float q = (d > 0.0f ? mn - o : mx - o) / d;
+    return d == 0.0f ? -SVO_F32_MAX : q;
+}
+__device__ static inline float axis_far(float o, float d, float mn, float mx)
+{
+    const float q = (d > 0.0f ? mx - o : mn - o) / d;
+    return d == 0.0f ? SVO_F32_MAX : q;
+}
+
+/* exit of tg_sparse_voxel_octree.c:704-707 (min over axes of the far plane) */
+__device__ static inline float box_exit(float px, float py, float pz, float dx, float dy, float dz, float mnx,
+                                        float mny, float mnz, float side)
+{
+    return fmin2(fmin2(axis_far(px, dx, mnx, mnx + side), axis_far(py, dy, mny, mny + side)),
+                 axis_far(pz, dz, mnz, mnz + side));
+}
+
+/* box_exit(...) > SVO_EPSILON for one axis, without dividing. fl(a / d) >
+ * 2^-23 holds exactly when a / d > 2^-23 (1 + 2^-24), the midpoint to the
+ * next float (a tie rounds to the even 2^-23). For a normal t = d 2^-23 no
+ * float lies in (|t|, |t| (1 + 2^-24)], so the test is a > t for d > 0 and
+ * a < t for d < 0. Components with |d| < 2^-100 take the division. */
+__device__ static inline bool axis_exit_gt_eps(float p, float d, float mn, float mx)
+{
+    if (d == 0.0f) return true; /* quotient F32_MAX */
+    if (fabsf(d) < 0x1p-100f) return axis_far(p, d, mn, mx) > SVO_EPSILON;
+    const float t = d * 0x1p-23f;
+    return d > 0.0f ? (mx - p) > t : (mn - p) < t;
+}
+
+/* tg_amanatides_woo's per-axis quantities: pos / d for d > 0, neg / -d for
+ * d < 0, F32_MAX for d == 0; one division. */
+__device__ static inline float dda_len(float pos, float neg, float d)
+{
+    const float q = (d > 0.0f ? pos : neg) / (d > 0.0f ? d : -d);
+    return d == 0.0f ? SVO_F32_MAX : q;
+}
+
+/* -1, 0 or 1 */
+__device__ static inline int dir_step(float d) { return d > 0.0f ? 1 : (d < 0.0f ? -1 : 0); }
+
+/* Per-lane ray state (22 registers). `node` packs the current node's
+ * coordinates (7 bits each: inner nodes sit at level <= depth - 6 <= 7), its
+ * level, the child slot being visited and the DDA phase flag; `vox` packs
+ * the leaf DDA voxel and iteration count. */
+struct trace_lane {
+    float    px, py, pz;   /* position relative to the root centre */
+    float    dx, dy, dz;
+    float    dist;
+    uint32_t steps;
+    uint32_t cur;          /* current node, SVO_NONE once popped empty */
+    uint32_t node;         /* cx | cy << 7 | cz << 14 | lvl << 21 | rc << 24 | dda << 27 */
+    uint32_t leaf;         /* leaf index during the DDA */
+    uint32_t vox;          /* x | y << 5 | z << 10 | it << 15 */
+    float    tmx, tmy, tmz, tdx, tdy, tdz;
+    unsigned long long bm; /* brick mask of the leaf */
+    uint32_t wi, wv;       /* last bit word index and value */
+};
+
+#define ND_CX(n)  ((n) & 127u)
+#define ND_CY(n)  (((n) >> 7) & 127u)
+#define ND_CZ(n)  (((n) >> 14) & 127u)
+#define ND_LVL(n) (((n) >> 21) & 7u)
+#define ND_RC(n)  (((n) >> 24) & 7u)
+#define ND_DDA    (1u << 27)
+
+__device__ static inline svo_res res_miss(uint32_t steps)
+{
+    svo_res r = {SVO_F32_MAX, SVO_NONE, 0u, 0u, steps, 0u};
+    return r;
+}
+
+/* Ray setup (tg_sparse_voxel_octree.c:570-594): false if the root box is
+ * missed (the result is then a miss with 0 steps). */
+__device__ static inline bool lane_init(const trace_params& P, float half, const svo_ray& r, trace_lane& L)
+{
+    const float ox = r.ox - (P.mnx + half), oy = r.oy - (P.mny + half), oz = r.oz - (P.mnz + half);
+    L.dx = r.dx; L.dy = r.dy; L.dz = r.dz;
+    L.steps = 0;
+    const float enter = fmax2(fmax2(axis_near(ox, L.dx, -half, half), axis_near(oy, L.dy, -half, half)),
+                              axis_near(oz, L.dz, -half, half));
+    const float exit_ = fmin2(fmin2(axis_far(ox, L.dx, -half, half), axis_far(oy, L.dy, -half, half)),
+                              axis_far(oz, L.dz, -half, half));
+    if (!(exit_ > 0.0f && enter <= exit_)) return false;
+    L.px = ox; L.py = oy; L.pz = oz;
+    if (enter > 0.0f) { L.px = ox + L.dx * enter; L.py = oy + L.dy * enter; L.pz = oz + L.dz * enter; }
+    L.dist = enter > 0.0f ? enter : 0.0f;
+    L.cur = 0;
+    L.node = 0;
+    /* DDA step lengths 1/|d| (tg_amanatides_woo), fixed per ray */
+    L.tdx = dda_len(1.0f, 1.0f, L.dx);
+    L.tdy = dda_len(1.0f, 1.0f, L.dy);
+    L.tdz = dda_len(1.0f, 1.0f, L.dz);
+    return true;
+}
+
+template <bool kLds>
+__device__ static inline uint2 inner_at(const uint2* __restrict__ g, const uint2* s, uint32_t i)
+{
+    if constexpr (kLds) return s[i];
+    else return g[i];
+}
+
+/* Brick masks follow the inner nodes in LDS. */
+template <bool kLds>
+__device__ static inline unsigned long long bricks_at(const unsigned long long* __restrict__ g, const uint2* s,
+                                                      uint32_t n_inner, uint32_t leaf)
+{
+    if constexpr (kLds) {
+        const uint2 v = s[n_inner + leaf];
+        return ((unsigned long long)v.y << 32) | v.x;
+    } else {
+        return g[leaf];
+    }
+}
+
+__device__ static inline uint32_t lane_rank(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+/* Minimum corner of child slot rc of the node packed in n. */
+__device__ static inline void child_min(uint32_t n, uint32_t rc, float half, float cs, float& x, float& y, float& z)
+{
+    x = -half + (float)(2u * ND_CX(n) + (rc & 1u)) * cs;
+    y = -half + (float)(2u * ND_CY(n) + ((rc >> 1) & 1u)) * cs;
+    z = -half + (float)(2u * ND_CZ(n) + (rc >> 2)) * cs;
+}
+
+/* One step of a lane: one iteration of the reference's node loop, or up to
+ * DDA_BURST iterations of the leaf DDA. Returns true when the ray is done;
+ * res is then its result. */
+template <bool kLds>
+__device__ static inline bool lane_step(const trace_params& P, const uint2* s_inner, float half, float maxd,
+                                        trace_lane& L, svo_res& res)
+{
+    bool adv = false;
+    if (!(L.node & ND_DDA)) {
+        /* one iteration of the reference's node loop */
+        if (L.cur == SVO_NONE || L.steps >= TGX_SVO_MAX_STEPS || L.dist > maxd) {
+            res = res_miss(L.steps);
+            return true;
+        }
+        L.steps++;
+        const uint2 nd = inner_at<kLds>(P.inner, s_inner, L.cur);
+        const uint32_t n = L.node;
+        const uint32_t lvl = ND_LVL(n);
+        const float ps = (float)(1u << (P.depth - lvl));
+        const float ce = ps * 0.5f;
+        const float pmx = -half + (float)ND_CX(n) * ps, pmy = -half + (float)ND_CY(n) * ps,
+                    pmz = -half + (float)ND_CZ(n) * ps;
+        const float hx = pmx + ce, hy = pmy + ce, hz = pmz + ce;
+        uint32_t rc = 0;
+        if (hx < L.px || (L.px == hx && L.dx > 0.0f)) rc |= 1u;
+        if (hy < L.py || (L.py == hy && L.dy > 0.0f)) rc |= 2u;
+        if (hz < L.pz || (L.pz == hz && L.dz > 0.0f)) rc |= 4u;
+        L.node = (n & ~(7u << 24)) | (rc << 24);
+        adv = true;
+        const uint32_t valid = nd.y >> 24;
+        if (valid & (1u << rc)) {
+            const uint32_t ci = nd.x + (uint32_t)__popc(valid & ((1u << rc) - 1u));
+            if (lvl + 1u < P.leaf_level) {
+                L.cur = ci;
+                L.node = (2u * ND_CX(n) + (rc & 1u)) | ((2u * ND_CY(n) + ((rc >> 1) & 1u)) << 7) |
+                         ((2u * ND_CZ(n) + (rc >> 2)) << 14) | ((lvl + 1u) << 21);
+                adv = false;
+            } else {
+                const uint32_t leaf = ci - P.n_inner;
+                const unsigned long long bm = bricks_at<kLds>(P.bricks, s_inner, P.n_inner, leaf);
+                if (bm != 0ull) {
+                    /* tg_amanatides_woo setup over the 32^3 block, entry voxel clamped */
+                    const float cmx = (rc & 1u) ? hx : pmx, cmy = (rc & 2u) ? hy : pmy, cmz = (rc & 4u) ? hz : pmz;
+                    const float gx = L.px - cmx, gy = L.py - cmy, gz = L.pz - cmz;
+                    const int x = (int)fmin2(fmax2(floorf(gx), 0.0f), 31.0f);
+                    const int y = (int)fmin2(fmax2(floorf(gy), 0.0f), 31.0f);
+                    const int z = (int)fmin2(fmax2(floorf(gz), 0.0f), 31.0f);
+                    L.tmx = dda_len((float)(x + 1) - gx, gx - (float)x, L.dx);
+                    L.tmy = dda_len((float)(y + 1) - gy, gy - (float)y, L.dy);
+                    L.tmz = dda_len((float)(z + 1) - gz, gz - (float)z, L.dz);
+                    L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10);
+                    L.bm = bm;
+                    L.leaf = leaf;
+                    L.wi = 0xFFFFFFFFu;
+                    L.node |= ND_DDA;
+                    adv = false;
+                }
+            }
+        }
+    }
+    if (L.node & ND_DDA) {
+        /* up to DDA_BURST iterations of the leaf DDA */
+        const uint32_t* grid = P.leaf_bits + (size_t)L.leaf * TGX_SVO_BLOCK_WORDS;
+        int x = (int)(L.vox & 31u), y = (int)((L.vox >> 5) & 31u), z = (int)((L.vox >> 10) & 31u);
+        uint32_t it = L.vox >> 15;
+        bool leave = false, hit = false;
+#pragma unroll 1
+        for (int k = 0; k < DDA_BURST; k++) {
+            if (it >= 4u * TGX_DDA_MAX_ITERATIONS) { leave = true; break; }
+            const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
+            if ((L.bm >> b) & 1ull) {
+                const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
+                if (wi != L.wi) { L.wi = wi; L.wv = grid[wi]; }
+                if ((L.wv >> (uint32_t)x) & 1u) { hit = true; break; }
+            }
+            /* step along x if tmx < tmy and tmx < tmz, else along y if
+             * !(tmx < tmy) and tmy < tmz, else along z (branch-free) */
+            const bool step_x = L.tmx < L.tmy && L.tmx < L.tmz;
+            const bool step_y = !(L.tmx < L.tmy) && L.tmy < L.tmz;
+            const bool step_z = !step_x && !step_y;
+            L.tmx = step_x ? L.tmx + L.tdx : L.tmx;
+            L.tmy = step_y ? L.tmy + L.tdy : L.tmy;
+            L.tmz = step_z ? L.tmz + L.tdz : L.tmz;
+            x += step_x ? dir_step(L.dx) : 0;
+            y += step_y ? dir_step(L.dy) : 0;
+            z += step_z ? dir_step(L.dz) : 0;
+            if (((uint32_t)x | (uint32_t)y | (uint32_t)z) >= 32u) { leave = true; break; }
+            it++;
+        }
+        if (hit) {
+            /* distance to the voxel's entry plane and the entry face */
+            float vmx, vmy, vmz;
+            child_min(L.node, ND_RC(L.node), half, 32.0f, vmx, vmy, vmz);
+            vmx = vmx + (float)x; vmy = vmy + (float)y; vmz = vmz + (float)z;
+            const float nx = axis_near(L.px, L.dx, vmx, vmx + 1.0f);
+            const float ny = axis_near(L.py, L.dy, vmy, vmy + 1.0f);
+            const float nz = axis_near(L.pz, L.dz, vmz, vmz + 1.0f);
+            const float enter = fmax2(fmax2(nx, ny), nz);
+            res.distance = L.dist + enter;
+            res.node = P.n_inner + L.leaf;
+            res.voxel = 1024u * (uint32_t)z + 32u * (uint32_t)y + (uint32_t)x;
+            uint32_t ax = nx > ny ? 0u : 1u;
+            const float nax = ax == 0u ? nx : ny;
+            if (!(nax > nz)) ax = 2u;
+            const float dc = ax == 0u ? L.dx : (ax == 1u ? L.dy : L.dz);
+            res.flags = (res.distance <= maxd ? 1u : 0u) | 2u | (ax << 2) | (dc > 0.0f ? 0u : 16u);
+            res.steps = L.steps;
+            res.pad = 0u;
+            return true;
+        }
+        if (leave) {
+            L.node &= ~ND_DDA;
+            adv = true;
+        } else {
+            L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10) | (it << 15);
+        }
+    }
+    if (adv) {
+        /* leave the child box and pop every node the position has left */
+        uint32_t n = L.node;
+        const float cs = (float)(1u << (P.depth - ND_LVL(n) - 1u));
+        float cmx, cmy, cmz;
+        child_min(n, ND_RC(n), half, cs, cmx, cmy, cmz);
+        const float t_adv = box_exit(L.px, L.py, L.pz, L.dx, L.dy, L.dz, cmx, cmy, cmz, cs) + SVO_EPSILON;
+        L.px = L.px + L.dx * t_adv;
+        L.py = L.py + L.dy * t_adv;
+        L.pz = L.pz + L.dz * t_adv;
+        L.dist += t_adv;
+        uint32_t cur = L.cur;
+        while (cur != SVO_NONE) {
+            const uint32_t lvl = ND_LVL(n);
+            const float ns = (float)(1u << (P.depth - lvl));
+            const float nmx = -half + (float)ND_CX(n) * ns, nmy = -half + (float)ND_CY(n) * ns,
+                        nmz = -half + (float)ND_CZ(n) * ns;
+            if (axis_exit_gt_eps(L.px, L.dx, nmx, nmx + ns) && axis_exit_gt_eps(L.py, L.dy, nmy, nmy + ns) &&
+                axis_exit_gt_eps(L.pz, L.dz, nmz, nmz + ns))
+                break;
+            const uint32_t par = inner_at<kLds>(P.inner, s_inner, cur).y & 0xFFFFFFu;
+            cur = par == 0xFFFFFFu ? SVO_NONE : par;
+            n = (ND_CX(n) >> 1) | ((ND_CY(n) >> 1) << 7) | ((ND_CZ(n) >> 1) << 14) | (((lvl - 1u) & 7u) << 21);
+        }
+        L.cur = cur;
+        L.node = n;
+    }
+    return false;
+}
+
+/* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd; the
+ * result of ray i goes to out[i]. `cursor` must be 0 at launch; hits (flag
+ * bit 0) are added to *hit_counter when it is not null. */
+template <bool kLds>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
+                                                          const uint32_t* __restrict__ n_rays_ptr, float maxd,
+                                                          uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
+                                                          uint32_t* __restrict__ hit_counter)
+{
+    extern __shared__ uint2 s_inner[];
+    if constexpr (kLds) {
+        for (uint32_t i = threadIdx.x; i < P.n_inner; i += blockDim.x) s_inner[i] = P.inner[i];
+        const uint2* gb = (const uint2*)P.bricks;
+        for (uint32_t i = threadIdx.x; i < P.n_leaves; i += blockDim.x) s_inner[P.n_inner + i] = gb[i];
+        __syncthreads();
+    }
+    const uint32_t n_rays = *n_rays_ptr;
+    const uint32_t lane = threadIdx.x & 63u;
+    const float half = (float)(1u << P.depth) * 0.5f;
+    trace_lane L;
+    uint32_t slot = 0, hits = 0;
+    bool active = false, exhausted = false;
+    while (true) {
+        /* wave-uniform refill once GI_REFILL_LANES lanes are idle */
+        const unsigned long long idle = __ballot(!active);
+        const uint32_t want = (uint32_t)__popcll(idle);
+        if (!exhausted && want >= GI_REFILL_LANES) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(cursor, want);
+            base = __shfl(base, 0);
+            if (base + want >= n_rays) exhausted = true;
+            if (!active) {
+                const uint32_t r = base + lane_rank(idle);
+                if (r < n_rays) {
+                    slot = r;
+                    active = lane_init(P, half, rays[r], L);
+                    if (!active) out[r] = res_miss(0u);
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        svo_res res;
+        if (active && lane_step<kLds>(P, s_inner, half, maxd, L, res)) {
+            out[slot] = res;
+            hits += res.flags & 1u;
+            active = false;
+        }
+    }
+    if (hit_counter) {
+        for (int off = 32; off > 0; off >>= 1) hits += __shfl_down(hits, off);
+        if (lane == 0 && hits) atomicAdd(hit_counter, hits);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* One-bounce GI                                                            */
+/* ------------------------------------------------------------------------ */
+
+struct gi_cam {
+    float pos[3];
+    float bl[3], br[3], tr[3], tl[3];
+};
+
+/* per-pixel surface kept for the resolve: albedo and a visibility flag */
+struct gi_surf {
+    float    albedo[3];
+    uint32_t valid;
+};
+
+/* counters: 1 bounce rays, 2 bounce hits, 3 shadow rays, 4 trace cursor
+ * (bounce), 5 trace cursor (shadow), 6 ray count of tgx_svo_trace, 7 shadow
+ * hits */
+#define GI_N_COUNTERS 8
+
+__global__ void gi_fill_k(uint32_t* p, uint64_t n, uint32_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+/* MurmurHash3 finaliser (util.inc:47-56) */
+__device__ static inline uint32_t gi_fmix32(uint32_t v)
+{
+    v ^= v >> 16; v *= 0x85ebca6bu; v ^= v >> 13; v *= 0xc2b2ae35u; v ^= v >> 16;
+    return v;
+}
+/* xorshift32 and its [lo, hi] float (util.inc:11-31) */
+__device__ static inline float gi_rand(uint32_t& s, float lo, float hi)
+{
+    uint32_t r = s;
+    r ^= r << 13; r ^= r >> 17; r ^= r << 5;
+    s = r;
+    const float u = (float)r / (float)0xFFFFFFFFu;
+    return u * (hi - lo) + lo;
+}
+__device__ static inline f3 gi_cosine_direction(f3 n, uint32_t& st)
+{
+    for (int attempt = 0; attempt < 32; attempt++) {
+        const float x = gi_rand(st, -1.0f, 1.0f);
+        const float y = gi_rand(st, -1.0f, 1.0f);
+        const float z = gi_rand(st, -1.0f, 1.0f);
+        const float l2 = x * x + y * y + z * z;
+        if (l2 > 1.0f || l2 < 1e-6f) continue;
+        const float l = sqrtf(l2);
+        const float dx = n.x + x / l, dy = n.y + y / l, dz = n.z + z / l;
+        const float dl2 = dx * dx + dy * dy + dz * dz;
+        if (dl2 < 1e-6f) continue;
+        const float dl = sqrtf(dl2);
+        f3 r = {dx / dl, dy / dl, dz / dl};
+        return r;
+    }
+    return n;
+}
+
+__device__ static inline float gi_sign(float a) { return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f); }
+
+__device__ static inline bool gi_slab(f3 o, f3 d, f3 mn, f3 mx, float& enter)
+{
+    const float nx = axis_near(o.x, d.x, mn.x, mx.x), ny = axis_near(o.y, d.y, mn.y, mx.y), nz = axis_near(o.z, d.z, mn.z, mx.z);
+    const float fx = axis_far(o.x, d.x, mn.x, mx.x), fy = axis_far(o.y, d.y, mn.y, mx.y), fz = axis_far(o.z, d.z, mn.z, mx.z);
+    enter = fmax2(fmax2(nx, ny), nz);
+    const float exit_ = fmin2(fmin2(fx, fy), fz);
+    return exit_ > 0.0f && enter <= exit_;
+}
+
+/* Primary surface of a visibility word (shading.frag:116-229). */
+__device__ static bool gi_surface(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
+                                  const uint32_t* __restrict__ c2o, const uint8_t* __restrict__ materials,
+                                  const tgx_lut_entry* __restrict__ luts, const gi_cam& cam, uint32_t W, uint32_t H,
+                                  uint32_t px, uint32_t py, uint64_t word, f3& p, f3& n, f3& albedo)
+{
+    const float depth = (float)(word >> 40) / 16777215.0f;
+    if (!(depth < 1.0f)) return false;
+    const uint32_t cptr = (uint32_t)(word >> 9) & 0x7FFFFFFFu;
+    const uint32_t v = (uint32_t)word & 511u;
+    const uint32_t cidx = cptrs[cptr];
+    const tgx_object ob = objects[c2o[cidx]];
+    const uint32_t rel = cptr - ob.first_cluster_pointer;
+    const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
+    const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
+    const tgx_m4 w2c = tgx_world_to_cluster(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_v4 o4 = tgx_m4_mulv4(w2c, cam.pos[0], cam.pos[1], cam.pos[2], 1.0f);
+    const float fx = ((float)px + 0.5f) / (float)W;
+    const float fy = 1.0f - ((float)py + 0.5f) / (float)H;
+    const float omy = 1.0f - fy, omx = 1.0f - fx;
+    const float lx = cam.bl[0] * omy + cam.tl[0] * fy, ly = cam.bl[1] * omy + cam.tl[1] * fy, lz = cam.bl[2] * omy + cam.tl[2] * fy;
+    const float qx = cam.br[0] * omy + cam.tr[0] * fy, qy = cam.br[1] * omy + cam.tr[1] * fy, qz = cam.br[2] * omy + cam.tr[2] * fy;
+    const float dwx = lx * omx + qx * fx, dwy = ly * omx + qy * fx, dwz = lz * omx + qz * fx;
+    const tgx_v4 d4 = tgx_m4_mulv4(w2c, dwx, dwy, dwz, 0.0f);
+    const float dm = sqrtf(d4.x * d4.x + d4.y * d4.y + d4.z * d4.z);
+    const f3 dms = {d4.x / dm, d4.y / dm, d4.z / dm};
+    const f3 orig = {o4.x, o4.y, o4.z};
+    const f3 vmn = {(float)(v % 8u), (float)((v / 8u) % 8u), (float)(v / 64u)};
+    const f3 vmx = {vmn.x + 1.0f, vmn.y + 1.0f, vmn.z + 1.0f};
+    float enter;
+    if (!gi_slab(orig, dms, vmn, vmx, enter)) return false;
+    f3 hit = orig;
+    if (enter > 0.0f) { hit.x = orig.x + dms.x * enter; hit.y = orig.y + dms.y * enter; hit.z = orig.z + dms.z * enter; }
+    f3 nn = {hit.x - (vmn.x + 0.5f), hit.y - (vmn.y + 0.5f), hit.z - (vmn.z + 0.5f)};
+    const float ax = fabsf(nn.x), ay = fabsf(nn.y), az = fabsf(nn.z);
+    if (ax > ay) {
+        nn.y = 0.0f;
+        if (ax > az) { nn.x = gi_sign(nn.x); nn.z = 0.0f; } else { nn.z = gi_sign(nn.z); nn.x = 0.0f; }
+    } else {
+        nn.x = 0.0f;
+        if (ay > az) { nn.y = gi_sign(nn.y); nn.z = 0.0f; } else { nn.z = gi_sign(nn.z); nn.y = 0.0f; }
+    }
+    tgx_m4 rot;
+    for (int i = 0; i < 16; i++) rot.m[i] = ob.rotation[i];
+    const tgx_v4 n4 = tgx_m4_mulv4(rot, nn.x, nn.y, nn.z, 0.0f);
+    const float nm = sqrtf(n4.x * n4.x + n4.y * n4.y + n4.z * n4.z);
+    n.x = n4.x / nm; n.y = n4.y / nm; n.z = n4.z / nm;
+    const tgx_m4 c2w = tgx_cluster_to_world(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
+    const tgx_v4 p4 = tgx_m4_mulv4(c2w, hit.x, hit.y, hit.z, 1.0f);
+    p.x = p4.x; p.y = p4.y; p.z = p4.z;
+    const uint32_t mat = materials ? (uint32_t)materials[(size_t)cidx * 512u + v] : 0u;
+    const tgx_lut_entry e = luts[(size_t)ob.lut_idx * TGX_LUT_ENTRIES + mat];
+    albedo.x = e.albedo[0]; albedo.y = e.albedo[1]; albedo.z = e.albedo[2];
+    return true;
+}
+
+/* Reserves `count` consecutive queue slots for the calling lane (counts may
+ * differ per lane) with one atomic per workgroup; returns the lane's first
+ * slot. Every thread of the workgroup must call it. */
+template <uint32_t kBlock>
+__device__ static inline uint32_t block_append(uint32_t* counter, uint32_t count)
+{
+    __shared__ uint32_t s_wave[kBlock / 64u + 1u];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = count;
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63u) s_wave[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kBlock / 64u; w++) {
+            const uint32_t c = s_wave[w];
+            s_wave[w] = total;
+            total += c;
+        }
+        s_wave[kBlock / 64u] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    return s_wave[kBlock / 64u] + s_wave[wave] + incl - count;
+}
+
+#define GI_BLOCK 256u
+#define SHADE_ITEMS 4u
+
+__global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const tgx_object* __restrict__ objects,
+                                                     const uint32_t* __restrict__ cptrs, const uint32_t* __restrict__ c2o,
+                                                     const uint8_t* __restrict__ materials,
+                                                     const tgx_lut_entry* __restrict__ luts, gi_cam cam, uint32_t W,
+                                                     uint32_t H, const uint64_t* __restrict__ vis, tgx_gi_params prm,
+                                                     gi_surf* __restrict__ surf, svo_ray* __restrict__ rays,
+                                                     float* __restrict__ contrib, uint32_t* __restrict__ counters)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    f3 p = {0, 0, 0}, n = {0, 0, 0}, alb = {0, 0, 0};
+    bool vis_ok = false;
+    if (pix < W * H) {
+        vis_ok = gi_surface(objects, cptrs, c2o, materials, luts, cam, W, H, pix % W, pix / W, vis[pix], p, n, alb);
+        gi_surf s;
+        s.albedo[0] = alb.x; s.albedo[1] = alb.y; s.albedo[2] = alb.z;
+        s.valid = vis_ok ? 1u : 0u;
+        surf[pix] = s;
+    }
+    const uint32_t first = block_append<GI_BLOCK>(&counters[1], vis_ok ? prm.spp : 0u);
+    if (vis_ok) {
+        const f3 ob = {p.x + GI_SQRT3 * n.x, p.y + GI_SQRT3 * n.y, p.z + GI_SQRT3 * n.z};
+        for (uint32_t k = 0; k < prm.spp; k++) {
+            uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
+            if (st == 0u) st = 0x6D2B79F5u;
+            const f3 d = gi_cosine_direction(n, st);
+            svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, pix * prm.spp + k, 0u};
+            rays[first + k] = r;
+            const size_t c = 3ull * ((size_t)pix * prm.spp + k);
+            contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
+        }
+    }
+}
+
+/* Cosine of a bounce hit's entry face with the sun; > 0 when the hit takes
+ * part in the estimate (hits at negative distance started inside a solid
+ * voxel and count as occluded). */
+__device__ static inline float shade_cos(const svo_res& r, const tgx_gi_params& prm)
+{
+    if (!(r.flags & 1u) || r.distance < 0.0f) return 0.0f;
+    const uint32_t ax = (r.flags >> 2) & 3u;
+    const float sign = (r.flags & 16u) ? 1.0f : -1.0f;
+    const float sc = ax == 0u ? prm.sun_dir[0] : (ax == 1u ? prm.sun_dir[1] : prm.sun_dir[2]);
+    return sign * sc;
+}
+
+/* Bounce hits that face the sun: albedo-weighted contribution and a shadow
+ * ray (started 1/64 voxel off the hit face). SHADE_ITEMS rays per thread. */
+__global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, const uint16_t* __restrict__ leaf_mats,
+                                                       const tgx_lut_entry* __restrict__ luts, tgx_gi_params prm,
+                                                       const svo_ray* __restrict__ rays,
+                                                       const uint32_t* __restrict__ n_rays_ptr,
+                                                       const svo_res* __restrict__ res, svo_ray* __restrict__ shadow,
+                                                       float* __restrict__ weight, uint32_t* __restrict__ counters)
+{
+    const uint32_t n_rays = *n_rays_ptr;
+    const uint32_t first = blockIdx.x * GI_BLOCK * SHADE_ITEMS + threadIdx.x;
+    uint32_t emit = 0, count = 0;
+    for (uint32_t k = 0; k < SHADE_ITEMS; k++) {
+        const uint32_t i = first + k * GI_BLOCK;
+        if (i < n_rays && shade_cos(res[i], prm) > 0.0f) { emit |= 1u << k; count++; }
+    }
+    uint32_t slot = block_append<GI_BLOCK>(&counters[3], count);
+    for (uint32_t k = 0; k < SHADE_ITEMS; k++) {
+        if (!(emit & (1u << k))) continue;
+        const uint32_t i = first + k * GI_BLOCK;
+        const svo_res r = res[i];
+        const float cosv = shade_cos(r, prm);
+        const uint32_t ax = (r.flags >> 2) & 3u;
+        const float sign = (r.flags & 16u) ? 1.0f : -1.0f;
+        const svo_ray b = rays[i];
+        float ox = b.ox + b.dx * r.distance, oy = b.oy + b.dy * r.distance, oz = b.oz + b.dz * r.distance;
+        const float off = sign * GI_SHADOW_OFFSET;
+        if (ax == 0u) ox = ox + off; else if (ax == 1u) oy = oy + off; else oz = oz + off;
+        const svo_ray sr = {ox, oy, oz, prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2], b.id, 0u};
+        shadow[slot] = sr;
+        const uint16_t m = leaf_mats[(size_t)(r.node - n_inner) * TGX_SVO_BLOCK_VOXELS + r.voxel];
+        const tgx_lut_entry e = luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
+        const float wgt = cosv * prm.sun_radiance * GI_INV_PI;
+        weight[3ull * slot] = e.albedo[0] * wgt;
+        weight[3ull * slot + 1] = e.albedo[1] * wgt;
+        weight[3ull * slot + 2] = e.albedo[2] * wgt;
+        slot++;
+    }
+}
+
+/* Unshadowed samples store their contribution at (pixel, sample). */
+__global__ __launch_bounds__(256) void gi_lit_k(const svo_ray* __restrict__ shadow, const uint32_t* __restrict__ n_shadow,
+                                                const svo_res* __restrict__ res, const float* __restrict__ weight,
+                                                float* __restrict__ contrib)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *n_shadow || (res[j].flags & 1u)) return;
+    const size_t c = 3ull * shadow[j].id;
+    contrib[c] = weight[3ull * j]; contrib[c + 1] = weight[3ull * j + 1]; contrib[c + 2] = weight[3ull * j + 2];
+}
+
+__global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf* __restrict__ surf, const float* __restrict__ contrib,
+                                                    uint32_t n_pix, uint32_t spp, float* __restrict__ radiance,
+                                                    float* __restrict__ irradiance)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= n_pix) return;
+    const gi_surf s = surf[pix];
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+    if (s.valid) {
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        const float* c = contrib + 3ull * (size_t)pix * spp;
+        for (uint32_t k = 0; k < spp; k++) { a0 = a0 + c[3 * k]; a1 = a1 + c[3 * k + 1]; a2 = a2 + c[3 * k + 2]; }
+        const float inv = 1.0f / (float)spp;
+        e0 = a0 * inv; e1 = a1 * inv; e2 = a2 * inv;
+        r0 = s.albedo[0] * e0; r1 = s.albedo[1] * e1; r2 = s.albedo[2] * e2;
+    }
+    radiance[3ull * pix] = r0; radiance[3ull * pix + 1] = r1; radiance[3ull * pix + 2] = r2;
+    if (irradiance) { irradiance[3ull * pix] = e0; irradiance[3ull * pix + 1] = e1; irradiance[3ull * pix + 2] = e2; }
+}
+
+/* svo_res -> tgx_svo_hit for the C ABI */
+__global__ void svo_hits_k(const svo_gnode* __restrict__ nodes, const svo_res* __restrict__ res, uint32_t n,
+                           tgx_svo_hit* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const svo_res r = res[i];
+    tgx_svo_hit h;
+    h.hit = r.flags & 1u;
+    h.distance = r.distance;
+    h.block[0] = h.block[1] = h.block[2] = 0u;
+    h.voxel = r.voxel;
+    h.normal[0] = h.normal[1] = h.normal[2] = 0;
+    if (r.flags & 2u) {
+        const svo_gnode cn = nodes[r.node];
+        h.block[0] = cn.cx; h.block[1] = cn.cy; h.block[2] = cn.cz;
+        h.normal[(r.flags >> 2) & 3u] = (r.flags & 16u) ? 1 : -1;
+    }
+    h.steps = r.steps;
+    out[i] = h;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Host side                                                                */
+/* ------------------------------------------------------------------------ */
+
+static trace_params params_of(tgx_context* ctx)
+{
+    trace_params P;
+    P.inner = (const uint2*)ctx->svo_inner.ptr;
+    P.bricks = (const unsigned long long*)ctx->svo_bricks.ptr;
+    P.leaf_bits = (const uint32_t*)ctx->svo_leaf_bits.ptr;
+    P.n_inner = ctx->svo_n_inner;
+    P.n_leaves = ctx->svo_info.n_leaves;
+    P.depth = ctx->svo_info.depth;
+    P.leaf_level = ctx->svo_info.depth - 5u;
+    P.mnx = ctx->svo_info.min[0]; P.mny = ctx->svo_info.min[1]; P.mnz = ctx->svo_info.min[2];
+    return P;
+}
+
+/* Persistent trace of the queue rays[0 .. *n_dev) (n_max bounds it): two
+ * 1024-lane workgroups per CU, the inner nodes in LDS when they fit. */
+static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t n_max, float maxd,
+                        uint32_t* cursor, svo_res* out, uint32_t* hit_counter)
+{
+    if (n_max == 0) return TGX_OK;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    const uint32_t want = (uint32_t)((n_max + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    const uint32_t cap = (uint32_t)cus * 2u;
+    const dim3 grid(want < cap ? want : cap);
+    const trace_params P = params_of(ctx);
+    {
+        tgx_launch_scope s(ctx, "gi_trace_k");
+        if ((uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX)
+            hipLaunchKernelGGL(gi_trace_k<true>, grid, dim3(TRACE_BLOCK), 8u * (P.n_inner + P.n_leaves), ctx->stream, P,
+                               rays, n_dev, maxd, cursor, out, hit_counter);
+        else
+            hipLaunchKernelGGL(gi_trace_k<false>, grid, dim3(TRACE_BLOCK), 0, ctx->stream, P, rays, n_dev, maxd, cursor,
+                               out, hit_counter);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
+    return TGX_OK;
+}
+
+extern "C" {
+
+int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n, float max_distance,
+                  tgx_svo_hit* out_host)
+{
+    if (!ctx || (n && (!origins || !directions || !out_host))) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (n == 0) return TGX_OK;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, sizeof(svo_ray) * (size_t)n, "svo rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_hits, (sizeof(svo_res) + sizeof(tgx_svo_hit)) * (size_t)n, "svo hits")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_N_COUNTERS, "gi counters")))
+        return rc;
+    std::vector<svo_ray> h(n);
+    for (uint32_t i = 0; i < n; i++)
+        h[i] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2], directions[3 * i], directions[3 * i + 1],
+                directions[3 * i + 2], i, 0u};
+    svo_ray* d_rays = (svo_ray*)ctx->svo_rays.ptr;
+    svo_res* d_res = (svo_res*)ctx->svo_hits.ptr;
+    tgx_svo_hit* d_hits = (tgx_svo_hit*)(d_res + n);
+    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_rays, h.data(), sizeof(svo_ray) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 6, 1ull, n);
+    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, max_distance, cnt + 4, d_res, nullptr))) return rc;
+    hipLaunchKernelGGL(svo_hits_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (const svo_gnode*)ctx->svo_nodes.ptr,
+                       d_res, n, d_hits);
+    TGX_LAUNCH_CHECK(ctx, "svo_hits_k");
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, d_hits, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
+int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
+                  const tgx_gi_params* prm, float* radiance_dev, float* irradiance_dev)
+{
+    if (!ctx || !cam || !vis_dev || !prm || !radiance_dev) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (!ctx->luts.ptr || ctx->n_luts == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "the scene has no LUTs");
+    if (w == 0 || h == 0 || (uint64_t)w * h * (prm->spp ? prm->spp : 1) >= (1ull << 31))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad resolution");
+    if (prm->spp < 1 || prm->spp > 64 || !(prm->max_distance > 0.0f))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "spp must be in [1, 64] and max_distance > 0");
+    const uint64_t n_pix = (uint64_t)w * h;
+    const uint64_t n_samples = n_pix * prm->spp;
+    ctx->gi_spp = prm->spp;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_N_COUNTERS, "gi counters")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_surf, sizeof(gi_surf) * n_pix, "gi surfaces")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_rays, sizeof(svo_ray) * n_samples, "gi bounce rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_res, sizeof(svo_res) * n_samples, "gi bounce results")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow, sizeof(svo_ray) * n_samples, "gi shadow rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow_res, sizeof(svo_res) * n_samples, "gi shadow results")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_weight, 12ull * n_samples, "gi weights")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_contrib, 12ull * n_samples, "gi contributions")))
+        return rc;
+    gi_cam gc;
+    for (int i = 0; i < 3; i++) {
+        gc.pos[i] = cam->camera[i];
+        gc.bl[i] = cam->ray_bl[i]; gc.br[i] = cam->ray_br[i]; gc.tr[i] = cam->ray_tr[i]; gc.tl[i] = cam->ray_tl[i];
+    }
+    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
+    svo_ray* rays = (svo_ray*)ctx->gi_rays.ptr;
+    svo_res* res = (svo_res*)ctx->gi_res.ptr;
+    svo_ray* shadow = (svo_ray*)ctx->gi_shadow.ptr;
+    svo_res* sres = (svo_res*)ctx->gi_shadow_res.ptr;
+    float* weight = (float*)ctx->gi_weight.ptr;
+    float* contrib = (float*)ctx->gi_contrib.ptr;
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    {
+        tgx_launch_scope s(ctx, "gi_gen_k");
+        hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream, (const tgx_object*)ctx->objects.ptr,
+                           (const uint32_t*)ctx->cluster_pointers.ptr, (const uint32_t*)ctx->cluster_idx_to_object.ptr,
+                           (const uint8_t*)ctx->materials.ptr, (const tgx_lut_entry*)ctx->luts.ptr, gc, w, h, vis_dev,
+                           *prm, (gi_surf*)ctx->gi_surf.ptr, rays, contrib, cnt);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
+    if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm->max_distance, cnt + 4, res, cnt + 2))) return rc;
+    {
+        tgx_launch_scope s(ctx, "gi_shade_k");
+        hipLaunchKernelGGL(gi_shade_k, dim3((uint32_t)((n_samples + GI_BLOCK * SHADE_ITEMS - 1) / (GI_BLOCK * SHADE_ITEMS))),
+                           dim3(GI_BLOCK), 0, ctx->stream, ctx->svo_n_inner, (const uint16_t*)ctx->svo_leaf_mats.ptr,
+                           (const tgx_lut_entry*)ctx->luts.ptr, *prm, rays, cnt + 1, res, shadow, weight, cnt);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_shade_k");
+    if ((rc = launch_trace(ctx, shadow, cnt + 3, (uint32_t)n_samples, prm->max_distance, cnt + 5, sres, cnt + 7))) return rc;
+    {
+        tgx_launch_scope s(ctx, "gi_lit_k");
+        hipLaunchKernelGGL(gi_lit_k, dim3(blocks_for(n_samples)), dim3(256), 0, ctx->stream, shadow, cnt + 3, sres,
+                           weight, contrib);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_lit_k");
+    {
+        tgx_launch_scope s(ctx, "gi_resolve_k");
+        hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream,
+                           (const gi_surf*)ctx->gi_surf.ptr, contrib, (uint32_t)n_pix, prm->spp, radiance_dev,
+                           irradiance_dev);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_resolve_k");
+    return TGX_OK;
+}
+
+int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out)
+{
+    if (!ctx || !out) return TGX_ERR_INVALID_ARG;
+    if (!ctx->gi_counts.ptr) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no GI call yet");
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t c[GI_N_COUNTERS];
+    TGX_HIP_CHECK(ctx, hipMemcpy(c, ctx->gi_counts.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    out->visible_pixels = ctx->gi_spp ? c[1] / ctx->gi_spp : 0u;
+    out->bounce_rays = c[1];
+    out->bounce_hits = c[2];
+    out->shadow_rays = c[3];
+    return TGX_OK;
+}
+
+int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* params,
+                     uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev)
+{
+    int rc = tgx_visibility_device(ctx, cam, 1, w, h, vis_dev);
+    if (rc) return rc;
+    return tgx_gi_device(ctx, cam, w, h, vis_dev, params, radiance_dev, irradiance_dev);
+}
+
+} /* extern "C" */

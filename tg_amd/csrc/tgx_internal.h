@@ -68,10 +68,21 @@ struct tgx_context {
     tgx_buffer   svo_list_off;   /* n_leaves + 1                            */
     tgx_buffer   svo_list;       /* cluster idx lists                       */
     tgx_buffer   svo_frames;     /* per-cptr cluster transforms (build)     */
+    tgx_buffer   svo_inner;      /* uint2 per inner node (traversal layout) */
+    tgx_buffer   svo_bricks;     /* u64 per leaf: 8^3 bricks holding solids */
+    uint32_t     svo_n_inner = 0;/* inner nodes = index of the first leaf   */
     tgx_buffer   svo_rays;       /* staging for tgx_svo_trace               */
     tgx_buffer   svo_hits;
     tgx_buffer   prim_tmp;       /* rocPRIM scratch                         */
-    tgx_buffer   gi_counts;      /* GI ray counters (4 x u64)               */
+    tgx_buffer   gi_counts;      /* GI queue counters and trace cursors     */
+    tgx_buffer   gi_surf;        /* per-pixel albedo + visible flag         */
+    tgx_buffer   gi_rays;        /* bounce ray queue                        */
+    tgx_buffer   gi_res;         /* bounce results                          */
+    tgx_buffer   gi_shadow;      /* shadow ray queue                        */
+    tgx_buffer   gi_shadow_res;  /* shadow results                          */
+    tgx_buffer   gi_weight;      /* RGB weight per shadow ray               */
+    tgx_buffer   gi_contrib;     /* RGB contribution per (pixel, sample)    */
+    uint32_t     gi_spp = 0;     /* spp of the last GI call                 */
 
     /* Profiling. */
     bool                           profiling = false;

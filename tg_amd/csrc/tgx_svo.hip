@@ -1,5 +1,5 @@
 /*
- * tgx_svo.hip — sparse voxel octree build and traversal on gfx950.
+ * tgx_svo.hip — sparse voxel octree build on gfx950.
  *
  * Build (replaces tg_svo_create, tg_sparse_voxel_octree.c:466-542, a
  * single-threaded depth-first recursion on the host):
@@ -23,30 +23,12 @@
  *                        sampled in the reference's exact fp32 order
  *                        (:242-315); solid bits collect in LDS, the first
  *                        cluster to set a voxel writes its material.
- * Traversal (tg_svo_traverse, :558-740): one lane per ray; the reference's
- * node stack is the path from the root to the current node, so the lane
- * keeps only the current node and walks parent links to pop; node boxes are
- * recomputed exactly from (level, coord).
+ * Traversal and GI live in tgx_gi.hip.
  */
-#include "tgx_internal.h"
+#include "tgx_svo_dev.h"
 #include "tgx_m4.h"
 
 #include <algorithm>
-
-#define SVO_F32_MAX 3.402823466e+38f
-#define SVO_EPSILON 1.192092896e-07f
-#define SVO_NONE    0xFFFFFFFFu
-
-/* Node layout on the device (32 B): canonical order, parent links. */
-struct svo_gnode {
-    uint32_t first_child;
-    uint32_t parent;
-    uint32_t leaf_idx;
-    uint32_t masks;       /* valid | leaf << 8 | level << 16 */
-    uint32_t cx, cy, cz;
-    uint32_t leaf_n;
-};
-static_assert(sizeof(svo_gnode) == 32, "svo_gnode layout");
 
 /* Per-cluster-pointer transforms (rows 0..2 of each m4). */
 struct svo_frame {
@@ -58,11 +40,6 @@ struct svo_frame {
     uint32_t pad[2];
 };
 static_assert(sizeof(svo_frame) == 160, "svo_frame layout");
-
-__device__ static inline float fmin2(float a, float b) { return a < b ? a : b; }
-__device__ static inline float fmax2(float a, float b) { return a > b ? a : b; }
-
-struct f3 { float x, y, z; };
 
 /* rows 0..2 of m times (x, y, z, w), tg_math.c:2428-2438 order */
 __device__ static inline f3 mulv(const float* m, float x, float y, float z, float w)
@@ -442,385 +419,43 @@ __global__ void svo_list_compact_k(const uint32_t* __restrict__ leaf_first_pair,
     for (uint32_t i = 0; i < leaf_n[l]; i++) list[o + i] = list_tmp[b + i];
 }
 
-/* ------------------------------------------------------------------------ */
-/* Traversal                                                                */
-/* ------------------------------------------------------------------------ */
-
-struct svo_ray_result {
-    uint32_t hit;
-    float    distance;
-    uint32_t node;       /* leaf node of the hit */
-    uint32_t voxel;
-    int      ax;         /* entry face axis */
-    int      sign;       /* entry face normal sign */
-    uint32_t steps;
-};
-
-__device__ static inline bool svo_slab(f3 o, f3 d, f3 mn, f3 mx, float& enter, float& exit_)
-{
-    const float ax = (d.x == 0.0f) ? -SVO_F32_MAX : ((mn.x - o.x) / d.x);
-    const float ay = (d.y == 0.0f) ? -SVO_F32_MAX : ((mn.y - o.y) / d.y);
-    const float az = (d.z == 0.0f) ? -SVO_F32_MAX : ((mn.z - o.z) / d.z);
-    const float bx = (d.x == 0.0f) ? SVO_F32_MAX : ((mx.x - o.x) / d.x);
-    const float by = (d.y == 0.0f) ? SVO_F32_MAX : ((mx.y - o.y) / d.y);
-    const float bz = (d.z == 0.0f) ? SVO_F32_MAX : ((mx.z - o.z) / d.z);
-    const float nx = fmin2(ax, bx), ny = fmin2(ay, by), nz = fmin2(az, bz);
-    const float fx = fmax2(ax, bx), fy = fmax2(ay, by), fz = fmax2(az, bz);
-    enter = fmax2(fmax2(nx, ny), nz);
-    exit_ = fmin2(fmin2(fx, fy), fz);
-    return exit_ > 0.0f && enter <= exit_;
-}
-
-/* min over axes of max((mn - p)/d, (mx - p)/d) with the zero-direction
- * alternatives of tgm_v3_div_zero_check (tg_sparse_voxel_octree.c:704-707). */
-__device__ static inline float svo_exit(f3 p, f3 d, f3 mn, f3 mx)
-{
-    const float ax = (d.x == 0.0f) ? -SVO_F32_MAX : ((mn.x - p.x) / d.x);
-    const float ay = (d.y == 0.0f) ? -SVO_F32_MAX : ((mn.y - p.y) / d.y);
-    const float az = (d.z == 0.0f) ? -SVO_F32_MAX : ((mn.z - p.z) / d.z);
-    const float bx = (d.x == 0.0f) ? SVO_F32_MAX : ((mx.x - p.x) / d.x);
-    const float by = (d.y == 0.0f) ? SVO_F32_MAX : ((mx.y - p.y) / d.y);
-    const float bz = (d.z == 0.0f) ? SVO_F32_MAX : ((mx.z - p.z) / d.z);
-    return fmin2(fmin2(fmax2(ax, bx), fmax2(ay, by)), fmax2(az, bz));
-}
-
-/* tg_amanatides_woo over one 32^3 block (entry voxel clamped both sides). */
-__device__ static inline bool svo_block_dda(f3 h, f3 d, const uint32_t* __restrict__ grid, int& ox, int& oy, int& oz)
-{
-    int x = (int)fmin2(fmax2(floorf(h.x), 0.0f), 31.0f);
-    int y = (int)fmin2(fmax2(floorf(h.y), 0.0f), 31.0f);
-    int z = (int)fmin2(fmax2(floorf(h.z), 0.0f), 31.0f);
-    int sx = 0, sy = 0, sz = 0;
-    float tmx = SVO_F32_MAX, tmy = SVO_F32_MAX, tmz = SVO_F32_MAX;
-    float tdx = SVO_F32_MAX, tdy = SVO_F32_MAX, tdz = SVO_F32_MAX;
-    if (d.x > 0.0f) { sx = 1; tmx = ((float)(x + 1) - h.x) / d.x; tdx = 1.0f / d.x; }
-    else if (d.x < 0.0f) { sx = -1; tmx = (h.x - (float)x) / -d.x; tdx = 1.0f / -d.x; }
-    if (d.y > 0.0f) { sy = 1; tmy = ((float)(y + 1) - h.y) / d.y; tdy = 1.0f / d.y; }
-    else if (d.y < 0.0f) { sy = -1; tmy = (h.y - (float)y) / -d.y; tdy = 1.0f / -d.y; }
-    if (d.z > 0.0f) { sz = 1; tmz = ((float)(z + 1) - h.z) / d.z; tdz = 1.0f / d.z; }
-    else if (d.z < 0.0f) { sz = -1; tmz = (h.z - (float)z) / -d.z; tdz = 1.0f / -d.z; }
-    for (uint32_t it = 0; it < 4u * TGX_DDA_MAX_ITERATIONS; it++) {
-        const uint32_t v = 1024u * (uint32_t)z + 32u * (uint32_t)y + (uint32_t)x;
-        if (grid[v >> 5] & (1u << (v & 31u))) { ox = x; oy = y; oz = z; return true; }
-        if (tmx < tmy) {
-            if (tmx < tmz) { tmx += tdx; x += sx; if (x < 0 || x >= 32) break; }
-            else           { tmz += tdz; z += sz; if (z < 0 || z >= 32) break; }
-        } else {
-            if (tmy < tmz) { tmy += tdy; y += sy; if (y < 0 || y >= 32) break; }
-            else           { tmz += tdz; z += sz; if (z < 0 || z >= 32) break; }
-        }
-    }
-    return false;
-}
-
-__device__ static inline f3 node_min(const svo_gnode& n, float half, uint32_t depth)
-{
-    const float side = (float)(1u << (depth - ((n.masks >> 16) & 0xFFu)));
-    f3 r = {-half + (float)n.cx * side, -half + (float)n.cy * side, -half + (float)n.cz * side};
-    return r;
-}
-
-/* tg_svo_traverse for one ray; origin and direction in world space. */
-__device__ svo_ray_result svo_trace_ray(const svo_gnode* __restrict__ nodes, const uint32_t* __restrict__ leaf_bits,
-                                        float mnx, float mny, float mnz, uint32_t depth, f3 origin, f3 d,
-                                        float max_distance)
-{
-    svo_ray_result res = {0u, SVO_F32_MAX, SVO_NONE, 0u, 0, 0, 0u};
-    const float side = (float)(1u << depth);
-    const float half = side * 0.5f;
-    const f3 o = {origin.x - (mnx + half), origin.y - (mny + half), origin.z - (mnz + half)};
-    const f3 rmin = {-half, -half, -half}, rmax = {half, half, half};
-    float enter, exit_;
-    if (!svo_slab(o, d, rmin, rmax, enter, exit_)) return res;
-    f3 pos = o;
-    if (enter > 0.0f) { pos.x = o.x + d.x * enter; pos.y = o.y + d.y * enter; pos.z = o.z + d.z * enter; }
-    float dist = enter > 0.0f ? enter : 0.0f;
-    uint32_t cur = 0;
-    uint32_t steps = 0;
-    while (cur != SVO_NONE) {
-        if (steps >= TGX_SVO_MAX_STEPS || dist > max_distance) break;
-        steps++;
-        const svo_gnode pn = nodes[cur];
-        const f3 pmn = node_min(pn, half, depth);
-        const float ps = (float)(1u << (depth - ((pn.masks >> 16) & 0xFFu)));
-        const f3 pmx = {pmn.x + ps, pmn.y + ps, pmn.z + ps};
-        const f3 ce = {(pmx.x - pmn.x) * 0.5f, (pmx.y - pmn.y) * 0.5f, (pmx.z - pmn.z) * 0.5f};
-        uint32_t rc = 0;
-        f3 cmin = pmn;
-        f3 cmax = {cmin.x + ce.x, cmin.y + ce.y, cmin.z + ce.z};
-        if (cmax.x < pos.x || (pos.x == cmax.x && d.x > 0.0f)) { rc += 1; cmin.x += ce.x; cmax.x += ce.x; }
-        if (cmax.y < pos.y || (pos.y == cmax.y && d.y > 0.0f)) { rc += 2; cmin.y += ce.y; cmax.y += ce.y; }
-        if (cmax.z < pos.z || (pos.z == cmax.z && d.z > 0.0f)) { rc += 4; cmin.z += ce.z; cmax.z += ce.z; }
-        bool advance = true;
-        const uint32_t valid = pn.masks & 0xFFu, leafm = (pn.masks >> 8) & 0xFFu;
-        if (valid & (1u << rc)) {
-            const uint32_t ci = pn.first_child + (uint32_t)__popc(valid & ((1u << rc) - 1u));
-            if (leafm & (1u << rc)) {
-                const svo_gnode cn = nodes[ci];
-                if (cn.leaf_n != 0u) {
-                    const f3 hg = {pos.x - cmin.x, pos.y - cmin.y, pos.z - cmin.z};
-                    int vx, vy, vz;
-                    if (svo_block_dda(hg, d, leaf_bits + (size_t)cn.leaf_idx * TGX_SVO_BLOCK_WORDS, vx, vy, vz)) {
-                        const f3 vmn = {cmin.x + (float)vx, cmin.y + (float)vy, cmin.z + (float)vz};
-                        const f3 vmx = {vmn.x + 1.0f, vmn.y + 1.0f, vmn.z + 1.0f};
-                        (void)svo_slab(pos, d, vmn, vmx, enter, exit_);
-                        res.distance = dist + enter;
-                        res.hit = res.distance <= max_distance ? 1u : 0u;
-                        res.node = ci;
-                        res.voxel = 1024u * (uint32_t)vz + 32u * (uint32_t)vy + (uint32_t)vx;
-                        /* entry face: largest near-plane distance */
-                        const float nx_ = fmin2((d.x == 0.0f) ? -SVO_F32_MAX : ((vmn.x - pos.x) / d.x), (d.x == 0.0f) ? SVO_F32_MAX : ((vmx.x - pos.x) / d.x));
-                        const float ny_ = fmin2((d.y == 0.0f) ? -SVO_F32_MAX : ((vmn.y - pos.y) / d.y), (d.y == 0.0f) ? SVO_F32_MAX : ((vmx.y - pos.y) / d.y));
-                        const float nz_ = fmin2((d.z == 0.0f) ? -SVO_F32_MAX : ((vmn.z - pos.z) / d.z), (d.z == 0.0f) ? SVO_F32_MAX : ((vmx.z - pos.z) / d.z));
-                        int ax = nx_ > ny_ ? 0 : 1;
-                        const float nax = ax == 0 ? nx_ : ny_;
-                        if (!(nax > nz_)) ax = 2;
-                        const float dc = ax == 0 ? d.x : (ax == 1 ? d.y : d.z);
-                        res.ax = ax;
-                        res.sign = dc > 0.0f ? -1 : 1;
-                        res.steps = steps;
-                        return res;
-                    }
-                }
-            } else {
-                advance = false;
-                cur = ci;
-            }
-        }
-        if (advance) {
-            exit_ = svo_exit(pos, d, cmin, cmax);
-            const float t_adv = exit_ + SVO_EPSILON;
-            pos.x = pos.x + d.x * t_adv;
-            pos.y = pos.y + d.y * t_adv;
-            pos.z = pos.z + d.z * t_adv;
-            dist += t_adv;
-            /* pop every node the position has left: walk parent links */
-            while (cur != SVO_NONE) {
-                const svo_gnode nn = nodes[cur];
-                const f3 nmn = node_min(nn, half, depth);
-                const float ns = (float)(1u << (depth - ((nn.masks >> 16) & 0xFFu)));
-                const f3 nmx = {nmn.x + ns, nmn.y + ns, nmn.z + ns};
-                if (svo_exit(pos, d, nmn, nmx) > SVO_EPSILON) break;
-                cur = nn.parent;
-            }
-        }
-    }
-    res.steps = steps;
-    return res;
-}
-
-__global__ __launch_bounds__(256) void svo_trace_k(const svo_gnode* __restrict__ nodes,
-                                                   const uint32_t* __restrict__ leaf_bits, float mnx, float mny,
-                                                   float mnz, uint32_t depth, const float* __restrict__ origins,
-                                                   const float* __restrict__ dirs, uint32_t n, float max_distance,
-                                                   tgx_svo_hit* __restrict__ out)
+/* Compact inner nodes for traversal (8 B): x = first child, y = parent
+ * (24 bits, 0xFFFFFF for the root) | child-valid mask << 24. */
+__global__ void svo_inner_k(const svo_gnode* __restrict__ nodes, uint32_t n_inner, uint2* __restrict__ inner)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const f3 o = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
-    const f3 d = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
-    const svo_ray_result r = svo_trace_ray(nodes, leaf_bits, mnx, mny, mnz, depth, o, d, max_distance);
-    tgx_svo_hit h;
-    h.hit = r.hit;
-    h.distance = r.distance;
-    h.block[0] = h.block[1] = h.block[2] = 0u;
-    h.voxel = r.voxel;
-    h.normal[0] = h.normal[1] = h.normal[2] = 0;
-    if (r.node != SVO_NONE) {
-        const svo_gnode cn = nodes[r.node];
-        h.block[0] = cn.cx; h.block[1] = cn.cy; h.block[2] = cn.cz;
-        h.normal[r.ax] = r.sign;
-    }
-    h.steps = r.steps;
-    out[i] = h;
+    if (i >= n_inner) return;
+    const svo_gnode n = nodes[i];
+    inner[i] = make_uint2(n.first_child, (n.parent & 0xFFFFFFu) | ((n.masks & 0xFFu) << 24));
 }
 
-/* ------------------------------------------------------------------------ */
-/* One-bounce GI (estimator documented at tgx_gi_params in tgx.h)            */
-/* ------------------------------------------------------------------------ */
-
-#define GI_SQRT3 1.73205080757f
-#define GI_SHADOW_OFFSET 0.015625f
-#define GI_INV_PI 0.318309886f
-
-struct gi_cam {
-    float pos[3];
-    float bl[3], br[3], tr[3], tl[3];
-};
-
-/* MurmurHash3 finaliser (util.inc:47-56) */
-__device__ static inline uint32_t gi_fmix32(uint32_t v)
+/* Per leaf, one bit per 8^3 brick (bx + 4 by + 16 bz) that holds a solid
+ * voxel. One wave per leaf, one lane per brick. */
+__global__ __launch_bounds__(64) void svo_bricks_k(const uint32_t* __restrict__ leaf_bits,
+                                                   unsigned long long* __restrict__ bricks)
 {
-    v ^= v >> 16; v *= 0x85ebca6bu; v ^= v >> 13; v *= 0xc2b2ae35u; v ^= v >> 16;
-    return v;
-}
-/* xorshift32 and its [lo, hi] float (util.inc:11-31) */
-__device__ static inline float gi_rand(uint32_t& s, float lo, float hi)
-{
-    uint32_t r = s;
-    r ^= r << 13; r ^= r >> 17; r ^= r << 5;
-    s = r;
-    const float u = (float)r / (float)0xFFFFFFFFu;
-    return u * (hi - lo) + lo;
-}
-__device__ static inline f3 gi_cosine_direction(f3 n, uint32_t& st)
-{
-    for (int attempt = 0; attempt < 32; attempt++) {
-        const float x = gi_rand(st, -1.0f, 1.0f);
-        const float y = gi_rand(st, -1.0f, 1.0f);
-        const float z = gi_rand(st, -1.0f, 1.0f);
-        const float l2 = x * x + y * y + z * z;
-        if (l2 > 1.0f || l2 < 1e-6f) continue;
-        const float l = sqrtf(l2);
-        const float dx = n.x + x / l, dy = n.y + y / l, dz = n.z + z / l;
-        const float dl2 = dx * dx + dy * dy + dz * dz;
-        if (dl2 < 1e-6f) continue;
-        const float dl = sqrtf(dl2);
-        f3 r = {dx / dl, dy / dl, dz / dl};
-        return r;
-    }
-    return n;
-}
-
-__device__ static inline float gi_sign(float a) { return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f); }
-
-/* Primary surface of a visibility word (shading.frag:116-229). */
-__device__ static bool gi_surface(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
-                                  const uint32_t* __restrict__ c2o, const uint8_t* __restrict__ materials,
-                                  const tgx_lut_entry* __restrict__ luts, const gi_cam& cam, uint32_t W, uint32_t H,
-                                  uint32_t px, uint32_t py, uint64_t word, f3& p, f3& n, f3& albedo)
-{
-    const float depth = (float)(word >> 40) / 16777215.0f;
-    if (!(depth < 1.0f)) return false;
-    const uint32_t cptr = (uint32_t)(word >> 9) & 0x7FFFFFFFu;
-    const uint32_t v = (uint32_t)word & 511u;
-    const uint32_t cidx = cptrs[cptr];
-    const tgx_object ob = objects[c2o[cidx]];
-    const uint32_t rel = cptr - ob.first_cluster_pointer;
-    const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
-    const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
-    const tgx_m4 w2c = tgx_world_to_cluster(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
-    const tgx_v4 o4 = tgx_m4_mulv4(w2c, cam.pos[0], cam.pos[1], cam.pos[2], 1.0f);
-    const float fx = ((float)px + 0.5f) / (float)W;
-    const float fy = 1.0f - ((float)py + 0.5f) / (float)H;
-    const float omy = 1.0f - fy, omx = 1.0f - fx;
-    const float lx = cam.bl[0] * omy + cam.tl[0] * fy, ly = cam.bl[1] * omy + cam.tl[1] * fy, lz = cam.bl[2] * omy + cam.tl[2] * fy;
-    const float qx = cam.br[0] * omy + cam.tr[0] * fy, qy = cam.br[1] * omy + cam.tr[1] * fy, qz = cam.br[2] * omy + cam.tr[2] * fy;
-    const float dwx = lx * omx + qx * fx, dwy = ly * omx + qy * fx, dwz = lz * omx + qz * fx;
-    const tgx_v4 d4 = tgx_m4_mulv4(w2c, dwx, dwy, dwz, 0.0f);
-    const float dm = sqrtf(d4.x * d4.x + d4.y * d4.y + d4.z * d4.z);
-    const f3 dms = {d4.x / dm, d4.y / dm, d4.z / dm};
-    const f3 orig = {o4.x, o4.y, o4.z};
-    const f3 vmn = {(float)(v % 8u), (float)((v / 8u) % 8u), (float)(v / 64u)};
-    const f3 vmx = {vmn.x + 1.0f, vmn.y + 1.0f, vmn.z + 1.0f};
-    float enter, exit_;
-    if (!svo_slab(orig, dms, vmn, vmx, enter, exit_)) return false;
-    f3 hit = orig;
-    if (enter > 0.0f) { hit.x = orig.x + dms.x * enter; hit.y = orig.y + dms.y * enter; hit.z = orig.z + dms.z * enter; }
-    f3 nn = {hit.x - (vmn.x + 0.5f), hit.y - (vmn.y + 0.5f), hit.z - (vmn.z + 0.5f)};
-    const float ax = fabsf(nn.x), ay = fabsf(nn.y), az = fabsf(nn.z);
-    if (ax > ay) {
-        nn.y = 0.0f;
-        if (ax > az) { nn.x = gi_sign(nn.x); nn.z = 0.0f; } else { nn.z = gi_sign(nn.z); nn.x = 0.0f; }
-    } else {
-        nn.x = 0.0f;
-        if (ay > az) { nn.y = gi_sign(nn.y); nn.z = 0.0f; } else { nn.z = gi_sign(nn.z); nn.y = 0.0f; }
-    }
-    tgx_m4 rot;
-    for (int i = 0; i < 16; i++) rot.m[i] = ob.rotation[i];
-    const tgx_v4 n4 = tgx_m4_mulv4(rot, nn.x, nn.y, nn.z, 0.0f);
-    const float nm = sqrtf(n4.x * n4.x + n4.y * n4.y + n4.z * n4.z);
-    n.x = n4.x / nm; n.y = n4.y / nm; n.z = n4.z / nm;
-    const tgx_m4 c2w = tgx_cluster_to_world(ob.rotation, ob.translation, ob.n_cluster_pointers_per_dim, rx, ry, rz);
-    const tgx_v4 p4 = tgx_m4_mulv4(c2w, hit.x, hit.y, hit.z, 1.0f);
-    p.x = p4.x; p.y = p4.y; p.z = p4.z;
-    const uint32_t mat = materials ? (uint32_t)materials[(size_t)cidx * 512u + v] : 0u;
-    const tgx_lut_entry e = luts[(size_t)ob.lut_idx * TGX_LUT_ENTRIES + mat];
-    albedo.x = e.albedo[0]; albedo.y = e.albedo[1]; albedo.z = e.albedo[2];
-    return true;
-}
-
-__device__ static inline unsigned long long gi_wave_sum(unsigned long long v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    return v;
-}
-
-__global__ __launch_bounds__(256) void gi_pixel_k(const tgx_object* __restrict__ objects,
-                                                  const uint32_t* __restrict__ cptrs, const uint32_t* __restrict__ c2o,
-                                                  const uint8_t* __restrict__ materials,
-                                                  const tgx_lut_entry* __restrict__ luts,
-                                                  const svo_gnode* __restrict__ nodes,
-                                                  const uint32_t* __restrict__ leaf_bits,
-                                                  const uint16_t* __restrict__ leaf_mats, float smx, float smy,
-                                                  float smz, uint32_t depth, gi_cam cam, uint32_t W, uint32_t H,
-                                                  const uint64_t* __restrict__ vis, tgx_gi_params prm,
-                                                  float* __restrict__ radiance, float* __restrict__ irradiance,
-                                                  unsigned long long* __restrict__ counts)
-{
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n_vis = 0, n_b = 0, n_bh = 0, n_sh = 0;
-    if (pix < W * H) {
-        const uint32_t px = pix % W, py = pix / W;
-        f3 p, n, alb;
-        float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
-        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
-        if (gi_surface(objects, cptrs, c2o, materials, luts, cam, W, H, px, py, vis[pix], p, n, alb)) {
-            n_vis = 1;
-            const f3 sun = {prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2]};
-            for (uint32_t k = 0; k < prm.spp; k++) {
-                uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
-                if (st == 0u) st = 0x6D2B79F5u;
-                const f3 d = gi_cosine_direction(n, st);
-                const f3 ob = {p.x + GI_SQRT3 * n.x, p.y + GI_SQRT3 * n.y, p.z + GI_SQRT3 * n.z};
-                const svo_ray_result hb = svo_trace_ray(nodes, leaf_bits, smx, smy, smz, depth, ob, d, prm.max_distance);
-                n_b++;
-                if (!hb.hit) continue;
-                n_bh++;
-                if (hb.distance < 0.0f) continue; /* started inside solid: occluded */
-                const float sc = hb.ax == 0 ? sun.x : (hb.ax == 1 ? sun.y : sun.z);
-                const float cosv = (float)hb.sign * sc;
-                if (!(cosv > 0.0f)) continue;
-                f3 os = {ob.x + d.x * hb.distance, ob.y + d.y * hb.distance, ob.z + d.z * hb.distance};
-                const float off = (float)hb.sign * GI_SHADOW_OFFSET;
-                if (hb.ax == 0) os.x = os.x + off; else if (hb.ax == 1) os.y = os.y + off; else os.z = os.z + off;
-                const svo_ray_result hs = svo_trace_ray(nodes, leaf_bits, smx, smy, smz, depth, os, sun, prm.max_distance);
-                n_sh++;
-                if (hs.hit) continue;
-                const uint16_t m = leaf_mats[(size_t)nodes[hb.node].leaf_idx * TGX_SVO_BLOCK_VOXELS + hb.voxel];
-                const tgx_lut_entry e = luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
-                const float wgt = cosv * prm.sun_radiance * GI_INV_PI;
-                acc0 = acc0 + e.albedo[0] * wgt;
-                acc1 = acc1 + e.albedo[1] * wgt;
-                acc2 = acc2 + e.albedo[2] * wgt;
-            }
-            const float inv = 1.0f / (float)prm.spp;
-            e0 = acc0 * inv; e1 = acc1 * inv; e2 = acc2 * inv;
-            r0 = alb.x * e0; r1 = alb.y * e1; r2 = alb.z * e2;
-        }
-        radiance[3 * (size_t)pix] = r0; radiance[3 * (size_t)pix + 1] = r1; radiance[3 * (size_t)pix + 2] = r2;
-        if (irradiance) { irradiance[3 * (size_t)pix] = e0; irradiance[3 * (size_t)pix + 1] = e1; irradiance[3 * (size_t)pix + 2] = e2; }
-    }
-    const unsigned long long a = gi_wave_sum(n_vis), b = gi_wave_sum(n_b), c = gi_wave_sum(n_bh), d = gi_wave_sum(n_sh);
-    if ((threadIdx.x & 63u) == 0u) {
-        if (a) atomicAdd(&counts[0], a);
-        if (b) atomicAdd(&counts[1], b);
-        if (c) atomicAdd(&counts[2], c);
-        if (d) atomicAdd(&counts[3], d);
-    }
+    const uint32_t t = threadIdx.x;
+    const uint32_t bx = t & 3u, by = (t >> 2) & 3u, bz = t >> 4;
+    const uint32_t* g = leaf_bits + (size_t)blockIdx.x * TGX_SVO_BLOCK_WORDS;
+    uint32_t acc = 0;
+    for (uint32_t z = 8u * bz; z < 8u * bz + 8u; z++)
+        for (uint32_t y = 8u * by; y < 8u * by + 8u; y++) acc |= (g[32u * z + y] >> (8u * bx)) & 0xFFu;
+    const unsigned long long m = __ballot(acc != 0u);
+    if (t == 0) bricks[blockIdx.x] = m;
 }
 
 /* ------------------------------------------------------------------------ */
 /* Host side                                                                */
 /* ------------------------------------------------------------------------ */
 
-static inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
-
 void tgx_svo_release(tgx_context* ctx)
 {
     tgx_buffer* bufs[] = {&ctx->svo_nodes, &ctx->svo_leaf_bits, &ctx->svo_leaf_mats, &ctx->svo_leaf_n,
                           &ctx->svo_list_off, &ctx->svo_list, &ctx->svo_frames, &ctx->svo_rays,
-                          &ctx->svo_hits, &ctx->prim_tmp, &ctx->gi_counts};
+                          &ctx->svo_hits, &ctx->prim_tmp, &ctx->gi_counts, &ctx->gi_surf,
+                          &ctx->gi_rays, &ctx->gi_res, &ctx->gi_shadow, &ctx->gi_shadow_res,
+                          &ctx->gi_weight, &ctx->gi_contrib, &ctx->svo_inner,
+                          &ctx->svo_bricks};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     ctx->svo_built = false;
 }
@@ -965,6 +600,8 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
     }
     const uint32_t n_leaves = lvl_nodes[leaf_level];
 
+    if (total_nodes >= 0xFFFFFFu)
+        return tgx_fail(ctx, TGX_ERR_CAPACITY, "the SVO has %u nodes; traversal holds node indices in 24 bits", total_nodes);
     /* Nodes: root, then each level linked to its parents. */
     if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_nodes, sizeof(svo_gnode) * (size_t)total_nodes, "svo nodes"))) return rc;
     svo_gnode* nodes = (svo_gnode*)ctx->svo_nodes.ptr;
@@ -1031,7 +668,21 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
                            (const uint32_t*)ctx->svo_list_off.ptr, n_leaves, list_tmp.as<uint32_t>(),
                            (uint32_t*)ctx->svo_list.ptr);
     TGX_LAUNCH_CHECK(ctx, "svo_list_compact_k");
+
+    /* Traversal layout: compact inner nodes and leaf brick masks. */
+    const uint32_t n_inner = lvl_base[leaf_level];
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_inner, 8ull * n_inner, "svo inner nodes")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_bricks, 8ull * nl, "svo leaf bricks")))
+        return rc;
+    hipLaunchKernelGGL(svo_inner_k, dim3(blocks_for(n_inner)), dim3(256), 0, ctx->stream, (const svo_gnode*)nodes,
+                       n_inner, (uint2*)ctx->svo_inner.ptr);
+    TGX_LAUNCH_CHECK(ctx, "svo_inner_k");
+    if (n_leaves)
+        hipLaunchKernelGGL(svo_bricks_k, dim3(n_leaves), dim3(64), 0, ctx->stream, (const uint32_t*)ctx->svo_leaf_bits.ptr,
+                           (unsigned long long*)ctx->svo_bricks.ptr);
+    TGX_LAUNCH_CHECK(ctx, "svo_bricks_k");
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->svo_n_inner = n_inner;
 
     ctx->svo_info.min[0] = mn[0]; ctx->svo_info.min[1] = mn[1]; ctx->svo_info.min[2] = mn[2];
     ctx->svo_info.depth = depth;
@@ -1082,89 +733,6 @@ int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes_out, uint32_t* leaf_bit
     if (list && in.n_list_entries)
         TGX_HIP_CHECK(ctx, hipMemcpy(list, ctx->svo_list.ptr, 4ull * in.n_list_entries, hipMemcpyDeviceToHost));
     return TGX_OK;
-}
-
-int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n, float max_distance,
-                  tgx_svo_hit* out_host)
-{
-    if (!ctx || (n && (!origins || !directions || !out_host))) return TGX_ERR_INVALID_ARG;
-    ctx->last_error.clear();
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
-    if (n == 0) return TGX_OK;
-    int rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, 24ull * n, "svo rays")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->svo_hits, sizeof(tgx_svo_hit) * (size_t)n, "svo hits")))
-        return rc;
-    float* d_o = (float*)ctx->svo_rays.ptr;
-    float* d_d = d_o + 3ull * n;
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_o, origins, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_d, directions, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
-    {
-        tgx_launch_scope s(ctx, "svo_trace_k");
-        hipLaunchKernelGGL(svo_trace_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (const svo_gnode*)ctx->svo_nodes.ptr,
-                           (const uint32_t*)ctx->svo_leaf_bits.ptr, ctx->svo_info.min[0], ctx->svo_info.min[1],
-                           ctx->svo_info.min[2], ctx->svo_info.depth, d_o, d_d, n, max_distance,
-                           (tgx_svo_hit*)ctx->svo_hits.ptr);
-    }
-    TGX_LAUNCH_CHECK(ctx, "svo_trace_k");
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, ctx->svo_hits.ptr, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
-    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return TGX_OK;
-}
-
-int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
-                  const tgx_gi_params* prm, float* radiance_dev, float* irradiance_dev)
-{
-    if (!ctx || !cam || !vis_dev || !prm || !radiance_dev) return TGX_ERR_INVALID_ARG;
-    ctx->last_error.clear();
-    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
-    if (!ctx->luts.ptr || ctx->n_luts == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "the scene has no LUTs");
-    if (w == 0 || h == 0 || (uint64_t)w * h >= (1ull << 31)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad resolution");
-    if (prm->spp < 1 || prm->spp > 64 || !(prm->max_distance > 0.0f))
-        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "spp must be in [1, 64] and max_distance > 0");
-    int rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 64, "gi counts"))) return rc;
-    gi_cam gc;
-    for (int i = 0; i < 3; i++) {
-        gc.pos[i] = cam->camera[i];
-        gc.bl[i] = cam->ray_bl[i]; gc.br[i] = cam->ray_br[i]; gc.tr[i] = cam->ray_tr[i]; gc.tl[i] = cam->ray_tl[i];
-    }
-    hipLaunchKernelGGL(svo_fill_k, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->gi_counts.ptr, 8ull, 0u);
-    {
-        tgx_launch_scope s(ctx, "gi_pixel_k");
-        hipLaunchKernelGGL(gi_pixel_k, dim3(blocks_for((uint64_t)w * h)), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
-                           (const tgx_lut_entry*)ctx->luts.ptr, (const svo_gnode*)ctx->svo_nodes.ptr,
-                           (const uint32_t*)ctx->svo_leaf_bits.ptr, (const uint16_t*)ctx->svo_leaf_mats.ptr,
-                           ctx->svo_info.min[0], ctx->svo_info.min[1], ctx->svo_info.min[2], ctx->svo_info.depth, gc,
-                           w, h, vis_dev, *prm, radiance_dev, irradiance_dev, (unsigned long long*)ctx->gi_counts.ptr);
-    }
-    TGX_LAUNCH_CHECK(ctx, "gi_pixel_k");
-    return TGX_OK;
-}
-
-int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out)
-{
-    if (!ctx || !out) return TGX_ERR_INVALID_ARG;
-    if (!ctx->gi_counts.ptr) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no GI call yet");
-    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    unsigned long long c[4];
-    TGX_HIP_CHECK(ctx, hipMemcpy(c, ctx->gi_counts.ptr, sizeof(c), hipMemcpyDeviceToHost));
-    out->visible_pixels = c[0];
-    out->bounce_rays = c[1];
-    out->bounce_hits = c[2];
-    out->shadow_rays = c[3];
-    return TGX_OK;
-}
-
-int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* params,
-                     uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev)
-{
-    int rc = tgx_visibility_device(ctx, cam, 1, w, h, vis_dev);
-    if (rc) return rc;
-    return tgx_gi_device(ctx, cam, w, h, vis_dev, params, radiance_dev, irradiance_dev);
 }
 
 } /* extern "C" */
