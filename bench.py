@@ -54,6 +54,88 @@ def trace_alg_bytes(gst, info) -> int:
     return 56 * rays + 2 * (32 * int(info["n_nodes"]) + 4096 * int(info["n_leaves"]))
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+NRC_FWD_FLOPS = 2 * (78 * 64 + 3 * 64 * 64 + 64 * 3)   # per sample, unpadded
+NRC_TRAIN_FLOPS = 3 * NRC_FWD_FLOPS                     # forward + two backward products
+
+
+def rel_mse(x, ref) -> float:
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(((x - ref) ** 2).sum() / (ref ** 2).sum())
+
+
+def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, plain_ms: float,
+                 plain_rays: int) -> dict:
+    """Config 3: the neural irradiance cache on the same view. Error of both
+    forms against a 256-spp FP64-accumulated reference (64 independent 4-spp
+    frames of the plain tracer); a second, independent 256-spp estimate
+    gives the reference's own noise floor."""
+    def irr_frame(frame: int) -> np.ndarray:
+        ctx.frame_device(cam, W, H, T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=frame), vis_d, rad_d, irr_d)
+        out = np.empty((H, W, 3), np.float32)
+        ctx.d2h(out, irr_d)
+        return out
+
+    refs = []
+    for base in (1000, 5000):
+        acc = np.zeros((H, W, 3))
+        for f in range(64):
+            acc += irr_frame(base + f)
+        refs.append(acc / 64)
+    ref, ref2 = refs
+    vis = np.empty((H, W), np.uint64)
+    ctx.d2h(vis, vis_d)
+    mask = vis != T.VISIBILITY_EMPTY
+    plain = irr_frame(7)
+    cfg = T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16, train_batch=1 << 12,
+                       train_spp=4, fallback_threshold=1.0, fallback_min_samples=1, cell_size=8)
+    ctx.nrc_create(cfg)
+    prm = T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=0)
+    for _ in range(pretrain):
+        ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_stats()
+    ctx.profile(False)
+    st = ctx.nrc_stats()
+    irr = np.empty((H, W, 3), np.float32)
+    ctx.d2h(irr, irr_d)
+    ms = 1e3 * elapsed / steps
+    k_ms = {k: v[1] / steps for k, v in prof.items()}
+    visible = int(st["visible_pixels"])
+    infer_ms = k_ms.get("nrc_infer_k", 0.0)
+    train_ms = sum(k_ms.get(k, 0.0) for k in ("nrc_fwdbwd_k", "nrc_wgrad_k", "nrc_adam_k"))
+    infer_tflops = NRC_FWD_FLOPS * visible / (infer_ms * 1e-3) / 1e12 if infer_ms else None
+    train_tflops = NRC_TRAIN_FLOPS * int(st["train_samples"]) / (train_ms * 1e-3) / 1e12 if train_ms else None
+    traced = W * H + int(st["train_rays"]) + int(st["fallback_rays"])
+    return {
+        "frame_ms": ms,
+        "plain_frame_ms": plain_ms,
+        "speedup_vs_plain": plain_ms / ms,
+        "rays_traced_per_frame": traced,
+        "rays_per_s_equivalent": plain_rays / (ms * 1e-3),
+        "rel_mse_vs_256spp": {"neural": rel_mse(irr[mask], ref[mask]), "plain_4spp": rel_mse(plain[mask], ref[mask]),
+                              "reference_noise_floor": rel_mse(ref2[mask], ref[mask])},
+        "visible_pixels": visible, "fallback_pixels": int(st["fallback_pixels"]),
+        "train_samples": int(st["train_samples"]), "adam_steps_total": int(st["adam_steps"]),
+        "train_loss": float(st["train_loss"]), "pretrain_frames": pretrain,
+        "kernels_ms_per_step": {k: v for k, v in k_ms.items() if k.startswith("nrc_")},
+        "mfma": {"infer_tflops": infer_tflops, "train_tflops": train_tflops, "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                 "infer_frac": infer_tflops / MFMA_BF16_PEAK_TFLOPS if infer_tflops else None,
+                 "train_frac": train_tflops / MFMA_BF16_PEAK_TFLOPS if train_tflops else None,
+                 "flops_per_sample": {"forward": NRC_FWD_FLOPS, "train": NRC_TRAIN_FLOPS}},
+        "config": {"network": "96(79 used)->64x4->3 ReLU, bf16 MFMA, fp32 Adam lr 1e-3", "train_samples": 1 << 16,
+                   "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512"},
+    }
+
+
 def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 12.0) -> dict:
     import oracle  # the CPU baseline leg is the only bench code that uses the oracle
 
@@ -99,6 +181,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", choices=("frame", "visibility"), default="frame")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-neural", action="store_true", help="skip the neural-cache measurement")
+    ap.add_argument("--neural-pretrain", type=int, default=200, help="untimed cache training frames")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would render whole "
@@ -199,6 +283,9 @@ def main() -> int:
     }
     if gst is not None:
         line["gi"] = {k: int(gst[k]) for k in gst.dtype.names}
+        if not args.no_neural:
+            line["neural"] = neural_bench(ctx, cam, T.seeded_sun(args.seed), vis_d, rad_d, irr_d, args.steps,
+                                          args.neural_pretrain, ms_per_step, rays_per_frame)
     for p in (vis_d, rad_d, irr_d):
         ctx.free(p)
     ctx.close()

@@ -320,6 +320,87 @@ int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
                      float* irradiance_dev);
 
 /* ------------------------------------------------------------------------ */
+/* Neural irradiance cache. No counterpart in the reference: a small MLP    */
+/* that predicts the irradiance E of tgx_gi_device from the primary hit,    */
+/* trained online on samples the SVO tracer generates, with the tracer as   */
+/* its fallback.                                                            */
+/*                                                                          */
+/* Network: 96 input features -> 64 -> 64 -> 64 -> 64 -> 3, ReLU after the  */
+/* four hidden layers, no biases, linear output. Features: for axis a and   */
+/* frequency k in 0..11, sin and cos of 2^k pi p_hat[a] at 24a + 2k (+1);   */
+/* normal (72..74), albedo (75..77), constant 1 (78), zeros (79..95), with  */
+/* p_hat = (p - origin) * scale in fp32. Weights and activations are BF16   */
+/* on MFMA with FP32 accumulation; master weights and Adam are FP32; the    */
+/* loss is the mean squared error over samples and RGB channels.            */
+/* ------------------------------------------------------------------------ */
+
+#define TGX_NRC_INPUTS      96u
+#define TGX_NRC_HIDDEN      64u
+#define TGX_NRC_N_PARAMS    18624u  /* W0[96][64], W1..W3[64][64], W4[64][3], row-major [in][out] */
+#define TGX_NRC_BATCH_ALIGN 512u    /* train_batch granularity */
+
+typedef struct tgx_nrc_config {
+    float    origin[3];             /* position encoding: p_hat = (p - origin) * scale */
+    float    scale;
+    float    learning_rate;         /* Adam (1e-3) */
+    float    beta1;                 /* 0.9 */
+    float    beta2;                 /* 0.99 */
+    float    epsilon;               /* 1e-8 */
+    uint32_t seed;                  /* weight initialisation */
+    uint32_t train_samples;         /* per frame of tgx_nrc_frame_device (65536) */
+    uint32_t train_batch;           /* samples per Adam step, multiple of 512 (16384) */
+    uint32_t train_spp;             /* SVO bounce samples per training target (4) */
+    float    fallback_threshold;    /* cell relative squared error above which pixels are traced */
+    uint32_t fallback_min_samples;  /* cells with fewer (decayed) training samples are traced */
+    uint32_t cell_size;             /* edge of an error-estimate cell in voxels, power of two */
+    uint32_t flags;                 /* TGX_NRC_FLAG_* */
+} tgx_nrc_config;
+
+#define TGX_NRC_FLAG_NO_FALLBACK    1u  /* never trace: every visible pixel uses the cache */
+#define TGX_NRC_FLAG_ALL_FALLBACK   2u  /* trace every visible pixel (plain GI through this path) */
+#define TGX_NRC_FLAG_NO_TRAINING    4u  /* frames run inference only */
+
+typedef struct tgx_nrc_stats {
+    uint64_t visible_pixels;
+    uint64_t fallback_pixels;       /* traced by the SVO tracer this frame */
+    uint64_t train_samples;
+    uint64_t train_rays;            /* bounce + shadow rays traced for training targets */
+    uint64_t fallback_rays;         /* bounce + shadow rays traced for fallback pixels */
+    uint32_t adam_steps;            /* total since tgx_nrc_create */
+    float    train_loss;            /* mean loss of the frame's training batches */
+} tgx_nrc_stats;
+
+/* (Re)creates the cache: He-uniform weights from cfg->seed, zero Adam state
+ * and an empty error estimate. */
+int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg);
+
+/* Master weights in the TGX_NRC_N_PARAMS layout (host arrays). set also
+ * refreshes the BF16 copies; Adam state is kept. */
+int tgx_nrc_get_params(tgx_context* ctx, float* params);
+int tgx_nrc_set_params(tgx_context* ctx, const float* params);
+
+/* Inference on n host samples of 9 floats (position, normal, albedo) into
+ * out[3 n] (RGB irradiance). Synchronous. */
+int tgx_nrc_infer(tgx_context* ctx, const float* samples, uint32_t n, float* out);
+
+/* One Adam step on n host samples (9 floats each) with targets[3 n]. The
+ * mean loss goes to *loss and, if grad is not null, the gradient (before
+ * the step, TGX_NRC_N_PARAMS floats) to grad. Synchronous. */
+int tgx_nrc_train_batch(tgx_context* ctx, const float* samples, const float* targets, uint32_t n,
+                        float* loss, float* grad);
+
+/* One frame with the cache: primary visibility into vis_dev, the surface of
+ * every visible pixel, cfg->train_samples training targets traced with the
+ * SVO estimator (gi->spp replaced by train_spp) at visible pixels chosen by
+ * a hash of (sample, frame), Adam steps of train_batch samples, inference
+ * for every visible pixel, and the plain SVO estimator (gi->spp) for pixels
+ * whose error-estimate cell is unreliable. radiance = albedo * E. */
+int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h,
+                         const tgx_gi_params* gi, uint64_t* vis_dev, float* radiance_dev,
+                         float* irradiance_dev);
+int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and profiling helpers (for callers without their own       */
 /* allocator; the reference's equivalents are tgvk_memory.c:182-240)        */
 /* ------------------------------------------------------------------------ */

@@ -69,6 +69,34 @@ def seeded_sun(seed: int) -> np.ndarray:
     az, el = rng.uniform(0, 2 * np.pi), np.radians(rng.uniform(25, 65))
     return np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
 
+NRC_N_PARAMS = 18624
+NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learning_rate", "<f4"),
+                             ("beta1", "<f4"), ("beta2", "<f4"), ("epsilon", "<f4"), ("seed", "<u4"),
+                             ("train_samples", "<u4"), ("train_batch", "<u4"), ("train_spp", "<u4"),
+                             ("fallback_threshold", "<f4"), ("fallback_min_samples", "<u4"),
+                             ("cell_size", "<u4"), ("flags", "<u4")])
+NRC_STATS_DTYPE = np.dtype([("visible_pixels", "<u8"), ("fallback_pixels", "<u8"), ("train_samples", "<u8"),
+                            ("train_rays", "<u8"), ("fallback_rays", "<u8"), ("adam_steps", "<u4"),
+                            ("train_loss", "<f4")])
+NRC_FLAG_NO_FALLBACK = 1
+NRC_FLAG_ALL_FALLBACK = 2
+NRC_FLAG_NO_TRAINING = 4
+
+
+def nrc_config(origin=(0.0, 0.0, 0.0), scale: float = 1.0 / 512, learning_rate: float = 1e-3, beta1: float = 0.9,
+               beta2: float = 0.99, epsilon: float = 1e-8, seed: int = 1, train_samples: int = 1 << 16,
+               train_batch: int = 1 << 12, train_spp: int = 4, fallback_threshold: float = 1.0,
+               fallback_min_samples: int = 1, cell_size: int = 8, flags: int = 0) -> np.ndarray:
+    c = np.zeros(1, NRC_CONFIG_DTYPE)
+    c["origin"] = origin
+    for k, v in (("scale", scale), ("learning_rate", learning_rate), ("beta1", beta1), ("beta2", beta2),
+                 ("epsilon", epsilon), ("seed", seed), ("train_samples", train_samples),
+                 ("train_batch", train_batch), ("train_spp", train_spp), ("fallback_threshold", fallback_threshold),
+                 ("fallback_min_samples", fallback_min_samples), ("cell_size", cell_size), ("flags", flags)):
+        c[k] = v
+    return c
+
+
 VISIBILITY_EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 GEN_SINGLE_BERNOULLI = 0
@@ -158,6 +186,13 @@ def load_library() -> ctypes.CDLL:
         "tgx_gi_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
         "tgx_gi_last_stats": (i32, [vp, vp]),
         "tgx_frame_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
+        "tgx_nrc_create": (i32, [vp, vp]),
+        "tgx_nrc_get_params": (i32, [vp, vp]),
+        "tgx_nrc_set_params": (i32, [vp, vp]),
+        "tgx_nrc_infer": (i32, [vp, vp, u32, vp]),
+        "tgx_nrc_train_batch": (i32, [vp, vp, vp, u32, vp, vp]),
+        "tgx_nrc_frame_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
+        "tgx_nrc_last_stats": (i32, [vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -393,6 +428,73 @@ class Context:
         st = np.zeros(1, GI_STATS_DTYPE)
         self._check(self.lib.tgx_gi_last_stats(self.h, _ptr(st)), "tgx_gi_last_stats")
         return st[0]
+
+    # -- neural irradiance cache ----------------------------------------------
+
+    def nrc_create(self, config: np.ndarray) -> None:
+        config = np.ascontiguousarray(config, dtype=NRC_CONFIG_DTYPE)
+        self._check(self.lib.tgx_nrc_create(self.h, _ptr(config)), "tgx_nrc_create")
+
+    def nrc_params(self) -> np.ndarray:
+        p = np.zeros(NRC_N_PARAMS, np.float32)
+        self._check(self.lib.tgx_nrc_get_params(self.h, _ptr(p)), "tgx_nrc_get_params")
+        return p
+
+    def nrc_set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == NRC_N_PARAMS
+        self._check(self.lib.tgx_nrc_set_params(self.h, _ptr(p)), "tgx_nrc_set_params")
+
+    @staticmethod
+    def _nrc_samples(p, n, albedo) -> np.ndarray:
+        return np.ascontiguousarray(np.concatenate([np.asarray(p, np.float32).reshape(-1, 3),
+                                                    np.asarray(n, np.float32).reshape(-1, 3),
+                                                    np.asarray(albedo, np.float32).reshape(-1, 3)], 1))
+
+    def nrc_infer(self, p, n, albedo) -> np.ndarray:
+        """Cache irradiance [N, 3] (clamped at 0) for positions, normals, albedos [N, 3]."""
+        x = self._nrc_samples(p, n, albedo)
+        out = np.zeros((x.shape[0], 3), np.float32)
+        self._check(self.lib.tgx_nrc_infer(self.h, _ptr(x), x.shape[0], _ptr(out)), "tgx_nrc_infer")
+        return out
+
+    def nrc_train_batch(self, p, n, albedo, targets, want_grad: bool = False):
+        """One Adam step; returns (loss, gradient before the step or None)."""
+        x = self._nrc_samples(p, n, albedo)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = ctypes.c_float()
+        g = np.zeros(NRC_N_PARAMS, np.float32) if want_grad else None
+        self._check(self.lib.tgx_nrc_train_batch(self.h, _ptr(x), _ptr(t), x.shape[0], ctypes.byref(loss), _ptr(g)),
+                    "tgx_nrc_train_batch")
+        return float(loss.value), g
+
+    def nrc_frame_device(self, cam: np.ndarray, w: int, h: int, params: np.ndarray, vis_dev: int, radiance_dev: int,
+                         irradiance_dev: int | None = None) -> None:
+        cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
+        self._check(self.lib.tgx_nrc_frame_device(self.h, _ptr(cam), w, h, _ptr(params), vis_dev, radiance_dev,
+                                                  irradiance_dev), "tgx_nrc_frame_device")
+
+    def nrc_stats(self):
+        st = np.zeros(1, NRC_STATS_DTYPE)
+        self._check(self.lib.tgx_nrc_last_stats(self.h, _ptr(st)), "tgx_nrc_last_stats")
+        return st[0]
+
+    def nrc_frame(self, cam: np.ndarray, w: int, h: int, params: np.ndarray):
+        """One cache frame: (vis[h, w], radiance[h, w, 3], irradiance[h, w, 3])."""
+        vis_d, rad_d, irr_d = self.alloc(8 * w * h), self.alloc(12 * w * h), self.alloc(12 * w * h)
+        try:
+            self.nrc_frame_device(cam, w, h, params, vis_d, rad_d, irr_d)
+            vis = np.empty((h, w), np.uint64)
+            rad = np.empty((h, w, 3), np.float32)
+            irr = np.empty((h, w, 3), np.float32)
+            self.d2h(vis, vis_d)
+            self.d2h(rad, rad_d)
+            self.d2h(irr, irr_d)
+        finally:
+            for q in (vis_d, rad_d, irr_d):
+                self.free(q)
+        return vis, rad, irr
 
     def frame(self, cam: np.ndarray, w: int, h: int, params: np.ndarray):
         """Visibility + GI of one camera: (vis[h, w], radiance[h, w, 3], irradiance[h, w, 3])."""

@@ -43,7 +43,7 @@
  * Contributions are stored per (pixel, sample) and summed in sample order,
  * so the fp32 result equals a sequential per-pixel loop bit for bit.
  */
-#include "tgx_svo_dev.h"
+#include "tgx_gi_dev.h"
 #include "tgx_m4.h"
 
 #define GI_SQRT3 1.73205080757f
@@ -435,29 +435,12 @@ struct gi_cam {
     float bl[3], br[3], tr[3], tl[3];
 };
 
-/* per-pixel surface kept for the resolve: albedo and a visibility flag */
-struct gi_surf {
-    float    albedo[3];
-    uint32_t valid;
-};
-
-/* counters: 1 bounce rays, 2 bounce hits, 3 shadow rays, 4 trace cursor
- * (bounce), 5 trace cursor (shadow), 6 ray count of tgx_svo_trace, 7 shadow
- * hits */
-#define GI_N_COUNTERS 8
-
 __global__ void gi_fill_k(uint32_t* p, uint64_t n, uint32_t v)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
-/* MurmurHash3 finaliser (util.inc:47-56) */
-__device__ static inline uint32_t gi_fmix32(uint32_t v)
-{
-    v ^= v >> 16; v *= 0x85ebca6bu; v ^= v >> 13; v *= 0xc2b2ae35u; v ^= v >> 16;
-    return v;
-}
 /* xorshift32 and its [lo, hi] float (util.inc:11-31) */
 __device__ static inline float gi_rand(uint32_t& s, float lo, float hi)
 {
@@ -553,67 +536,71 @@ __device__ static bool gi_surface(const tgx_object* __restrict__ objects, const 
     return true;
 }
 
-/* Reserves `count` consecutive queue slots for the calling lane (counts may
- * differ per lane) with one atomic per workgroup; returns the lane's first
- * slot. Every thread of the workgroup must call it. */
-template <uint32_t kBlock>
-__device__ static inline uint32_t block_append(uint32_t* counter, uint32_t count)
-{
-    __shared__ uint32_t s_wave[kBlock / 64u + 1u];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t incl = count;
-    for (uint32_t off = 1; off < 64u; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    if (lane == 63u) s_wave[wave] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t total = 0;
-        for (uint32_t w = 0; w < kBlock / 64u; w++) {
-            const uint32_t c = s_wave[w];
-            s_wave[w] = total;
-            total += c;
-        }
-        s_wave[kBlock / 64u] = total ? atomicAdd(counter, total) : 0u;
-    }
-    __syncthreads();
-    return s_wave[kBlock / 64u] + s_wave[wave] + incl - count;
-}
-
 #define GI_BLOCK 256u
 #define SHADE_ITEMS 4u
 
-__global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const tgx_object* __restrict__ objects,
-                                                     const uint32_t* __restrict__ cptrs, const uint32_t* __restrict__ c2o,
-                                                     const uint8_t* __restrict__ materials,
-                                                     const tgx_lut_entry* __restrict__ luts, gi_cam cam, uint32_t W,
-                                                     uint32_t H, const uint64_t* __restrict__ vis, tgx_gi_params prm,
-                                                     gi_surf* __restrict__ surf, svo_ray* __restrict__ rays,
-                                                     float* __restrict__ contrib, uint32_t* __restrict__ counters)
+/* Surface pass: the primary hit of every pixel (shading.frag:116-229) and
+ * its visibility flag. Outputs of pixels without a hit are cleared. */
+__global__ __launch_bounds__(GI_BLOCK) void gi_surface_k(const tgx_object* __restrict__ objects,
+                                                         const uint32_t* __restrict__ cptrs,
+                                                         const uint32_t* __restrict__ c2o,
+                                                         const uint8_t* __restrict__ materials,
+                                                         const tgx_lut_entry* __restrict__ luts, gi_cam cam,
+                                                         uint32_t W, uint32_t H, const uint64_t* __restrict__ vis,
+                                                         gi_surf_rec* __restrict__ surf, uint32_t* __restrict__ flag,
+                                                         float* __restrict__ radiance, float* __restrict__ irradiance)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= W * H) return;
     f3 p = {0, 0, 0}, n = {0, 0, 0}, alb = {0, 0, 0};
-    bool vis_ok = false;
-    if (pix < W * H) {
-        vis_ok = gi_surface(objects, cptrs, c2o, materials, luts, cam, W, H, pix % W, pix / W, vis[pix], p, n, alb);
-        gi_surf s;
-        s.albedo[0] = alb.x; s.albedo[1] = alb.y; s.albedo[2] = alb.z;
-        s.valid = vis_ok ? 1u : 0u;
-        surf[pix] = s;
+    const bool ok = gi_surface(objects, cptrs, c2o, materials, luts, cam, W, H, pix % W, pix / W, vis[pix], p, n, alb);
+    gi_surf_rec r;
+    r.p[0] = p.x; r.p[1] = p.y; r.p[2] = p.z;
+    r.n[0] = n.x; r.n[1] = n.y; r.n[2] = n.z;
+    r.albedo[0] = alb.x; r.albedo[1] = alb.y; r.albedo[2] = alb.z;
+    r.valid = ok ? 1u : 0u;
+    surf[pix] = r;
+    flag[pix] = ok ? 1u : 0u;
+    if (!ok) {
+        if (radiance) { radiance[3ull * pix] = 0.0f; radiance[3ull * pix + 1] = 0.0f; radiance[3ull * pix + 2] = 0.0f; }
+        if (irradiance) { irradiance[3ull * pix] = 0.0f; irradiance[3ull * pix + 1] = 0.0f; irradiance[3ull * pix + 2] = 0.0f; }
     }
-    const uint32_t first = block_append<GI_BLOCK>(&counters[1], vis_ok ? prm.spp : 0u);
-    if (vis_ok) {
-        const f3 ob = {p.x + GI_SQRT3 * n.x, p.y + GI_SQRT3 * n.y, p.z + GI_SQRT3 * n.z};
-        for (uint32_t k = 0; k < prm.spp; k++) {
-            uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
-            if (st == 0u) st = 0x6D2B79F5u;
-            const f3 d = gi_cosine_direction(n, st);
-            svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, pix * prm.spp + k, 0u};
-            rays[first + k] = r;
-            const size_t c = 3ull * ((size_t)pix * prm.spp + k);
-            contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
-        }
+}
+
+/* Visible pixels in ascending order from the flags and their exclusive scan. */
+__global__ void gi_list_k(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, uint32_t n,
+                          uint32_t* __restrict__ list, uint32_t* __restrict__ count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) list[off[i]] = i;
+    if (i == n - 1) *count = off[i] + flag[i];
+}
+
+/* spp cosine bounce rays per listed pixel, from p + sqrt(3) n; ray id =
+ * list entry * spp + sample. */
+__global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restrict__ surf,
+                                                     const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ count_ptr, tgx_gi_params prm,
+                                                     svo_ray* __restrict__ rays, float* __restrict__ contrib,
+                                                     uint32_t* __restrict__ counters)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = i < *count_ptr;
+    const uint32_t first = block_append<GI_BLOCK>(&counters[1], ok ? prm.spp : 0u);
+    if (!ok) return;
+    const uint32_t pix = list[i];
+    const gi_surf_rec sr = surf[pix];
+    const f3 n = {sr.n[0], sr.n[1], sr.n[2]};
+    const f3 ob = {sr.p[0] + GI_SQRT3 * n.x, sr.p[1] + GI_SQRT3 * n.y, sr.p[2] + GI_SQRT3 * n.z};
+    for (uint32_t k = 0; k < prm.spp; k++) {
+        uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
+        if (st == 0u) st = 0x6D2B79F5u;
+        const f3 d = gi_cosine_direction(n, st);
+        const svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, i * prm.spp + k, 0u};
+        rays[first + k] = r;
+        const size_t c = 3ull * ((size_t)i * prm.spp + k);
+        contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
     }
 }
 
@@ -680,23 +667,38 @@ __global__ __launch_bounds__(256) void gi_lit_k(const svo_ray* __restrict__ shad
     contrib[c] = weight[3ull * j]; contrib[c + 1] = weight[3ull * j + 1]; contrib[c + 2] = weight[3ull * j + 2];
 }
 
-__global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf* __restrict__ surf, const float* __restrict__ contrib,
-                                                    uint32_t n_pix, uint32_t spp, float* __restrict__ radiance,
-                                                    float* __restrict__ irradiance)
+/* Per list entry: E = (sum of its sample contributions in sample order) /
+ * spp, radiance = albedo * E. */
+__global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf_rec* __restrict__ surf,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count_ptr,
+                                                    const float* __restrict__ contrib, uint32_t spp,
+                                                    float* __restrict__ e_entry, float* __restrict__ e_var,
+                                                    float* __restrict__ radiance, float* __restrict__ irradiance)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= n_pix) return;
-    const gi_surf s = surf[pix];
-    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
-    if (s.valid) {
-        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-        const float* c = contrib + 3ull * (size_t)pix * spp;
-        for (uint32_t k = 0; k < spp; k++) { a0 = a0 + c[3 * k]; a1 = a1 + c[3 * k + 1]; a2 = a2 + c[3 * k + 2]; }
-        const float inv = 1.0f / (float)spp;
-        e0 = a0 * inv; e1 = a1 * inv; e2 = a2 * inv;
-        r0 = s.albedo[0] * e0; r1 = s.albedo[1] * e1; r2 = s.albedo[2] * e2;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count_ptr) return;
+    const uint32_t pix = list[i];
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    const float* c = contrib + 3ull * (size_t)i * spp;
+    for (uint32_t k = 0; k < spp; k++) { a0 = a0 + c[3 * k]; a1 = a1 + c[3 * k + 1]; a2 = a2 + c[3 * k + 2]; }
+    const float inv = 1.0f / (float)spp;
+    const float e0 = a0 * inv, e1 = a1 * inv, e2 = a2 * inv;
+    if (e_entry) { e_entry[3ull * i] = e0; e_entry[3ull * i + 1] = e1; e_entry[3ull * i + 2] = e2; }
+    if (e_var) {
+        float q = 0.0f;
+        for (uint32_t k = 0; k < spp; k++) {
+            const float x0 = c[3 * k] - e0, x1 = c[3 * k + 1] - e1, x2 = c[3 * k + 2] - e2;
+            q += x0 * x0 + x1 * x1 + x2 * x2;
+        }
+        e_var[i] = spp > 1u ? q / ((float)spp * (float)(spp - 1u)) : 0.0f;
     }
-    radiance[3ull * pix] = r0; radiance[3ull * pix + 1] = r1; radiance[3ull * pix + 2] = r2;
+    if (radiance) {
+        const gi_surf_rec s = surf[pix];
+        radiance[3ull * pix] = s.albedo[0] * e0;
+        radiance[3ull * pix + 1] = s.albedo[1] * e1;
+        radiance[3ull * pix + 2] = s.albedo[2] * e2;
+    }
     if (irradiance) { irradiance[3ull * pix] = e0; irradiance[3ull * pix + 1] = e1; irradiance[3ull * pix + 2] = e2; }
 }
 
@@ -765,84 +767,76 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
     return TGX_OK;
 }
 
-extern "C" {
-
-int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n, float max_distance,
-                  tgx_svo_hit* out_host)
+int tgx_gi_validate(tgx_context* ctx, uint32_t w, uint32_t h, const tgx_gi_params* prm)
 {
-    if (!ctx || (n && (!origins || !directions || !out_host))) return TGX_ERR_INVALID_ARG;
-    ctx->last_error.clear();
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
-    if (n == 0) return TGX_OK;
-    int rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, sizeof(svo_ray) * (size_t)n, "svo rays")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->svo_hits, (sizeof(svo_res) + sizeof(tgx_svo_hit)) * (size_t)n, "svo hits")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_N_COUNTERS, "gi counters")))
-        return rc;
-    std::vector<svo_ray> h(n);
-    for (uint32_t i = 0; i < n; i++)
-        h[i] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2], directions[3 * i], directions[3 * i + 1],
-                directions[3 * i + 2], i, 0u};
-    svo_ray* d_rays = (svo_ray*)ctx->svo_rays.ptr;
-    svo_res* d_res = (svo_res*)ctx->svo_hits.ptr;
-    tgx_svo_hit* d_hits = (tgx_svo_hit*)(d_res + n);
-    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_rays, h.data(), sizeof(svo_ray) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
-    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 6, 1ull, n);
-    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, max_distance, cnt + 4, d_res, nullptr))) return rc;
-    hipLaunchKernelGGL(svo_hits_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (const svo_gnode*)ctx->svo_nodes.ptr,
-                       d_res, n, d_hits);
-    TGX_LAUNCH_CHECK(ctx, "svo_hits_k");
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, d_hits, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
-    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return TGX_OK;
-}
-
-int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
-                  const tgx_gi_params* prm, float* radiance_dev, float* irradiance_dev)
-{
-    if (!ctx || !cam || !vis_dev || !prm || !radiance_dev) return TGX_ERR_INVALID_ARG;
-    ctx->last_error.clear();
     if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
     if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
     if (!ctx->luts.ptr || ctx->n_luts == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "the scene has no LUTs");
-    if (w == 0 || h == 0 || (uint64_t)w * h * (prm->spp ? prm->spp : 1) >= (1ull << 31))
-        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad resolution");
     if (prm->spp < 1 || prm->spp > 64 || !(prm->max_distance > 0.0f))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "spp must be in [1, 64] and max_distance > 0");
+    if (w == 0 || h == 0 || (uint64_t)w * h * prm->spp >= (1ull << 31))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad resolution");
+    return TGX_OK;
+}
+
+int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
+                    float* radiance_dev, float* irradiance_dev)
+{
     const uint64_t n_pix = (uint64_t)w * h;
-    const uint64_t n_samples = n_pix * prm->spp;
-    ctx->gi_spp = prm->spp;
     int rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_N_COUNTERS, "gi counters")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_surf, sizeof(gi_surf) * n_pix, "gi surfaces")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_rays, sizeof(svo_ray) * n_samples, "gi bounce rays")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_res, sizeof(svo_res) * n_samples, "gi bounce results")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow, sizeof(svo_ray) * n_samples, "gi shadow rays")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow_res, sizeof(svo_res) * n_samples, "gi shadow results")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_weight, 12ull * n_samples, "gi weights")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_contrib, 12ull * n_samples, "gi contributions")))
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_CNT_TOTAL, "gi counters")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_surf, sizeof(gi_surf_rec) * n_pix, "gi surfaces")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_list, 12ull * n_pix, "gi pixel list")))
         return rc;
     gi_cam gc;
     for (int i = 0; i < 3; i++) {
         gc.pos[i] = cam->camera[i];
         gc.bl[i] = cam->ray_bl[i]; gc.br[i] = cam->ray_br[i]; gc.tr[i] = cam->ray_tr[i]; gc.tl[i] = cam->ray_tl[i];
     }
-    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
+    uint32_t* list = (uint32_t*)ctx->gi_list.ptr;
+    uint32_t* flag = list + n_pix;
+    uint32_t* off = flag + n_pix;
+    {
+        tgx_launch_scope s(ctx, "gi_surface_k");
+        hipLaunchKernelGGL(gi_surface_k, dim3(blocks_for(n_pix)), dim3(GI_BLOCK), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
+                           (const tgx_lut_entry*)ctx->luts.ptr, gc, w, h, vis_dev, (gi_surf_rec*)ctx->gi_surf.ptr,
+                           flag, radiance_dev, irradiance_dev);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_surface_k");
+    if ((rc = tgx_prim_exclusive_scan_u32(ctx, flag, off, (uint32_t)n_pix))) return rc;
+    hipLaunchKernelGGL(gi_list_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream, flag, off, (uint32_t)n_pix, list,
+                       (uint32_t*)ctx->gi_counts.ptr + GI_CNT_VISIBLE);
+    TGX_LAUNCH_CHECK(ctx, "gi_list_k");
+    return TGX_OK;
+}
+
+int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t* list, const uint32_t* count_dev,
+                      uint32_t max_count, uint32_t* cnt, float* e_entry, float* e_var, float* radiance_dev,
+                      float* irradiance_dev)
+{
+    const uint64_t n_samples = (uint64_t)max_count * prm->spp;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_rays, sizeof(svo_ray) * n_samples, "gi bounce rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_res, sizeof(svo_res) * n_samples, "gi bounce results")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow, sizeof(svo_ray) * n_samples, "gi shadow rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow_res, sizeof(svo_res) * n_samples, "gi shadow results")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_weight, 12ull * n_samples, "gi weights")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_contrib, 12ull * n_samples, "gi contributions")))
+        return rc;
     svo_ray* rays = (svo_ray*)ctx->gi_rays.ptr;
     svo_res* res = (svo_res*)ctx->gi_res.ptr;
     svo_ray* shadow = (svo_ray*)ctx->gi_shadow.ptr;
     svo_res* sres = (svo_res*)ctx->gi_shadow_res.ptr;
     float* weight = (float*)ctx->gi_weight.ptr;
     float* contrib = (float*)ctx->gi_contrib.ptr;
+    const gi_surf_rec* surf = (const gi_surf_rec*)ctx->gi_surf.ptr;
     hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
     {
         tgx_launch_scope s(ctx, "gi_gen_k");
-        hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream, (const tgx_object*)ctx->objects.ptr,
-                           (const uint32_t*)ctx->cluster_pointers.ptr, (const uint32_t*)ctx->cluster_idx_to_object.ptr,
-                           (const uint8_t*)ctx->materials.ptr, (const tgx_lut_entry*)ctx->luts.ptr, gc, w, h, vis_dev,
-                           *prm, (gi_surf*)ctx->gi_surf.ptr, rays, contrib, cnt);
+        hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, list, count_dev,
+                           *prm, rays, contrib, cnt);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
     if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm->max_distance, cnt + 4, res, cnt + 2))) return rc;
@@ -862,12 +856,59 @@ int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t 
     TGX_LAUNCH_CHECK(ctx, "gi_lit_k");
     {
         tgx_launch_scope s(ctx, "gi_resolve_k");
-        hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream,
-                           (const gi_surf*)ctx->gi_surf.ptr, contrib, (uint32_t)n_pix, prm->spp, radiance_dev,
-                           irradiance_dev);
+        hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(max_count)), dim3(256), 0, ctx->stream, surf, list, count_dev,
+                           contrib, prm->spp, e_entry, e_var, radiance_dev, irradiance_dev);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_resolve_k");
     return TGX_OK;
+}
+
+extern "C" {
+
+int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n, float max_distance,
+                  tgx_svo_hit* out_host)
+{
+    if (!ctx || (n && (!origins || !directions || !out_host))) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (n == 0) return TGX_OK;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, sizeof(svo_ray) * (size_t)n, "svo rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_hits, (sizeof(svo_res) + sizeof(tgx_svo_hit)) * (size_t)n, "svo hits")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_CNT_TOTAL, "gi counters")))
+        return rc;
+    std::vector<svo_ray> h(n);
+    for (uint32_t i = 0; i < n; i++)
+        h[i] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2], directions[3 * i], directions[3 * i + 1],
+                directions[3 * i + 2], i, 0u};
+    svo_ray* d_rays = (svo_ray*)ctx->svo_rays.ptr;
+    svo_res* d_res = (svo_res*)ctx->svo_hits.ptr;
+    tgx_svo_hit* d_hits = (tgx_svo_hit*)(d_res + n);
+    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr + GI_CNT_SVO;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_rays, h.data(), sizeof(svo_ray) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 6, 1ull, n);
+    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, max_distance, cnt + 4, d_res, nullptr))) return rc;
+    hipLaunchKernelGGL(svo_hits_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (const svo_gnode*)ctx->svo_nodes.ptr,
+                       d_res, n, d_hits);
+    TGX_LAUNCH_CHECK(ctx, "svo_hits_k");
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, d_hits, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
+int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
+                  const tgx_gi_params* prm, float* radiance_dev, float* irradiance_dev)
+{
+    if (!ctx || !cam || !vis_dev || !prm || !radiance_dev) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    int rc;
+    if ((rc = tgx_gi_validate(ctx, w, h, prm))) return rc;
+    if ((rc = tgx_gi_surfaces(ctx, cam, w, h, vis_dev, radiance_dev, irradiance_dev))) return rc;
+    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
+    ctx->gi_spp = prm->spp;
+    return tgx_gi_trace_list(ctx, prm, (const uint32_t*)ctx->gi_list.ptr, cnt + GI_CNT_VISIBLE, w * h,
+                             cnt + GI_CNT_PLAIN, nullptr, nullptr, radiance_dev, irradiance_dev);
 }
 
 int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out)
@@ -875,12 +916,12 @@ int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out)
     if (!ctx || !out) return TGX_ERR_INVALID_ARG;
     if (!ctx->gi_counts.ptr) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no GI call yet");
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    uint32_t c[GI_N_COUNTERS];
+    uint32_t c[GI_CNT_TOTAL];
     TGX_HIP_CHECK(ctx, hipMemcpy(c, ctx->gi_counts.ptr, sizeof(c), hipMemcpyDeviceToHost));
-    out->visible_pixels = ctx->gi_spp ? c[1] / ctx->gi_spp : 0u;
-    out->bounce_rays = c[1];
-    out->bounce_hits = c[2];
-    out->shadow_rays = c[3];
+    out->visible_pixels = c[GI_CNT_VISIBLE];
+    out->bounce_rays = c[GI_CNT_PLAIN + 1];
+    out->bounce_hits = c[GI_CNT_PLAIN + 2];
+    out->shadow_rays = c[GI_CNT_PLAIN + 3];
     return TGX_OK;
 }
 

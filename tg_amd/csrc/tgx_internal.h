@@ -75,7 +75,8 @@ struct tgx_context {
     tgx_buffer   svo_hits;
     tgx_buffer   prim_tmp;       /* rocPRIM scratch                         */
     tgx_buffer   gi_counts;      /* GI queue counters and trace cursors     */
-    tgx_buffer   gi_surf;        /* per-pixel albedo + visible flag         */
+    tgx_buffer   gi_surf;        /* per-pixel surface records               */
+    tgx_buffer   gi_list;        /* visible pixel list, flags, scan         */
     tgx_buffer   gi_rays;        /* bounce ray queue                        */
     tgx_buffer   gi_res;         /* bounce results                          */
     tgx_buffer   gi_shadow;      /* shadow ray queue                        */
@@ -83,6 +84,23 @@ struct tgx_context {
     tgx_buffer   gi_weight;      /* RGB weight per shadow ray               */
     tgx_buffer   gi_contrib;     /* RGB contribution per (pixel, sample)    */
     uint32_t     gi_spp = 0;     /* spp of the last GI call                 */
+
+    /* Neural irradiance cache (tgx_nrc.hip). */
+    bool           nrc_ready = false;
+    tgx_nrc_config nrc_cfg = {};
+    uint32_t       nrc_steps = 0;    /* Adam steps taken                    */
+    uint32_t       nrc_frame = 0;    /* frames run                          */
+    bool           nrc_fb_traced = false; /* last frame traced a fallback list */
+    tgx_buffer     nrc_master, nrc_m, nrc_v;   /* fp32 [N_PARAMS]           */
+    tgx_buffer     nrc_fwd, nrc_bwd; /* bf16 MFMA A-operand fragments       */
+    tgx_buffer     nrc_partials;     /* fp32 [chunks][N_PARAMS]             */
+    tgx_buffer     nrc_grad;         /* fp32 [N_PARAMS], last step          */
+    tgx_buffer     nrc_train;        /* bf16 [feature][batch] activations   */
+    tgx_buffer     nrc_loss;         /* fp32 per training sample            */
+    tgx_buffer     nrc_samples, nrc_targets, nrc_out; /* staging            */
+    tgx_buffer     nrc_cells;        /* error-estimate table                */
+    tgx_buffer     nrc_queue;        /* frame lists and counters            */
+    tgx_nrc_stats  nrc_stats = {};
 
     /* Profiling. */
     bool                           profiling = false;
@@ -95,6 +113,7 @@ int  tgx_prim_exclusive_scan_u32(tgx_context* ctx, const uint32_t* in, uint32_t*
 int  tgx_prim_sort_pairs_u32(tgx_context* ctx, const uint32_t* keys_in, uint32_t* keys_out,
                              const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, unsigned end_bit);
 void tgx_svo_release(tgx_context* ctx);
+void tgx_nrc_release(tgx_context* ctx);
 int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
 void tgx_buffer_release(tgx_buffer* b);
 

@@ -115,6 +115,7 @@ int tgx_destroy(tgx_context* ctx)
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     tgx_svo_release(ctx);
+    tgx_nrc_release(ctx);
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
     if (ctx->vis_stats_event) (void)hipEventDestroy(ctx->vis_stats_event);
     (void)hipStreamDestroy(ctx->stream);
