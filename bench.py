@@ -22,7 +22,12 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
+import signal
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -136,6 +141,94 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     }
 
 
+COPY_BYTES = 256 << 20  # known-size copy that calibrates the HBM byte counters
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def probe(args) -> int:
+    """Short run for the byte counters: one warm-up and `steps` frames of
+    the workload, then a COPY_BYTES device-to-device copy (tgx_copy_k)."""
+    params = T.noise_objects_params(seed=args.seed)
+    scene = T.generate_scene(params)
+    cam = T.generate_camera(params, 0, W, H)
+    mn = np.floor(cam[0]["camera"][:3]) - float(1 << (DEPTH - 1))
+    prm = T.gi_params(T.seeded_sun(args.seed), spp=SPP, max_distance=1024.0)
+    ctx = T.Context(0)
+    ctx.upload(scene)
+    vis_d, rad_d, irr_d = ctx.alloc(W * H * 8), ctx.alloc(W * H * 12), ctx.alloc(W * H * 12)
+    if args.workload == "frame":
+        ctx.svo_build(mn, DEPTH)
+    for _ in range(1 + args.steps):
+        if args.workload == "frame":
+            ctx.frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+        else:
+            ctx.visibility_device(cam, W, H, vis_d)
+    a, b = ctx.alloc(COPY_BYTES), ctx.alloc(COPY_BYTES)
+    ctx.d2d(b, a, COPY_BYTES)
+    ctx.synchronize()
+    for q in (vis_d, rad_d, irr_d, a, b):
+        ctx.free(q)
+    ctx.close()
+    return 0
+
+
+def counter_per_dispatch(db: Path, counter: str) -> dict:
+    """Mean per-dispatch value of `counter` by kernel name (short name)."""
+    con = sqlite3.connect(str(db))
+    rows = con.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                       "where counter_name = ? group by dispatch_id", (counter,)).fetchall()
+    con.close()
+    acc: dict = {}
+    for name, _, v in rows:
+        short = name.split("(")[0].replace("void ", "").split("<")[0].strip()
+        acc.setdefault(short, []).append(float(v))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def measure_traffic(workload: str, kernel: str, steps: int = 2) -> dict | None:
+    """HBM bytes per launch of `kernel`: rocprofv3 FETCH_SIZE and WRITE_SIZE
+    passes (KiB per dispatch) over a probe run, each scaled by what the same
+    counter reads for a copy of COPY_BYTES (tgx_copy_k). Runs rocprofv3 as a
+    child process; returns None if it is unavailable or fails."""
+    rp = shutil.which("rocprofv3")
+    if rp is None:
+        return None
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        log(f"rocprofv3 --pmc {counter} pass")
+        with tempfile.TemporaryDirectory(prefix="tgx_pmc_", dir="/tmp") as d:
+            cmd = [rp, "--pmc", counter, "-d", d, "-o", "run", "--", sys.executable, str(ROOT / "bench.py"),
+                   "--probe", "--workload", workload, "--steps", str(steps)]
+            proc = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                                    stderr=subprocess.DEVNULL, start_new_session=True)
+            try:
+                proc.wait(timeout=150)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)
+                proc.wait()
+                log(f"{counter} pass timed out")
+                return None
+            dbs = sorted(Path(d).rglob("*.db"))
+            if not dbs:
+                log(f"{counter} pass left no results (exit {proc.returncode})")
+                return None
+            vals = counter_per_dispatch(dbs[0], counter)
+        if kernel not in vals or "tgx_copy_k" not in vals or vals["tgx_copy_k"] <= 0:
+            log(f"{counter}: kernels {sorted(vals)} lack {kernel} or the copy")
+            return None
+        got[counter] = (vals[kernel], vals["tgx_copy_k"])
+    f_k, f_c = got["FETCH_SIZE"]
+    w_k, w_c = got["WRITE_SIZE"]
+    fetch_scale = COPY_BYTES / 1024.0 / f_c
+    write_scale = COPY_BYTES / 1024.0 / w_c
+    return {"bytes": 1024.0 * (f_k * fetch_scale + w_k * write_scale),
+            "fetch_kib": f_k, "write_kib": w_k, "copy_fetch_kib": f_c, "copy_write_kib": w_c,
+            "copy_bytes": COPY_BYTES, "fetch_scale": fetch_scale, "write_scale": write_scale}
+
+
 def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 12.0) -> dict:
     import oracle  # the CPU baseline leg is the only bench code that uses the oracle
 
@@ -183,7 +276,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-cache measurement")
     ap.add_argument("--neural-pretrain", type=int, default=200, help="untimed cache training frames")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 byte-counter passes")
+    ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.probe:
+        return probe(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would render whole "
               "cameras or screen tiles of its own; see the sharding design). Run with --gpus 1.", file=sys.stderr)
@@ -281,15 +378,23 @@ def main() -> int:
                                "tile_entries": int(vstats.tile_entries)},
         "svo": {"build_ms": svo_build_ms, "nodes": int(info["n_nodes"]), "leaves": int(info["n_leaves"])},
     }
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step")
     if gst is not None:
         line["gi"] = {k: int(gst[k]) for k in gst.dtype.names}
         if not args.no_neural:
+            log("neural cache measurement")
             line["neural"] = neural_bench(ctx, cam, T.seeded_sun(args.seed), vis_d, rad_d, irr_d, args.steps,
                                           args.neural_pretrain, ms_per_step, rays_per_frame)
     for p in (vis_d, rad_d, irr_d):
         ctx.free(p)
     ctx.close()
+    if not args.no_traffic:
+        tr = measure_traffic(args.workload, dominant)
+        if tr is not None:
+            line["roofline"]["traffic"] = tr["bytes"]
+            line["roofline"]["traffic_detail"] = tr
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(scene, params, cam, mn, prm, args.workload)
     print(json.dumps(line), flush=True)
     return 0

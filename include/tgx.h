@@ -409,6 +409,10 @@ int tgx_device_alloc(tgx_context* ctx, size_t bytes, void** out_dev);
 int tgx_device_free(tgx_context* ctx, void* dev);
 int tgx_memcpy_d2h(tgx_context* ctx, void* dst_host, const void* src_dev, size_t bytes);
 int tgx_memcpy_h2d(tgx_context* ctx, void* dst_dev, const void* src_host, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (tgx_copy_k: 16-byte
+ * loads and stores, bytes a multiple of 16); returns without waiting. Also
+ * the known-size reference for HBM byte counters. */
+int tgx_memcpy_d2d(tgx_context* ctx, void* dst_dev, const void* src_dev, size_t bytes);
 
 /* Per-kernel timing with HIP events recorded on the context's stream.
  * When enabled, every kernel launch of the context is bracketed by events;

@@ -170,6 +170,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_device_free": (i32, [vp, vp]),
         "tgx_memcpy_d2h": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "tgx_memcpy_h2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
+        "tgx_memcpy_d2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "tgx_profile_enable": (i32, [vp, i32]),
         "tgx_profile_reset": (i32, [vp]),
         "tgx_profile_get": (i32, [vp, ctypes.POINTER(KernelStat), u32, ctypes.POINTER(u32)]),
@@ -516,6 +517,9 @@ class Context:
         p = ctypes.c_void_p()
         self._check(self.lib.tgx_device_alloc(self.h, nbytes, ctypes.byref(p)), "tgx_device_alloc")
         return p.value
+
+    def d2d(self, dst: int, src: int, nbytes: int) -> None:
+        self._check(self.lib.tgx_memcpy_d2d(self.h, dst, src, nbytes), "tgx_memcpy_d2d")
 
     def free(self, ptr: int) -> None:
         self._check(self.lib.tgx_device_free(self.h, ptr), "tgx_device_free")

@@ -74,6 +74,13 @@ tgx_launch_scope::~tgx_launch_scope()
     ctx->pending.push_back({stat_idx, start, stop});
 }
 
+/* Grid-stride 16-byte copy. */
+__global__ __launch_bounds__(256) void tgx_copy_k(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 extern "C" {
 
 int tgx_create(int device, tgx_context** out_ctx)
@@ -223,6 +230,21 @@ int tgx_memcpy_h2d(tgx_context* ctx, void* dst_dev, const void* src_host, size_t
     if (!ctx) return TGX_ERR_INVALID_ARG;
     TGX_HIP_CHECK(ctx, hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, ctx->stream));
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
+int tgx_memcpy_d2d(tgx_context* ctx, void* dst_dev, const void* src_dev, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst_dev || !src_dev)) || bytes % 16) return TGX_ERR_INVALID_ARG;
+    if (bytes == 0) return TGX_OK;
+    const uint64_t n = bytes / 16;
+    const uint64_t blocks = (n + 255) / 256;
+    const uint32_t grid = blocks < 65536 ? (uint32_t)blocks : 65536u;
+    {
+        tgx_launch_scope s(ctx, "tgx_copy_k");
+        hipLaunchKernelGGL(tgx_copy_k, dim3(grid), dim3(256), 0, ctx->stream, (uint4*)dst_dev, (const uint4*)src_dev, n);
+    }
+    TGX_LAUNCH_CHECK(ctx, "tgx_copy_k");
     return TGX_OK;
 }
 
