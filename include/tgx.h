@@ -320,6 +320,23 @@ int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
                      float* irradiance_dev);
 
 /* ------------------------------------------------------------------------ */
+/* Tuning: launch shapes and queue policies. Results never depend on them;  */
+/* tgx_autotune picks them by timing.                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct tgx_tuning {
+    uint32_t trace_sort;          /* bit 0: sort bounce rays, bit 1: sort shadow rays,
+                                     by (direction octant, origin cell) before tracing (0) */
+    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (16) */
+    uint32_t trace_blocks_per_cu; /* persistent 1024-lane trace workgroups per CU, 1..2 (2) */
+    uint32_t trace_vote;          /* 1: a wave runs only node steps or only DDA bursts per
+                                     iteration, by majority of its lanes (0) */
+} tgx_tuning;
+
+int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);
+int tgx_tuning_set(tgx_context* ctx, const tgx_tuning* tuning);
+
+/* ------------------------------------------------------------------------ */
 /* Neural irradiance cache. No counterpart in the reference: a small MLP    */
 /* that predicts the irradiance E of tgx_gi_device from the primary hit,    */
 /* trained online on samples the SVO tracer generates, with the tracer as   */

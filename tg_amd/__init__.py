@@ -69,6 +69,9 @@ def seeded_sun(seed: int) -> np.ndarray:
     az, el = rng.uniform(0, 2 * np.pi), np.radians(rng.uniform(25, 65))
     return np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
 
+TUNING_DTYPE = np.dtype([("trace_sort", "<u4"), ("trace_refill", "<u4"), ("trace_blocks_per_cu", "<u4"),
+                         ("trace_vote", "<u4")])
+
 NRC_N_PARAMS = 18624
 NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learning_rate", "<f4"),
                              ("beta1", "<f4"), ("beta2", "<f4"), ("epsilon", "<f4"), ("seed", "<u4"),
@@ -171,6 +174,8 @@ def load_library() -> ctypes.CDLL:
         "tgx_memcpy_d2h": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "tgx_memcpy_h2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "tgx_memcpy_d2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
+        "tgx_tuning_get": (i32, [vp, vp]),
+        "tgx_tuning_set": (i32, [vp, vp]),
         "tgx_profile_enable": (i32, [vp, i32]),
         "tgx_profile_reset": (i32, [vp]),
         "tgx_profile_get": (i32, [vp, ctypes.POINTER(KernelStat), u32, ctypes.POINTER(u32)]),
@@ -517,6 +522,17 @@ class Context:
         p = ctypes.c_void_p()
         self._check(self.lib.tgx_device_alloc(self.h, nbytes, ctypes.byref(p)), "tgx_device_alloc")
         return p.value
+
+    def tuning(self) -> np.ndarray:
+        t = np.zeros(1, TUNING_DTYPE)
+        self._check(self.lib.tgx_tuning_get(self.h, _ptr(t)), "tgx_tuning_get")
+        return t
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning()
+        for k, v in kw.items():
+            t[k] = v
+        self._check(self.lib.tgx_tuning_set(self.h, _ptr(t)), "tgx_tuning_set")
 
     def d2d(self, dst: int, src: int, nbytes: int) -> None:
         self._check(self.lib.tgx_memcpy_d2d(self.h, dst, src, nbytes), "tgx_memcpy_d2d")

@@ -29,7 +29,7 @@
  * gi_trace_k is persistent: each lane runs one ray as a state machine
  * whose steps are either one node step of the reference's loop or up to
  * DDA_BURST iterations of the leaf DDA, so lanes of a wave stay busy while
- * they are in different phases. Whenever GI_REFILL_LANES lanes of a wave
+ * they are in different phases. Whenever `refill` (tuning) lanes of a wave
  * have finished, the wave takes that many new rays from the queue with one
  * atomic (ballot + lane rank).
  *
@@ -53,7 +53,6 @@
 #define TRACE_BLOCK 1024
 #define TRACE_LDS_NODES_MAX 8000u   /* inner nodes + leaf brick masks in LDS (64000 B) */
 #define DDA_BURST 8
-#define GI_REFILL_LANES 16u
 
 /* ------------------------------------------------------------------------ */
 /* Traversal                                                                */
@@ -138,6 +137,17 @@ __device__ static inline int dir_step(float d) { return d > 0.0f ? 1 : (d < 0.0f
  * coordinates (7 bits each: inner nodes sit at level <= depth - 6 <= 7), its
  * level, the child slot being visited and the DDA phase flag; `vox` packs
  * the leaf DDA voxel and iteration count. */
+/* axis_exit_gt_eps as a 0/1 integer for |d| >= 2^-100 or d == 0: the sign of
+ * the exact-sign difference a - t (IEEE subtraction is 0 only for equal
+ * operands, and rounding keeps the sign). Callers handle |d| < 2^-100. */
+__device__ static inline uint32_t axis_exit_fast(float p, float d, float mn, float mx)
+{
+    const float a = (d > 0.0f ? mx : mn) - p;
+    const float t = d * 0x1p-23f;
+    const float u = d > 0.0f ? a - t : t - a;
+    return u > 0.0f ? 1u : (d == 0.0f ? 1u : 0u);
+}
+
 struct trace_lane {
     float    px, py, pz;   /* position relative to the root centre */
     float    dx, dy, dz;
@@ -222,20 +232,55 @@ __device__ static inline void child_min(uint32_t n, uint32_t rc, float half, flo
     z = -half + (float)(2u * ND_CZ(n) + (rc >> 2)) * cs;
 }
 
+/* Pops every node the position has left: a wave-uniform loop over the
+ * lanes in `adv`, walking parent links; cur / n are the current node and
+ * its packed coordinates. kDivide uses the dividing exit test (needed for
+ * direction components below 2^-100). */
+template <bool kLds, bool kDivide>
+__device__ static inline void pop_walk(const trace_params& P, const uint2* s_inner, float half, const trace_lane& L,
+                                       bool adv, uint32_t& cur, uint32_t& n)
+{
+    uint32_t popping = adv ? 1u : 0u;
+    while (true) {
+        const uint32_t live = popping & (cur != SVO_NONE ? 1u : 0u);
+        if (!__any(live)) break;
+        const uint32_t lvl = ND_LVL(n);
+        const float ns = (float)(1u << (P.depth - lvl));
+        const float nmx = -half + (float)ND_CX(n) * ns, nmy = -half + (float)ND_CY(n) * ns,
+                    nmz = -half + (float)ND_CZ(n) * ns;
+        uint32_t inside;
+        if constexpr (kDivide)
+            inside = (axis_exit_gt_eps(L.px, L.dx, nmx, nmx + ns) && axis_exit_gt_eps(L.py, L.dy, nmy, nmy + ns) &&
+                      axis_exit_gt_eps(L.pz, L.dz, nmz, nmz + ns)) ? 1u : 0u;
+        else
+            inside = axis_exit_fast(L.px, L.dx, nmx, nmx + ns) & axis_exit_fast(L.py, L.dy, nmy, nmy + ns) &
+                     axis_exit_fast(L.pz, L.dz, nmz, nmz + ns);
+        const uint32_t up = live & (inside ^ 1u);
+        const uint32_t par = inner_at<kLds>(P.inner, s_inner, live ? cur : 0u).y & 0xFFFFFFu;
+        cur = up ? (par == 0xFFFFFFu ? SVO_NONE : par) : cur;
+        n = up ? ((ND_CX(n) >> 1) | ((ND_CY(n) >> 1) << 7) | ((ND_CZ(n) >> 1) << 14) | (((lvl - 1u) & 7u) << 21)) : n;
+        popping = up;
+    }
+}
+
 /* One step of a lane: one iteration of the reference's node loop, or up to
  * DDA_BURST iterations of the leaf DDA. Returns true when the ray is done;
  * res is then its result. */
 template <bool kLds>
 __device__ static inline bool lane_step(const trace_params& P, const uint2* s_inner, float half, float maxd,
-                                        trace_lane& L, svo_res& res)
+                                        trace_lane& L, svo_res& res, bool active, bool allow_node, bool allow_dda)
 {
-    bool adv = false;
-    if (!(L.node & ND_DDA)) {
+    bool adv = false, done = false;
+    allow_node = allow_node && active;
+    allow_dda = allow_dda && active;
+    if (!(L.node & ND_DDA) && allow_node && (L.cur == SVO_NONE || L.steps >= TGX_SVO_MAX_STEPS || L.dist > maxd)) {
+        res = res_miss(L.steps);
+        done = true;
+        allow_node = false;
+        allow_dda = false;
+    }
+    if (!(L.node & ND_DDA) && allow_node) {
         /* one iteration of the reference's node loop */
-        if (L.cur == SVO_NONE || L.steps >= TGX_SVO_MAX_STEPS || L.dist > maxd) {
-            res = res_miss(L.steps);
-            return true;
-        }
         L.steps++;
         const uint2 nd = inner_at<kLds>(P.inner, s_inner, L.cur);
         const uint32_t n = L.node;
@@ -282,91 +327,115 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
             }
         }
     }
-    if (L.node & ND_DDA) {
-        /* up to DDA_BURST iterations of the leaf DDA */
+    /* Up to DDA_BURST iterations of the leaf DDA. The loop is wave-uniform
+     * (one __any exit) and lanes that stop are predicated off, which keeps
+     * the scalar unit free of per-lane exit bookkeeping. */
+    const bool in_dda = (L.node & ND_DDA) && allow_dda;
+    bool hit = false, leave = false;
+    int x = (int)(L.vox & 31u), y = (int)((L.vox >> 5) & 31u), z = (int)((L.vox >> 10) & 31u);
+    uint32_t it = L.vox >> 15;
+    if (__any(in_dda)) {
         const uint32_t* grid = P.leaf_bits + (size_t)L.leaf * TGX_SVO_BLOCK_WORDS;
-        int x = (int)(L.vox & 31u), y = (int)((L.vox >> 5) & 31u), z = (int)((L.vox >> 10) & 31u);
-        uint32_t it = L.vox >> 15;
-        bool leave = false, hit = false;
+        const int sx = dir_step(L.dx), sy = dir_step(L.dy), sz = dir_step(L.dz);
+        /* Lane conditions are kept as 0/1 integers in vector registers:
+         * booleans combined with && / || would become 64-bit lane masks
+         * combined on the CU's shared scalar unit, which then limits the
+         * loop. Only single comparisons feed selects. */
+        uint32_t live = in_dda ? 1u : 0u, hit_i = 0u, leave_i = 0u;
 #pragma unroll 1
         for (int k = 0; k < DDA_BURST; k++) {
-            if (it >= 4u * TGX_DDA_MAX_ITERATIONS) { leave = true; break; }
+            /* tg_amanatides_woo: stop at the iteration cap (leave), on a
+             * solid voxel (hit) or after a step out of the block (leave) */
+            const uint32_t cap = it >> 8;            /* it <= 4 * TGX_DDA_MAX_ITERATIONS = 256 */
             const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
-            if ((L.bm >> b) & 1ull) {
-                const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
-                if (wi != L.wi) { L.wi = wi; L.wv = grid[wi]; }
-                if ((L.wv >> (uint32_t)x) & 1u) { hit = true; break; }
-            }
+            const uint32_t inb = live & (cap ^ 1u) & (uint32_t)(L.bm >> b);
+            const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
+            if ((inb & 1u) && wi != L.wi) { L.wi = wi; L.wv = grid[wi]; }
+            const uint32_t h = inb & (L.wv >> (uint32_t)x) & 1u;
             /* step along x if tmx < tmy and tmx < tmz, else along y if
-             * !(tmx < tmy) and tmy < tmz, else along z (branch-free) */
-            const bool step_x = L.tmx < L.tmy && L.tmx < L.tmz;
-            const bool step_y = !(L.tmx < L.tmy) && L.tmy < L.tmz;
-            const bool step_z = !step_x && !step_y;
-            L.tmx = step_x ? L.tmx + L.tdx : L.tmx;
-            L.tmy = step_y ? L.tmy + L.tdy : L.tmy;
-            L.tmz = step_z ? L.tmz + L.tdz : L.tmz;
-            x += step_x ? dir_step(L.dx) : 0;
-            y += step_y ? dir_step(L.dy) : 0;
-            z += step_z ? dir_step(L.dz) : 0;
-            if (((uint32_t)x | (uint32_t)y | (uint32_t)z) >= 32u) { leave = true; break; }
-            it++;
+             * !(tmx < tmy) and tmy < tmz, else along z */
+            const bool xy = L.tmx < L.tmy;
+            const float m1 = xy ? L.tmx : L.tmy;
+            const uint32_t axis = m1 < L.tmz ? (xy ? 0u : 1u) : 2u;
+            const uint32_t move = live & (cap ^ 1u) & (h ^ 1u);
+            const float tsum = (axis == 0u ? L.tmx : (axis == 1u ? L.tmy : L.tmz)) +
+                               (axis == 0u ? L.tdx : (axis == 1u ? L.tdy : L.tdz));
+            const int dstep = axis == 0u ? sx : (axis == 1u ? sy : sz);
+            const uint32_t mx = move & (axis == 0u ? 1u : 0u);
+            const uint32_t my = move & (axis == 1u ? 1u : 0u);
+            const uint32_t mz = move & (axis == 2u ? 1u : 0u);
+            L.tmx = mx ? tsum : L.tmx;
+            L.tmy = my ? tsum : L.tmy;
+            L.tmz = mz ? tsum : L.tmz;
+            x += (int)mx * dstep;
+            y += (int)my * dstep;
+            z += (int)mz * dstep;
+            const uint32_t outside = (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
+            const uint32_t lv = live & (cap | (move & outside));
+            it += move & (outside ^ 1u);
+            hit_i |= h;
+            leave_i |= lv;
+            live &= (h | lv) ^ 1u;
+            if (!__any(live)) break;
         }
-        if (hit) {
-            /* distance to the voxel's entry plane and the entry face */
-            float vmx, vmy, vmz;
-            child_min(L.node, ND_RC(L.node), half, 32.0f, vmx, vmy, vmz);
-            vmx = vmx + (float)x; vmy = vmy + (float)y; vmz = vmz + (float)z;
-            const float nx = axis_near(L.px, L.dx, vmx, vmx + 1.0f);
-            const float ny = axis_near(L.py, L.dy, vmy, vmy + 1.0f);
-            const float nz = axis_near(L.pz, L.dz, vmz, vmz + 1.0f);
-            const float enter = fmax2(fmax2(nx, ny), nz);
-            res.distance = L.dist + enter;
-            res.node = P.n_inner + L.leaf;
-            res.voxel = 1024u * (uint32_t)z + 32u * (uint32_t)y + (uint32_t)x;
-            uint32_t ax = nx > ny ? 0u : 1u;
-            const float nax = ax == 0u ? nx : ny;
-            if (!(nax > nz)) ax = 2u;
-            const float dc = ax == 0u ? L.dx : (ax == 1u ? L.dy : L.dz);
-            res.flags = (res.distance <= maxd ? 1u : 0u) | 2u | (ax << 2) | (dc > 0.0f ? 0u : 16u);
-            res.steps = L.steps;
-            res.pad = 0u;
-            return true;
-        }
+        hit = hit_i != 0u;
+        leave = leave_i != 0u;
+        if (in_dda && !hit && !leave) L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10) | (it << 15);
         if (leave) {
             L.node &= ~ND_DDA;
             adv = true;
-        } else {
-            L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10) | (it << 15);
         }
     }
-    if (adv) {
-        /* leave the child box and pop every node the position has left */
+    if (hit) {
+        /* distance to the voxel's entry plane and the entry face */
+        float vmx, vmy, vmz;
+        child_min(L.node, ND_RC(L.node), half, 32.0f, vmx, vmy, vmz);
+        vmx = vmx + (float)x; vmy = vmy + (float)y; vmz = vmz + (float)z;
+        const float nx = axis_near(L.px, L.dx, vmx, vmx + 1.0f);
+        const float ny = axis_near(L.py, L.dy, vmy, vmy + 1.0f);
+        const float nz = axis_near(L.pz, L.dz, vmz, vmz + 1.0f);
+        const float enter = fmax2(fmax2(nx, ny), nz);
+        res.distance = L.dist + enter;
+        res.node = P.n_inner + L.leaf;
+        res.voxel = 1024u * (uint32_t)z + 32u * (uint32_t)y + (uint32_t)x;
+        uint32_t ax = nx > ny ? 0u : 1u;
+        const float nax = ax == 0u ? nx : ny;
+        if (!(nax > nz)) ax = 2u;
+        const float dc = ax == 0u ? L.dx : (ax == 1u ? L.dy : L.dz);
+        res.flags = (res.distance <= maxd ? 1u : 0u) | 2u | (ax << 2) | (dc > 0.0f ? 0u : 16u);
+        res.steps = L.steps;
+        res.pad = 0u;
+        done = true;
+    }
+    if (__any(adv)) {
+        /* leave the child box, then pop every node the position has left
+         * (wave-uniform loop, lanes predicated) */
         uint32_t n = L.node;
-        const float cs = (float)(1u << (P.depth - ND_LVL(n) - 1u));
-        float cmx, cmy, cmz;
-        child_min(n, ND_RC(n), half, cs, cmx, cmy, cmz);
-        const float t_adv = box_exit(L.px, L.py, L.pz, L.dx, L.dy, L.dz, cmx, cmy, cmz, cs) + SVO_EPSILON;
-        L.px = L.px + L.dx * t_adv;
-        L.py = L.py + L.dy * t_adv;
-        L.pz = L.pz + L.dz * t_adv;
-        L.dist += t_adv;
         uint32_t cur = L.cur;
-        while (cur != SVO_NONE) {
-            const uint32_t lvl = ND_LVL(n);
-            const float ns = (float)(1u << (P.depth - lvl));
-            const float nmx = -half + (float)ND_CX(n) * ns, nmy = -half + (float)ND_CY(n) * ns,
-                        nmz = -half + (float)ND_CZ(n) * ns;
-            if (axis_exit_gt_eps(L.px, L.dx, nmx, nmx + ns) && axis_exit_gt_eps(L.py, L.dy, nmy, nmy + ns) &&
-                axis_exit_gt_eps(L.pz, L.dz, nmz, nmz + ns))
-                break;
-            const uint32_t par = inner_at<kLds>(P.inner, s_inner, cur).y & 0xFFFFFFu;
-            cur = par == 0xFFFFFFu ? SVO_NONE : par;
-            n = (ND_CX(n) >> 1) | ((ND_CY(n) >> 1) << 7) | ((ND_CZ(n) >> 1) << 14) | (((lvl - 1u) & 7u) << 21);
+        if (adv) {
+            const float cs = (float)(1u << (P.depth - ND_LVL(n) - 1u));
+            float cmx, cmy, cmz;
+            child_min(n, ND_RC(n), half, cs, cmx, cmy, cmz);
+            const float t_adv = box_exit(L.px, L.py, L.pz, L.dx, L.dy, L.dz, cmx, cmy, cmz, cs) + SVO_EPSILON;
+            L.px = L.px + L.dx * t_adv;
+            L.py = L.py + L.dy * t_adv;
+            L.pz = L.pz + L.dz * t_adv;
+            L.dist += t_adv;
         }
-        L.cur = cur;
-        L.node = n;
+        /* a wave with a direction component below 2^-100 in an advancing
+         * lane takes the dividing pop test; all others the exact
+         * division-free one */
+        const bool tiny = adv && (fabsf(L.dx) < 0x1p-100f || fabsf(L.dy) < 0x1p-100f || fabsf(L.dz) < 0x1p-100f);
+        if (__any(tiny))
+            pop_walk<kLds, true>(P, s_inner, half, L, adv, cur, n);
+        else
+            pop_walk<kLds, false>(P, s_inner, half, L, adv, cur, n);
+        if (adv) {
+            L.cur = cur;
+            L.node = n;
+        }
     }
-    return false;
+    return done;
 }
 
 /* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd; the
@@ -376,7 +445,8 @@ template <bool kLds>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
-                                                          uint32_t* __restrict__ hit_counter)
+                                                          uint32_t* __restrict__ hit_counter, uint32_t refill,
+                                                          uint32_t vote)
 {
     extern __shared__ uint2 s_inner[];
     if constexpr (kLds) {
@@ -388,14 +458,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
     const uint32_t n_rays = *n_rays_ptr;
     const uint32_t lane = threadIdx.x & 63u;
     const float half = (float)(1u << P.depth) * 0.5f;
-    trace_lane L;
+    trace_lane L = {};
     uint32_t slot = 0, hits = 0;
     bool active = false, exhausted = false;
     while (true) {
-        /* wave-uniform refill once GI_REFILL_LANES lanes are idle */
+        /* wave-uniform refill once `refill` lanes are idle */
         const unsigned long long idle = __ballot(!active);
         const uint32_t want = (uint32_t)__popcll(idle);
-        if (!exhausted && want >= GI_REFILL_LANES) {
+        if (!exhausted && want >= refill) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(cursor, want);
             base = __shfl(base, 0);
@@ -413,8 +483,17 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             if (exhausted) break;
             continue;
         }
+        /* phase vote (tuning): a wave runs only node steps or only DDA bursts
+         * in an iteration, whichever mode more of its active lanes are in */
+        bool allow_node = true, allow_dda = true;
+        if (vote) {
+            const uint32_t n_dda = (uint32_t)__popcll(__ballot(active && (L.node & ND_DDA)));
+            const uint32_t n_all = (uint32_t)__popcll(__ballot(active));
+            allow_dda = 2u * n_dda >= n_all;
+            allow_node = !allow_dda;
+        }
         svo_res res;
-        if (active && lane_step<kLds>(P, s_inner, half, maxd, L, res)) {
+        if (lane_step<kLds>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda) && active) {
             out[slot] = res;
             hits += res.flags & 1u;
             active = false;
@@ -644,7 +723,7 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, const u
         float ox = b.ox + b.dx * r.distance, oy = b.oy + b.dy * r.distance, oz = b.oz + b.dz * r.distance;
         const float off = sign * GI_SHADOW_OFFSET;
         if (ax == 0u) ox = ox + off; else if (ax == 1u) oy = oy + off; else oz = oz + off;
-        const svo_ray sr = {ox, oy, oz, prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2], b.id, 0u};
+        const svo_ray sr = {ox, oy, oz, prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2], b.id, slot};
         shadow[slot] = sr;
         const uint16_t m = leaf_mats[(size_t)(r.node - n_inner) * TGX_SVO_BLOCK_VOXELS + r.voxel];
         const tgx_lut_entry e = luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
@@ -656,15 +735,16 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, const u
     }
 }
 
-/* Unshadowed samples store their contribution at (pixel, sample). */
+/* Unshadowed samples store their contribution at (entry, sample). */
 __global__ __launch_bounds__(256) void gi_lit_k(const svo_ray* __restrict__ shadow, const uint32_t* __restrict__ n_shadow,
                                                 const svo_res* __restrict__ res, const float* __restrict__ weight,
                                                 float* __restrict__ contrib)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= *n_shadow || (res[j].flags & 1u)) return;
-    const size_t c = 3ull * shadow[j].id;
-    contrib[c] = weight[3ull * j]; contrib[c + 1] = weight[3ull * j + 1]; contrib[c + 2] = weight[3ull * j + 2];
+    const svo_ray r = shadow[j];
+    const size_t c = 3ull * r.id, w = 3ull * r.pad; /* pad = the ray's weight slot */
+    contrib[c] = weight[w]; contrib[c + 1] = weight[w + 1]; contrib[c + 2] = weight[w + 2];
 }
 
 /* Per list entry: E = (sum of its sample contributions in sample order) /
@@ -700,6 +780,42 @@ __global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf_rec* __restric
         radiance[3ull * pix + 2] = s.albedo[2] * e2;
     }
     if (irradiance) { irradiance[3ull * pix] = e0; irradiance[3ull * pix + 1] = e1; irradiance[3ull * pix + 2] = e2; }
+}
+
+/* Sort key of a queued ray: its direction octant, then the Morton code of
+ * the 8-voxel cell of its origin (relative to the SVO root, 256 cells per
+ * axis), so rays that leave one region in similar directions end up in the
+ * same wave. Slots past the queue get the largest key. */
+__device__ static inline uint32_t spread8(uint32_t v)
+{
+    v &= 0xFFu;
+    v = (v | (v << 8)) & 0x0000F00Fu;
+    v = (v | (v << 4)) & 0x000C30C3u;
+    v = (v | (v << 2)) & 0x00249249u;
+    return v;
+}
+
+__global__ void gi_ray_keys_k(const svo_ray* __restrict__ rays, const uint32_t* __restrict__ n_ptr, uint32_t cap,
+                              float mnx, float mny, float mnz, float cell_scale, uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    idx[i] = i;
+    if (i >= *n_ptr) { keys[i] = 0xFFFFFFFFu; return; }
+    const svo_ray r = rays[i];
+    const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+    const uint32_t cx = (uint32_t)fminf(fmaxf((r.ox - mnx) * cell_scale, 0.0f), 255.0f);
+    const uint32_t cy = (uint32_t)fminf(fmaxf((r.oy - mny) * cell_scale, 0.0f), 255.0f);
+    const uint32_t cz = (uint32_t)fminf(fmaxf((r.oz - mnz) * cell_scale, 0.0f), 255.0f);
+    keys[i] = (oct << 24) | spread8(cx) | (spread8(cy) << 1) | (spread8(cz) << 2);
+}
+
+__global__ void gi_gather_rays_k(const svo_ray* __restrict__ src, const uint32_t* __restrict__ idx,
+                                 const uint32_t* __restrict__ n_ptr, svo_ray* __restrict__ dst)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *n_ptr) dst[i] = src[idx[i]];
 }
 
 /* svo_res -> tgx_svo_hit for the C ABI */
@@ -742,8 +858,9 @@ static trace_params params_of(tgx_context* ctx)
     return P;
 }
 
-/* Persistent trace of the queue rays[0 .. *n_dev) (n_max bounds it): two
- * 1024-lane workgroups per CU, the inner nodes in LDS when they fit. */
+/* Persistent trace of the queue rays[0 .. *n_dev) (n_max bounds it):
+ * 1024-lane workgroups (tuning: per CU), the traversal layout in LDS when it
+ * fits. */
 static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t n_max, float maxd,
                         uint32_t* cursor, svo_res* out, uint32_t* hit_counter)
 {
@@ -751,19 +868,44 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     const uint32_t want = (uint32_t)((n_max + TRACE_BLOCK - 1) / TRACE_BLOCK);
-    const uint32_t cap = (uint32_t)cus * 2u;
+    const uint32_t cap = (uint32_t)cus * ctx->tuning.trace_blocks_per_cu;
     const dim3 grid(want < cap ? want : cap);
     const trace_params P = params_of(ctx);
+    const uint32_t refill = ctx->tuning.trace_refill;
+    const uint32_t vote = ctx->tuning.trace_vote;
     {
         tgx_launch_scope s(ctx, "gi_trace_k");
         if ((uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX)
             hipLaunchKernelGGL(gi_trace_k<true>, grid, dim3(TRACE_BLOCK), 8u * (P.n_inner + P.n_leaves), ctx->stream, P,
-                               rays, n_dev, maxd, cursor, out, hit_counter);
+                               rays, n_dev, maxd, cursor, out, hit_counter, refill, vote);
         else
             hipLaunchKernelGGL(gi_trace_k<false>, grid, dim3(TRACE_BLOCK), 0, ctx->stream, P, rays, n_dev, maxd, cursor,
-                               out, hit_counter);
+                               out, hit_counter, refill, vote);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
+    return TGX_OK;
+}
+
+/* Reorders the ray queue rays[0 .. *n_dev) into `sorted` by gi_ray_keys_k
+ * (cap bounds the count). */
+static int sort_rays(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t cap, svo_ray* sorted)
+{
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_sort, 16ull * cap, "gi ray sort"))) return rc;
+    uint32_t* keys = (uint32_t*)ctx->gi_sort.ptr;
+    uint32_t* idx = keys + cap;
+    uint32_t* keys2 = idx + cap;
+    uint32_t* idx2 = keys2 + cap;
+    hipLaunchKernelGGL(gi_ray_keys_k, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, rays, n_dev, cap,
+                       ctx->svo_info.min[0], ctx->svo_info.min[1], ctx->svo_info.min[2],
+                       256.0f / (float)(1u << ctx->svo_info.depth), keys, idx);
+    TGX_LAUNCH_CHECK(ctx, "gi_ray_keys_k");
+    if ((rc = tgx_prim_sort_pairs_u32(ctx, keys, keys2, idx, idx2, cap, 27u))) return rc;
+    {
+        tgx_launch_scope s(ctx, "gi_gather_rays_k");
+        hipLaunchKernelGGL(gi_gather_rays_k, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, rays, idx2, n_dev, sorted);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_gather_rays_k");
     return TGX_OK;
 }
 
@@ -839,6 +981,11 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
                            *prm, rays, contrib, cnt);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
+    if (ctx->tuning.trace_sort & 1u) {
+        if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_rays2, sizeof(svo_ray) * n_samples, "gi sorted rays"))) return rc;
+        if ((rc = sort_rays(ctx, rays, cnt + 1, (uint32_t)n_samples, (svo_ray*)ctx->gi_rays2.ptr))) return rc;
+        rays = (svo_ray*)ctx->gi_rays2.ptr;
+    }
     if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm->max_distance, cnt + 4, res, cnt + 2))) return rc;
     {
         tgx_launch_scope s(ctx, "gi_shade_k");
@@ -847,6 +994,12 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
                            (const tgx_lut_entry*)ctx->luts.ptr, *prm, rays, cnt + 1, res, shadow, weight, cnt);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_shade_k");
+    if (ctx->tuning.trace_sort & 2u) {
+        /* bounce rays are consumed; their buffer takes the sorted shadow rays */
+        svo_ray* dst = (svo_ray*)ctx->gi_rays.ptr;
+        if ((rc = sort_rays(ctx, shadow, cnt + 3, (uint32_t)n_samples, dst))) return rc;
+        shadow = dst;
+    }
     if ((rc = launch_trace(ctx, shadow, cnt + 3, (uint32_t)n_samples, prm->max_distance, cnt + 5, sres, cnt + 7))) return rc;
     {
         tgx_launch_scope s(ctx, "gi_lit_k");

@@ -83,7 +83,10 @@ struct tgx_context {
     tgx_buffer   gi_shadow_res;  /* shadow results                          */
     tgx_buffer   gi_weight;      /* RGB weight per shadow ray               */
     tgx_buffer   gi_contrib;     /* RGB contribution per (pixel, sample)    */
+    tgx_buffer   gi_rays2;       /* sorted bounce rays                      */
+    tgx_buffer   gi_sort;        /* sort keys and indices                   */
     uint32_t     gi_spp = 0;     /* spp of the last GI call                 */
+    tgx_tuning   tuning = {0u, 16u, 2u, 0u};
 
     /* Neural irradiance cache (tgx_nrc.hip). */
     bool           nrc_ready = false;

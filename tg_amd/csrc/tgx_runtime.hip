@@ -248,6 +248,23 @@ int tgx_memcpy_d2d(tgx_context* ctx, void* dst_dev, const void* src_dev, size_t 
     return TGX_OK;
 }
 
+int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out)
+{
+    if (!ctx || !out) return TGX_ERR_INVALID_ARG;
+    *out = ctx->tuning;
+    return TGX_OK;
+}
+
+int tgx_tuning_set(tgx_context* ctx, const tgx_tuning* t)
+{
+    if (!ctx || !t) return TGX_ERR_INVALID_ARG;
+    if (t->trace_sort > 3u || t->trace_refill < 1u || t->trace_refill > 64u || t->trace_blocks_per_cu < 1u ||
+        t->trace_blocks_per_cu > 2u || t->trace_vote > 1u)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "tuning out of range");
+    ctx->tuning = *t;
+    return TGX_OK;
+}
+
 int tgx_profile_enable(tgx_context* ctx, int enable)
 {
     if (!ctx) return TGX_ERR_INVALID_ARG;
