@@ -20,6 +20,7 @@ library's stream) and the CPU oracle timed on this host as the baseline.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import shutil
@@ -139,6 +140,115 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "config": {"network": "96(79 used)->64x4->3 ReLU, bf16 MFMA, fp32 Adam lr 1e-3", "train_samples": 1 << 16,
                    "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512"},
     }
+
+
+# Batched stress frame: 4096 noise-filled objects of 1..32 clusters per axis
+# in a 4096^3 world, 64 LUTs, a depth-12 SVO over the world, 16 seeded
+# cameras at 1920x1080 in one captured frame.
+BATCH_OBJECTS, BATCH_EXTENT, BATCH_WORLD, BATCH_LUTS = 4096, 32, 4096.0, 64
+BATCH_DEPTH, BATCH_CAMS = 12, 16
+
+
+def batch_bench(args, steps: int, pretrain: int) -> dict:
+    """Both forms of the batched frame as captured graphs: plain SVO GI and
+    the neural cache with fallback. Each is autotuned (uncaptured frames),
+    captured, replayed `pretrain` times untimed, then `steps` times timed.
+    Per-kernel times come from one uncaptured, event-profiled frame after
+    the timed replays (same kernels and inputs)."""
+    params = T.noise_objects_params(seed=args.seed, n_objects=BATCH_OBJECTS, max_extent=BATCH_EXTENT,
+                                    world=BATCH_WORLD, n_luts=BATCH_LUTS)
+    ctx = T.Context(0)
+    log("batch: device scene generation")
+    t0 = time.perf_counter()
+    ctx.generate_scene_device(params)
+    ctx.synchronize()
+    gen_ms = 1e3 * (time.perf_counter() - t0)
+    cams = np.concatenate([T.generate_camera(params, i, W, H) for i in range(BATCH_CAMS)])
+    mn = np.floor(cams[0]["camera"][:3]) - float(1 << (BATCH_DEPTH - 1))
+    log(f"batch: SVO build, depth {BATCH_DEPTH}")
+    t0 = time.perf_counter()
+    ctx.svo_build(mn, BATCH_DEPTH)
+    build_ms = 1e3 * (time.perf_counter() - t0)
+    info = ctx.svo_info()
+    prm = T.gi_params(T.seeded_sun(args.seed), spp=SPP, max_distance=1024.0)
+    n_px = BATCH_CAMS * W * H
+    vis_d, rad_d, irr_d = ctx.alloc(8 * n_px), ctx.alloc(12 * n_px), ctx.alloc(12 * n_px)
+    sizes = T._GenSizes()
+    T.load_library().tgx_gen_scene_sizes(ctypes.byref(params), ctypes.byref(sizes))
+    out = {"scene": {"objects": BATCH_OBJECTS, "clusters": int(sizes.n_clusters), "world": BATCH_WORLD,
+                     "luts": BATCH_LUTS, "generate_ms": gen_ms},
+           "svo": {"depth": BATCH_DEPTH, "build_ms": build_ms, "nodes": int(info["n_nodes"]),
+                   "leaves": int(info["n_leaves"])},
+           "cameras": BATCH_CAMS, "width": W, "height": H, "spp": SPP}
+    plain_rays = None
+    for mode, flags in (("plain", 0), ("neural", T.FRAME_NEURAL)):
+        if flags:
+            ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / BATCH_WORLD, train_samples=1 << 16,
+                                        train_batch=1 << 12, train_spp=4, fallback_threshold=1.0,
+                                        fallback_min_samples=1, cell_size=8))
+        log(f"batch {mode}: autotune")
+        t0 = time.perf_counter()
+        best = ctx.autotune(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags, frames=1)
+        tune_s = time.perf_counter() - t0
+        log(f"batch {mode}: capture, {pretrain} untimed replays")
+        t0 = time.perf_counter()
+        g = ctx.frame_graph(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
+        capture_s = time.perf_counter() - t0
+        try:
+            for _ in range(pretrain):
+                g.launch()
+            ctx.synchronize()
+            log(f"batch {mode}: {steps} timed replays")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.launch()
+            ctx.synchronize()
+            elapsed = time.perf_counter() - t0
+            st = g.stats()
+        finally:
+            g.destroy()
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.frame_batch(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
+        ctx.synchronize()
+        prof = ctx.profile_stats()
+        ctx.profile(False)
+        ms = 1e3 * elapsed / steps
+        traced_gi = int(st["bounce_rays"]) + int(st["shadow_rays"]) + int(st["train_rays"])
+        traced = int(st["primary_rays"]) + traced_gi
+        if flags == 0:
+            plain_rays = traced
+        k_ms = {k: v[1] for k, v in prof.items()}
+        dominant = max(k_ms, key=k_ms.get)
+        launches, total_ms = prof[dominant]
+        avg_ms = total_ms / launches
+        if dominant == "gi_trace_k":
+            # per ray: its 32-B queue record read and 24-B result written (the
+            # SVO reads, at most the 8 B/inner node + 4104 B/leaf structure
+            # per launch, are left out: a lower bound)
+            alg = 56 * traced_gi / launches
+        else:
+            alg = None
+        achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+        out[mode] = {
+            "frame_ms": ms,
+            "rays_traced_per_frame": traced,
+            "rays_traced_per_s": traced / (ms * 1e-3),
+            "stats": {k: int(st[k]) for k in st.dtype.names},
+            "tuning": {k: int(best[k][0]) for k in best.dtype.names},
+            "autotune_s": tune_s, "capture_s": capture_s,
+            "kernels_ms_per_frame": k_ms,
+            "roofline": {"bound": "hbm", "kernel": dominant, "kernel_avg_ms": avg_ms, "launches_per_frame": launches,
+                         "alg_bytes_per_launch": alg, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS if achieved else None},
+        }
+    if plain_rays is not None:
+        out["neural"]["rays_per_s_equivalent"] = plain_rays / (out["neural"]["frame_ms"] * 1e-3)
+        out["neural"]["speedup_vs_plain"] = out["plain"]["frame_ms"] / out["neural"]["frame_ms"]
+    for q in (vis_d, rad_d, irr_d):
+        ctx.free(q)
+    ctx.close()
+    return out
 
 
 COPY_BYTES = 256 << 20  # known-size copy that calibrates the HBM byte counters
@@ -272,7 +382,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=("frame", "visibility"), default="frame")
+    ap.add_argument("--workload", choices=("frame", "visibility", "batch"), default="frame")
+    ap.add_argument("--batch-pretrain", type=int, default=20, help="untimed replays of each batched frame graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-cache measurement")
     ap.add_argument("--neural-pretrain", type=int, default=200, help="untimed cache training frames")
@@ -285,6 +396,11 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would render whole "
               "cameras or screen tiles of its own; see the sharding design). Run with --gpus 1.", file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        b = batch_bench(args, args.steps, args.batch_pretrain)
+        print(json.dumps(b), flush=True)
+        return 0
 
     params = T.noise_objects_params(seed=args.seed)
     scene = T.generate_scene(params)

@@ -149,6 +149,12 @@ int         tgx_synchronize(tgx_context* ctx);
  * pointer or object index is out of range. */
 int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* scene);
 
+/* Copies the context's scene arrays back to host memory (each pointer may
+ * be NULL; sizes are those of the uploaded or generated scene). */
+int tgx_scene_export(tgx_context* ctx, tgx_object* objects, uint32_t* cluster_pointers,
+                     uint32_t* cluster_idx_to_object, uint32_t* bits, uint8_t* materials,
+                     tgx_lut_entry* luts);
+
 /* ------------------------------------------------------------------------ */
 /* Primary visibility                                                       */
 /* ------------------------------------------------------------------------ */
@@ -335,6 +341,59 @@ typedef struct tgx_tuning {
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);
 int tgx_tuning_set(tgx_context* ctx, const tgx_tuning* tuning);
+
+/* ------------------------------------------------------------------------ */
+/* Batched frames as captured HIP graphs. The reference renders one camera  */
+/* per tg_raytracer_render (tgvk_raytracer.c:1144-1553), recording and      */
+/* submitting its visibility, SVO and shading command buffers every frame;  */
+/* here a frame of n cameras is captured once and replayed.                 */
+/* ------------------------------------------------------------------------ */
+
+#define TGX_FRAME_NEURAL 1u   /* per camera the cache frame (tgx_nrc_frame_device) instead of tgx_gi_device */
+
+typedef struct tgx_frame_graph tgx_frame_graph;
+
+typedef struct tgx_frame_stats {
+    uint64_t primary_rays;          /* n_cams * w * h */
+    uint64_t visible_pixels;
+    uint64_t bounce_rays;           /* plain GI, or the cache's fallback pixels */
+    uint64_t shadow_rays;
+    uint64_t bounce_hits;
+    uint64_t train_rays;            /* bounce + shadow rays of training targets */
+    uint64_t fallback_pixels;
+} tgx_frame_stats;
+
+/* Runs one frame of n_cams cameras at w x h without capture (the work a
+ * graph records, see below); with stats_out it waits and fills the totals. */
+int tgx_frame_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                    const tgx_gi_params* gi, uint32_t flags, uint64_t* vis_dev, float* radiance_dev,
+                    float* irradiance_dev, tgx_frame_stats* stats_out);
+
+/* Captures one frame of n_cams cameras at w x h: visibility of all cameras
+ * into vis_dev (n_cams * w * h words, camera-major), then per camera c GI
+ * (or the cache frame) into radiance_dev + 3 c w h (and irradiance_dev,
+ * which may be null). Runs the frame once uncaptured first (its results
+ * are those of the first replay's inputs, advanced by one cache frame).
+ * While any graph exists the context refuses calls that would allocate,
+ * free or move buffers (scene upload, SVO build, cache creation, larger
+ * resolutions). Captured frames trace rays unsorted (trace_sort ignored). */
+int tgx_frame_graph_create(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                           const tgx_gi_params* gi, uint32_t flags, uint64_t* vis_dev, float* radiance_dev,
+                           float* irradiance_dev, tgx_frame_graph** out);
+/* Replays the frame on the context stream; returns without waiting. */
+int tgx_frame_graph_launch(tgx_context* ctx, tgx_frame_graph* g);
+/* Waits and returns the totals of the last replay. */
+int tgx_frame_graph_stats(tgx_context* ctx, tgx_frame_graph* g, tgx_frame_stats* out);
+int tgx_frame_graph_destroy(tgx_context* ctx, tgx_frame_graph* g);
+
+/* Times the frame (uncaptured, `frames` runs per candidate after one
+ * untimed run) over trace workgroups per CU {1, 2} x refill thresholds
+ * {4, 8, 16, 24, 32}, keeps the fastest in the context's tuning and
+ * returns it in best_out (may be null). Every candidate gives the same
+ * results; cache frames advance training once per run. */
+int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                 const tgx_gi_params* gi, uint32_t flags, uint64_t* vis_dev, float* radiance_dev,
+                 float* irradiance_dev, uint32_t frames, tgx_tuning* best_out);
 
 /* ------------------------------------------------------------------------ */
 /* Neural irradiance cache. No counterpart in the reference: a small MLP    */

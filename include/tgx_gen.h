@@ -84,6 +84,14 @@ void tgx_camera_forward(float pitch, float yaw, float out_forward[3]);
 int tgx_gen_camera(const tgx_gen_params* params, uint32_t camera_index,
                    uint32_t w, uint32_t h, tgx_camera* out);
 
+/* Builds the scene of `params` directly in the context's device buffers,
+ * replacing any previous scene, as if tgx_gen_scene_fill's arrays had been
+ * passed to tgx_scene_upload (identical bytes). NOISE_OBJECTS scenes are
+ * voxelised on the GPU (the per-voxel noise of the 9.4 GB material array
+ * of a 4096-object scene would take minutes on the host); the
+ * single-object scene is generated on the host and uploaded. */
+int tgx_gen_scene_device(tgx_context* ctx, const tgx_gen_params* params);
+
 /* splitmix64 stream seed for (seed, purpose). */
 uint64_t tgx_gen_stream_seed(uint64_t seed, uint64_t purpose);
 

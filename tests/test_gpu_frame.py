@@ -1,0 +1,222 @@
+"""Batched multi-camera frames: the device scene generator against the host
+generator, captured frame graphs against uncaptured frames and the
+visibility oracle, the context's buffer lock while a graph exists, and the
+autotuner (whose candidates must not change any result).
+
+The scene has the batched workload's shape at a size the oracle finishes
+in seconds: objects of 1..32 clusters per axis, value-noise solids, 64
+LUTs, several seeded cameras at the world centre."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tg_amd as T
+
+pytestmark = pytest.mark.gpu
+
+W, H = 320, 180
+N_CAMS = 4
+DEPTH = 10
+
+
+def say(msg: str) -> None:
+    print(f"[gpu] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_params(n_objects: int = 12):
+    return T.noise_objects_params(seed=7, n_objects=n_objects, max_extent=32, world=512.0, n_luts=64)
+
+
+@pytest.fixture(scope="module")
+def batch_scene():
+    p = batch_params()
+    sc = T.generate_scene(p)
+    cams = np.concatenate([T.generate_camera(p, i, W, H) for i in range(N_CAMS)])
+    mn = np.floor(cams[0]["camera"][:3]) - float(1 << (DEPTH - 1))
+    return p, sc, cams, mn
+
+
+class Buffers:
+    def __init__(self, ctx, n_cams: int, w: int, h: int):
+        self.ctx, self.n, self.w, self.h = ctx, n_cams, w, h
+        self.vis = ctx.alloc(8 * n_cams * w * h)
+        self.rad = ctx.alloc(12 * n_cams * w * h)
+        self.irr = ctx.alloc(12 * n_cams * w * h)
+
+    def read(self):
+        vis = np.empty((self.n, self.h, self.w), np.uint64)
+        rad = np.empty((self.n, self.h, self.w, 3), np.float32)
+        irr = np.empty((self.n, self.h, self.w, 3), np.float32)
+        self.ctx.d2h(vis, self.vis)
+        self.ctx.d2h(rad, self.rad)
+        self.ctx.d2h(irr, self.irr)
+        return vis, rad, irr
+
+    def free(self):
+        for q in (self.vis, self.rad, self.irr):
+            self.ctx.free(q)
+
+
+def setup_scene(ctx, p, mn):
+    say("device scene generation + SVO build")
+    ctx.generate_scene_device(p)
+    ctx.svo_build(mn, DEPTH)
+
+
+@pytest.mark.parametrize("kind", ["noise", "single"])
+def test_device_scene_equals_host_scene(ctx, kind):
+    p = batch_params(20) if kind == "noise" else T.single_object_params(3)
+    say(f"device generator vs host generator ({kind})")
+    ctx.generate_scene_device(p)
+    got = ctx.scene_export(p)
+    ref = T.generate_scene(p)
+    for name in ("objects", "cluster_pointers", "cluster_idx_to_object", "bits", "materials", "luts"):
+        a, b = getattr(got, name), getattr(ref, name)
+        assert a.shape == b.shape and a.tobytes() == b.tobytes(), name
+    if kind == "noise":
+        assert 0.3 < ref.solid_fraction() < 0.5
+        n = ref.objects["n_cluster_pointers_per_dim"]
+        assert n.min() >= 1 and n.max() <= 32 and n.max() > 8
+
+
+def test_frame_graph_plain_matches_uncaptured_and_oracle(ctx, batch_scene):
+    p, sc, cams, mn = batch_scene
+    setup_scene(ctx, p, mn)
+    prm = T.gi_params(T.seeded_sun(4), spp=2, frame=3)
+    buf = Buffers(ctx, N_CAMS, W, H)
+    try:
+        say(f"frame graph: {N_CAMS} cameras {W}x{H}, plain GI")
+        g = ctx.frame_graph(cams, W, H, prm, buf.vis, buf.rad, buf.irr)
+        try:
+            g.launch()
+            st1 = g.stats()
+            vis1, rad1, irr1 = buf.read()
+            g.launch()
+            st2 = g.stats()
+            vis2, rad2, irr2 = buf.read()
+        finally:
+            g.destroy()
+        # plain GI draws the samples of prm.frame: replays are identical
+        assert vis1.tobytes() == vis2.tobytes() and rad1.tobytes() == rad2.tobytes()
+        assert irr1.tobytes() == irr2.tobytes() and tuple(st1) == tuple(st2)
+        osc = O.OracleScene(sc)
+        visible = bounce = shadow = hits = 0
+        for c in range(N_CAMS):
+            say(f"uncaptured frame + oracle visibility, camera {c}")
+            v, r, i = ctx.frame(cams[c:c + 1], W, H, prm)
+            gs = ctx.gi_stats()
+            ref_vis, _ = O.visibility_culled(osc, cams[c:c + 1], W, H)
+            assert np.array_equal(vis1[c], ref_vis), f"camera {c}: visibility differs from the oracle"
+            assert np.array_equal(vis1[c], v)
+            assert rad1[c].tobytes() == r.tobytes(), f"camera {c}: radiance differs from the uncaptured frame"
+            assert irr1[c].tobytes() == i.tobytes()
+            visible += int(gs["visible_pixels"])
+            bounce += int(gs["bounce_rays"])
+            shadow += int(gs["shadow_rays"])
+            hits += int(gs["bounce_hits"])
+        assert st1["primary_rays"] == N_CAMS * W * H
+        assert st1["visible_pixels"] == visible == int((vis1 != T.VISIBILITY_EMPTY).sum())
+        assert (st1["bounce_rays"], st1["shadow_rays"], st1["bounce_hits"]) == (bounce, shadow, hits)
+        assert st1["train_rays"] == 0 and visible > 0.3 * N_CAMS * W * H and rad1.sum() > 0
+    finally:
+        buf.free()
+
+
+def test_graph_locks_context_buffers(ctx, batch_scene):
+    p, sc, cams, mn = batch_scene
+    setup_scene(ctx, p, mn)
+    prm = T.gi_params(T.seeded_sun(4), spp=1)
+    buf = Buffers(ctx, 2, W, H)
+    try:
+        g = ctx.frame_graph(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr)
+        try:
+            with pytest.raises(T.TgxError):
+                ctx.svo_build(mn, DEPTH)
+            with pytest.raises(T.TgxError):
+                ctx.nrc_create(T.nrc_config())
+            big = Buffers(ctx, 1, 4 * W, 4 * H)
+            try:
+                with pytest.raises(T.TgxError):
+                    ctx.frame_device(cams[0:1], 4 * W, 4 * H, prm, big.vis, big.rad, big.irr)
+            finally:
+                big.free()
+            g.launch()
+            assert g.stats()["primary_rays"] == 2 * W * H
+        finally:
+            g.destroy()
+        ctx.svo_build(mn, DEPTH)  # unlocked again
+    finally:
+        buf.free()
+
+
+def test_frame_graph_neural_matches_uncaptured(ctx, batch_scene):
+    p, sc, cams, mn = batch_scene
+    setup_scene(ctx, p, mn)
+    prm = T.gi_params(T.seeded_sun(4), spp=1, frame=5)
+    n_cams, replays = 3, 2
+    cfg = T.nrc_config(origin=tuple(mn), scale=1.0 / (1 << DEPTH), train_samples=4096, train_batch=2048, seed=9)
+    steps_per_frame = n_cams * 4096 // 2048
+    buf = Buffers(ctx, n_cams, W, H)
+    try:
+        ctx.nrc_create(cfg)
+        say(f"frame graph: {n_cams} cameras {W}x{H}, neural cache, {replays} replays")
+        g = ctx.frame_graph(cams[:n_cams], W, H, prm, buf.vis, buf.rad, buf.irr, flags=T.FRAME_NEURAL)
+        try:
+            for _ in range(replays):
+                g.launch()
+            st = g.stats()
+            vis_g, rad_g, irr_g = buf.read()
+            assert ctx.nrc_stats()["adam_steps"] == (1 + replays) * steps_per_frame
+            w_g = ctx.nrc_params()
+        finally:
+            g.destroy()
+        assert st["train_rays"] > 0 and st["visible_pixels"] == int((vis_g != T.VISIBILITY_EMPTY).sum())
+        # the same frames uncaptured, camera by camera, from a fresh cache
+        ctx.nrc_create(cfg)
+        one = Buffers(ctx, 1, W, H)
+        try:
+            for f in range(1 + replays):
+                for c in range(n_cams):
+                    ctx.nrc_frame_device(cams[c:c + 1], W, H, prm, one.vis, one.rad, one.irr)
+                    if f == replays:
+                        v, r, i = one.read()
+                        assert np.array_equal(v[0], vis_g[c])
+                        assert r[0].tobytes() == rad_g[c].tobytes(), f"camera {c}: cache radiance differs"
+                        assert i[0].tobytes() == irr_g[c].tobytes()
+        finally:
+            one.free()
+        assert ctx.nrc_params().tobytes() == w_g.tobytes()
+        assert ctx.nrc_stats()["adam_steps"] == (1 + replays) * steps_per_frame
+    finally:
+        buf.free()
+
+
+def test_autotune_keeps_results(ctx, batch_scene):
+    p, sc, cams, mn = batch_scene
+    setup_scene(ctx, p, mn)
+    prm = T.gi_params(T.seeded_sun(4), spp=2, frame=1)
+    ctx.set_tuning(trace_sort=0, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0)
+    buf = Buffers(ctx, 2, W, H)
+    try:
+        g = ctx.frame_graph(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr)
+        g.launch()
+        before = buf.read()
+        g.destroy()
+        say("autotune over 10 candidates, 2 cameras")
+        best = ctx.autotune(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr, frames=1)
+        assert tuple(ctx.tuning()[0]) == tuple(best[0])
+        assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 8, 16, 24, 32)
+        assert best["trace_sort"][0] == 0 and best["trace_vote"][0] == 0
+        g = ctx.frame_graph(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr)
+        g.launch()
+        after = buf.read()
+        g.destroy()
+        for a, b in zip(before, after):
+            assert a.tobytes() == b.tobytes()
+    finally:
+        ctx.set_tuning(trace_sort=0, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0)
+        buf.free()

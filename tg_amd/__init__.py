@@ -81,6 +81,11 @@ NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learni
 NRC_STATS_DTYPE = np.dtype([("visible_pixels", "<u8"), ("fallback_pixels", "<u8"), ("train_samples", "<u8"),
                             ("train_rays", "<u8"), ("fallback_rays", "<u8"), ("adam_steps", "<u4"),
                             ("train_loss", "<f4")])
+FRAME_NEURAL = 1
+FRAME_STATS_DTYPE = np.dtype([("primary_rays", "<u8"), ("visible_pixels", "<u8"), ("bounce_rays", "<u8"),
+                              ("shadow_rays", "<u8"), ("bounce_hits", "<u8"), ("train_rays", "<u8"),
+                              ("fallback_pixels", "<u8")])
+
 NRC_FLAG_NO_FALLBACK = 1
 NRC_FLAG_ALL_FALLBACK = 2
 NRC_FLAG_NO_TRAINING = 4
@@ -174,6 +179,8 @@ def load_library() -> ctypes.CDLL:
         "tgx_memcpy_d2h": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "tgx_memcpy_h2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "tgx_memcpy_d2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
+        "tgx_gen_scene_device": (i32, [vp, ctypes.POINTER(_GenParams)]),
+        "tgx_scene_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "tgx_tuning_get": (i32, [vp, vp]),
         "tgx_tuning_set": (i32, [vp, vp]),
         "tgx_profile_enable": (i32, [vp, i32]),
@@ -199,6 +206,12 @@ def load_library() -> ctypes.CDLL:
         "tgx_nrc_train_batch": (i32, [vp, vp, vp, u32, vp, vp]),
         "tgx_nrc_frame_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
         "tgx_nrc_last_stats": (i32, [vp, vp]),
+        "tgx_frame_graph_create": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, ctypes.POINTER(vp)]),
+        "tgx_frame_batch": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, vp]),
+        "tgx_frame_graph_launch": (i32, [vp, vp]),
+        "tgx_frame_graph_stats": (i32, [vp, vp, vp]),
+        "tgx_frame_graph_destroy": (i32, [vp, vp]),
+        "tgx_autotune": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, u32, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -359,6 +372,28 @@ class Context:
         desc = scene.desc()
         self._check(self.lib.tgx_scene_upload(self.h, ctypes.byref(desc)), "tgx_scene_upload")
 
+    def generate_scene_device(self, params: _GenParams) -> None:
+        """Builds the scene of `params` in device memory (tgx_gen_scene_device)."""
+        self._scene_ref = None
+        self._check(self.lib.tgx_gen_scene_device(self.h, ctypes.byref(params)), "tgx_gen_scene_device")
+
+    def scene_export(self, params: _GenParams) -> Scene:
+        """The context's scene arrays, sized by the generator parameters."""
+        sizes = _GenSizes()
+        rc = self.lib.tgx_gen_scene_sizes(ctypes.byref(params), ctypes.byref(sizes))
+        if rc:
+            raise TgxError(rc, "tgx_gen_scene_sizes")
+        s = Scene(objects=np.zeros(sizes.n_objects, OBJECT_DTYPE),
+                  cluster_pointers=np.zeros(sizes.n_cluster_pointers, np.uint32),
+                  cluster_idx_to_object=np.zeros(sizes.n_clusters, np.uint32),
+                  bits=np.zeros(16 * sizes.n_clusters, np.uint32),
+                  materials=np.zeros(512 * sizes.n_clusters, np.uint8),
+                  luts=np.zeros(256 * sizes.n_luts, LUT_DTYPE))
+        self._check(self.lib.tgx_scene_export(self.h, _ptr(s.objects), _ptr(s.cluster_pointers),
+                                              _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials),
+                                              _ptr(s.luts)), "tgx_scene_export")
+        return s
+
     def visibility(self, cams: np.ndarray, w: int, h: int) -> np.ndarray:
         cams = np.ascontiguousarray(cams, dtype=CAMERA_DTYPE)
         out = np.empty((cams.size, h, w), np.uint64)
@@ -518,6 +553,39 @@ class Context:
                 self.free(p)
         return vis, rad, irr
 
+    # -- batched frames as captured graphs --------------------------------------
+
+    def frame_graph(self, cams: np.ndarray, w: int, h: int, params: np.ndarray, vis_dev: int, radiance_dev: int,
+                    irradiance_dev: int | None = None, flags: int = 0) -> "FrameGraph":
+        """Captures a frame of len(cams) cameras (runs it once uncaptured first)."""
+        cams = np.ascontiguousarray(np.atleast_1d(cams), dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
+        g = ctypes.c_void_p()
+        self._check(self.lib.tgx_frame_graph_create(self.h, _ptr(cams), cams.shape[0], w, h, _ptr(params), flags,
+                                                    vis_dev, radiance_dev, irradiance_dev, ctypes.byref(g)),
+                    "tgx_frame_graph_create")
+        return FrameGraph(self, g.value)
+
+    def frame_batch(self, cams: np.ndarray, w: int, h: int, params: np.ndarray, vis_dev: int, radiance_dev: int,
+                    irradiance_dev: int | None = None, flags: int = 0, stats: bool = False):
+        """One uncaptured frame of len(cams) cameras; with stats=True waits and returns its totals."""
+        cams = np.ascontiguousarray(np.atleast_1d(cams), dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
+        st = np.zeros(1, FRAME_STATS_DTYPE) if stats else None
+        self._check(self.lib.tgx_frame_batch(self.h, _ptr(cams), cams.shape[0], w, h, _ptr(params), flags, vis_dev,
+                                             radiance_dev, irradiance_dev, _ptr(st)), "tgx_frame_batch")
+        return st[0] if stats else None
+
+    def autotune(self, cams: np.ndarray, w: int, h: int, params: np.ndarray, vis_dev: int, radiance_dev: int,
+                 irradiance_dev: int | None = None, flags: int = 0, frames: int = 2) -> np.ndarray:
+        """Times the frame over the tuning candidates; keeps and returns the fastest."""
+        cams = np.ascontiguousarray(np.atleast_1d(cams), dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
+        best = np.zeros(1, TUNING_DTYPE)
+        self._check(self.lib.tgx_autotune(self.h, _ptr(cams), cams.shape[0], w, h, _ptr(params), flags, vis_dev,
+                                          radiance_dev, irradiance_dev, frames, _ptr(best)), "tgx_autotune")
+        return best
+
     def alloc(self, nbytes: int) -> int:
         p = ctypes.c_void_p()
         self._check(self.lib.tgx_device_alloc(self.h, nbytes, ctypes.byref(p)), "tgx_device_alloc")
@@ -560,3 +628,29 @@ class Context:
         buf = (KernelStat * 64)()
         self._check(self.lib.tgx_profile_get(self.h, buf, 64, ctypes.byref(n)), "tgx_profile_get")
         return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(min(n.value, 64))}
+
+
+class FrameGraph:
+    """A captured multi-camera frame (tgx_frame_graph_*)."""
+
+    def __init__(self, ctx: Context, handle: int):
+        self.ctx, self.handle = ctx, handle
+
+    def launch(self) -> None:
+        self.ctx._check(self.ctx.lib.tgx_frame_graph_launch(self.ctx.h, self.handle), "tgx_frame_graph_launch")
+
+    def stats(self):
+        st = np.zeros(1, FRAME_STATS_DTYPE)
+        self.ctx._check(self.ctx.lib.tgx_frame_graph_stats(self.ctx.h, self.handle, _ptr(st)), "tgx_frame_graph_stats")
+        return st[0]
+
+    def destroy(self) -> None:
+        if self.handle and self.ctx.h:
+            self.ctx._check(self.ctx.lib.tgx_frame_graph_destroy(self.ctx.h, self.handle), "tgx_frame_graph_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()

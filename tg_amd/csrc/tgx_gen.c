@@ -3,9 +3,12 @@
  * Declarations and stream layout: tgx_gen.h.
  */
 #include "tgx_gen.h"
+#include "tgx_gen_int.h"
 #include "tgx_m4.h"
+#include "tgx_noise.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define TGX_PI_D 3.14159265358979323846
@@ -21,13 +24,7 @@ enum {
     STREAM_LUT_IDX   = 8
 };
 
-static uint64_t splitmix64_next(uint64_t* s)
-{
-    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
+#define splitmix64_next tgx_splitmix64_next
 
 uint64_t tgx_gen_stream_seed(uint64_t seed, uint64_t purpose)
 {
@@ -42,48 +39,6 @@ static double rng_unit(uint64_t* s) { return (double)(splitmix64_next(s) >> 11) 
 static float rng_unit_f(uint64_t* s) { return (float)(splitmix64_next(s) >> 40) * (1.0f / 16777216.0f); }
 /* Uniform integer in [0, n). */
 static uint32_t rng_below(uint64_t* s, uint32_t n) { return (uint32_t)(((splitmix64_next(s) >> 32) * (uint64_t)n) >> 32); }
-
-/* ------------------------------------------------------------------------ */
-/* Value noise: lattice values hashed from integer coordinates, trilinear   */
-/* interpolation with smoothstep weights.                                   */
-/* ------------------------------------------------------------------------ */
-
-static double lattice(uint64_t key, int64_t x, int64_t y, int64_t z)
-{
-    uint64_t h = key;
-    h ^= (uint64_t)x * 0x9E3779B97F4A7C15ull;
-    h ^= (uint64_t)y * 0xC2B2AE3D27D4EB4Full;
-    h ^= (uint64_t)z * 0x165667B19E3779F9ull;
-    (void)splitmix64_next(&h);
-    return (double)(splitmix64_next(&h) >> 11) * (1.0 / 9007199254740992.0);
-}
-
-static double smooth(double t) { return t * t * (3.0 - 2.0 * t); }
-
-static double value_noise(uint64_t key, double x, double y, double z, double period)
-{
-    const double px = x / period, py = y / period, pz = z / period;
-    const double fx0 = floor(px), fy0 = floor(py), fz0 = floor(pz);
-    const int64_t ix = (int64_t)fx0, iy = (int64_t)fy0, iz = (int64_t)fz0;
-    const double tx = smooth(px - fx0), ty = smooth(py - fy0), tz = smooth(pz - fz0);
-    double c[2][2][2];
-    for (int dz = 0; dz < 2; dz++)
-        for (int dy = 0; dy < 2; dy++)
-            for (int dx = 0; dx < 2; dx++)
-                c[dz][dy][dx] = lattice(key, ix + dx, iy + dy, iz + dz);
-    const double x00 = c[0][0][0] + (c[0][0][1] - c[0][0][0]) * tx;
-    const double x10 = c[0][1][0] + (c[0][1][1] - c[0][1][0]) * tx;
-    const double x01 = c[1][0][0] + (c[1][0][1] - c[1][0][0]) * tx;
-    const double x11 = c[1][1][0] + (c[1][1][1] - c[1][1][0]) * tx;
-    const double y0 = x00 + (x10 - x00) * ty;
-    const double y1 = x01 + (x11 - x01) * ty;
-    return y0 + (y1 - y0) * tz;
-}
-
-/* Threshold below which value noise leaves ~40% of voxels solid (measured
- * over 2^22 samples of this noise; the fraction of a generated scene is
- * checked by the tests). */
-#define TGX_NOISE_SOLID_THRESHOLD 0.4493
 
 /* ------------------------------------------------------------------------ */
 
@@ -145,7 +100,7 @@ static void random_rotation(uint64_t* s, float m[16])
     m[15] = 1.0f;
 }
 
-static void fill_luts(const tgx_gen_params* p, tgx_lut_entry* luts)
+void tgx_gen_luts(const tgx_gen_params* p, tgx_lut_entry* luts)
 {
     uint64_t s = tgx_gen_stream_seed(p->seed, STREAM_LUT);
     for (uint32_t i = 0; i < TGX_LUT_ENTRIES * p->n_luts; i++) {
@@ -181,16 +136,13 @@ static void fill_single(const tgx_gen_params* p, tgx_object* objects, uint32_t* 
         materials[i] = (uint8_t)(splitmix64_next(&sm) >> 56);
 }
 
-static void fill_noise(const tgx_gen_params* p, tgx_object* objects, uint32_t* cptrs,
-                       uint32_t* c2o, uint32_t* bits, uint8_t* materials)
+int tgx_gen_noise_objects(const tgx_gen_params* p, tgx_object* objects, double* noise_offsets)
 {
+    if (!params_ok(p) || p->kind != TGX_GEN_NOISE_OBJECTS || !objects || !noise_offsets) return TGX_ERR_INVALID_ARG;
     uint64_t se = tgx_gen_stream_seed(p->seed, STREAM_EXTENTS);
     uint64_t st = tgx_gen_stream_seed(p->seed, STREAM_TRANSFORM);
     uint64_t sl = tgx_gen_stream_seed(p->seed, STREAM_LUT_IDX);
-    const uint64_t key_solid = tgx_gen_stream_seed(p->seed, STREAM_SOLID);
-    const uint64_t key_mat = tgx_gen_stream_seed(p->seed, STREAM_MATERIAL);
     uint64_t sn = tgx_gen_stream_seed(p->seed, STREAM_NOISE);
-
     uint32_t cptr = 0;
     for (uint32_t oi = 0; oi < p->n_objects; oi++) {
         tgx_object* o = &objects[oi];
@@ -207,11 +159,33 @@ static void fill_noise(const tgx_gen_params* p, tgx_object* objects, uint32_t* c
         o->translation[2] = (float)(rng_unit(&st) * (double)p->world);
         o->lut_idx = p->n_luts > 1 ? rng_below(&sl, p->n_luts) : 0u;
         /* Per-object offset into the noise field, so objects differ. */
-        const double ox = rng_unit(&sn) * 65536.0, oy = rng_unit(&sn) * 65536.0, oz = rng_unit(&sn) * 65536.0;
+        noise_offsets[3 * oi] = rng_unit(&sn) * 65536.0;
+        noise_offsets[3 * oi + 1] = rng_unit(&sn) * 65536.0;
+        noise_offsets[3 * oi + 2] = rng_unit(&sn) * 65536.0;
+        cptr += n[0] * n[1] * n[2];
+    }
+    return TGX_OK;
+}
 
+uint64_t tgx_gen_noise_key(uint64_t seed, int material)
+{
+    return tgx_gen_stream_seed(seed, material ? STREAM_MATERIAL : STREAM_SOLID);
+}
+
+static void fill_noise(const tgx_gen_params* p, tgx_object* objects, uint32_t* cptrs,
+                       uint32_t* c2o, uint32_t* bits, uint8_t* materials)
+{
+    const uint64_t key_solid = tgx_gen_noise_key(p->seed, 0);
+    const uint64_t key_mat = tgx_gen_noise_key(p->seed, 1);
+    double* off = (double*)malloc(sizeof(double) * 3 * p->n_objects);
+    if (!off) return;
+    (void)tgx_gen_noise_objects(p, objects, off);
+    for (uint32_t oi = 0; oi < p->n_objects; oi++) {
+        const tgx_object* o = &objects[oi];
+        const uint32_t* n = o->n_cluster_pointers_per_dim;
         const uint32_t nc = n[0] * n[1] * n[2];
         for (uint32_t rc = 0; rc < nc; rc++) {
-            const uint32_t c = cptr + rc; /* cluster idx == pointer */
+            const uint32_t c = o->first_cluster_pointer + rc; /* cluster idx == pointer */
             cptrs[c] = c;
             c2o[c] = oi;
             const uint32_t cx = rc % n[0], cy = (rc / n[0]) % n[1], cz = rc / (n[0] * n[1]);
@@ -219,17 +193,14 @@ static void fill_noise(const tgx_gen_params* p, tgx_object* objects, uint32_t* c
             memset(w, 0, TGX_BITS_WORDS_PER_CLUSTER * sizeof(uint32_t));
             uint8_t* mat = &materials[(size_t)c * TGX_VOXELS_PER_CLUSTER];
             for (uint32_t v = 0; v < TGX_VOXELS_PER_CLUSTER; v++) {
-                const double x = ox + (double)(8u * cx + (v & 7u)) + 0.5;
-                const double y = oy + (double)(8u * cy + ((v >> 3) & 7u)) + 0.5;
-                const double z = oz + (double)(8u * cz + (v >> 6)) + 0.5;
-                if (value_noise(key_solid, x, y, z, 32.0) < TGX_NOISE_SOLID_THRESHOLD)
-                    w[v >> 5] |= 1u << (v & 31u);
-                double m = value_noise(key_mat, x, y, z, 16.0) * 256.0;
-                mat[v] = (uint8_t)(m >= 255.0 ? 255.0 : m);
+                int solid;
+                tgx_noise_voxel(key_solid, key_mat, off[3 * oi], off[3 * oi + 1], off[3 * oi + 2], cx, cy, cz, v,
+                                &solid, &mat[v]);
+                if (solid) w[v >> 5] |= 1u << (v & 31u);
             }
         }
-        cptr += nc;
     }
+    free(off);
 }
 
 int tgx_gen_scene_fill(const tgx_gen_params* p, tgx_object* objects, uint32_t* cluster_pointers,
@@ -243,7 +214,7 @@ int tgx_gen_scene_fill(const tgx_gen_params* p, tgx_object* objects, uint32_t* c
         fill_single(p, objects, cluster_pointers, cluster_idx_to_object, bits, materials);
     else
         fill_noise(p, objects, cluster_pointers, cluster_idx_to_object, bits, materials);
-    fill_luts(p, luts);
+    tgx_gen_luts(p, luts);
     return TGX_OK;
 }
 

@@ -661,9 +661,12 @@ __global__ void gi_list_k(const uint32_t* __restrict__ flag, const uint32_t* __r
 __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restrict__ surf,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count_ptr, tgx_gi_params prm,
+                                                     const uint32_t* __restrict__ frame_salt,
                                                      svo_ray* __restrict__ rays, float* __restrict__ contrib,
                                                      uint32_t* __restrict__ counters)
 {
+    /* a device counter can advance the frame of replayed work */
+    if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + 1u);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < *count_ptr;
     const uint32_t first = block_append<GI_BLOCK>(&counters[1], ok ? prm.spp : 0u);
@@ -956,7 +959,7 @@ int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_
 
 int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t* list, const uint32_t* count_dev,
                       uint32_t max_count, uint32_t* cnt, float* e_entry, float* e_var, float* radiance_dev,
-                      float* irradiance_dev)
+                      float* irradiance_dev, const uint32_t* frame_salt)
 {
     const uint64_t n_samples = (uint64_t)max_count * prm->spp;
     int rc;
@@ -978,7 +981,7 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
     {
         tgx_launch_scope s(ctx, "gi_gen_k");
         hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, list, count_dev,
-                           *prm, rays, contrib, cnt);
+                           *prm, frame_salt, rays, contrib, cnt);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
     if (ctx->tuning.trace_sort & 1u) {
@@ -1061,7 +1064,7 @@ int tgx_gi_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t 
     uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
     ctx->gi_spp = prm->spp;
     return tgx_gi_trace_list(ctx, prm, (const uint32_t*)ctx->gi_list.ptr, cnt + GI_CNT_VISIBLE, w * h,
-                             cnt + GI_CNT_PLAIN, nullptr, nullptr, radiance_dev, irradiance_dev);
+                             cnt + GI_CNT_PLAIN, nullptr, nullptr, radiance_dev, irradiance_dev, nullptr);
 }
 
 int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out)

@@ -78,9 +78,12 @@ int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_
  * entry i, the irradiance E goes to e_entry[3 i] and the variance of that
  * mean (summed over RGB, 0 for spp 1) to e_var[i]; per pixel, albedo * E
  * goes to radiance_dev and E to irradiance_dev (each may be null).
- * counters is a block of GI_N_COUNTERS in gi_counts, cleared here. */
+ * counters is a block of GI_N_COUNTERS in gi_counts, cleared here. With
+ * frame_salt (device u32, may be null) the bounce samples of frame
+ * prm->frame + 0x632BE5AB (*frame_salt + 1) are drawn, so replayed work can
+ * draw fresh samples. */
 int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t* list, const uint32_t* count_dev,
                       uint32_t max_count, uint32_t* counters, float* e_entry, float* e_var, float* radiance_dev,
-                      float* irradiance_dev);
+                      float* irradiance_dev, const uint32_t* frame_salt);
 
 #endif

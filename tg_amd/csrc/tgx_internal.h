@@ -15,7 +15,7 @@
 
 #include "../../include/tgx.h"
 
-/* Device buffer that only grows. */
+/* Device buffer that only grows (never while a frame graph exists). */
 struct tgx_buffer {
     void*  ptr = nullptr;
     size_t bytes = 0;
@@ -91,8 +91,6 @@ struct tgx_context {
     /* Neural irradiance cache (tgx_nrc.hip). */
     bool           nrc_ready = false;
     tgx_nrc_config nrc_cfg = {};
-    uint32_t       nrc_steps = 0;    /* Adam steps taken                    */
-    uint32_t       nrc_frame = 0;    /* frames run                          */
     bool           nrc_fb_traced = false; /* last frame traced a fallback list */
     tgx_buffer     nrc_master, nrc_m, nrc_v;   /* fp32 [N_PARAMS]           */
     tgx_buffer     nrc_fwd, nrc_bwd; /* bf16 MFMA A-operand fragments       */
@@ -103,7 +101,13 @@ struct tgx_context {
     tgx_buffer     nrc_samples, nrc_targets, nrc_out; /* staging            */
     tgx_buffer     nrc_cells;        /* error-estimate table                */
     tgx_buffer     nrc_queue;        /* frame lists and counters            */
+    tgx_buffer     nrc_dev;          /* device frame and Adam-step counters */
     tgx_nrc_stats  nrc_stats = {};
+
+    /* Captured frames alive: while > 0, buffers are neither grown nor freed
+     * (the graphs hold their addresses). */
+    uint32_t     n_graphs = 0;
+    tgx_buffer   frame_acc;      /* totals of the last tgx_frame_batch     */
 
     /* Profiling. */
     bool                           profiling = false;
@@ -117,6 +121,18 @@ int  tgx_prim_sort_pairs_u32(tgx_context* ctx, const uint32_t* keys_in, uint32_t
                              const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, unsigned end_bit);
 void tgx_svo_release(tgx_context* ctx);
 void tgx_nrc_release(tgx_context* ctx);
+/* Visibility camera records for captured frames: tgx_vis_cams_upload
+ * allocates and fills them (free with hipFree); tgx_visibility_record is
+ * tgx_visibility_device without host work (no camera upload, no list
+ * capacity update, no statistics readback), for stream capture after one
+ * ordinary call at the same size. */
+int  tgx_vis_cams_upload(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                         void** dev_cams);
+int  tgx_visibility_record(tgx_context* ctx, const void* dev_cams, const tgx_camera* cams, uint32_t n_cams,
+                           uint32_t w, uint32_t h, uint64_t* out_dev);
+/* The cache frame after visibility (vis_dev already holds the camera's words). */
+extern "C" int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* gi,
+                            const uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev);
 int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
 void tgx_buffer_release(tgx_buffer* b);
 

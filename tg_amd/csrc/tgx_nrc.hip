@@ -401,15 +401,26 @@ __global__ void nrc_pack_k(const float* __restrict__ master, __bf16* __restrict_
 
 struct nrc_adam_args {
     float lr, beta1, beta2, eps;
-    float inv_bc1, inv_bc2; /* 1 / (1 - beta^t) */
 };
 
+/* device counters (nrc_dev): 0 frames run, 1 Adam steps taken */
+#define NRC_DEV_FRAME 0
+#define NRC_DEV_STEP  1
+
+__global__ void nrc_tick_k(uint32_t* counter) { *counter += 1u; }
+
+/* One Adam step; the step number t comes from the device counter (advanced
+ * by nrc_tick_k before this launch), so replayed graphs keep counting. */
 __global__ void nrc_adam_k(const float* __restrict__ partial, uint32_t n_chunks, nrc_adam_args a,
-                           float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
-                           float* __restrict__ grad, __bf16* __restrict__ fwd, __bf16* __restrict__ bwd)
+                           const uint32_t* __restrict__ dev, float* __restrict__ master, float* __restrict__ m,
+                           float* __restrict__ v, float* __restrict__ grad, __bf16* __restrict__ fwd,
+                           __bf16* __restrict__ bwd)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NRC_N) return;
+    const double t = (double)dev[NRC_DEV_STEP];
+    const float inv_bc1 = (float)(1.0 / (1.0 - pow((double)a.beta1, t)));
+    const float inv_bc2 = (float)(1.0 / (1.0 - pow((double)a.beta2, t)));
     float gsum = 0.0f;
     for (uint32_t c = 0; c < n_chunks; c++) gsum += partial[(size_t)c * NRC_N + i];
     grad[i] = gsum;
@@ -417,7 +428,7 @@ __global__ void nrc_adam_k(const float* __restrict__ partial, uint32_t n_chunks,
     const float vi = a.beta2 * v[i] + (1.0f - a.beta2) * gsum * gsum;
     m[i] = mi;
     v[i] = vi;
-    const float w = master[i] - a.lr * (mi * a.inv_bc1) / (sqrtf(vi * a.inv_bc2) + a.eps);
+    const float w = master[i] - a.lr * (mi * inv_bc1) / (sqrtf(vi * inv_bc2) + a.eps);
     master[i] = w;
     int f, b;
     param_slots(i, f, b);
@@ -490,8 +501,10 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_infer_k(const bf16x8* __restric
 
 /* Training pixels: entry j takes visible pixel hash(j, frame) % n_visible. */
 __global__ void nrc_pick_k(const uint32_t* __restrict__ vis_list, const uint32_t* __restrict__ n_vis_ptr,
-                           uint32_t n_samples, uint32_t frame, uint32_t* __restrict__ out, uint32_t* __restrict__ count)
+                           uint32_t n_samples, const uint32_t* __restrict__ dev, uint32_t* __restrict__ out,
+                           uint32_t* __restrict__ count)
 {
+    const uint32_t frame = dev[NRC_DEV_FRAME];
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nv = *n_vis_ptr;
     if (j == 0) *count = nv ? n_samples : 0u;
@@ -541,7 +554,8 @@ void tgx_nrc_release(tgx_context* ctx)
 {
     tgx_buffer* bufs[] = {&ctx->nrc_master, &ctx->nrc_m, &ctx->nrc_v, &ctx->nrc_fwd, &ctx->nrc_bwd,
                           &ctx->nrc_partials, &ctx->nrc_grad, &ctx->nrc_train, &ctx->nrc_loss,
-                          &ctx->nrc_samples, &ctx->nrc_targets, &ctx->nrc_out, &ctx->nrc_cells, &ctx->nrc_queue};
+                          &ctx->nrc_samples, &ctx->nrc_targets, &ctx->nrc_out, &ctx->nrc_cells, &ctx->nrc_queue,
+                          &ctx->nrc_dev};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     ctx->nrc_ready = false;
 }
@@ -628,15 +642,16 @@ int train_step(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, 
                            (float*)ctx->nrc_partials.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_wgrad_k");
-    ctx->nrc_steps++;
     nrc_adam_args a;
     a.lr = c.learning_rate; a.beta1 = c.beta1; a.beta2 = c.beta2; a.eps = c.epsilon;
-    a.inv_bc1 = (float)(1.0 / (1.0 - std::pow((double)c.beta1, (double)ctx->nrc_steps)));
-    a.inv_bc2 = (float)(1.0 / (1.0 - std::pow((double)c.beta2, (double)ctx->nrc_steps)));
+    uint32_t* dev = (uint32_t*)ctx->nrc_dev.ptr;
+    hipLaunchKernelGGL(nrc_tick_k, dim3(1), dim3(1), 0, ctx->stream, dev + NRC_DEV_STEP);
+    TGX_LAUNCH_CHECK(ctx, "nrc_tick_k");
     {
         tgx_launch_scope s(ctx, "nrc_adam_k");
         hipLaunchKernelGGL(nrc_adam_k, dim3(nrc_blocks(NRC_N, 256)), dim3(256), 0, ctx->stream,
-                           (const float*)ctx->nrc_partials.ptr, n_chunks, a, (float*)ctx->nrc_master.ptr,
+                           (const float*)ctx->nrc_partials.ptr, n_chunks, a, (const uint32_t*)dev,
+                           (float*)ctx->nrc_master.ptr,
                            (float*)ctx->nrc_m.ptr, (float*)ctx->nrc_v.ptr, (float*)ctx->nrc_grad.ptr,
                            (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr);
     }
@@ -715,6 +730,8 @@ int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg)
     if (c.train_spp < 1 || c.train_spp > 64 || c.cell_size == 0 || (c.cell_size & (c.cell_size - 1)) ||
         c.cell_size > (1u << 16) || c.fallback_min_samples > (1u << 20))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "train_spp in [1, 64], cell_size a power of two");
+    if (ctx->n_graphs > 0)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "a frame graph uses the cache: destroy it before re-creating");
     tgx_nrc_release(ctx);
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->nrc_master, 4ull * NRC_N, "nrc weights")) ||
@@ -723,11 +740,10 @@ int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg)
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_grad, 4ull * NRC_N, "nrc gradient")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_fwd, 2ull * NRC_FWD_ELEMS, "nrc forward fragments")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_bwd, 2ull * NRC_BWD_ELEMS, "nrc backward fragments")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_cells, sizeof(nrc_cell) * (size_t)NRC_CELLS, "nrc error cells")))
+        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_cells, sizeof(nrc_cell) * (size_t)NRC_CELLS, "nrc error cells")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_dev, 64, "nrc counters")))
         return rc;
     ctx->nrc_cfg = c;
-    ctx->nrc_steps = 0;
-    ctx->nrc_frame = 0;
     ctx->nrc_stats = {};
     std::vector<float> p;
     init_params(c.seed, p);
@@ -736,6 +752,7 @@ int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg)
     TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_v.ptr, 0, 4ull * NRC_N, ctx->stream));
     TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_grad.ptr, 0, 4ull * NRC_N, ctx->stream));
     TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_cells.ptr, 0, sizeof(nrc_cell) * (size_t)NRC_CELLS, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_dev.ptr, 0, 64, ctx->stream));
     if ((rc = pack(ctx))) return rc;
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->nrc_ready = true;
@@ -809,8 +826,8 @@ int tgx_nrc_train_batch(tgx_context* ctx, const float* samples, const float* tar
     return TGX_OK;
 }
 
-int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* gi,
-                         uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev)
+int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* gi,
+                           const uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev)
 {
     if (!ctx || !cam || !gi || !vis_dev || !radiance_dev) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
@@ -837,7 +854,6 @@ int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, ui
     float* v_train = e_train + 3ull * n_train;
     nrc_cell* cells = (nrc_cell*)ctx->nrc_cells.ptr;
 
-    if ((rc = tgx_visibility_device(ctx, cam, 1, w, h, vis_dev))) return rc;
     if ((rc = tgx_gi_surfaces(ctx, cam, w, h, vis_dev, radiance_dev, irradiance_dev))) return rc;
     uint32_t* gcnt = (uint32_t*)ctx->gi_counts.ptr;
     const uint32_t* vis_list = (const uint32_t*)ctx->gi_list.ptr;
@@ -852,14 +868,15 @@ int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, ui
         hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
         TGX_LAUNCH_CHECK(ctx, "nrc_decay_k");
         hipLaunchKernelGGL(nrc_pick_k, dim3(nrc_blocks(n_train, 256)), dim3(256), 0, ctx->stream, vis_list, n_vis,
-                           n_train, ctx->nrc_frame, train_list, qc);
+                           n_train, (const uint32_t*)ctx->nrc_dev.ptr, train_list, qc);
         TGX_LAUNCH_CHECK(ctx, "nrc_pick_k");
         tgx_gi_params tp = *gi;
         tp.spp = c.train_spp;
-        /* fresh bounce samples every frame, decorrelated from the displayed frame's */
-        tp.frame = (gi->frame ^ 0x5BD1E995u) + 0x632BE5ABu * (ctx->nrc_frame + 1u);
+        /* fresh bounce samples every frame (device frame counter), decorrelated
+         * from the displayed frame's */
+        tp.frame = gi->frame ^ 0x5BD1E995u;
         if ((rc = tgx_gi_trace_list(ctx, &tp, train_list, qc, n_train, gcnt + GI_CNT_TRAIN, e_train, v_train, nullptr,
-                                    nullptr)))
+                                    nullptr, (const uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME)))
             return rc;
         for (uint32_t first = 0; first < n_train; first += c.train_batch)
             if ((rc = train_step(ctx, surf, train_list, qc, first, c.train_batch, c.train_batch, e_train, v_train,
@@ -888,12 +905,22 @@ int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, ui
                            vis_list, n_vis, fb_list, qc + 1);
         TGX_LAUNCH_CHECK(ctx, "nrc_fallback_list_k");
         if ((rc = tgx_gi_trace_list(ctx, gi, fb_list, qc + 1, n_pix, gcnt + GI_CNT_PLAIN, nullptr, nullptr,
-                                    radiance_dev, irradiance_dev)))
+                                    radiance_dev, irradiance_dev, nullptr)))
             return rc;
     }
-    ctx->nrc_frame++;
+    hipLaunchKernelGGL(nrc_tick_k, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME);
+    TGX_LAUNCH_CHECK(ctx, "nrc_tick_k");
     ctx->nrc_fb_traced = use_cells || all_fb;
     return TGX_OK;
+}
+
+int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* gi,
+                         uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev)
+{
+    if (!ctx || !cam || !gi || !vis_dev || !radiance_dev) return TGX_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = tgx_visibility_device(ctx, cam, 1, w, h, vis_dev))) return rc;
+    return tgx_nrc_frame_from_vis(ctx, cam, w, h, gi, vis_dev, radiance_dev, irradiance_dev);
 }
 
 int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out)
@@ -903,8 +930,9 @@ int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out)
     if ((rc = check_ready(ctx))) return rc;
     if (!ctx->nrc_queue.ptr || !ctx->gi_counts.ptr) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no cache frame yet");
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    uint32_t qc[64], gc[GI_CNT_TOTAL];
+    uint32_t qc[64], gc[GI_CNT_TOTAL], dv[2];
     TGX_HIP_CHECK(ctx, hipMemcpy(qc, ctx->nrc_queue.ptr, sizeof(qc), hipMemcpyDeviceToHost));
+    TGX_HIP_CHECK(ctx, hipMemcpy(dv, ctx->nrc_dev.ptr, sizeof(dv), hipMemcpyDeviceToHost));
     TGX_HIP_CHECK(ctx, hipMemcpy(gc, ctx->gi_counts.ptr, sizeof(gc), hipMemcpyDeviceToHost));
     tgx_nrc_stats s = {};
     s.visible_pixels = gc[GI_CNT_VISIBLE];
@@ -913,7 +941,7 @@ int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out)
     s.train_samples = ctx->nrc_stats.train_samples;
     s.train_rays = s.train_samples ? (uint64_t)gc[GI_CNT_TRAIN + 1] + gc[GI_CNT_TRAIN + 3] : 0u;
     s.fallback_rays = fb ? (uint64_t)gc[GI_CNT_PLAIN + 1] + gc[GI_CNT_PLAIN + 3] : 0u;
-    s.adam_steps = ctx->nrc_steps;
+    s.adam_steps = dv[NRC_DEV_STEP];
     float lsum;
     memcpy(&lsum, &qc[8], 4);
     s.train_loss = s.train_samples ? lsum / (3.0f * (float)s.train_samples) : 0.0f;

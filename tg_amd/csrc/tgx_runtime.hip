@@ -22,6 +22,9 @@ int tgx_fail(tgx_context* ctx, int code, const char* fmt, ...)
 int tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what)
 {
     if (bytes <= b->bytes && b->ptr) return TGX_OK;
+    if (ctx->n_graphs > 0)
+        return tgx_fail(ctx, TGX_ERR_CAPACITY,
+                        "%s needs %zu bytes, but a frame graph holds the context's buffers: destroy it first", what, bytes);
     if (b->ptr) {
         /* The old contents are dead; wait for in-flight users first. */
         hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -119,7 +122,8 @@ int tgx_destroy(tgx_context* ctx)
                           &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters,
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
-                          &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev};
+                          &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,
+                          &ctx->frame_acc};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     tgx_svo_release(ctx);
     tgx_nrc_release(ctx);

@@ -501,6 +501,8 @@ extern "C" {
 
 int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t flags)
 {
+    if (ctx && ctx->n_graphs > 0)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "a frame graph uses the SVO: destroy it before rebuilding");
     if (!ctx || !mn) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
     if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");

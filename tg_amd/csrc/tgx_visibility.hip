@@ -585,7 +585,7 @@ static bool make_cam(const tgx_camera& c, vis_cam& v)
 }
 
 static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t W,
-                            uint32_t H, uint64_t* out_dev)
+                            uint32_t H, uint64_t* out_dev, const vis_cam* resident)
 {
     const uint32_t tiles_x = (W + VIS_TILE - 1) / VIS_TILE, tiles_y = (H + VIS_TILE - 1) / VIS_TILE;
     const uint32_t tiles_per_cam = tiles_x * tiles_y;
@@ -596,14 +596,17 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
         return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
     const uint32_t n_tiles = (uint32_t)n_tiles64;
 
-    std::vector<vis_cam> vc(n_cams);
-    for (uint32_t i = 0; i < n_cams; i++) {
-        make_cam(cams[i], vc[i]);
-        vc[i].rec_base = i * n_cptr;
-        vc[i].tile_base = i * tiles_per_cam;
-    }
     int rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_cams, sizeof(vis_cam) * n_cams, "vis cams"))) return rc;
+    std::vector<vis_cam> vc;
+    if (!resident) {
+        vc.resize(n_cams);
+        for (uint32_t i = 0; i < n_cams; i++) {
+            make_cam(cams[i], vc[i]);
+            vc[i].rec_base = i * n_cptr;
+            vc[i].tile_base = i * tiles_per_cam;
+        }
+        if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_cams, sizeof(vis_cam) * n_cams, "vis cams"))) return rc;
+    }
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_clusters, sizeof(vis_cluster) * n_slots, "vis records"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_rec_flags, 4ull * n_slots, "vis record flags"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_global_keys, 8ull * n_slots, "vis global keys"))) return rc;
@@ -611,7 +614,7 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_count, 4ull * n_tiles, "tile counts"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
 
-    vis_cam* d_cams = (vis_cam*)ctx->vis_cams.ptr;
+    const vis_cam* d_cams = resident ? resident : (const vis_cam*)ctx->vis_cams.ptr;
     /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts */
     uint32_t* d_counters = (uint32_t*)ctx->vis_counters.ptr;
     uint32_t* d_global_count = d_counters + n_cams + 8;
@@ -620,7 +623,9 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     uint32_t* d_count = (uint32_t*)ctx->vis_tile_count.ptr;
     uint32_t* d_off = (uint32_t*)ctx->vis_tile_off.ptr;
     vis_cluster* d_recs = (vis_cluster*)ctx->vis_clusters.ptr;
-    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_cams, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice, ctx->stream));
+    if (!resident)
+        TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->vis_cams.ptr, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice,
+                                          ctx->stream));
 
     {
         tgx_launch_scope s(ctx, "vis_fill_u32");
@@ -692,7 +697,12 @@ static int list_capacity_update(tgx_context* ctx, uint32_t w, uint32_t h)
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_stats_dev, 64, "vis stats"))) return rc;
     uint64_t want = ctx->vis_list_capacity;
     const uint64_t tiles = (uint64_t)((w + VIS_TILE - 1) / VIS_TILE) * ((h + VIS_TILE - 1) / VIS_TILE);
-    if (want == 0) want = tiles * 64u > (1u << 22) ? tiles * 64u : (1u << 22);
+    if (want == 0) {
+        /* first call: room for 64 entries per tile, two per cluster pointer
+         * and at least 2^22, so that large scenes rarely start overflowed */
+        want = tiles * 64u > (1u << 22) ? tiles * 64u : (1u << 22);
+        if (2ull * ctx->n_cluster_pointers > want) want = 2ull * ctx->n_cluster_pointers;
+    }
     if (ctx->vis_list_capacity_fixed) want = ctx->vis_list_capacity_fixed;
     else if (ctx->vis_stats_pending && hipEventQuery(ctx->vis_stats_event) == hipSuccess) {
         ctx->vis_stats_pending = false;
@@ -716,6 +726,49 @@ static uint32_t batch_cams(const tgx_context* ctx, uint32_t n_cams)
     return (uint32_t)(b < n_cams ? b : n_cams);
 }
 
+int tgx_vis_cams_upload(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                        void** dev_cams)
+{
+    int rc = check_args(ctx, cams, n_cams, w, h);
+    if (rc) return rc;
+    const uint32_t per = batch_cams(ctx, n_cams);
+    const uint32_t tiles_per_cam = ((w + VIS_TILE - 1) / VIS_TILE) * ((h + VIS_TILE - 1) / VIS_TILE);
+    std::vector<vis_cam> vc(n_cams);
+    for (uint32_t i = 0; i < n_cams; i++) {
+        const uint32_t j = i % per; /* index within its batch */
+        make_cam(cams[i], vc[i]);
+        vc[i].rec_base = j * ctx->n_cluster_pointers;
+        vc[i].tile_base = j * tiles_per_cam;
+    }
+    void* d = nullptr;
+    TGX_HIP_CHECK(ctx, hipMalloc(&d, sizeof(vis_cam) * n_cams));
+    const hipError_t e = hipMemcpy(d, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return tgx_fail(ctx, TGX_ERR_HIP, "camera upload: %s", hipGetErrorString(e));
+    }
+    *dev_cams = d;
+    return TGX_OK;
+}
+
+int tgx_visibility_record(tgx_context* ctx, const void* dev_cams, const tgx_camera* cams, uint32_t n_cams, uint32_t w,
+                          uint32_t h, uint64_t* out_dev)
+{
+    int rc = check_args(ctx, cams, n_cams, w, h);
+    if (rc) return rc;
+    if (!ctx->vis_tile_list.ptr || !ctx->vis_stats_dev.ptr)
+        return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "record a frame only after one tgx_visibility_device call");
+    hipLaunchKernelGGL(vis_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->vis_stats_dev.ptr, 8u, 0u);
+    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    const uint32_t per = batch_cams(ctx, n_cams);
+    const vis_cam* rc_cams = (const vis_cam*)dev_cams;
+    for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
+        const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
+        if ((rc = visibility_batch(ctx, cams + c0, n, w, h, out_dev + (size_t)c0 * w * h, rc_cams + c0))) return rc;
+    }
+    return TGX_OK;
+}
+
 extern "C" {
 
 int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w,
@@ -735,7 +788,7 @@ int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_c
     const uint32_t per = batch_cams(ctx, n_cams);
     for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
         const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
-        rc = visibility_batch(ctx, cams + c0, n, w, h, out_dev + (size_t)c0 * w * h);
+        rc = visibility_batch(ctx, cams + c0, n, w, h, out_dev + (size_t)c0 * w * h, nullptr);
         if (rc) return rc;
     }
     /* Frame statistics travel back asynchronously; the next call (or
