@@ -121,7 +121,7 @@ def test_frame_graph_plain_matches_uncaptured_and_oracle(ctx, batch_scene):
         assert st1["primary_rays"] == N_CAMS * W * H
         assert st1["visible_pixels"] == visible == int((vis1 != T.VISIBILITY_EMPTY).sum())
         assert (st1["bounce_rays"], st1["shadow_rays"], st1["bounce_hits"]) == (bounce, shadow, hits)
-        assert st1["train_rays"] == 0 and visible > 0.3 * N_CAMS * W * H and rad1.sum() > 0
+        assert st1["train_rays"] == 0 and visible > 0.05 * N_CAMS * W * H and rad1.sum() > 0
     finally:
         buf.free()
 
