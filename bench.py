@@ -1,7 +1,7 @@
 """Benchmark: primary visibility + one-bounce SVO GI at 1920x1080 over 512
 noise-filled objects (the metric's workload).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload frame|visibility]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload frame|visibility|batch]
 
 One step = one frame of the hot path on one GPU:
   - primary visibility: cull/bin of every cluster pointer, tile lists and
@@ -16,6 +16,12 @@ The scene arrays and the SVO are resident in HBM before the timed region
 Prints one JSON line: rays/s (primary + bounce + shadow rays per frame over
 the frame time), the roofline of the dominant kernel (HIP events on the
 library's stream) and the CPU oracle timed on this host as the baseline.
+
+The same line carries the neural-cache frame on the same view (`neural`)
+and the batched stress frame (`batch`): 4096 objects of 1-32 clusters per
+axis in a 4096^3 world, a depth-12 SVO, 16 cameras at 1080p, each form
+captured as one hipGraph after autotuning, with the oracle's visibility of
+sampled rows of every camera compared word for word.
 """
 from __future__ import annotations
 
@@ -147,6 +153,9 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
 # cameras at 1920x1080 in one captured frame.
 BATCH_OBJECTS, BATCH_EXTENT, BATCH_WORLD, BATCH_LUTS = 4096, 32, 4096.0, 64
 BATCH_DEPTH, BATCH_CAMS = 12, 16
+BATCH_WORKLOAD = ("batched frame: 4096 noise-filled objects of 1-32 clusters per axis in a 4096^3 world, 64 LUTs, "
+                  "depth-12 SVO over the world, 16 seeded cameras at 1920x1080 at the world centre, one captured "
+                  "hipGraph per form (plain SVO GI at 4 spp; neural cache with fallback), autotuned trace shape")
 
 
 def batch_bench(args, steps: int, pretrain: int) -> dict:
@@ -192,21 +201,32 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
         tune_s = time.perf_counter() - t0
         log(f"batch {mode}: capture, {pretrain} untimed replays")
         t0 = time.perf_counter()
-        g = ctx.frame_graph(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
+        g = None if args.batch_no_graph else ctx.frame_graph(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
         capture_s = time.perf_counter() - t0
+
+        def run():
+            if g is None:
+                ctx.frame_batch(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
+            else:
+                g.launch()
         try:
             for _ in range(pretrain):
-                g.launch()
+                run()
             ctx.synchronize()
-            log(f"batch {mode}: {steps} timed replays")
+            log(f"batch {mode}: {steps} timed {'frames' if g is None else 'replays'}")
             t0 = time.perf_counter()
             for _ in range(steps):
-                g.launch()
+                run()
             ctx.synchronize()
             elapsed = time.perf_counter() - t0
-            st = g.stats()
+            st = (ctx.frame_batch(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags, stats=True) if g is None
+                  else g.stats())
+            if flags == 0:
+                vis_host = np.empty((BATCH_CAMS, H, W), np.uint64)
+                ctx.d2h(vis_host, vis_d)
         finally:
-            g.destroy()
+            if g is not None:
+                g.destroy()
         ctx.profile(True)
         ctx.profile_reset()
         ctx.frame_batch(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
@@ -236,7 +256,7 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
             "rays_traced_per_s": traced / (ms * 1e-3),
             "stats": {k: int(st[k]) for k in st.dtype.names},
             "tuning": {k: int(best[k][0]) for k in best.dtype.names},
-            "autotune_s": tune_s, "capture_s": capture_s,
+            "autotune_s": tune_s, "capture_s": capture_s, "captured": g is not None,
             "kernels_ms_per_frame": k_ms,
             "roofline": {"bound": "hbm", "kernel": dominant, "kernel_avg_ms": avg_ms, "launches_per_frame": launches,
                          "alg_bytes_per_launch": alg, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -247,8 +267,43 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
         out["neural"]["speedup_vs_plain"] = out["plain"]["frame_ms"] / out["neural"]["frame_ms"]
     for q in (vis_d, rad_d, irr_d):
         ctx.free(q)
-    ctx.close()
+    if not args.no_cpu_baseline:
+        log("batch: scene export for the CPU baseline")
+        scene = ctx.scene_export(params)
+        ctx.close()
+        log("batch: cpu baseline")
+        out["cpu_baseline"] = batch_cpu_baseline(scene, cams, vis_host)
+        del scene
+    else:
+        ctx.close()
     return out
+
+
+def batch_cpu_baseline(scene, cams, vis_gpu, row_step: int = 8, budget_s: float = 25.0) -> dict:
+    """The oracle's culled visibility of every `row_step`-th row of each
+    camera of the batched frame (as many cameras as fit in the budget), on
+    the host cores. The same rows of the GPU's visibility buffer are compared
+    word for word."""
+    import oracle  # the CPU baseline leg is the only bench code that uses the oracle
+
+    oscene = oracle.OracleScene(scene)
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1), 16)
+    rows = len(range(0, H, row_step))
+    exact, rays = [], 0
+    t0 = time.perf_counter()
+    for c in range(cams.shape[0]):
+        ref, _ = oracle.visibility_culled(oscene, cams[c:c + 1], W, H, row_step=row_step, n_threads=threads)
+        exact.append(bool(np.array_equal(ref[::row_step], vis_gpu[c, ::row_step])))
+        rays += W * rows
+        if time.perf_counter() - t0 > budget_s:
+            break
+    elapsed = time.perf_counter() - t0
+    return {"value": rays / elapsed, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle culled visibility (cull of all {scene.n_clusters} clusters per camera, then every "
+                      f"{row_step}th row, {rows} rows x {W}) of {len(exact)} of the {cams.shape[0]} cameras, "
+                      f"{elapsed:.1f} s on {threads} threads",
+            "visibility_check": {"cameras_checked": len(exact), "row_step": row_step,
+                                 "bit_exact": bool(exact) and all(exact), "per_camera": exact}}
 
 
 COPY_BYTES = 256 << 20  # known-size copy that calibrates the HBM byte counters
@@ -356,24 +411,28 @@ def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 1
         rate = rays / (time.perf_counter() - t0)
         return {"value": rate, "unit": "rays/s", "cores": threads, "kind": "port",
                 "sample": f"oracle culled visibility, {ci} full 1920x1080 frames on {threads} threads"}
-    # Frame workload: visibility of the whole frame, GI on every k-th row.
+    # Frame workload: whole frames (visibility, then GI of every pixel with
+    # the frame's own bounce samples), repeated until the budget is spent.
     t0 = time.perf_counter()
     svo = oracle.OracleSvo(oscene, mn, DEPTH)
     t_build = time.perf_counter() - t0
-    row_step = 8
-    t0 = time.perf_counter()
-    vis, _ = oracle.visibility_culled(oscene, cam, W, H, n_threads=threads)
-    t_vis = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, _, cnt = svo.gi(cam, W, H, vis, prm, row_step=row_step)
-    t_gi = time.perf_counter() - t0
-    gi_rays = int(cnt[1] + cnt[3])
-    rays = W * H + gi_rays * row_step
-    frame_s = t_vis + t_gi * row_step
-    return {"value": rays / frame_s, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle visibility of one full 1920x1080 frame ({t_vis:.2f} s) plus oracle GI of every "
-                      f"{row_step}th row ({gi_rays} bounce+shadow rays, {t_gi:.2f} s, scaled x{row_step}) on "
-                      f"{threads} threads; oracle SVO build {t_build:.2f} s"}
+    frames, rays, t_vis, t_gi = 0, 0, 0.0, 0.0
+    vis = None
+    while t_vis + t_gi < budget_s:
+        t0 = time.perf_counter()
+        vis, _ = oracle.visibility_culled(oscene, cam, W, H, n_threads=threads)
+        t_vis += time.perf_counter() - t0
+        p = prm.copy()
+        p["frame"] = frames
+        t0 = time.perf_counter()
+        _, _, cnt = svo.gi(cam, W, H, vis, p)
+        t_gi += time.perf_counter() - t0
+        rays += W * H + int(cnt[1] + cnt[3])
+        frames += 1
+    return {"value": rays / (t_vis + t_gi), "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} whole 1920x1080 frames of the oracle (visibility {t_vis:.2f} s, GI at {SPP} spp "
+                      f"{t_gi:.2f} s, {rays} primary+bounce+shadow rays) on {threads} threads; oracle SVO build "
+                      f"{t_build:.2f} s"}
 
 
 def main() -> int:
@@ -384,6 +443,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", choices=("frame", "visibility", "batch"), default="frame")
     ap.add_argument("--batch-pretrain", type=int, default=20, help="untimed replays of each batched frame graph")
+    ap.add_argument("--batch-steps", type=int, default=5, help="timed replays of each batched frame graph")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 16-camera frame")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="time uncaptured batched frames (the same launches) instead of graph replays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-cache measurement")
     ap.add_argument("--neural-pretrain", type=int, default=200, help="untimed cache training frames")
@@ -399,7 +462,17 @@ def main() -> int:
 
     if args.workload == "batch":
         b = batch_bench(args, args.steps, args.batch_pretrain)
-        print(json.dumps(b), flush=True)
+        line = {
+            "metric": "rays/s (primary DDA + SVO bounce, 1080p); neural vs plain GI at <=2% rel-MSE",
+            "value": b["plain"]["rays_traced_per_s"], "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.batch_pretrain, "ms_per_step": b["plain"]["frame_ms"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": BATCH_WORKLOAD, "cameras": BATCH_CAMS, "width": W, "height": H},
+            "roofline": b["plain"]["roofline"], "batch": b,
+        }
+        if "cpu_baseline" in b:
+            line["cpu_baseline"] = b["cpu_baseline"]
+        print(json.dumps(line), flush=True)
         return 0
 
     params = T.noise_objects_params(seed=args.seed)
@@ -504,6 +577,10 @@ def main() -> int:
     for p in (vis_d, rad_d, irr_d):
         ctx.free(p)
     ctx.close()
+    if not args.no_batch and args.workload == "frame":
+        log("batched 16-camera frame")
+        line["batch"] = batch_bench(args, args.batch_steps, args.batch_pretrain)
+        line["batch"]["workload"] = BATCH_WORKLOAD
     if not args.no_traffic:
         tr = measure_traffic(args.workload, dominant)
         if tr is not None:

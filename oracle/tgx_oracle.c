@@ -505,22 +505,30 @@ static int cluster_is_empty(const tgxo_scene* s, uint32_t cidx)
     return 1;
 }
 
+/* The clusters that can produce a hit and their pixel rectangles, in
+ * pointer order. Each pointer is tested independently (in parallel); the
+ * survivors are then compacted in order. */
 static uint32_t collect_clusters(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h,
                                  culled_cluster* list)
 {
     const cam_basis b = make_basis(cam);
-    uint32_t n = 0;
-    for (uint32_t c = 0; c < s->n_cluster_pointers; c++) {
+    const int64_t n_cptr = (int64_t)s->n_cluster_pointers;
+#pragma omp parallel for schedule(static, 4096)
+    for (int64_t ci = 0; ci < n_cptr; ci++) {
+        const uint32_t c = (uint32_t)ci;
         const uint32_t cidx = s->cluster_pointers[c];
+        list[c].cptr = UINT32_MAX;
         if (cluster_is_empty(s, cidx)) continue; /* no solid bit: never a hit */
         const tgx_object* o = &s->objects[s->cluster_idx_to_object[cidx]];
         uint32_t r[4];
         if (!rect_of(&b, o, c - o->first_cluster_pointer, w, h, r)) continue;
-        list[n].frame = cluster_frame(s, cam, c);
-        list[n].cptr = c;
-        memcpy(list[n].rect, r, sizeof(r));
-        n++;
+        list[c].frame = cluster_frame(s, cam, c);
+        list[c].cptr = c;
+        memcpy(list[c].rect, r, sizeof(r));
     }
+    uint32_t n = 0;
+    for (uint32_t c = 0; c < s->n_cluster_pointers; c++)
+        if (list[c].cptr != UINT32_MAX) list[n++] = list[c];
     return n;
 }
 
@@ -528,14 +536,14 @@ uint64_t tgxo_visibility_culled(const tgxo_scene* s, const tgx_camera* cam, uint
                                 uint32_t h, uint32_t row_step, int n_threads, uint64_t* out)
 {
     if (row_step == 0) row_step = 1;
-    culled_cluster* list = (culled_cluster*)malloc(sizeof(*list) * (s->n_cluster_pointers + 1u));
-    const uint32_t n = collect_clusters(s, cam, w, h, list);
-    uint64_t pairs = 0;
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #else
     (void)n_threads;
 #endif
+    culled_cluster* list = (culled_cluster*)malloc(sizeof(*list) * (s->n_cluster_pointers + 1u));
+    const uint32_t n = collect_clusters(s, cam, w, h, list);
+    uint64_t pairs = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : pairs)
     for (int64_t yy = 0; yy < (int64_t)h; yy += row_step) {
         const uint32_t y = (uint32_t)yy;
