@@ -326,6 +326,40 @@ int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
                      float* irradiance_dev);
 
 /* ------------------------------------------------------------------------ */
+/* Shading pass over the visibility buffer (shading.frag:114-337)           */
+/* ------------------------------------------------------------------------ */
+
+/* Views, numbered as tg's TG_DEBUG_SHOW_* (shading.frag:16-25). */
+#define TGX_SHADE_LIT               0u  /* ambient + GGX-lit LUT albedo (shading.frag:308-322) */
+#define TGX_SHADE_OBJECT_INDEX      1u  /* hash colour of the object index */
+#define TGX_SHADE_DEPTH             2u  /* min(1, 8 depth) as grey */
+#define TGX_SHADE_CLUSTER_INDEX     3u  /* hash colour of the cluster index */
+#define TGX_SHADE_VOXEL_INDEX       4u  /* hash colour of the 9-bit voxel index */
+#define TGX_SHADE_BLOCKS            5u  /* as CLUSTER_INDEX (shading.frag:244) */
+#define TGX_SHADE_COLOR_LUT_INDEX   6u  /* hash colour of the material (LUT entry) */
+#define TGX_SHADE_COLOR             7u  /* LUT albedo */
+#define TGX_SHADE_NORMAL            8u  /* face normal * 0.5 + 0.5 */
+#define TGX_SHADE_SHADING           9u  /* lit with white albedo (shading.frag:291-306) */
+
+typedef struct tgx_shade_params {
+    uint32_t view;            /* TGX_SHADE_* */
+    float    light_dir[3];    /* normalised here; tg: (0, 0.8, 0.3) */
+    float    light_radiance;  /* tg: 3.0 */
+    float    ambient;         /* ambient = ambient * albedo; tg: 0.1 */
+    float    metallic;        /* tg: 0.1 */
+    float    roughness;       /* tg: 0.8; negative: the LUT entry's roughness */
+    float    miss[4];         /* RGBA of pixels without a hit; tg: (1, 0, 1, 1) */
+} tgx_shade_params;
+
+/* Shades every pixel of one camera's visibility buffer vis_dev (w * h words
+ * from tgx_visibility_device) into rgba_dev (4 floats per pixel, row-major),
+ * as shading.frag's full-screen pass does (out_color). tg's LUT is RGBA8;
+ * this path reads the fp32 albedo (and optionally roughness) of the LUT
+ * entry. Returns without waiting for the GPU. */
+int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h,
+                     const uint64_t* vis_dev, const tgx_shade_params* params, float* rgba_dev);
+
+/* ------------------------------------------------------------------------ */
 /* Tuning: launch shapes and queue policies. Results never depend on them;  */
 /* tgx_autotune picks them by timing.                                        */
 /* ------------------------------------------------------------------------ */

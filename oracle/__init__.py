@@ -58,6 +58,8 @@ def load() -> ctypes.CDLL:
         "tgxo_aabb_obb_ignore_contact": (i32, [vp, vp, vp]),
         "tgxo_svo_node_key": (u64, [u32, vp]),
         "tgxo_gi": (None, [vp, vp, u32, u32, vp, vp, u32, vp, vp, vp]),
+        "tgxo_shade": (None, [vp, vp, u32, u32, vp, vp, vp]),
+        "tgxo_shade_pixel": (None, [vp, vp, u32, u32, u32, u32, u64, vp, vp]),
         "tgxo_gi_surface": (i32, [vp, vp, u32, u32, u32, u32, u64, vp, vp, vp]),
         "tgxo_gi_seed": (u32, [u32, u32, u32, u32]),
         "tgxo_gi_direction": (None, [vp, u32, vp]),
@@ -103,6 +105,21 @@ def visibility_culled(oscene: OracleScene, cam: np.ndarray, w: int, h: int, row_
     out = np.full((h, w), 0xFFFFFFFFFFFFFFFF, np.uint64)
     pairs = load().tgxo_visibility_culled(oscene.ref, _p(cam), w, h, row_step, n_threads, _p(out))
     return out, int(pairs)
+
+
+def shade(oscene: OracleScene, cam: np.ndarray, w: int, h: int, vis: np.ndarray, params: np.ndarray) -> np.ndarray:
+    """Shading pass of one camera: RGBA float32 [h, w, 4]."""
+    vis = np.ascontiguousarray(vis, np.uint64)
+    out = np.zeros((h, w, 4), np.float32)
+    load().tgxo_shade(oscene.ref, _p(cam), w, h, _p(vis), _p(params), _p(out))
+    return out
+
+
+def shade_pixel(oscene: OracleScene, cam: np.ndarray, w: int, h: int, x: int, y: int, word: int,
+                params: np.ndarray) -> np.ndarray:
+    out = np.zeros(4, np.float32)
+    load().tgxo_shade_pixel(oscene.ref, _p(cam), w, h, x, y, int(word), _p(params), _p(out))
+    return out
 
 
 def vis_pair(oscene: OracleScene, cam: np.ndarray, w: int, h: int, x: int, y: int, cptr: int) -> int:

@@ -100,6 +100,15 @@ void      tgxo_gi_direction(const float n[3], uint32_t seed, float out[3]);
 int       tgxo_aabb_obb_ignore_contact(const float mn[3], const float mx[3], const float corners[24]);
 uint64_t  tgxo_svo_node_key(uint32_t level, const uint32_t coord[3]);
 
+/* ------------------------------------------------------------------------ */
+/* Shading pass (tgx_oracle_shade.c, shading.frag:114-337)                   */
+/* ------------------------------------------------------------------------ */
+
+void tgxo_shade_pixel(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h, uint32_t px,
+                      uint32_t py, uint64_t word, const tgx_shade_params* params, float out[4]);
+void tgxo_shade(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
+                const tgx_shade_params* params, float* rgba);
+
 #ifdef __cplusplus
 }
 #endif

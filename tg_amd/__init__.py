@@ -81,6 +81,27 @@ NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learni
 NRC_STATS_DTYPE = np.dtype([("visible_pixels", "<u8"), ("fallback_pixels", "<u8"), ("train_samples", "<u8"),
                             ("train_rays", "<u8"), ("fallback_rays", "<u8"), ("adam_steps", "<u4"),
                             ("train_loss", "<f4")])
+SHADE_PARAMS_DTYPE = np.dtype([("view", "<u4"), ("light_dir", "<f4", (3,)), ("light_radiance", "<f4"),
+                               ("ambient", "<f4"), ("metallic", "<f4"), ("roughness", "<f4"), ("miss", "<f4", (4,))])
+SHADE_LIT, SHADE_OBJECT_INDEX, SHADE_DEPTH, SHADE_CLUSTER_INDEX, SHADE_VOXEL_INDEX = 0, 1, 2, 3, 4
+SHADE_BLOCKS, SHADE_COLOR_LUT_INDEX, SHADE_COLOR, SHADE_NORMAL, SHADE_SHADING = 5, 6, 7, 8, 9
+
+
+def shade_params(view: int = SHADE_LIT, light_dir=(0.0, 0.8, 0.3), light_radiance: float = 3.0,
+                 ambient: float = 0.1, metallic: float = 0.1, roughness: float = 0.8,
+                 miss=(1.0, 0.0, 1.0, 1.0)) -> np.ndarray:
+    """Shading parameters; the defaults are tg's constants (shading.frag:308-322)."""
+    p = np.zeros(1, SHADE_PARAMS_DTYPE)
+    p["view"] = view
+    p["light_dir"] = light_dir
+    p["light_radiance"] = light_radiance
+    p["ambient"] = ambient
+    p["metallic"] = metallic
+    p["roughness"] = roughness
+    p["miss"] = miss
+    return p
+
+
 FRAME_NEURAL = 1
 FRAME_STATS_DTYPE = np.dtype([("primary_rays", "<u8"), ("visible_pixels", "<u8"), ("bounce_rays", "<u8"),
                               ("shadow_rays", "<u8"), ("bounce_hits", "<u8"), ("train_rays", "<u8"),
@@ -208,6 +229,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_nrc_last_stats": (i32, [vp, vp]),
         "tgx_frame_graph_create": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, ctypes.POINTER(vp)]),
         "tgx_frame_batch": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, vp]),
+        "tgx_shade_device": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "tgx_frame_graph_launch": (i32, [vp, vp]),
         "tgx_frame_graph_stats": (i32, [vp, vp, vp]),
         "tgx_frame_graph_destroy": (i32, [vp, vp]),
@@ -552,6 +574,28 @@ class Context:
             for p in (vis_d, rad_d, irr_d):
                 self.free(p)
         return vis, rad, irr
+
+    # -- shading pass ---------------------------------------------------------
+
+    def shade_device(self, cam: np.ndarray, w: int, h: int, vis_dev: int, params: np.ndarray, rgba_dev: int) -> None:
+        cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)
+        params = np.ascontiguousarray(params, dtype=SHADE_PARAMS_DTYPE)
+        self._check(self.lib.tgx_shade_device(self.h, _ptr(cam), w, h, vis_dev, _ptr(params), rgba_dev),
+                    "tgx_shade_device")
+
+    def shade(self, cam: np.ndarray, w: int, h: int, vis: np.ndarray, params: np.ndarray) -> np.ndarray:
+        """Shades a host visibility buffer [h, w]: RGBA float32 [h, w, 4]."""
+        vis = np.ascontiguousarray(vis, np.uint64)
+        vis_d, out_d = self.alloc(8 * w * h), self.alloc(16 * w * h)
+        try:
+            self.h2d(vis_d, vis)
+            self.shade_device(cam, w, h, vis_d, params, out_d)
+            out = np.empty((h, w, 4), np.float32)
+            self.d2h(out, out_d)
+        finally:
+            self.free(vis_d)
+            self.free(out_d)
+        return out
 
     # -- batched frames as captured graphs --------------------------------------
 

@@ -342,31 +342,38 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
          * combined on the CU's shared scalar unit, which then limits the
          * loop. Only single comparisons feed selects. */
         uint32_t live = in_dda ? 1u : 0u, hit_i = 0u, leave_i = 0u;
+        /* The loop works on register copies of the lane state: a select
+         * between two fields of L would be folded into a load through a
+         * selected address, which forces L into scratch memory. */
+        float tmx = L.tmx, tmy = L.tmy, tmz = L.tmz;
+        const float tdx = L.tdx, tdy = L.tdy, tdz = L.tdz;
+        const unsigned long long bm = L.bm;
+        uint32_t wi_last = L.wi, wv = L.wv;
 #pragma unroll 1
         for (int k = 0; k < DDA_BURST; k++) {
             /* tg_amanatides_woo: stop at the iteration cap (leave), on a
              * solid voxel (hit) or after a step out of the block (leave) */
             const uint32_t cap = it >> 8;            /* it <= 4 * TGX_DDA_MAX_ITERATIONS = 256 */
             const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
-            const uint32_t inb = live & (cap ^ 1u) & (uint32_t)(L.bm >> b);
+            const uint32_t inb = live & (cap ^ 1u) & (uint32_t)(bm >> b);
             const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
-            if ((inb & 1u) && wi != L.wi) { L.wi = wi; L.wv = grid[wi]; }
-            const uint32_t h = inb & (L.wv >> (uint32_t)x) & 1u;
+            if ((inb & 1u) && wi != wi_last) { wi_last = wi; wv = grid[wi]; }
+            const uint32_t h = inb & (wv >> (uint32_t)x) & 1u;
             /* step along x if tmx < tmy and tmx < tmz, else along y if
              * !(tmx < tmy) and tmy < tmz, else along z */
-            const bool xy = L.tmx < L.tmy;
-            const float m1 = xy ? L.tmx : L.tmy;
-            const uint32_t axis = m1 < L.tmz ? (xy ? 0u : 1u) : 2u;
+            const bool xy = tmx < tmy;
+            const float m1 = xy ? tmx : tmy;
+            const uint32_t axis = m1 < tmz ? (xy ? 0u : 1u) : 2u;
             const uint32_t move = live & (cap ^ 1u) & (h ^ 1u);
-            const float tsum = (axis == 0u ? L.tmx : (axis == 1u ? L.tmy : L.tmz)) +
-                               (axis == 0u ? L.tdx : (axis == 1u ? L.tdy : L.tdz));
+            const float tsum = (axis == 0u ? tmx : (axis == 1u ? tmy : tmz)) +
+                               (axis == 0u ? tdx : (axis == 1u ? tdy : tdz));
             const int dstep = axis == 0u ? sx : (axis == 1u ? sy : sz);
             const uint32_t mx = move & (axis == 0u ? 1u : 0u);
             const uint32_t my = move & (axis == 1u ? 1u : 0u);
             const uint32_t mz = move & (axis == 2u ? 1u : 0u);
-            L.tmx = mx ? tsum : L.tmx;
-            L.tmy = my ? tsum : L.tmy;
-            L.tmz = mz ? tsum : L.tmz;
+            tmx = mx ? tsum : tmx;
+            tmy = my ? tsum : tmy;
+            tmz = mz ? tsum : tmz;
             x += (int)mx * dstep;
             y += (int)my * dstep;
             z += (int)mz * dstep;
@@ -378,6 +385,8 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
             live &= (h | lv) ^ 1u;
             if (!__any(live)) break;
         }
+        L.tmx = tmx; L.tmy = tmy; L.tmz = tmz;
+        L.wi = wi_last; L.wv = wv;
         hit = hit_i != 0u;
         leave = leave_i != 0u;
         if (in_dda && !hit && !leave) L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10) | (it << 15);
@@ -401,8 +410,8 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         uint32_t ax = nx > ny ? 0u : 1u;
         const float nax = ax == 0u ? nx : ny;
         if (!(nax > nz)) ax = 2u;
-        const float dc = ax == 0u ? L.dx : (ax == 1u ? L.dy : L.dz);
-        res.flags = (res.distance <= maxd ? 1u : 0u) | 2u | (ax << 2) | (dc > 0.0f ? 0u : 16u);
+        const uint32_t neg_x = L.dx > 0.0f ? 0u : 16u, neg_y = L.dy > 0.0f ? 0u : 16u, neg_z = L.dz > 0.0f ? 0u : 16u;
+        res.flags = (res.distance <= maxd ? 1u : 0u) | 2u | (ax << 2) | (ax == 0u ? neg_x : (ax == 1u ? neg_y : neg_z));
         res.steps = L.steps;
         res.pad = 0u;
         done = true;
