@@ -306,6 +306,33 @@ def batch_cpu_baseline(scene, cams, vis_gpu, row_step: int = 8, budget_s: float 
                                  "bit_exact": bool(exact) and all(exact), "per_camera": exact}}
 
 
+def shading_bench(ctx, cam, vis_d, steps: int) -> dict:
+    """The shading pass (shading.frag's full-screen pass, lit view) over the
+    frame's visibility buffer. Algorithmic bytes per launch: the 8-B word
+    read and the 16-B RGBA written per pixel, plus per visible pixel its
+    4-B cluster pointer entry, 4-B cluster->object entry and 1-B material."""
+    rgba = ctx.alloc(16 * W * H)
+    prm = T.shade_params()
+    ctx.shade_device(cam, W, H, vis_d, prm, rgba)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        ctx.shade_device(cam, W, H, vis_d, prm, rgba)
+    ctx.synchronize()
+    launches, total = ctx.profile_stats()["shade_k"]
+    ctx.profile(False)
+    ctx.free(rgba)
+    vis = np.empty((H, W), np.uint64)
+    ctx.d2h(vis, vis_d)
+    visible = int((vis != T.VISIBILITY_EMPTY).sum())
+    alg = 24 * W * H + 9 * visible
+    avg_ms = total / launches
+    gbs = alg / (avg_ms * 1e-3) / 1e9
+    return {"kernel": "shade_k", "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": alg, "achieved_gbs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "view": "lit (tg's default constants)", "visible_pixels": visible}
+
+
 COPY_BYTES = 256 << 20  # known-size copy that calibrates the HBM byte counters
 
 
@@ -568,6 +595,7 @@ def main() -> int:
         "svo": {"build_ms": svo_build_ms, "nodes": int(info["n_nodes"]), "leaves": int(info["n_leaves"])},
     }
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step")
+    line["shading"] = shading_bench(ctx, cam, vis_d, args.steps)
     if gst is not None:
         line["gi"] = {k: int(gst[k]) for k in gst.dtype.names}
         if not args.no_neural:
