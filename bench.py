@@ -333,6 +333,37 @@ def shading_bench(ctx, cam, vis_d, steps: int) -> dict:
             "frac": gbs / HBM_PEAK_GBS, "view": "lit (tg's default constants)", "visible_pixels": visible}
 
 
+def svo_visibility_bench(ctx, cam, steps: int) -> dict:
+    """SVO primary visibility (the second producer of the visibility word)
+    over the 1080p frame: primary rays per second through the SVO."""
+    out = ctx.alloc(8 * W * H)
+    ctx.svo_visibility_device(cam, W, H, out)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.svo_visibility_device(cam, W, H, out)
+    ctx.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    vis = np.empty((H, W), np.uint64)
+    ctx.d2h(vis, out)
+    ctx.free(out)
+    return {"ms": ms, "rays_per_s": W * H / (ms * 1e-3), "hit_pixels": int((vis != T.VISIBILITY_EMPTY).sum())}
+
+
+def pick_bench(ctx, cam, vis_d, n: int = 4096) -> dict:
+    """Pick readback: latency of one synchronous call for 1 and for n pixels."""
+    rng = np.random.default_rng(0)
+    xy = np.stack([rng.integers(0, W, n), rng.integers(0, H, n)], 1).astype(np.uint32)
+    ctx.pick(vis_d, W, H, xy[:1], cam)
+    res = {}
+    for k in (1, n):
+        t0 = time.perf_counter()
+        for _ in range(20):
+            ctx.pick(vis_d, W, H, xy[:k], cam)
+        res[f"us_per_call_{k}"] = 1e6 * (time.perf_counter() - t0) / 20
+    return res
+
+
 COPY_BYTES = 256 << 20  # known-size copy that calibrates the HBM byte counters
 
 
@@ -596,6 +627,8 @@ def main() -> int:
     }
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step")
     line["shading"] = shading_bench(ctx, cam, vis_d, args.steps)
+    line["svo_visibility"] = svo_visibility_bench(ctx, cam, args.steps)
+    line["pick"] = pick_bench(ctx, cam, vis_d)
     if gst is not None:
         line["gi"] = {k: int(gst[k]) for k in gst.dtype.names}
         if not args.no_neural:

@@ -115,7 +115,8 @@ typedef enum tgx_status {
     TGX_ERR_NO_SCENE       = 4,
     TGX_ERR_NO_DEVICE      = 5,
     TGX_ERR_OUT_OF_MEMORY  = 6,
-    TGX_ERR_NOT_BUILT      = 7
+    TGX_ERR_NOT_BUILT      = 7,
+    TGX_ERR_IO             = 8
 } tgx_status;
 
 typedef struct tgx_context tgx_context;
@@ -154,6 +155,27 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* scene);
 int tgx_scene_export(tgx_context* ctx, tgx_object* objects, uint32_t* cluster_pointers,
                      uint32_t* cluster_idx_to_object, uint32_t* bits, uint8_t* materials,
                      tgx_lut_entry* luts);
+
+/* Scene files (host only; no context needed). tg keeps its scene in memory
+ * (tgvk_raytracer.c:805-992); this format stores the same arrays byte for
+ * byte with an FNV-1a 64 checksum. save: TGX_ERR_IO if the file cannot be
+ * written. counts / load: TGX_ERR_IO if it cannot be opened,
+ * TGX_ERR_INVALID_ARG if it is not a scene file, is truncated or fails the
+ * checksum. load fills caller arrays sized by tgx_scene_file_counts
+ * (materials may be NULL; a file without materials loads zeros). */
+typedef struct tgx_scene_counts {
+    uint32_t n_objects;
+    uint32_t n_cluster_pointers;
+    uint32_t n_clusters;
+    uint32_t n_luts;
+    uint32_t has_materials;
+} tgx_scene_counts;
+
+int tgx_scene_save(const char* path, const tgx_scene_desc* scene);
+int tgx_scene_file_counts(const char* path, tgx_scene_counts* out);
+int tgx_scene_load(const char* path, tgx_object* objects, uint32_t* cluster_pointers,
+                   uint32_t* cluster_idx_to_object, uint32_t* bits, uint8_t* materials,
+                   tgx_lut_entry* luts);
 
 /* ------------------------------------------------------------------------ */
 /* Primary visibility                                                       */
@@ -266,6 +288,18 @@ int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes, uint32_t* leaf_bits,
 int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* directions, uint32_t n,
                   float max_distance, tgx_svo_hit* out_host);
 
+/* SVO primary visibility, the second producer of the visibility word
+ * (debug_visibility_svo.frag:27-70): one ray per pixel from the camera
+ * along the normalised corner-ray lerp, traced through the SVO (the
+ * traversal of tgx_svo_trace, no range limit). A hit at distance t with
+ * d = t / far <= 1 writes u64(d * 16777215) << 40 | leaf node index << 9 |
+ * (block voxel index % 512), the node index in the canonical order of
+ * tgx_svo_export; other pixels get TGX_VISIBILITY_EMPTY. The reference's
+ * GLSL traversal reports enter/far in a mixed frame; this follows the C
+ * traversal's distance (tg_sparse_voxel_octree.c:558-740). */
+int tgx_svo_visibility_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h,
+                              uint64_t* vis_dev);
+
 /* ------------------------------------------------------------------------ */
 /* One-bounce global illumination through the SVO                           */
 /* ------------------------------------------------------------------------ */
@@ -324,6 +358,27 @@ int tgx_gi_last_stats(tgx_context* ctx, tgx_gi_stats* out);
 int tgx_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h,
                      const tgx_gi_params* params, uint64_t* vis_dev, float* radiance_dev,
                      float* irradiance_dev);
+
+/* Pick readback: replaces tg_raytracer_get_hovered_voxel
+ * (tgvk_raytracer.c:1605-1655), which reads one visibility word back and
+ * returns its 31-bit field as a "cluster index" although the GPU wrote a
+ * cluster pointer (the caller then indexes cluster_idx_to_object with it,
+ * tg_application.c:228). Here each result carries the pointer, the cluster
+ * index it maps to and the object, all resolved on the device. */
+typedef struct tgx_pick_result {
+    uint32_t hit;              /* depth < 1 */
+    float    depth;            /* (word >> 40) / 16777215 */
+    uint32_t cluster_pointer;  /* the word's 31-bit field (UINT32_MAX on a miss) */
+    uint32_t cluster_idx;      /* cluster_pointers[cluster_pointer] */
+    uint32_t object_idx;       /* cluster_idx_to_object[cluster_idx] */
+    uint32_t voxel;            /* 9-bit voxel index 64 z + 8 y + x (UINT32_MAX on a miss) */
+    float    position[3];      /* camera + depth far dir (shading.frag:231), if cam given */
+} tgx_pick_result;
+
+/* Picks n pixels (xy[2 i], xy[2 i + 1]) of a w x h visibility buffer on the
+ * device; cam may be NULL (position then 0). Synchronous. */
+int tgx_pick(tgx_context* ctx, const tgx_camera* cam, const uint64_t* vis_dev, uint32_t w, uint32_t h,
+             const uint32_t* xy, uint32_t n, tgx_pick_result* out_host);
 
 /* ------------------------------------------------------------------------ */
 /* Shading pass over the visibility buffer (shading.frag:114-337)           */

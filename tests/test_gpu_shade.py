@@ -48,3 +48,52 @@ def test_shade_rejects_bad_arguments(ctx):
         ctx.shade(cam, 8, 8, vis, T.shade_params(light_dir=(0.0, 0.0, 0.0)))
     out = ctx.shade(cam, 8, 8, vis, T.shade_params())
     assert np.all(out == np.array([1.0, 0.0, 1.0, 1.0], np.float32))
+
+
+def lerp_dir_f32(cam, x, y, w, h):
+    """tg_lerp_corner_ray_directions_nn in fp32, the kernels' operation order."""
+    f = np.float32
+    fx = (f(x) + f(0.5)) / f(w)
+    fy = f(1.0) - (f(y) + f(0.5)) / f(h)
+    omy, omx = f(1.0) - fy, f(1.0) - fx
+    c = cam[0]
+    left = c["ray_bl"][:3] * omy + c["ray_tl"][:3] * fy
+    right = c["ray_br"][:3] * omy + c["ray_tr"][:3] * fy
+    return left * omx + right * fx
+
+
+def test_pick_resolves_pointer_index_object_and_position(ctx):
+    p = T.noise_objects_params(seed=2)
+    sc = T.generate_scene(p)
+    w, h = 480, 270
+    cam = T.generate_camera(p, 0, w, h)
+    ctx.upload(sc)
+    vis_d = ctx.alloc(8 * w * h)
+    try:
+        ctx.visibility_device(cam, w, h, vis_d)
+        vis = np.empty((h, w), np.uint64)
+        ctx.d2h(vis, vis_d)
+        rng = np.random.default_rng(4)
+        xy = np.stack([rng.integers(0, w, 2000), rng.integers(0, h, 2000)], 1).astype(np.uint32)
+        say("pick 2000 pixels")
+        got = ctx.pick(vis_d, w, h, xy, cam)
+        for (x, y), r in zip(xy, got):
+            word = int(vis[y, x])
+            hit, depth, cptr, vox = T.decode(word)
+            assert bool(r["hit"]) == hit and np.float32(r["depth"]) == np.float32(depth)
+            if not hit:
+                assert r["cluster_pointer"] == r["cluster_idx"] == r["object_idx"] == r["voxel"] == 0xFFFFFFFF
+                continue
+            cidx = int(sc.cluster_pointers[cptr])
+            assert (r["cluster_pointer"], r["voxel"], r["cluster_idx"]) == (cptr, vox, cidx)
+            assert r["object_idx"] == sc.cluster_idx_to_object[cidx]
+            t = np.float32(depth) * cam[0]["far_plane"]
+            pos = cam[0]["camera"][:3] + t * lerp_dir_f32(cam, int(x), int(y), w, h)
+            assert r["position"].tobytes() == pos.astype(np.float32).tobytes()
+        assert got["hit"].sum() > 100
+        no_cam = ctx.pick(vis_d, w, h, xy[:10])
+        assert np.all(no_cam["position"] == 0)
+        with pytest.raises(T.TgxError):
+            ctx.pick(vis_d, w, h, [[w, 0]])
+    finally:
+        ctx.free(vis_d)

@@ -102,3 +102,51 @@ def test_svo_error_paths(ctx):
         assert e.value.code == 7
     finally:
         fresh.close()
+
+
+def svo_visibility_reference(svo: O.OracleSvo, cam: np.ndarray, w: int, h: int) -> np.ndarray:
+    """debug_visibility_svo.frag:27-70 over the oracle traversal: rays in the
+    kernels' fp32 operation order, words packed as the reference packs them."""
+    f = np.float32
+    c = cam[0]
+    ys, xs = np.mgrid[0:h, 0:w]
+    fx = (xs.astype(f) + f(0.5)) / f(w)
+    fy = f(1.0) - (ys.astype(f) + f(0.5)) / f(h)
+    omy, omx = f(1.0) - fy, f(1.0) - fx
+    d = np.empty((h, w, 3), f)
+    for a in range(3):
+        left = c["ray_bl"][a] * omy + c["ray_tl"][a] * fy
+        right = c["ray_br"][a] * omy + c["ray_tr"][a] * fy
+        d[..., a] = left * omx + right * fx
+    m = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2])
+    d = d / m[..., None]
+    o = np.broadcast_to(c["camera"][:3], d.shape)
+    hits = svo.trace(o.reshape(-1, 3), d.reshape(-1, 3))
+    e = svo.export()
+    leaf_level = int(svo.info()["depth"]) - 5
+    index = {(int(n["level"]),) + tuple(int(v) for v in n["coord"]): i for i, n in enumerate(e.nodes)}
+    out = np.full(w * h, T.VISIBILITY_EMPTY, np.uint64)
+    far = f(c["far_plane"])
+    for i in np.nonzero(hits["hit"])[0]:
+        dd = f(hits["distance"][i]) / far
+        if dd <= f(1.0):
+            node = index[(leaf_level,) + tuple(int(v) for v in hits["block"][i])]
+            out[i] = (int(f(dd * f(16777215.0))) << 40) | ((node & 0x7FFFFFFF) << 9) | (int(hits["voxel"][i]) % 512)
+    return out.reshape(h, w)
+
+
+@pytest.mark.parametrize("w,h", [(160, 90), (480, 270)])
+def test_svo_visibility_matches_oracle(ctx, w, h):
+    p = T.noise_objects_params(seed=1)
+    sc = T.generate_scene(p)
+    cam = T.generate_camera(p, 0, w, h)
+    mn = np.floor(cam[0]["camera"][:3]) - 1024.0
+    ctx.upload(sc)
+    ctx.svo_build(mn, 11)
+    say(f"svo visibility {w}x{h}")
+    got = ctx.svo_visibility(cam, w, h)
+    svo = O.OracleSvo(O.OracleScene(sc), mn, 11)
+    ref = svo_visibility_reference(svo, cam, w, h)
+    bad = got != ref
+    assert not bad.any(), f"{int(bad.sum())} words differ, first at {np.argwhere(bad)[:3].tolist()}"
+    assert (got != T.VISIBILITY_EMPTY).mean() > 0.3

@@ -102,6 +102,11 @@ def shade_params(view: int = SHADE_LIT, light_dir=(0.0, 0.8, 0.3), light_radianc
     return p
 
 
+PICK_DTYPE = np.dtype([("hit", "<u4"), ("depth", "<f4"), ("cluster_pointer", "<u4"), ("cluster_idx", "<u4"),
+                       ("object_idx", "<u4"), ("voxel", "<u4"), ("position", "<f4", (3,))])
+SCENE_COUNTS_DTYPE = np.dtype([("n_objects", "<u4"), ("n_cluster_pointers", "<u4"), ("n_clusters", "<u4"),
+                               ("n_luts", "<u4"), ("has_materials", "<u4")])
+
 FRAME_NEURAL = 1
 FRAME_STATS_DTYPE = np.dtype([("primary_rays", "<u8"), ("visible_pixels", "<u8"), ("bounce_rays", "<u8"),
                               ("shadow_rays", "<u8"), ("bounce_hits", "<u8"), ("train_rays", "<u8"),
@@ -133,7 +138,7 @@ GEN_NOISE_OBJECTS = 1
 
 STATUS = {0: "TGX_OK", 1: "TGX_ERR_INVALID_ARG", 2: "TGX_ERR_CAPACITY", 3: "TGX_ERR_HIP",
           4: "TGX_ERR_NO_SCENE", 5: "TGX_ERR_NO_DEVICE", 6: "TGX_ERR_OUT_OF_MEMORY",
-          7: "TGX_ERR_NOT_BUILT"}
+          7: "TGX_ERR_NOT_BUILT", 8: "TGX_ERR_IO"}
 
 
 class TgxError(RuntimeError):
@@ -230,6 +235,11 @@ def load_library() -> ctypes.CDLL:
         "tgx_frame_graph_create": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, ctypes.POINTER(vp)]),
         "tgx_frame_batch": (i32, [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, vp]),
         "tgx_shade_device": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "tgx_pick": (i32, [vp, vp, vp, u32, u32, vp, u32, vp]),
+        "tgx_svo_visibility_device": (i32, [vp, vp, u32, u32, vp]),
+        "tgx_scene_save": (i32, [ctypes.c_char_p, ctypes.POINTER(_SceneDesc)]),
+        "tgx_scene_file_counts": (i32, [ctypes.c_char_p, vp]),
+        "tgx_scene_load": (i32, [ctypes.c_char_p, vp, vp, vp, vp, vp, vp]),
         "tgx_frame_graph_launch": (i32, [vp, vp]),
         "tgx_frame_graph_stats": (i32, [vp, vp, vp]),
         "tgx_frame_graph_destroy": (i32, [vp, vp]),
@@ -312,6 +322,34 @@ def generate_scene(params: _GenParams) -> Scene:
                                 _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials), _ptr(s.luts))
     if rc:
         raise TgxError(rc, "tgx_gen_scene_fill")
+    return s
+
+
+def save_scene(path, scene: "Scene") -> None:
+    """Writes a scene file (tgx_scene_save)."""
+    rc = load_library().tgx_scene_save(str(path).encode(), ctypes.byref(scene.desc()))
+    if rc:
+        raise TgxError(rc, f"tgx_scene_save({path})")
+
+
+def load_scene(path) -> "Scene":
+    """Reads a scene file (tgx_scene_file_counts + tgx_scene_load)."""
+    lib = load_library()
+    c = np.zeros(1, SCENE_COUNTS_DTYPE)
+    rc = lib.tgx_scene_file_counts(str(path).encode(), _ptr(c))
+    if rc:
+        raise TgxError(rc, f"tgx_scene_file_counts({path})")
+    c = c[0]
+    s = Scene(objects=np.zeros(int(c["n_objects"]), OBJECT_DTYPE),
+              cluster_pointers=np.zeros(int(c["n_cluster_pointers"]), np.uint32),
+              cluster_idx_to_object=np.zeros(int(c["n_clusters"]), np.uint32),
+              bits=np.zeros(16 * int(c["n_clusters"]), np.uint32),
+              materials=np.zeros(512 * int(c["n_clusters"]), np.uint8),
+              luts=np.zeros(256 * int(c["n_luts"]), LUT_DTYPE))
+    rc = lib.tgx_scene_load(str(path).encode(), _ptr(s.objects), _ptr(s.cluster_pointers),
+                            _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials), _ptr(s.luts))
+    if rc:
+        raise TgxError(rc, f"tgx_scene_load({path})")
     return s
 
 
@@ -574,6 +612,31 @@ class Context:
             for p in (vis_d, rad_d, irr_d):
                 self.free(p)
         return vis, rad, irr
+
+    def svo_visibility_device(self, cam: np.ndarray, w: int, h: int, vis_dev: int) -> None:
+        cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)
+        self._check(self.lib.tgx_svo_visibility_device(self.h, _ptr(cam), w, h, vis_dev), "tgx_svo_visibility_device")
+
+    def svo_visibility(self, cam: np.ndarray, w: int, h: int) -> np.ndarray:
+        """SVO primary visibility words [h, w] (tgx_svo_visibility_device)."""
+        d = self.alloc(8 * w * h)
+        try:
+            self.svo_visibility_device(cam, w, h, d)
+            out = np.empty((h, w), np.uint64)
+            self.d2h(out, d)
+        finally:
+            self.free(d)
+        return out
+
+    # -- pick readback ----------------------------------------------------------
+
+    def pick(self, vis_dev: int, w: int, h: int, xy, cam: np.ndarray | None = None) -> np.ndarray:
+        """Decodes pixels (x, y) of a device visibility buffer: PICK_DTYPE [n]."""
+        xy = np.ascontiguousarray(np.asarray(xy, np.uint32).reshape(-1, 2))
+        out = np.zeros(xy.shape[0], PICK_DTYPE)
+        cam_a = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE) if cam is not None else None
+        self._check(self.lib.tgx_pick(self.h, _ptr(cam_a), vis_dev, w, h, _ptr(xy), xy.shape[0], _ptr(out)), "tgx_pick")
+        return out
 
     # -- shading pass ---------------------------------------------------------
 

@@ -852,6 +852,43 @@ __global__ void svo_hits_k(const svo_gnode* __restrict__ nodes, const svo_res* _
     out[i] = h;
 }
 
+/* SVO primary visibility (debug_visibility_svo.frag:27-70): one ray per
+ * pixel from the camera along the normalised corner-ray lerp
+ * (tg_lerp_corner_ray_directions, common.inc:48-53). */
+__global__ void svo_primary_rays_k(gi_cam cam, uint32_t W, uint32_t H, svo_ray* __restrict__ rays)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= W * H) return;
+    const uint32_t px = pix % W, py = pix / W;
+    const float fx = ((float)px + 0.5f) / (float)W;
+    const float fy = 1.0f - ((float)py + 0.5f) / (float)H;
+    const float omy = 1.0f - fy, omx = 1.0f - fx;
+    const float lx = cam.bl[0] * omy + cam.tl[0] * fy, ly = cam.bl[1] * omy + cam.tl[1] * fy, lz = cam.bl[2] * omy + cam.tl[2] * fy;
+    const float qx = cam.br[0] * omy + cam.tr[0] * fy, qy = cam.br[1] * omy + cam.tr[1] * fy, qz = cam.br[2] * omy + cam.tr[2] * fy;
+    const float dx = lx * omx + qx * fx, dy = ly * omx + qy * fx, dz = lz * omx + qz * fx;
+    const float m = sqrtf(dx * dx + dy * dy + dz * dz);
+    svo_ray r = {cam.pos[0], cam.pos[1], cam.pos[2], dx / m, dy / m, dz / m, pix, 0u};
+    rays[pix] = r;
+}
+
+/* The visibility word of an SVO hit (debug_visibility_svo.frag:59-68):
+ * depth24 = u64(d 16777215) for d = distance / far <= 1, then the leaf
+ * node's index (canonical order) in place of the cluster pointer and the
+ * block voxel index mod 512. */
+__global__ void svo_vis_pack_k(const svo_res* __restrict__ res, uint32_t n, float far_plane, uint64_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const svo_res r = res[i];
+    uint64_t word = TGX_VISIBILITY_EMPTY;
+    if (r.flags & 2u) {
+        const float d = r.distance / far_plane;
+        if (d <= 1.0f)
+            word = ((uint64_t)(d * 16777215.0f) << 40) | ((uint64_t)(r.node & 0x7FFFFFFFu) << 9) | (uint64_t)(r.voxel % 512u);
+    }
+    out[i] = word;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Host side                                                                */
 /* ------------------------------------------------------------------------ */
@@ -1059,6 +1096,48 @@ int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* direction
     TGX_LAUNCH_CHECK(ctx, "svo_hits_k");
     TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, d_hits, sizeof(tgx_svo_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
+int tgx_svo_visibility_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, uint64_t* vis_dev)
+{
+    if (!ctx) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!cam || !vis_dev) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "null argument");
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (w == 0 || h == 0 || w > 65535 || h > 65535 || (uint64_t)w * h >= (1ull << 31))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "invalid resolution %ux%u", w, h);
+    if (!(cam->far_plane > 0.0f)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "far plane must be > 0");
+    const uint32_t n = w * h;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, sizeof(svo_ray) * (size_t)n, "svo rays")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_hits, sizeof(svo_res) * (size_t)n, "svo hits")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_CNT_TOTAL, "gi counters")))
+        return rc;
+    gi_cam gc;
+    for (int a = 0; a < 3; a++) {
+        gc.pos[a] = cam->camera[a];
+        gc.bl[a] = cam->ray_bl[a]; gc.br[a] = cam->ray_br[a];
+        gc.tr[a] = cam->ray_tr[a]; gc.tl[a] = cam->ray_tl[a];
+    }
+    svo_ray* d_rays = (svo_ray*)ctx->svo_rays.ptr;
+    svo_res* d_res = (svo_res*)ctx->svo_hits.ptr;
+    uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr + GI_CNT_SVO;
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 6, 1ull, n);
+    {
+        tgx_launch_scope s(ctx, "svo_primary_rays_k");
+        hipLaunchKernelGGL(svo_primary_rays_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, gc, w, h, d_rays);
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_primary_rays_k");
+    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, SVO_F32_MAX, cnt + 4, d_res, nullptr))) return rc;
+    {
+        tgx_launch_scope s(ctx, "svo_vis_pack_k");
+        hipLaunchKernelGGL(svo_vis_pack_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, d_res, n, cam->far_plane,
+                           vis_dev);
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_vis_pack_k");
     return TGX_OK;
 }
 

@@ -108,6 +108,7 @@ struct tgx_context {
      * (the graphs hold their addresses). */
     uint32_t     n_graphs = 0;
     tgx_buffer   frame_acc;      /* totals of the last tgx_frame_batch     */
+    tgx_buffer   pick_buf;       /* tgx_pick coordinates and results       */
 
     /* Profiling. */
     bool                           profiling = false;

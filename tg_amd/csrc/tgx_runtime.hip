@@ -123,7 +123,7 @@ int tgx_destroy(tgx_context* ctx)
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,
-                          &ctx->frame_acc};
+                          &ctx->frame_acc, &ctx->pick_buf};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     tgx_svo_release(ctx);
     tgx_nrc_release(ctx);

@@ -6,6 +6,8 @@
  * float4; the pass is a streaming read of the visibility buffer plus
  * gathers of the per-cluster maps, object records and materials.
  *
+ * The pick readback (tgx_pick) decodes single pixels the same way.
+ *
  * Arithmetic follows shading.frag's operation order in fp32 (no contraction),
  * so the oracle's transcription gives identical bits. GLSL leaves the
  * precision of pow() to the implementation; pow(x, 5.0) is evaluated here
@@ -192,7 +194,85 @@ __global__ __launch_bounds__(256) void shade_k(const tgx_object* __restrict__ ob
     out[pix] = c;
 }
 
+/* Pick readback: one lane per requested pixel. */
+__global__ void pick_k(const uint32_t* __restrict__ cptrs, const uint32_t* __restrict__ c2o, shade_cam cam,
+                       uint32_t has_cam, uint32_t W, uint32_t H, const uint64_t* __restrict__ vis,
+                       const uint32_t* __restrict__ xy, uint32_t n, tgx_pick_result* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t px = xy[2 * i], py = xy[2 * i + 1];
+    const uint64_t word = vis[(size_t)py * W + px];
+    tgx_pick_result r;
+    r.depth = (float)(word >> 40) / 16777215.0f;
+    r.position[0] = r.position[1] = r.position[2] = 0.0f;
+    if (r.depth < 1.0f) {
+        r.hit = 1u;
+        r.cluster_pointer = (uint32_t)(word >> 9) & 0x7FFFFFFFu;
+        r.voxel = (uint32_t)word & 511u;
+        r.cluster_idx = cptrs[r.cluster_pointer];
+        r.object_idx = c2o[r.cluster_idx];
+        if (has_cam) {
+            const float fx = ((float)px + 0.5f) / (float)W;
+            const float fy = 1.0f - ((float)py + 0.5f) / (float)H;
+            const float omy = 1.0f - fy, omx = 1.0f - fx;
+            const float lx = cam.bl[0] * omy + cam.tl[0] * fy, ly = cam.bl[1] * omy + cam.tl[1] * fy,
+                        lz = cam.bl[2] * omy + cam.tl[2] * fy;
+            const float qx = cam.br[0] * omy + cam.tr[0] * fy, qy = cam.br[1] * omy + cam.tr[1] * fy,
+                        qz = cam.br[2] * omy + cam.tr[2] * fy;
+            const float t = r.depth * cam.far_plane;
+            r.position[0] = cam.pos[0] + t * (lx * omx + qx * fx);
+            r.position[1] = cam.pos[1] + t * (ly * omx + qy * fx);
+            r.position[2] = cam.pos[2] + t * (lz * omx + qz * fx);
+        }
+    } else {
+        r.hit = 0u;
+        r.cluster_pointer = r.cluster_idx = r.object_idx = r.voxel = 0xFFFFFFFFu;
+    }
+    out[i] = r;
+}
+
+static void cam_of(const tgx_camera* cam, shade_cam& sc)
+{
+    for (int a = 0; a < 3; a++) {
+        sc.pos[a] = cam->camera[a];
+        sc.bl[a] = cam->ray_bl[a]; sc.br[a] = cam->ray_br[a];
+        sc.tr[a] = cam->ray_tr[a]; sc.tl[a] = cam->ray_tl[a];
+    }
+    sc.far_plane = cam->far_plane;
+}
+
 extern "C" {
+
+int tgx_pick(tgx_context* ctx, const tgx_camera* cam, const uint64_t* vis_dev, uint32_t w, uint32_t h,
+             const uint32_t* xy, uint32_t n, tgx_pick_result* out_host)
+{
+    if (!ctx) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (n == 0) return TGX_OK;
+    if (!vis_dev || !xy || !out_host) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "null argument");
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
+    if (n > (1u << 20)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "at most 2^20 picks per call");
+    for (uint32_t i = 0; i < n; i++)
+        if (xy[2 * i] >= w || xy[2 * i + 1] >= h)
+            return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "pick %u at (%u, %u) is outside %ux%u", i, xy[2 * i],
+                            xy[2 * i + 1], w, h);
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->pick_buf, (8ull + sizeof(tgx_pick_result)) * n, "picks"))) return rc;
+    uint32_t* d_xy = (uint32_t*)ctx->pick_buf.ptr;
+    tgx_pick_result* d_out = (tgx_pick_result*)(d_xy + 2ull * n);
+    shade_cam sc = {};
+    if (cam) cam_of(cam, sc);
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_xy, xy, 8ull * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(pick_k, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)ctx->cluster_pointers.ptr, (const uint32_t*)ctx->cluster_idx_to_object.ptr, sc,
+                       cam ? 1u : 0u, w, h, vis_dev, d_xy, n, d_out);
+    TGX_LAUNCH_CHECK(ctx, "pick_k");
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(out_host, d_out, sizeof(tgx_pick_result) * n, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
 
 int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
                      const tgx_shade_params* prm, float* rgba_dev)
@@ -205,12 +285,7 @@ int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "invalid resolution %ux%u", w, h);
     if (prm->view > TGX_SHADE_SHADING) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "unknown view %u", prm->view);
     shade_cam sc;
-    for (int a = 0; a < 3; a++) {
-        sc.pos[a] = cam->camera[a];
-        sc.bl[a] = cam->ray_bl[a]; sc.br[a] = cam->ray_br[a];
-        sc.tr[a] = cam->ray_tr[a]; sc.tl[a] = cam->ray_tl[a];
-    }
-    sc.far_plane = cam->far_plane;
+    cam_of(cam, sc);
     shade_consts k;
     k.view = prm->view;
     /* normalize(light_dir) as GLSL does: v / sqrt(dot(v, v)) */
