@@ -156,13 +156,10 @@ int tgx_scene_export(tgx_context* ctx, tgx_object* objects, uint32_t* cluster_po
                      uint32_t* cluster_idx_to_object, uint32_t* bits, uint8_t* materials,
                      tgx_lut_entry* luts);
 
-/* Scene files (host only; no context needed). tg keeps its scene in memory
- * (tgvk_raytracer.c:805-992); this format stores the same arrays byte for
- * byte with an FNV-1a 64 checksum. save: TGX_ERR_IO if the file cannot be
- * written. counts / load: TGX_ERR_IO if it cannot be opened,
- * TGX_ERR_INVALID_ARG if it is not a scene file, is truncated or fails the
- * checksum. load fills caller arrays sized by tgx_scene_file_counts
- * (materials may be NULL; a file without materials loads zeros). */
+/* Array sizes of the context's scene as tgx_scene_export writes them. For
+ * a dynamic scene: n_objects and n_clusters are the capacities (records of
+ * free object slots are zero, tg_object_is_initialized at
+ * tgvk_raytracer.c:1069-1077) and n_cluster_pointers the live pointers. */
 typedef struct tgx_scene_counts {
     uint32_t n_objects;
     uint32_t n_cluster_pointers;
@@ -170,7 +167,46 @@ typedef struct tgx_scene_counts {
     uint32_t n_luts;
     uint32_t has_materials;
 } tgx_scene_counts;
+int tgx_scene_get_counts(tgx_context* ctx, tgx_scene_counts* out);
 
+/* ------------------------------------------------------------------------ */
+/* Dynamic scenes: objects created and destroyed one at a time              */
+/* ------------------------------------------------------------------------ */
+
+/* Empty scene with fixed capacities and the LUT array (the scene half of
+ * tg_raytracer_create, tgvk_raytracer.c:687-712): object and cluster
+ * indices are handed out from LIFO free stacks that start as
+ * [capacity - 1, ..., 0]. Replaces any uploaded scene. */
+int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t cluster_capacity,
+                      const tgx_lut_entry* luts, uint32_t n_luts);
+
+/* tg_raytracer_create_object (tgvk_raytracer.c:805-868): pops an object
+ * index and n = nx ny nz cluster indices, appends n cluster pointers after
+ * the live ones, writes the object record (first pointer, extent in
+ * clusters, translation, column-major rotation, LUT index) and both cluster
+ * maps. bits (16 u32 per cluster) and materials (512 B per cluster), in
+ * relative-pointer order x fastest, fill the new clusters (NULL: empty /
+ * zero). tg fixes the rotation (7 degrees times the index about y) and
+ * fills voxels on the host; here the caller passes both. */
+int tgx_object_create(tgx_context* ctx, const uint32_t n_clusters_per_dim[3], const float translation[3],
+                      const float rotation[16], uint32_t lut_idx, const uint32_t* bits,
+                      const uint8_t* materials, uint32_t* out_object_idx);
+
+/* tg_raytracer_destroy_object (tgvk_raytracer.c:994-1067): pushes the
+ * object's cluster indices back in pointer order, shifts every later
+ * cluster pointer down over the gap (on the device), lowers the first
+ * pointer of every later object, clears the record and pushes the object
+ * index back. The SVO is not updated: build it again before tracing. Both
+ * calls are refused while a frame graph exists. */
+int tgx_object_destroy(tgx_context* ctx, uint32_t object_idx);
+
+/* Scene files (host only; no context needed). tg keeps its scene in memory
+ * (tgvk_raytracer.c:805-992); this format stores the same arrays byte for
+ * byte with an FNV-1a 64 checksum. save: TGX_ERR_IO if the file cannot be
+ * written. counts / load: TGX_ERR_IO if it cannot be opened,
+ * TGX_ERR_INVALID_ARG if it is not a scene file, is truncated or fails the
+ * checksum. load fills caller arrays sized by tgx_scene_file_counts
+ * (materials may be NULL; a file without materials loads zeros). */
 int tgx_scene_save(const char* path, const tgx_scene_desc* scene);
 int tgx_scene_file_counts(const char* path, tgx_scene_counts* out);
 int tgx_scene_load(const char* path, tgx_object* objects, uint32_t* cluster_pointers,

@@ -237,6 +237,10 @@ def load_library() -> ctypes.CDLL:
         "tgx_shade_device": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "tgx_pick": (i32, [vp, vp, vp, u32, u32, vp, u32, vp]),
         "tgx_svo_visibility_device": (i32, [vp, vp, u32, u32, vp]),
+        "tgx_scene_get_counts": (i32, [vp, vp]),
+        "tgx_scene_reserve": (i32, [vp, u32, u32, vp, u32]),
+        "tgx_object_create": (i32, [vp, vp, vp, vp, u32, vp, vp, ctypes.POINTER(u32)]),
+        "tgx_object_destroy": (i32, [vp, u32]),
         "tgx_scene_save": (i32, [ctypes.c_char_p, ctypes.POINTER(_SceneDesc)]),
         "tgx_scene_file_counts": (i32, [ctypes.c_char_p, vp]),
         "tgx_scene_load": (i32, [ctypes.c_char_p, vp, vp, vp, vp, vp, vp]),
@@ -612,6 +616,51 @@ class Context:
             for p in (vis_d, rad_d, irr_d):
                 self.free(p)
         return vis, rad, irr
+
+    # -- dynamic scenes ---------------------------------------------------------
+
+    def scene_reserve(self, object_capacity: int, cluster_capacity: int, luts: np.ndarray) -> None:
+        luts = np.ascontiguousarray(luts, dtype=LUT_DTYPE)
+        assert luts.size % 256 == 0
+        self._check(self.lib.tgx_scene_reserve(self.h, object_capacity, cluster_capacity, _ptr(luts), luts.size // 256),
+                    "tgx_scene_reserve")
+
+    def object_create(self, n_clusters_per_dim, translation, rotation, lut_idx: int = 0, bits=None,
+                      materials=None) -> int:
+        n = np.ascontiguousarray(n_clusters_per_dim, np.uint32)
+        t = np.ascontiguousarray(translation, np.float32)
+        r = np.ascontiguousarray(rotation, np.float32)
+        b = None if bits is None else np.ascontiguousarray(bits, np.uint32)
+        m = None if materials is None else np.ascontiguousarray(materials, np.uint8)
+        k = int(np.prod(n.astype(np.uint64)))
+        assert b is None or b.size == 16 * k
+        assert m is None or m.size == 512 * k
+        out = ctypes.c_uint32()
+        self._check(self.lib.tgx_object_create(self.h, _ptr(n), _ptr(t), _ptr(r), lut_idx, _ptr(b), _ptr(m),
+                                               ctypes.byref(out)), "tgx_object_create")
+        return int(out.value)
+
+    def object_destroy(self, object_idx: int) -> None:
+        self._check(self.lib.tgx_object_destroy(self.h, object_idx), "tgx_object_destroy")
+
+    def scene_counts(self):
+        c = np.zeros(1, SCENE_COUNTS_DTYPE)
+        self._check(self.lib.tgx_scene_get_counts(self.h, _ptr(c)), "tgx_scene_get_counts")
+        return c[0]
+
+    def scene_arrays(self) -> Scene:
+        """The context's scene arrays (tgx_scene_export, sized by tgx_scene_get_counts)."""
+        c = self.scene_counts()
+        s = Scene(objects=np.zeros(int(c["n_objects"]), OBJECT_DTYPE),
+                  cluster_pointers=np.zeros(int(c["n_cluster_pointers"]), np.uint32),
+                  cluster_idx_to_object=np.zeros(int(c["n_clusters"]), np.uint32),
+                  bits=np.zeros(16 * int(c["n_clusters"]), np.uint32),
+                  materials=np.zeros(512 * int(c["n_clusters"]), np.uint8),
+                  luts=np.zeros(256 * int(c["n_luts"]), LUT_DTYPE))
+        self._check(self.lib.tgx_scene_export(self.h, _ptr(s.objects), _ptr(s.cluster_pointers),
+                                              _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials),
+                                              _ptr(s.luts)), "tgx_scene_export")
+        return s
 
     def svo_visibility_device(self, cam: np.ndarray, w: int, h: int, vis_dev: int) -> None:
         cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)

@@ -27,7 +27,7 @@ ORACLE_DIR = ROOT / "oracle"
 
 ARCH = os.environ.get("TGX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["tgx_runtime.hip", "tgx_visibility.hip", "tgx_svo.hip", "tgx_gi.hip", "tgx_nrc.hip", "tgx_gen_dev.hip", "tgx_frame.hip", "tgx_shade.hip", "tgx_prims.hip"]
+HIP_SOURCES = ["tgx_runtime.hip", "tgx_visibility.hip", "tgx_svo.hip", "tgx_gi.hip", "tgx_nrc.hip", "tgx_gen_dev.hip", "tgx_frame.hip", "tgx_shade.hip", "tgx_dynamic.hip", "tgx_prims.hip"]
 C_SOURCES = ["tgx_gen.c", "tgx_io.c"]
 
 HIPCC_FLAGS = [

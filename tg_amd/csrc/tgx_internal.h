@@ -110,6 +110,14 @@ struct tgx_context {
     tgx_buffer   frame_acc;      /* totals of the last tgx_frame_batch     */
     tgx_buffer   pick_buf;       /* tgx_pick coordinates and results       */
 
+    /* Dynamic scene (tgx_dynamic.hip): free stacks, host copy of the
+     * object records, staging and compaction scratch. */
+    bool                   dynamic = false;
+    bool                   svo_stale = false; /* scene edited since the last SVO build */
+    std::vector<tgx_object> dyn_objects;
+    std::vector<uint32_t>  dyn_free_objects, dyn_free_clusters;
+    tgx_buffer             dyn_stage, dyn_tmp;
+
     /* Profiling. */
     bool                           profiling = false;
     std::vector<tgx_kernel_stat>   stats;

@@ -123,7 +123,7 @@ int tgx_destroy(tgx_context* ctx)
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,
-                          &ctx->frame_acc, &ctx->pick_buf};
+                          &ctx->frame_acc, &ctx->pick_buf, &ctx->dyn_stage, &ctx->dyn_tmp};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     tgx_svo_release(ctx);
     tgx_nrc_release(ctx);
@@ -187,6 +187,10 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
     }
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     int rc;
+    ctx->dynamic = false;
+    ctx->dyn_objects.clear();
+    ctx->dyn_free_objects.clear();
+    ctx->dyn_free_clusters.clear();
     if ((rc = upload(ctx, &ctx->objects, s->objects, sizeof(tgx_object) * s->n_objects, "objects"))) return rc;
     if ((rc = upload(ctx, &ctx->cluster_pointers, s->cluster_pointers, 4ull * s->n_cluster_pointers, "cluster pointers"))) return rc;
     if ((rc = upload(ctx, &ctx->cluster_idx_to_object, s->cluster_idx_to_object, 4ull * s->n_clusters, "cluster->object"))) return rc;

@@ -515,6 +515,7 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
     tgx_svo_release(ctx);
     int rc;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
+    if (n_cptr == 0) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "the scene has no clusters");
     const uint32_t leaf_level = depth - 5u;
 
     if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_frames, sizeof(svo_frame) * (size_t)n_cptr, "svo frames"))) return rc;
@@ -693,6 +694,7 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
     ctx->svo_info.n_list_entries = n_list;
     ctx->svo_info.flags = flags;
     ctx->svo_built = true;
+    ctx->svo_stale = false;
     return TGX_OK;
 }
 

@@ -591,6 +591,15 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     const uint32_t tiles_per_cam = tiles_x * tiles_y;
     const uint64_t n_tiles64 = (uint64_t)tiles_per_cam * n_cams;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
+    if (n_cptr == 0) {
+        /* a dynamic scene without objects: every pixel is empty */
+        const uint64_t words = 2ull * n_cams * W * H;
+        if (words >= (1ull << 32)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
+        hipLaunchKernelGGL(vis_fill_u32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (uint32_t*)out_dev, (uint32_t)words, 0xFFFFFFFFu);
+        TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+        return TGX_OK;
+    }
     const uint64_t n_slots = (uint64_t)n_cptr * n_cams;
     if (n_tiles64 >= (1ull << 31) || n_slots >= (1ull << 32))
         return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
