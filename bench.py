@@ -350,6 +350,39 @@ def svo_visibility_bench(ctx, cam, steps: int) -> dict:
     return {"ms": ms, "rays_per_s": W * H / (ms * 1e-3), "hit_pixels": int((vis != T.VISIBILITY_EMPTY).sum())}
 
 
+def dynamic_bench(ctx, scene) -> dict:
+    """The frame's scene rebuilt as a dynamic scene, object by object, then
+    the first object destroyed (every later pointer shifts) and created
+    again: wall time per call, each call synchronous."""
+    objs = scene.objects
+    ctx.scene_reserve(objs.size, scene.n_clusters, scene.luts)
+    t0 = time.perf_counter()
+    for o in range(objs.size):
+        ob = objs[o]
+        n = int(np.prod(ob["n_cluster_pointers_per_dim"].astype(np.int64)))
+        f = int(ob["first_cluster_pointer"])
+        cidx = scene.cluster_pointers[f:f + n].astype(np.int64)
+        bits = scene.bits.reshape(-1, 16)[cidx]
+        mats = scene.materials.reshape(-1, 512)[cidx]
+        ctx.object_create(ob["n_cluster_pointers_per_dim"], ob["translation"], ob["rotation"], int(ob["lut_idx"]),
+                          bits, mats)
+    create_all = time.perf_counter() - t0
+    ob = objs[0]
+    t0 = time.perf_counter()
+    ctx.object_destroy(0)
+    destroy_s = time.perf_counter() - t0
+    n = int(np.prod(ob["n_cluster_pointers_per_dim"].astype(np.int64)))
+    cidx = scene.cluster_pointers[:n].astype(np.int64)
+    t0 = time.perf_counter()
+    ctx.object_create(ob["n_cluster_pointers_per_dim"], ob["translation"], ob["rotation"], int(ob["lut_idx"]),
+                      scene.bits.reshape(-1, 16)[cidx], scene.materials.reshape(-1, 512)[cidx])
+    create_s = time.perf_counter() - t0
+    live = int(ctx.scene_counts()["n_cluster_pointers"])
+    return {"objects": int(objs.size), "cluster_pointers": live, "create_all_ms": 1e3 * create_all,
+            "destroy_first_us": 1e6 * destroy_s, "create_us": 1e6 * create_s,
+            "note": "destroying object 0 shifts every later cluster pointer on the device"}
+
+
 def pick_bench(ctx, cam, vis_d, n: int = 4096) -> dict:
     """Pick readback: latency of one synchronous call for 1 and for n pixels."""
     rng = np.random.default_rng(0)
@@ -637,6 +670,9 @@ def main() -> int:
                                           args.neural_pretrain, ms_per_step, rays_per_frame)
     for p in (vis_d, rad_d, irr_d):
         ctx.free(p)
+    if args.workload == "frame":
+        log("dynamic scene edits")
+        line["dynamic"] = dynamic_bench(ctx, scene)
     ctx.close()
     if not args.no_batch and args.workload == "frame":
         log("batched 16-camera frame")
