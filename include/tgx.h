@@ -460,8 +460,10 @@ typedef struct tgx_tuning {
                                      by (direction octant, origin cell) before tracing (0) */
     uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (16) */
     uint32_t trace_blocks_per_cu; /* persistent 1024-lane trace workgroups per CU, 1..2 (2) */
-    uint32_t trace_vote;          /* 1: a wave runs only node steps or only DDA bursts per
-                                     iteration, by majority of its lanes (0) */
+    uint32_t trace_vote;          /* 0: every iteration runs node steps and DDA bursts for the
+                                     lanes needing them; k in 1..64: a wave runs only DDA
+                                     bursts once >= k/64 of its active lanes are in a leaf,
+                                     else only node steps (0) */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);

@@ -51,8 +51,8 @@
 #define GI_INV_PI 0.318309886f
 
 #define TRACE_BLOCK 1024
+#define DDA_BURST 8u
 #define TRACE_LDS_NODES_MAX 8000u   /* inner nodes + leaf brick masks in LDS (64000 B) */
-#define DDA_BURST 8
 
 /* ------------------------------------------------------------------------ */
 /* Traversal                                                                */
@@ -350,40 +350,32 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         const unsigned long long bm = L.bm;
         uint32_t wi_last = L.wi, wv = L.wv;
 #pragma unroll 1
-        for (int k = 0; k < DDA_BURST; k++) {
-            /* tg_amanatides_woo: stop at the iteration cap (leave), on a
-             * solid voxel (hit) or after a step out of the block (leave) */
-            const uint32_t cap = it >> 8;            /* it <= 4 * TGX_DDA_MAX_ITERATIONS = 256 */
+        for (uint32_t k = 0; k < DDA_BURST; k++) {
+            const uint32_t cap = it >> 8;
             const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
             const uint32_t inb = live & (cap ^ 1u) & (uint32_t)(bm >> b);
             const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
             if ((inb & 1u) && wi != wi_last) { wi_last = wi; wv = grid[wi]; }
             const uint32_t h = inb & (wv >> (uint32_t)x) & 1u;
-            /* step along x if tmx < tmy and tmx < tmz, else along y if
-             * !(tmx < tmy) and tmy < tmz, else along z */
-            const bool xy = tmx < tmy;
-            const float m1 = xy ? tmx : tmy;
-            const uint32_t axis = m1 < tmz ? (xy ? 0u : 1u) : 2u;
+            const float m = fminf(fminf(tmx, tmy), tmz);
+            const uint32_t sz_ = tmz == m ? 1u : 0u;
+            const uint32_t sy_ = (tmy == m ? 1u : 0u) & (sz_ ^ 1u);
+            const uint32_t sx_ = (sz_ | sy_) ^ 1u;
             const uint32_t move = live & (cap ^ 1u) & (h ^ 1u);
-            const float tsum = (axis == 0u ? tmx : (axis == 1u ? tmy : tmz)) +
-                               (axis == 0u ? tdx : (axis == 1u ? tdy : tdz));
-            const int dstep = axis == 0u ? sx : (axis == 1u ? sy : sz);
-            const uint32_t mx = move & (axis == 0u ? 1u : 0u);
-            const uint32_t my = move & (axis == 1u ? 1u : 0u);
-            const uint32_t mz = move & (axis == 2u ? 1u : 0u);
-            tmx = mx ? tsum : tmx;
-            tmy = my ? tsum : tmy;
-            tmz = mz ? tsum : tmz;
-            x += (int)mx * dstep;
-            y += (int)my * dstep;
-            z += (int)mz * dstep;
+            const uint32_t mx = move & sx_, my = move & sy_, mz = move & sz_;
+            tmx = mx ? tmx + tdx : tmx;
+            tmy = my ? tmy + tdy : tmy;
+            tmz = mz ? tmz + tdz : tmz;
+            x = mx ? x + sx : x;
+            y = my ? y + sy : y;
+            z = mz ? z + sz : z;
             const uint32_t outside = (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
             const uint32_t lv = live & (cap | (move & outside));
             it += move & (outside ^ 1u);
             hit_i |= h;
             leave_i |= lv;
             live &= (h | lv) ^ 1u;
-            if (!__any(live)) break;
+            if (!__any(live != 0u)) break;
         }
         L.tmx = tmx; L.tmy = tmy; L.tmz = tmz;
         L.wi = wi_last; L.wv = wv;
@@ -492,13 +484,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             if (exhausted) break;
             continue;
         }
-        /* phase vote (tuning): a wave runs only node steps or only DDA bursts
-         * in an iteration, whichever mode more of its active lanes are in */
+        /* phase vote (tuning): a wave runs only DDA bursts in an iteration
+         * once at least vote/64 of its active lanes are in a leaf DDA, and
+         * only node steps otherwise */
         bool allow_node = true, allow_dda = true;
         if (vote) {
             const uint32_t n_dda = (uint32_t)__popcll(__ballot(active && (L.node & ND_DDA)));
             const uint32_t n_all = (uint32_t)__popcll(__ballot(active));
-            allow_dda = 2u * n_dda >= n_all;
+            allow_dda = 64u * n_dda >= vote * n_all;
             allow_node = !allow_dda;
         }
         svo_res res;
