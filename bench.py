@@ -161,7 +161,8 @@ BATCH_WORKLOAD = ("batched frame: 4096 noise-filled objects of 1-32 clusters per
 def batch_bench(args, steps: int, pretrain: int) -> dict:
     """Both forms of the batched frame as captured graphs: plain SVO GI and
     the neural cache with fallback. Each is autotuned (uncaptured frames),
-    captured, replayed `pretrain` times untimed, then `steps` times timed.
+    captured, replayed untimed (3 times for plain GI, `pretrain` times for
+    the cache, which trains meanwhile), then `steps` times timed.
     Per-kernel times come from one uncaptured, event-profiled frame after
     the timed replays (same kernels and inputs)."""
     params = T.noise_objects_params(seed=args.seed, n_objects=BATCH_OBJECTS, max_extent=BATCH_EXTENT,
@@ -199,7 +200,8 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
         t0 = time.perf_counter()
         best = ctx.autotune(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags, frames=1)
         tune_s = time.perf_counter() - t0
-        log(f"batch {mode}: capture, {pretrain} untimed replays")
+        untimed = pretrain if flags else 3  # the cache trains while it replays; plain GI only warms up
+        log(f"batch {mode}: capture, {untimed} untimed replays")
         t0 = time.perf_counter()
         g = None if args.batch_no_graph else ctx.frame_graph(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags)
         capture_s = time.perf_counter() - t0
@@ -210,7 +212,7 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
             else:
                 g.launch()
         try:
-            for _ in range(pretrain):
+            for _ in range(untimed):
                 run()
             ctx.synchronize()
             log(f"batch {mode}: {steps} timed {'frames' if g is None else 'replays'}")
@@ -256,7 +258,7 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
             "rays_traced_per_s": traced / (ms * 1e-3),
             "stats": {k: int(st[k]) for k in st.dtype.names},
             "tuning": {k: int(best[k][0]) for k in best.dtype.names},
-            "autotune_s": tune_s, "capture_s": capture_s, "captured": g is not None,
+            "autotune_s": tune_s, "capture_s": capture_s, "captured": g is not None, "untimed_replays": untimed,
             "kernels_ms_per_frame": k_ms,
             "roofline": {"bound": "hbm", "kernel": dominant, "kernel_avg_ms": avg_ms, "launches_per_frame": launches,
                          "alg_bytes_per_launch": alg, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -533,7 +535,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", choices=("frame", "visibility", "batch"), default="frame")
-    ap.add_argument("--batch-pretrain", type=int, default=20, help="untimed replays of each batched frame graph")
+    ap.add_argument("--batch-pretrain", type=int, default=60, help="untimed (training) replays of the batched cache frame")
     ap.add_argument("--batch-steps", type=int, default=5, help="timed replays of each batched frame graph")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 16-camera frame")
     ap.add_argument("--batch-no-graph", action="store_true",
