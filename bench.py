@@ -352,10 +352,12 @@ def svo_visibility_bench(ctx, cam, steps: int) -> dict:
     return {"ms": ms, "rays_per_s": W * H / (ms * 1e-3), "hit_pixels": int((vis != T.VISIBILITY_EMPTY).sum())}
 
 
-def dynamic_bench(ctx, scene) -> dict:
+def dynamic_bench(ctx, scene, mn, depth: int) -> dict:
     """The frame's scene rebuilt as a dynamic scene, object by object, then
     the first object destroyed (every later pointer shifts) and created
-    again: wall time per call, each call synchronous."""
+    again: wall time per call, each call synchronous. Then the SVO after
+    that edit: tgx_svo_update (reusing the clean leaves of a build before
+    the edit) against a full tgx_svo_build of the same scene."""
     objs = scene.objects
     ctx.scene_reserve(objs.size, scene.n_clusters, scene.luts)
     t0 = time.perf_counter()
@@ -369,6 +371,7 @@ def dynamic_bench(ctx, scene) -> dict:
         ctx.object_create(ob["n_cluster_pointers_per_dim"], ob["translation"], ob["rotation"], int(ob["lut_idx"]),
                           bits, mats)
     create_all = time.perf_counter() - t0
+    ctx.svo_build(mn, depth)
     ob = objs[0]
     t0 = time.perf_counter()
     ctx.object_destroy(0)
@@ -380,9 +383,23 @@ def dynamic_bench(ctx, scene) -> dict:
                       scene.bits.reshape(-1, 16)[cidx], scene.materials.reshape(-1, 512)[cidx])
     create_s = time.perf_counter() - t0
     live = int(ctx.scene_counts()["n_cluster_pointers"])
+    t0 = time.perf_counter()
+    reused, voxelized = ctx.svo_update()
+    update_s = time.perf_counter() - t0
+    upd = ctx.svo_export()
+    t0 = time.perf_counter()
+    ctx.svo_build(mn, depth)
+    build_s = time.perf_counter() - t0
+    full = ctx.svo_export()
+    same = all(np.array_equal(upd[k], full[k]) for k in ("nodes", "bits", "materials", "leaf_n", "list_offsets", "list"))
+    if not same:
+        raise SystemExit("tgx_svo_update differs from tgx_svo_build after the edit")
     return {"objects": int(objs.size), "cluster_pointers": live, "create_all_ms": 1e3 * create_all,
             "destroy_first_us": 1e6 * destroy_s, "create_us": 1e6 * create_s,
-            "note": "destroying object 0 shifts every later cluster pointer on the device"}
+            "svo_update_ms": 1e3 * update_s, "svo_full_build_ms": 1e3 * build_s,
+            "svo_leaves_reused": reused, "svo_leaves_voxelized": voxelized, "svo_update_equals_build": same,
+            "note": "destroying object 0 shifts every later cluster pointer on the device; the SVO update "
+                    "follows that destroy and re-create of object 0"}
 
 
 def pick_bench(ctx, cam, vis_d, n: int = 4096) -> dict:
@@ -674,7 +691,7 @@ def main() -> int:
         ctx.free(p)
     if args.workload == "frame":
         log("dynamic scene edits")
-        line["dynamic"] = dynamic_bench(ctx, scene)
+        line["dynamic"] = dynamic_bench(ctx, scene, mn, DEPTH)
     ctx.close()
     if not args.no_batch and args.workload == "frame":
         log("batched 16-camera frame")

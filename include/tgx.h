@@ -305,6 +305,19 @@ typedef struct tgx_svo_hit {
  * in its list that sampled it. min must be integral; depth in
  * [TGX_SVO_MIN_DEPTH, TGX_SVO_MAX_DEPTH]. */
 int tgx_svo_build(tgx_context* ctx, const float min[3], uint32_t depth, uint32_t flags);
+
+/* Rebuilds the SVO after object creates and destroys, with the last
+ * build's min, depth and flags: the incremental update the reference
+ * promises (README.md:43) but does not have (tg_svo_create runs once,
+ * tgvk_raytracer.c:1187-1196). The nodes, lists and inner levels are
+ * rebuilt; a leaf whose key and candidate clusters (indices, in pointer
+ * order) equal those of a leaf of the last build, none of them placed since,
+ * copies that leaf instead of re-sampling 32^3 voxels. The result equals
+ * tgx_svo_build's on the same scene, byte for byte. Without a previous
+ * build on the current dynamic scene (tgx_scene_reserve) every leaf is
+ * sampled. reused_leaves / voxelized_leaves (may be NULL) receive the
+ * split. TGX_ERR_NOT_BUILT without a previous tgx_svo_build. */
+int tgx_svo_update(tgx_context* ctx, uint32_t* reused_leaves, uint32_t* voxelized_leaves);
 int tgx_svo_get_info(tgx_context* ctx, tgx_svo_info* out);
 
 /* Copies the SVO out in canonical form: nodes[n_nodes]; per leaf bits

@@ -2,7 +2,8 @@
 tgx_object_destroy) against the bookkeeping model of oracle/dynamic.py:
 after every create and destroy the device arrays equal the model's, the
 visibility of the edited scene equals the oracle's on the model arrays, and
-an SVO rebuilt after the edits equals the oracle's."""
+an SVO rebuilt after the edits, in full or by tgx_svo_update, equals the
+oracle's."""
 from __future__ import annotations
 
 import sys
@@ -103,3 +104,62 @@ def test_dynamic_scene_errors(ctx):
     assert int(ctx.scene_arrays().objects[b]["first_cluster_pointer"]) == 0
     ctx.object_destroy(b)
     assert ctx.scene_counts()["n_cluster_pointers"] == 0
+
+
+SVO_KEYS = ("nodes", "bits", "materials", "leaf_n", "list_offsets", "list")
+
+
+def test_svo_update_equals_full_build(ctx):
+    """tgx_svo_update after creates and destroys equals a full tgx_svo_build
+    of the edited scene byte for byte (and the oracle's SVO), and reuses the
+    leaves an edit leaves untouched."""
+    rng = np.random.default_rng(33)
+    luts = random_luts(rng)
+    ctx.scene_reserve(OBJ_CAP, CL_CAP, luts)
+    model = DynamicScene(OBJ_CAP, CL_CAP, luts)
+    mn = np.array([-128.0, -128.0, -128.0], np.float32)
+
+    def create():
+        n = tuple(int(v) for v in rng.integers(1, 4, 3))
+        k = int(np.prod(n))
+        t = rng.uniform(-100, 100, 3).astype(np.float32)
+        rot = S.random_rotation(rng)
+        lut = int(rng.integers(3))
+        bits = np.packbits(rng.random(512 * k) < 0.4, bitorder="little").view(np.uint32)
+        mats = rng.integers(0, 256, 512 * k, dtype=np.uint8)
+        o = ctx.object_create(n, t, rot, lut, bits, mats)
+        assert o == model.create(n, t, rot, lut, bits, mats)
+        return o
+
+    alive = [create() for _ in range(8)]
+    with pytest.raises(T.TgxError):
+        ctx.svo_update()  # nothing built yet
+    ctx.svo_build(mn, 8)
+    total_reused = 0
+    for step in range(6):
+        if step % 2 == 0:
+            o = alive.pop(int(rng.integers(len(alive))))
+            say(f"svo update step {step}: destroy {o}")
+            ctx.object_destroy(o)
+            model.destroy(o)
+        else:
+            alive.append(create())
+            say(f"svo update step {step}: create {alive[-1]}")
+        reused, voxelized = ctx.svo_update()
+        info = ctx.svo_info()
+        assert reused + voxelized == int(info["n_leaves"])
+        total_reused += reused
+        upd = ctx.svo_export()
+        ref = O.OracleSvo(O.OracleScene(model.scene()), mn, 8).export()
+        for k in SVO_KEYS:
+            assert np.array_equal(upd[k], getattr(ref, k)), f"step {step}: {k} differs from the oracle"
+        if step in (1, 4):
+            # a full build in between: the next update compares against it
+            ctx.svo_build(mn, 8)
+            full = ctx.svo_export()
+            for k in SVO_KEYS:
+                assert np.array_equal(upd[k], full[k]), f"step {step}: {k} differs from a full build"
+    assert total_reused > 0
+    # an update right after a build reuses every leaf
+    reused, voxelized = ctx.svo_update()
+    assert voxelized == 0 and reused == int(ctx.svo_info()["n_leaves"])

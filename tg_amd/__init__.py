@@ -219,6 +219,7 @@ def load_library() -> ctypes.CDLL:
                                   ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
         "tgx_gen_stream_seed": (u64, [u64, u64]),
         "tgx_svo_build": (i32, [vp, vp, u32, u32]),
+        "tgx_svo_update": (i32, [vp, vp, vp]),
         "tgx_svo_get_info": (i32, [vp, vp]),
         "tgx_svo_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "tgx_svo_trace": (i32, [vp, vp, vp, u32, ctypes.c_float, vp]),
@@ -482,6 +483,12 @@ class Context:
     def svo_build(self, mn, depth: int, flags: int = 0) -> None:
         m = np.ascontiguousarray(mn, np.float32)
         self._check(self.lib.tgx_svo_build(self.h, _ptr(m), depth, flags), "tgx_svo_build")
+
+    def svo_update(self) -> tuple[int, int]:
+        """tgx_svo_update: rebuild after scene edits; (reused, voxelized) leaves."""
+        r = np.zeros(2, np.uint32)
+        self._check(self.lib.tgx_svo_update(self.h, _ptr(r[0:]), _ptr(r[1:])), "tgx_svo_update")
+        return int(r[0]), int(r[1])
 
     def svo_info(self):
         i = np.zeros(1, SVO_INFO_DTYPE)

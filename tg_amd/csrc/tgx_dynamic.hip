@@ -28,7 +28,7 @@ __global__ void dyn_copy_k(const uint32_t* __restrict__ src, uint32_t* __restric
 __global__ void dyn_place_k(const uint32_t* __restrict__ idx, uint32_t n, uint32_t first, uint32_t object,
                             const uint32_t* __restrict__ stage_bits, const uint8_t* __restrict__ stage_mats,
                             uint32_t* __restrict__ cptrs, uint32_t* __restrict__ c2o, uint32_t* __restrict__ bits,
-                            uint8_t* __restrict__ mats)
+                            uint8_t* __restrict__ mats, uint8_t* __restrict__ dirty)
 {
     const uint32_t k = blockIdx.x;
     if (k >= n) return;
@@ -36,6 +36,7 @@ __global__ void dyn_place_k(const uint32_t* __restrict__ idx, uint32_t n, uint32
     if (threadIdx.x == 0) {
         cptrs[first + k] = c;
         c2o[c] = object;
+        dirty[c] = 1u;
     }
     if (threadIdx.x < 16u) bits[(size_t)c * 16u + threadIdx.x] = stage_bits ? stage_bits[(size_t)k * 16u + threadIdx.x] : 0u;
     for (uint32_t v = threadIdx.x; v < 512u; v += blockDim.x)
@@ -73,6 +74,7 @@ int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t clust
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     ctx->has_scene = false;
     ctx->dynamic = false;
+    ctx->svo_prev_valid = false;
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->objects, sizeof(tgx_object) * (size_t)object_capacity, "objects")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->cluster_pointers, 4ull * cluster_capacity, "cluster pointers")) ||
@@ -80,7 +82,8 @@ int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t clust
         (rc = tgx_buffer_reserve(ctx, &ctx->bits, 64ull * cluster_capacity, "solid bits")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->materials, 512ull * cluster_capacity, "materials")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->luts, sizeof(tgx_lut_entry) * TGX_LUT_ENTRIES * (size_t)n_luts, "luts")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->dyn_tmp, 4ull * cluster_capacity, "compaction scratch")))
+        (rc = tgx_buffer_reserve(ctx, &ctx->dyn_tmp, 4ull * cluster_capacity, "compaction scratch")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->cl_dirty, 4ull * ((cluster_capacity + 3ull) / 4ull), "cluster dirty flags")))
         return rc;
     ctx->dyn_objects.assign(object_capacity, tgx_object{});
     ctx->dyn_free_objects.resize(object_capacity);
@@ -91,6 +94,7 @@ int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t clust
     TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->cluster_idx_to_object.ptr, 0, 4ull * cluster_capacity, ctx->stream));
     TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->bits.ptr, 0, 64ull * cluster_capacity, ctx->stream));
     TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->materials.ptr, 0, 512ull * cluster_capacity, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->cl_dirty.ptr, 0, 4ull * ((cluster_capacity + 3ull) / 4ull), ctx->stream));
     TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->luts.ptr, luts, sizeof(tgx_lut_entry) * TGX_LUT_ENTRIES * (size_t)n_luts,
                                       hipMemcpyHostToDevice, ctx->stream));
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -149,7 +153,8 @@ int tgx_object_create(tgx_context* ctx, const uint32_t n_clusters_per_dim[3], co
     if ((rc = sync_objects(ctx, o))) return rc;
     hipLaunchKernelGGL(dyn_place_k, dim3(n), dim3(128), 0, ctx->stream, d_idx, n, ob.first_cluster_pointer, o,
                        bits ? d_bits : nullptr, materials ? d_mats : nullptr, (uint32_t*)ctx->cluster_pointers.ptr,
-                       (uint32_t*)ctx->cluster_idx_to_object.ptr, (uint32_t*)ctx->bits.ptr, (uint8_t*)ctx->materials.ptr);
+                       (uint32_t*)ctx->cluster_idx_to_object.ptr, (uint32_t*)ctx->bits.ptr, (uint8_t*)ctx->materials.ptr,
+                       (uint8_t*)ctx->cl_dirty.ptr);
     TGX_LAUNCH_CHECK(ctx, "dyn_place_k");
     /* the host arrays and the staged copies must outlive the copies */
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));

@@ -90,6 +90,11 @@ int tgx_gen_scene_device(tgx_context* ctx, const tgx_gen_params* p)
                  off[3ull * o + 1], off[3ull * o + 2], 0.0};
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     ctx->has_scene = false;
+    ctx->dynamic = false;
+    ctx->svo_prev_valid = false;
+    ctx->dyn_objects.clear();
+    ctx->dyn_free_objects.clear();
+    ctx->dyn_free_clusters.clear();
     if ((rc = tgx_buffer_reserve(ctx, &ctx->objects, sizeof(tgx_object) * sz.n_objects, "objects")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->cluster_pointers, 4ull * sz.n_cluster_pointers, "cluster pointers")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->cluster_idx_to_object, 4ull * sz.n_clusters, "cluster->object")) ||

@@ -117,6 +117,20 @@ struct tgx_context {
     std::vector<tgx_object> dyn_objects;
     std::vector<uint32_t>  dyn_free_objects, dyn_free_clusters;
     tgx_buffer             dyn_stage, dyn_tmp;
+    tgx_buffer             cl_dirty;          /* u8 per cluster: placed since the last SVO build */
+
+    /* What tgx_svo_update compares against: the last build's leaf keys,
+     * per-leaf candidate offsets and candidate cluster indices, and (during
+     * an update) the last build's leaf data. */
+    bool         svo_prev_valid = false;
+    uint32_t     svo_prev_leaves = 0;
+    uint32_t     svo_reused = 0;
+    tgx_buffer   svo_prev_keys, svo_prev_off, svo_prev_cand;
+    tgx_buffer   svo_old_bits, svo_old_mats, svo_old_n, svo_old_list_off, svo_old_list;
+    /* Temporary arrays of a build: one arena, plus spills freed at its end. */
+    tgx_buffer          svo_arena;
+    size_t              svo_arena_used = 0, svo_arena_want = 0, svo_arena_spill_bytes = 0;
+    std::vector<void*>  svo_arena_spill;
 
     /* Profiling. */
     bool                           profiling = false;

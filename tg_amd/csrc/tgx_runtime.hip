@@ -123,7 +123,7 @@ int tgx_destroy(tgx_context* ctx)
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,
-                          &ctx->frame_acc, &ctx->pick_buf, &ctx->dyn_stage, &ctx->dyn_tmp};
+                          &ctx->frame_acc, &ctx->pick_buf, &ctx->dyn_stage, &ctx->dyn_tmp, &ctx->cl_dirty};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     tgx_svo_release(ctx);
     tgx_nrc_release(ctx);
@@ -188,6 +188,7 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     int rc;
     ctx->dynamic = false;
+    ctx->svo_prev_valid = false;
     ctx->dyn_objects.clear();
     ctx->dyn_free_objects.clear();
     ctx->dyn_free_clusters.clear();

@@ -324,19 +324,44 @@ __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ 
                                                   const uint8_t* __restrict__ materials, float mnx, float mny,
                                                   float mnz, uint32_t quirk, uint32_t* __restrict__ leaf_bits,
                                                   uint16_t* __restrict__ leaf_mats, uint32_t* __restrict__ leaf_n,
-                                                  uint32_t* __restrict__ list_tmp, svo_gnode* __restrict__ nodes_rw)
+                                                  uint32_t* __restrict__ list_tmp, svo_gnode* __restrict__ nodes_rw,
+                                                  const uint32_t* __restrict__ reuse, const uint32_t* __restrict__ old_bits,
+                                                  const uint16_t* __restrict__ old_mats, const uint32_t* __restrict__ old_n,
+                                                  const uint32_t* __restrict__ old_list_off,
+                                                  const uint32_t* __restrict__ old_list)
 {
     __shared__ uint32_t lbits[TGX_SVO_BLOCK_WORDS];
     __shared__ int any_flag;
     __shared__ uint32_t n_contrib, last_cidx;
     const uint32_t leaf = blockIdx.x;
     const uint32_t tid = threadIdx.x;
+    if (reuse && reuse[leaf] != SVO_NONE) {
+        /* an update: same candidate clusters in the same order, none of
+         * them new, so the previous build's leaf is this leaf */
+        const uint32_t j = reuse[leaf];
+        const uint4* sb = (const uint4*)(old_bits + (size_t)j * TGX_SVO_BLOCK_WORDS);
+        uint4* db = (uint4*)(leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS);
+        for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS / 4u; w += blockDim.x) db[w] = sb[w];
+        const uint4* sm = (const uint4*)(old_mats + (size_t)j * TGX_SVO_BLOCK_VOXELS);
+        uint4* dm = (uint4*)(leaf_mats + (size_t)leaf * TGX_SVO_BLOCK_VOXELS);
+        for (uint32_t w = tid; w < TGX_SVO_BLOCK_VOXELS / 8u; w += blockDim.x) dm[w] = sm[w];
+        const uint32_t n = old_n[j], b = leaf_first_pair[leaf], o = old_list_off[j];
+        for (uint32_t i = tid; i < n; i += blockDim.x) list_tmp[b + i] = old_list[o + i];
+        if (tid == 0) {
+            leaf_n[leaf] = n;
+            nodes_rw[leaf_base + leaf].leaf_n = n;
+        }
+        return;
+    }
     const svo_gnode nd = nodes[leaf_base + leaf];
     const float pminx = mnx + (float)nd.cx * 32.0f, pminy = mny + (float)nd.cy * 32.0f, pminz = mnz + (float)nd.cz * 32.0f;
     const float pmaxx = pminx + 32.0f, pmaxy = pminy + 32.0f, pmaxz = pminz + 32.0f;
     for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS; w += blockDim.x) lbits[w] = 0u;
     if (tid == 0) { n_contrib = 0u; last_cidx = SVO_NONE; }
     uint16_t* mats = leaf_mats + (size_t)leaf * TGX_SVO_BLOCK_VOXELS;
+    /* materials of empty voxels are 0; the barrier below orders these
+     * stores before the sampled ones */
+    for (uint32_t w = tid; w < TGX_SVO_BLOCK_VOXELS / 8u; w += blockDim.x) ((uint4*)mats)[w] = make_uint4(0u, 0u, 0u, 0u);
     const uint32_t b = leaf_first_pair[leaf];
     const uint32_t e = leaf + 1 < n_leaves ? leaf_first_pair[leaf + 1] : n_pairs;
     __syncthreads();
@@ -409,6 +434,56 @@ __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ 
     }
 }
 
+/* Update: for each new leaf, the previous build's leaf with the same key
+ * whose candidate clusters (cluster indices in pointer order) are the same
+ * and none of them placed since that build; SVO_NONE otherwise. */
+__global__ __launch_bounds__(256) void svo_match_k(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ first,
+                                                   uint32_t n_pairs, uint32_t n_leaves,
+                                                   const uint32_t* __restrict__ pair_cptr,
+                                                   const svo_frame* __restrict__ frames, const uint8_t* __restrict__ dirty,
+                                                   const uint32_t* __restrict__ prev_keys, uint32_t n_prev,
+                                                   const uint32_t* __restrict__ prev_off,
+                                                   const uint32_t* __restrict__ prev_cand, uint32_t* __restrict__ reuse,
+                                                   uint32_t* __restrict__ n_reused)
+{
+    /* one wave per leaf: the search is wave-uniform, the candidates are
+     * compared 64 at a time */
+    const uint32_t l = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (l >= n_leaves) return;
+    const uint32_t key = keys[l];
+    uint32_t lo = 0, hi = n_prev;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prev_keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    uint32_t r = SVO_NONE;
+    if (lo < n_prev && prev_keys[lo] == key) {
+        const uint32_t b = first[l], e = l + 1 < n_leaves ? first[l + 1] : n_pairs;
+        const uint32_t pb = prev_off[lo], pe = prev_off[lo + 1];
+        if (e - b == pe - pb) {
+            uint32_t bad = 0u;
+            for (uint32_t k = lane; k < e - b; k += 64u) {
+                const uint32_t c = frames[pair_cptr[b + k]].cidx;
+                bad |= (c != prev_cand[pb + k] || dirty[c] != 0u) ? 1u : 0u;
+            }
+            if (__ballot(bad != 0u) == 0ull) r = lo;
+        }
+    }
+    if (lane == 0) {
+        reuse[l] = r;
+        if (r != SVO_NONE) atomicAdd(n_reused, 1u);
+    }
+}
+
+/* candidate cluster pointers -> cluster indices (kept for the next update) */
+__global__ void svo_cand_k(const uint32_t* __restrict__ pair_cptr, uint32_t n, const svo_frame* __restrict__ frames,
+                           uint32_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = frames[pair_cptr[i]].cidx;
+}
+
 __global__ void svo_list_compact_k(const uint32_t* __restrict__ leaf_first_pair, const uint32_t* __restrict__ leaf_n,
                                    const uint32_t* __restrict__ list_off, uint32_t n_leaves,
                                    const uint32_t* __restrict__ list_tmp, uint32_t* __restrict__ list)
@@ -457,23 +532,68 @@ void tgx_svo_release(tgx_context* ctx)
                           &ctx->gi_weight, &ctx->gi_contrib, &ctx->gi_list, &ctx->gi_rays2, &ctx->gi_sort, &ctx->svo_inner,
                           &ctx->svo_bricks};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
+    tgx_buffer* prev[] = {&ctx->svo_prev_keys, &ctx->svo_prev_off, &ctx->svo_prev_cand, &ctx->svo_old_bits,
+                          &ctx->svo_old_mats, &ctx->svo_old_n, &ctx->svo_old_list_off, &ctx->svo_old_list,
+                          &ctx->svo_arena};
+    for (tgx_buffer* b : prev) tgx_buffer_release(b);
     ctx->svo_built = false;
+    ctx->svo_prev_valid = false;
 }
 
 namespace {
-/* Temporary device arrays of one build. */
+/* Temporary device arrays of one build, carved from the context's build
+ * arena (kept between builds, so a rebuild of a similar scene allocates
+ * nothing). What does not fit is allocated on its own and freed when the
+ * build ends, and the arena grows to the whole need for the next build. */
 struct dbuf {
     void* p = nullptr;
-    ~dbuf() { if (p) (void)hipFree(p); }
     template <typename T> T* as() { return (T*)p; }
 };
 int dalloc(tgx_context* ctx, dbuf& b, size_t bytes, const char* what)
 {
-    if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
-    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 4);
-    if (e != hipSuccess) return tgx_fail(ctx, TGX_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) for %s", bytes, what);
+    bytes = (bytes + 255u) & ~(size_t)255u;
+    if (bytes == 0) bytes = 256;
+    if (ctx->svo_arena.ptr && ctx->svo_arena_used + bytes <= ctx->svo_arena.bytes) {
+        b.p = (char*)ctx->svo_arena.ptr + ctx->svo_arena_used;
+        ctx->svo_arena_used += bytes;
+        return TGX_OK;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return tgx_fail(ctx, TGX_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) for %s", bytes, what);
+    }
+    ctx->svo_arena_spill.push_back(b.p);
+    ctx->svo_arena_spill_bytes += bytes;
     return TGX_OK;
 }
+/* Opens the arena for one build; on every exit frees the spills and
+ * records the size the next build needs. */
+struct arena_scope {
+    tgx_context* ctx;
+    explicit arena_scope(tgx_context* c) : ctx(c)
+    {
+        ctx->svo_arena_used = 0;
+        ctx->svo_arena_spill_bytes = 0;
+        if (ctx->svo_arena_want > ctx->svo_arena.bytes) {
+            tgx_buffer_release(&ctx->svo_arena);
+            if (tgx_buffer_reserve(ctx, &ctx->svo_arena, ctx->svo_arena_want, "svo build arena")) {
+                tgx_buffer_release(&ctx->svo_arena);  /* the build spills instead */
+                ctx->last_error.clear();
+            }
+        }
+    }
+    ~arena_scope()
+    {
+        if (!ctx->svo_arena_spill.empty()) {
+            (void)hipStreamSynchronize(ctx->stream);
+            for (void* p : ctx->svo_arena_spill) (void)hipFree(p);
+            ctx->svo_arena_spill.clear();
+            const size_t need = ctx->svo_arena_used + ctx->svo_arena_spill_bytes;
+            ctx->svo_arena_want = need + need / 4u;
+        }
+    }
+};
 /* Pinned scratch of the SVO code: bytes [64, 128) of the context's pinned
  * page (bytes [0, 64) hold the visibility statistics readback). */
 inline uint32_t* svo_pinned(tgx_context* ctx) { return (uint32_t*)((char*)ctx->host_pinned + 64); }
@@ -488,18 +608,39 @@ int read_u32(tgx_context* ctx, const uint32_t* dev, uint32_t* out)
 int scan_total(tgx_context* ctx, const uint32_t* counts, const uint32_t* offs, uint32_t n, uint32_t* total)
 {
     if (n == 0) { *total = 0; return TGX_OK; }
-    uint32_t a, b;
-    int rc;
-    if ((rc = read_u32(ctx, offs + n - 1, &a))) return rc;
-    if ((rc = read_u32(ctx, counts + n - 1, &b))) return rc;
-    *total = a + b;
+    uint32_t* h = svo_pinned(ctx);
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(h, offs + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(h + 1, counts + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *total = h[0] + h[1];
     return TGX_OK;
 }
 }  // namespace
 
 extern "C" {
 
+static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t flags, bool update);
+
 int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t flags)
+{
+    return svo_build(ctx, mn, depth, flags, false);
+}
+
+int tgx_svo_update(tgx_context* ctx, uint32_t* reused_leaves, uint32_t* voxelized_leaves)
+{
+    if (!ctx) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built: call tgx_svo_build first");
+    const float mn[3] = {ctx->svo_info.min[0], ctx->svo_info.min[1], ctx->svo_info.min[2]};
+    const int rc = svo_build(ctx, mn, ctx->svo_info.depth, ctx->svo_info.flags, ctx->dynamic && ctx->svo_prev_valid);
+    if (!rc) {
+        if (reused_leaves) *reused_leaves = ctx->svo_reused;
+        if (voxelized_leaves) *voxelized_leaves = ctx->svo_info.n_leaves - ctx->svo_reused;
+    }
+    return rc;
+}
+
+static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t flags, bool update)
 {
     if (ctx && ctx->n_graphs > 0)
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "a frame graph uses the SVO: destroy it before rebuilding");
@@ -512,7 +653,20 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
         if (!(mn[a] == floorf(mn[a])) || fabsf(mn[a]) + (float)(1u << depth) >= 16777216.0f)
             return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "svo min must be integral and |min| + 2^depth < 2^24");
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    tgx_svo_release(ctx);
+    ctx->svo_reused = 0;
+    ctx->svo_built = false;
+    /* The buffers stay allocated and are reused when large enough. An
+     * update keeps the last build's leaves aside (double buffering). */
+    if (update) {
+        std::swap(ctx->svo_leaf_bits, ctx->svo_old_bits);
+        std::swap(ctx->svo_leaf_mats, ctx->svo_old_mats);
+        std::swap(ctx->svo_leaf_n, ctx->svo_old_n);
+        std::swap(ctx->svo_list_off, ctx->svo_old_list_off);
+        std::swap(ctx->svo_list, ctx->svo_old_list);
+    } else {
+        ctx->svo_prev_valid = false;
+    }
+    arena_scope arena(ctx);
     int rc;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
     if (n_cptr == 0) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "the scene has no clusters");
@@ -642,12 +796,19 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
     const uint32_t n_leaf_pairs = lvl_n[leaf_level];
     dbuf list_tmp;
     if ((rc = dalloc(ctx, list_tmp, 4ull * (n_leaf_pairs ? n_leaf_pairs : 1), "svo list"))) return rc;
-    {
-        const uint64_t words = (uint64_t)TGX_SVO_BLOCK_VOXELS * nl / 2u;
-        hipLaunchKernelGGL(svo_fill_k, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (uint32_t*)ctx->svo_leaf_mats.ptr, words, 0u);
+    dbuf reuse;
+    if (update && n_leaves) {
+        if ((rc = dalloc(ctx, reuse, 4ull * n_leaves + 4, "svo reuse"))) return rc;
+        uint32_t* n_reused = reuse.as<uint32_t>() + n_leaves;
+        hipLaunchKernelGGL(svo_fill_k, dim3(1), dim3(64), 0, ctx->stream, n_reused, 1ull, 0u);
+        hipLaunchKernelGGL(svo_match_k, dim3((n_leaves + 3u) / 4u), dim3(256), 0, ctx->stream,
+                           lvl_node_keys[leaf_level].as<uint32_t>(), lvl_first[leaf_level].as<uint32_t>(), n_leaf_pairs,
+                           n_leaves, lvl_scptr[leaf_level].as<uint32_t>(), frames, (const uint8_t*)ctx->cl_dirty.ptr, (const uint32_t*)ctx->svo_prev_keys.ptr,
+                           ctx->svo_prev_leaves, (const uint32_t*)ctx->svo_prev_off.ptr,
+                           (const uint32_t*)ctx->svo_prev_cand.ptr, reuse.as<uint32_t>(), n_reused);
+        TGX_LAUNCH_CHECK(ctx, "svo_match_k");
+        if ((rc = read_u32(ctx, n_reused, &ctx->svo_reused))) return rc;
     }
-    TGX_LAUNCH_CHECK(ctx, "svo_fill_k");
     if (n_leaves) {
         tgx_launch_scope s(ctx, "svo_leaf_k");
         hipLaunchKernelGGL(svo_leaf_k, dim3(n_leaves), dim3(256), 0, ctx->stream, nodes, lvl_base[leaf_level],
@@ -655,7 +816,10 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
                            n_leaves, frames, (const uint32_t*)ctx->bits.ptr, (const uint8_t*)ctx->materials.ptr,
                            mn[0], mn[1], mn[2], (flags & TGX_SVO_FLAG_REFERENCE_CORNER_QUIRK) ? 1u : 0u,
                            (uint32_t*)ctx->svo_leaf_bits.ptr, (uint16_t*)ctx->svo_leaf_mats.ptr,
-                           (uint32_t*)ctx->svo_leaf_n.ptr, list_tmp.as<uint32_t>(), nodes);
+                           (uint32_t*)ctx->svo_leaf_n.ptr, list_tmp.as<uint32_t>(), nodes,
+                           update ? reuse.as<uint32_t>() : nullptr, (const uint32_t*)ctx->svo_old_bits.ptr,
+                           (const uint16_t*)ctx->svo_old_mats.ptr, (const uint32_t*)ctx->svo_old_n.ptr,
+                           (const uint32_t*)ctx->svo_old_list_off.ptr, (const uint32_t*)ctx->svo_old_list.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "svo_leaf_k");
     if ((rc = tgx_prim_exclusive_scan_u32(ctx, (const uint32_t*)ctx->svo_leaf_n.ptr, (uint32_t*)ctx->svo_list_off.ptr, n_leaves))) return rc;
@@ -684,7 +848,33 @@ int tgx_svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32_t 
         hipLaunchKernelGGL(svo_bricks_k, dim3(n_leaves), dim3(64), 0, ctx->stream, (const uint32_t*)ctx->svo_leaf_bits.ptr,
                            (unsigned long long*)ctx->svo_bricks.ptr);
     TGX_LAUNCH_CHECK(ctx, "svo_bricks_k");
+    if (ctx->dynamic) {
+        /* what the next tgx_svo_update compares against */
+        if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_prev_keys, 4ull * nl, "svo previous keys")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->svo_prev_off, 4ull * (nl + 1), "svo previous offsets")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->svo_prev_cand, 4ull * (n_leaf_pairs ? n_leaf_pairs : 1), "svo previous candidates")))
+            return rc;
+        if (n_leaves) {
+            TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->svo_prev_keys.ptr, lvl_node_keys[leaf_level].p, 4ull * n_leaves,
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+            TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->svo_prev_off.ptr, lvl_first[leaf_level].p, 4ull * n_leaves,
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        hipLaunchKernelGGL(svo_fill_k, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->svo_prev_off.ptr + n_leaves,
+                           1ull, n_leaf_pairs);
+        if (n_leaf_pairs)
+            hipLaunchKernelGGL(svo_cand_k, dim3(blocks_for(n_leaf_pairs)), dim3(256), 0, ctx->stream,
+                               lvl_scptr[leaf_level].as<uint32_t>(), n_leaf_pairs, frames,
+                               (uint32_t*)ctx->svo_prev_cand.ptr);
+        TGX_LAUNCH_CHECK(ctx, "svo_cand_k");
+        const uint64_t dirty_words = (ctx->n_clusters + 3ull) / 4ull;
+        hipLaunchKernelGGL(svo_fill_k, dim3((uint32_t)((dirty_words + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (uint32_t*)ctx->cl_dirty.ptr, dirty_words, 0u);
+        TGX_LAUNCH_CHECK(ctx, "svo_fill_k");
+        ctx->svo_prev_leaves = n_leaves;
+    }
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->svo_prev_valid = ctx->dynamic;
     ctx->svo_n_inner = n_inner;
 
     ctx->svo_info.min[0] = mn[0]; ctx->svo_info.min[1] = mn[1]; ctx->svo_info.min[2] = mn[2];
