@@ -439,15 +439,36 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
     return done;
 }
 
+/* Bounce queue traced with its shadow rays in the same launch (GI
+ * estimator, one or two lists): when a lane's bounce ray hits a face that
+ * faces the sun, the lane writes the sample's albedo-weighted contribution
+ * and continues with the shadow ray itself; an occluded shadow ray clears
+ * the contribution again. Same arithmetic as gi_shade_k + gi_lit_k.
+ * contrib == nullptr: plain queue tracing. Ray ids at or above split belong
+ * to the second list; per list, bounce hits go to stats[2], shadow rays to
+ * stats[3] and shadow hits to stats[7]. */
+struct trace_fuse {
+    const uint16_t*      leaf_mats;
+    const tgx_lut_entry* luts;
+    float*               contrib;
+    uint32_t*            stats0;
+    uint32_t*            stats1;
+    uint32_t             split;
+    float                sun0, sun1, sun2;
+    float                sun_radiance;
+};
+
+#define TRACE_SHADOW 0x80000000u   /* slot of a lane tracing a shadow ray: id | TRACE_SHADOW */
+
 /* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd; the
- * result of ray i goes to out[i]. `cursor` must be 0 at launch; hits (flag
- * bit 0) are added to *hit_counter when it is not null. */
+ * result of ray i goes to out[i] (plain tracing). `cursor` must be 0 at
+ * launch; hits (flag bit 0) are added to *hit_counter when it is not null. */
 template <bool kLds>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
                                                           uint32_t* __restrict__ hit_counter, uint32_t refill,
-                                                          uint32_t vote)
+                                                          uint32_t vote, const trace_fuse F)
 {
     extern __shared__ uint2 s_inner[];
     if constexpr (kLds) {
@@ -460,7 +481,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
     const uint32_t lane = threadIdx.x & 63u;
     const float half = (float)(1u << P.depth) * 0.5f;
     trace_lane L = {};
-    uint32_t slot = 0, hits = 0;
+    uint32_t slot = 0;
+    /* wave totals: bounce hits, shadow rays and shadow hits per list */
+    uint32_t n_hit0 = 0, n_hit1 = 0, n_sh0 = 0, n_sh1 = 0, n_shhit0 = 0, n_shhit1 = 0;
     bool active = false, exhausted = false;
     while (true) {
         /* wave-uniform refill once `refill` lanes are idle */
@@ -476,7 +499,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                 if (r < n_rays) {
                     slot = r;
                     active = lane_init(P, half, rays[r], L);
-                    if (!active) out[r] = res_miss(0u);
+                    if (!active && !F.contrib) out[r] = res_miss(0u);
                 }
             }
         }
@@ -495,15 +518,70 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             allow_node = !allow_dda;
         }
         svo_res res;
+        /* per-lane events of this iteration, counted below at a uniform point:
+         * 1 bounce hit, 2 shadow ray started, 4 shadow hit; << 3 for the
+         * second list */
+        uint32_t ev = 0u;
         if (lane_step<kLds>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda) && active) {
-            out[slot] = res;
-            hits += res.flags & 1u;
             active = false;
+            if (!F.contrib) {
+                out[slot] = res;
+                ev = res.flags & 1u;
+            } else if (slot & TRACE_SHADOW) {
+                const uint32_t id = slot & ~TRACE_SHADOW;
+                if (res.flags & 1u) {
+                    /* occluded: the sample contributes nothing */
+                    float* c = F.contrib + 3ull * id;
+                    c[0] = 0.0f; c[1] = 0.0f; c[2] = 0.0f;
+                    ev = id >= F.split ? 32u : 4u;
+                }
+            } else {
+                const svo_ray b = rays[slot];
+                const uint32_t sh = b.id >= F.split ? 3u : 0u;
+                ev = (res.flags & 1u) << sh;
+                /* cosine of the hit face with the sun (gi_shade_k) */
+                const uint32_t ax = (res.flags >> 2) & 3u;
+                const float sign = (res.flags & 16u) ? 1.0f : -1.0f;
+                const float sc = ax == 0u ? F.sun0 : (ax == 1u ? F.sun1 : F.sun2);
+                const float cosv = ((res.flags & 1u) && !(res.distance < 0.0f)) ? sign * sc : 0.0f;
+                if (cosv > 0.0f) {
+                    float ox = b.ox + b.dx * res.distance, oy = b.oy + b.dy * res.distance,
+                          oz = b.oz + b.dz * res.distance;
+                    const float off = sign * GI_SHADOW_OFFSET;
+                    if (ax == 0u) ox = ox + off; else if (ax == 1u) oy = oy + off; else oz = oz + off;
+                    const uint16_t m = F.leaf_mats[(size_t)(res.node - P.n_inner) * TGX_SVO_BLOCK_VOXELS + res.voxel];
+                    const tgx_lut_entry e = F.luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
+                    const float wgt = cosv * F.sun_radiance * GI_INV_PI;
+                    float* c = F.contrib + 3ull * b.id;
+                    c[0] = e.albedo[0] * wgt; c[1] = e.albedo[1] * wgt; c[2] = e.albedo[2] * wgt;
+                    const svo_ray sr = {ox, oy, oz, F.sun0, F.sun1, F.sun2, b.id, 0u};
+                    ev |= 2u << sh;
+                    slot = b.id | TRACE_SHADOW;
+                    /* a shadow ray that misses the root box is unoccluded */
+                    active = lane_init(P, half, sr, L);
+                }
+            }
+        }
+        if (__any(ev != 0u)) {
+            n_hit0 += (uint32_t)__popcll(__ballot(ev & 1u));
+            n_sh0 += (uint32_t)__popcll(__ballot(ev & 2u));
+            n_shhit0 += (uint32_t)__popcll(__ballot(ev & 4u));
+            n_hit1 += (uint32_t)__popcll(__ballot(ev & 8u));
+            n_sh1 += (uint32_t)__popcll(__ballot(ev & 16u));
+            n_shhit1 += (uint32_t)__popcll(__ballot(ev & 32u));
         }
     }
-    if (hit_counter) {
-        for (int off = 32; off > 0; off >>= 1) hits += __shfl_down(hits, off);
-        if (lane == 0 && hits) atomicAdd(hit_counter, hits);
+    if (lane == 0) {
+        if (!F.contrib) {
+            if (hit_counter && n_hit0) atomicAdd(hit_counter, n_hit0);
+        } else {
+            if (n_hit0) atomicAdd(&F.stats0[2], n_hit0);
+            if (n_sh0) atomicAdd(&F.stats0[3], n_sh0);
+            if (n_shhit0) atomicAdd(&F.stats0[7], n_shhit0);
+            if (n_hit1) atomicAdd(&F.stats1[2], n_hit1);
+            if (n_sh1) atomicAdd(&F.stats1[3], n_sh1);
+            if (n_shhit1) atomicAdd(&F.stats1[7], n_shhit1);
+        }
     }
 }
 
@@ -659,19 +737,22 @@ __global__ void gi_list_k(const uint32_t* __restrict__ flag, const uint32_t* __r
 }
 
 /* spp cosine bounce rays per listed pixel, from p + sqrt(3) n; ray id =
- * list entry * spp + sample. */
+ * id_base + list entry * spp + sample (id_base separates the lists that
+ * share a queue). stats (may be null) counts the list's bounce rays. */
 __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restrict__ surf,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count_ptr, tgx_gi_params prm,
                                                      const uint32_t* __restrict__ frame_salt,
                                                      svo_ray* __restrict__ rays, float* __restrict__ contrib,
-                                                     uint32_t* __restrict__ counters)
+                                                     uint32_t* __restrict__ counters, uint32_t id_base,
+                                                     uint32_t* __restrict__ stats)
 {
     /* a device counter can advance the frame of replayed work */
     if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + 1u);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < *count_ptr;
     const uint32_t first = block_append<GI_BLOCK>(&counters[1], ok ? prm.spp : 0u);
+    if (stats) (void)block_append<GI_BLOCK>(&stats[1], ok ? prm.spp : 0u);
     if (!ok) return;
     const uint32_t pix = list[i];
     const gi_surf_rec sr = surf[pix];
@@ -681,9 +762,9 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restri
         uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
         if (st == 0u) st = 0x6D2B79F5u;
         const f3 d = gi_cosine_direction(n, st);
-        const svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, i * prm.spp + k, 0u};
+        const svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, id_base + i * prm.spp + k, 0u};
         rays[first + k] = r;
-        const size_t c = 3ull * ((size_t)i * prm.spp + k);
+        const size_t c = 3ull * ((size_t)id_base + (size_t)i * prm.spp + k);
         contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
     }
 }
@@ -701,22 +782,41 @@ __device__ static inline float shade_cos(const svo_res& r, const tgx_gi_params& 
 }
 
 /* Bounce hits that face the sun: albedo-weighted contribution and a shadow
- * ray (started 1/64 voxel off the hit face). SHADE_ITEMS rays per thread. */
+ * ray (started 1/64 voxel off the hit face). SHADE_ITEMS rays per thread.
+ * With two lists in the queue (stats0 and stats1 not null), each list's
+ * bounce hits and shadow rays are counted apart; ids below split belong to
+ * the first. */
 __global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, const uint16_t* __restrict__ leaf_mats,
                                                        const tgx_lut_entry* __restrict__ luts, tgx_gi_params prm,
                                                        const svo_ray* __restrict__ rays,
                                                        const uint32_t* __restrict__ n_rays_ptr,
                                                        const svo_res* __restrict__ res, svo_ray* __restrict__ shadow,
-                                                       float* __restrict__ weight, uint32_t* __restrict__ counters)
+                                                       float* __restrict__ weight, uint32_t* __restrict__ counters,
+                                                       uint32_t split, uint32_t* __restrict__ stats0,
+                                                       uint32_t* __restrict__ stats1)
 {
     const uint32_t n_rays = *n_rays_ptr;
     const uint32_t first = blockIdx.x * GI_BLOCK * SHADE_ITEMS + threadIdx.x;
-    uint32_t emit = 0, count = 0;
+    uint32_t emit = 0, count = 0, hits1 = 0, hits0 = 0, emit1 = 0;
     for (uint32_t k = 0; k < SHADE_ITEMS; k++) {
         const uint32_t i = first + k * GI_BLOCK;
-        if (i < n_rays && shade_cos(res[i], prm) > 0.0f) { emit |= 1u << k; count++; }
+        if (i >= n_rays) continue;
+        const svo_res r = res[i];
+        const bool second = stats0 && rays[i].id >= split;
+        if (r.flags & 1u) { if (second) hits1++; else hits0++; }
+        if (shade_cos(r, prm) > 0.0f) {
+            emit |= 1u << k;
+            count++;
+            if (second) emit1++;
+        }
     }
     uint32_t slot = block_append<GI_BLOCK>(&counters[3], count);
+    if (stats0) {
+        (void)block_append<GI_BLOCK>(&stats0[2], hits0);
+        (void)block_append<GI_BLOCK>(&stats0[3], count - emit1);
+        (void)block_append<GI_BLOCK>(&stats1[2], hits1);
+        (void)block_append<GI_BLOCK>(&stats1[3], emit1);
+    }
     for (uint32_t k = 0; k < SHADE_ITEMS; k++) {
         if (!(emit & (1u << k))) continue;
         const uint32_t i = first + k * GI_BLOCK;
@@ -904,8 +1004,9 @@ static trace_params params_of(tgx_context* ctx)
  * 1024-lane workgroups (tuning: per CU), the traversal layout in LDS when it
  * fits. */
 static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t n_max, float maxd,
-                        uint32_t* cursor, svo_res* out, uint32_t* hit_counter)
+                        uint32_t* cursor, svo_res* out, uint32_t* hit_counter, const trace_fuse* fuse = nullptr)
 {
+    const trace_fuse F = fuse ? *fuse : trace_fuse{};
     if (n_max == 0) return TGX_OK;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
@@ -919,10 +1020,10 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
         tgx_launch_scope s(ctx, "gi_trace_k");
         if ((uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX)
             hipLaunchKernelGGL(gi_trace_k<true>, grid, dim3(TRACE_BLOCK), 8u * (P.n_inner + P.n_leaves), ctx->stream, P,
-                               rays, n_dev, maxd, cursor, out, hit_counter, refill, vote);
+                               rays, n_dev, maxd, cursor, out, hit_counter, refill, vote, F);
         else
             hipLaunchKernelGGL(gi_trace_k<false>, grid, dim3(TRACE_BLOCK), 0, ctx->stream, P, rays, n_dev, maxd, cursor,
-                               out, hit_counter, refill, vote);
+                               out, hit_counter, refill, vote, F);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
     return TGX_OK;
@@ -996,11 +1097,17 @@ int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_
     return TGX_OK;
 }
 
-int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t* list, const uint32_t* count_dev,
-                      uint32_t max_count, uint32_t* cnt, float* e_entry, float* e_var, float* radiance_dev,
-                      float* irradiance_dev, const uint32_t* frame_salt)
+int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_sets)
 {
-    const uint64_t n_samples = (uint64_t)max_count * prm->spp;
+    if (n_sets == 0 || n_sets > 2) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "1 or 2 trace sets");
+    uint64_t n_samples = 0;
+    uint32_t id_base[2] = {0u, 0u};
+    for (uint32_t k = 0; k < n_sets; k++) {
+        id_base[k] = (uint32_t)n_samples;
+        n_samples += (uint64_t)sets[k].max_count * sets[k].prm.spp;
+    }
+    if (n_samples >= (1ull << 31)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%llu samples exceed 2^31",
+                                                   (unsigned long long)n_samples);
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_rays, sizeof(svo_ray) * n_samples, "gi bounce rays")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->gi_res, sizeof(svo_res) * n_samples, "gi bounce results")) ||
@@ -1016,24 +1123,50 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
     float* weight = (float*)ctx->gi_weight.ptr;
     float* contrib = (float*)ctx->gi_contrib.ptr;
     const gi_surf_rec* surf = (const gi_surf_rec*)ctx->gi_surf.ptr;
-    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
-    {
+    /* one list: its own counters are the queues; two: the shared block */
+    const bool two = n_sets == 2;
+    uint32_t* cnt = two ? (uint32_t*)ctx->gi_counts.ptr + GI_CNT_QUEUE : sets[0].counters;
+    for (uint32_t k = 0; k < n_sets; k++)
+        hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, sets[k].counters, (uint64_t)GI_N_COUNTERS, 0u);
+    if (two) hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    for (uint32_t k = 0; k < n_sets; k++) {
+        const gi_trace_set& t = sets[k];
+        if (t.max_count == 0) continue;
         tgx_launch_scope s(ctx, "gi_gen_k");
-        hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, list, count_dev,
-                           *prm, frame_salt, rays, contrib, cnt);
+        hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(t.max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, t.list,
+                           t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k],
+                           two ? t.counters : nullptr);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
+    /* shade, trace and lit use what the lists share: sun, radiance, range */
+    const tgx_gi_params& prm = sets[0].prm;
     if (ctx->tuning.trace_sort & 1u) {
         if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_rays2, sizeof(svo_ray) * n_samples, "gi sorted rays"))) return rc;
         if ((rc = sort_rays(ctx, rays, cnt + 1, (uint32_t)n_samples, (svo_ray*)ctx->gi_rays2.ptr))) return rc;
         rays = (svo_ray*)ctx->gi_rays2.ptr;
     }
-    if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm->max_distance, cnt + 4, res, cnt + 2))) return rc;
+    if (!(ctx->tuning.trace_sort & 2u)) {
+        /* bounce and shadow rays in one launch */
+        trace_fuse F = {};
+        F.leaf_mats = (const uint16_t*)ctx->svo_leaf_mats.ptr;
+        F.luts = (const tgx_lut_entry*)ctx->luts.ptr;
+        F.contrib = contrib;
+        F.stats0 = two ? sets[0].counters : cnt;
+        F.stats1 = two ? sets[1].counters : cnt;
+        F.split = two ? id_base[1] : 0xFFFFFFFFu;
+        F.sun0 = prm.sun_dir[0]; F.sun1 = prm.sun_dir[1]; F.sun2 = prm.sun_dir[2];
+        F.sun_radiance = prm.sun_radiance;
+        if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, nullptr, &F)))
+            return rc;
+    } else {
+    if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, two ? nullptr : cnt + 2)))
+        return rc;
     {
         tgx_launch_scope s(ctx, "gi_shade_k");
         hipLaunchKernelGGL(gi_shade_k, dim3((uint32_t)((n_samples + GI_BLOCK * SHADE_ITEMS - 1) / (GI_BLOCK * SHADE_ITEMS))),
                            dim3(GI_BLOCK), 0, ctx->stream, ctx->svo_n_inner, (const uint16_t*)ctx->svo_leaf_mats.ptr,
-                           (const tgx_lut_entry*)ctx->luts.ptr, *prm, rays, cnt + 1, res, shadow, weight, cnt);
+                           (const tgx_lut_entry*)ctx->luts.ptr, prm, rays, cnt + 1, res, shadow, weight, cnt,
+                           id_base[1], two ? sets[0].counters : nullptr, two ? sets[1].counters : nullptr);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_shade_k");
     if (ctx->tuning.trace_sort & 2u) {
@@ -1042,20 +1175,33 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
         if ((rc = sort_rays(ctx, shadow, cnt + 3, (uint32_t)n_samples, dst))) return rc;
         shadow = dst;
     }
-    if ((rc = launch_trace(ctx, shadow, cnt + 3, (uint32_t)n_samples, prm->max_distance, cnt + 5, sres, cnt + 7))) return rc;
+    if ((rc = launch_trace(ctx, shadow, cnt + 3, (uint32_t)n_samples, prm.max_distance, cnt + 5, sres, cnt + 7))) return rc;
     {
         tgx_launch_scope s(ctx, "gi_lit_k");
         hipLaunchKernelGGL(gi_lit_k, dim3(blocks_for(n_samples)), dim3(256), 0, ctx->stream, shadow, cnt + 3, sres,
                            weight, contrib);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_lit_k");
-    {
+    }
+    for (uint32_t k = 0; k < n_sets; k++) {
+        const gi_trace_set& t = sets[k];
+        if (t.max_count == 0) continue;
         tgx_launch_scope s(ctx, "gi_resolve_k");
-        hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(max_count)), dim3(256), 0, ctx->stream, surf, list, count_dev,
-                           contrib, prm->spp, e_entry, e_var, radiance_dev, irradiance_dev);
+        hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(t.max_count)), dim3(256), 0, ctx->stream, surf, t.list,
+                           t.count_dev, contrib + 3ull * id_base[k], t.prm.spp, t.e_entry, t.e_var, t.radiance_dev,
+                           t.irradiance_dev);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_resolve_k");
     return TGX_OK;
+}
+
+int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t* list, const uint32_t* count_dev,
+                      uint32_t max_count, uint32_t* cnt, float* e_entry, float* e_var, float* radiance_dev,
+                      float* irradiance_dev, const uint32_t* frame_salt)
+{
+    const gi_trace_set t = {*prm, list, count_dev, max_count, cnt, e_entry, e_var, radiance_dev, irradiance_dev,
+                            frame_salt};
+    return tgx_gi_trace_sets(ctx, &t, 1);
 }
 
 extern "C" {

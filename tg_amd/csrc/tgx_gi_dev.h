@@ -27,7 +27,8 @@ static_assert(sizeof(gi_surf_rec) == 40, "gi_surf_rec layout");
 #define GI_CNT_TRAIN   8u   /* the cache's training targets */
 #define GI_CNT_SVO     16u  /* tgx_svo_trace */
 #define GI_CNT_VISIBLE 24u  /* visible pixels of the last surface pass */
-#define GI_CNT_TOTAL   32u
+#define GI_CNT_QUEUE   32u  /* shared ray queues of tgx_gi_trace_sets with two sets */
+#define GI_CNT_TOTAL   40u
 
 /* Reserves `count` consecutive queue slots for the calling lane (counts may
  * differ per lane) with one atomic per workgroup; returns the lane's first
@@ -85,5 +86,29 @@ int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_
 int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t* list, const uint32_t* count_dev,
                       uint32_t max_count, uint32_t* counters, float* e_entry, float* e_var, float* radiance_dev,
                       float* irradiance_dev, const uint32_t* frame_salt);
+
+/* One list of tgx_gi_trace_sets: the arguments of tgx_gi_trace_list. */
+struct gi_trace_set {
+    tgx_gi_params   prm;
+    const uint32_t* list;
+    const uint32_t* count_dev;
+    uint32_t        max_count;
+    uint32_t*       counters;
+    float*          e_entry;
+    float*          e_var;
+    float*          radiance_dev;
+    float*          irradiance_dev;
+    const uint32_t* frame_salt;
+};
+
+/* Traces one or two lists through shared ray queues: one bounce launch and
+ * one shadow launch for both, so a small list does not pay a launch tail of
+ * its own. Each list's results equal those of tgx_gi_trace_list on it
+ * alone (every sample is drawn from its own list entry, sample and frame,
+ * and contributions are summed per entry in sample order). The lists share
+ * prm except spp and frame. Per-list counters hold bounce rays (1), bounce
+ * hits (2) and shadow rays (3); with two sets the queues use the block at
+ * GI_CNT_QUEUE. */
+int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_sets);
 
 #endif

@@ -862,29 +862,8 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
     hipLaunchKernelGGL(nrc_fill_k, dim3(1), dim3(64), 0, ctx->stream, qc, 64ull, 0u);
     TGX_LAUNCH_CHECK(ctx, "nrc_fill_k");
 
-    /* training: targets traced by the SVO estimator, then Adam steps */
-    ctx->nrc_stats.train_samples = 0;
-    if (n_train) {
-        hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
-        TGX_LAUNCH_CHECK(ctx, "nrc_decay_k");
-        hipLaunchKernelGGL(nrc_pick_k, dim3(nrc_blocks(n_train, 256)), dim3(256), 0, ctx->stream, vis_list, n_vis,
-                           n_train, (const uint32_t*)ctx->nrc_dev.ptr, train_list, qc);
-        TGX_LAUNCH_CHECK(ctx, "nrc_pick_k");
-        tgx_gi_params tp = *gi;
-        tp.spp = c.train_spp;
-        /* fresh bounce samples every frame (device frame counter), decorrelated
-         * from the displayed frame's */
-        tp.frame = gi->frame ^ 0x5BD1E995u;
-        if ((rc = tgx_gi_trace_list(ctx, &tp, train_list, qc, n_train, gcnt + GI_CNT_TRAIN, e_train, v_train, nullptr,
-                                    nullptr, (const uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME)))
-            return rc;
-        for (uint32_t first = 0; first < n_train; first += c.train_batch)
-            if ((rc = train_step(ctx, surf, train_list, qc, first, c.train_batch, c.train_batch, e_train, v_train,
-                                 cells, (float*)(qc + 8))))
-                return rc;
-        ctx->nrc_stats.train_samples = n_train;
-    }
-    /* inference for every visible pixel; unreliable cells fall back */
+    /* Inference with the cache as the last frame left it; pixels of cells
+     * it does not trust are listed for the SVO estimator. */
     const bool all_fb = (c.flags & TGX_NRC_FLAG_ALL_FALLBACK) != 0;
     const bool use_cells = !(c.flags & TGX_NRC_FLAG_NO_FALLBACK) && !all_fb;
     if (!all_fb) {
@@ -892,7 +871,8 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
                                use_cells ? cells : nullptr, fb_flag)))
             return rc;
     }
-    if (use_cells || all_fb) {
+    const bool fallback = use_cells || all_fb;
+    if (fallback) {
         if (all_fb) {
             hipLaunchKernelGGL(nrc_fill_k, dim3(nrc_blocks(n_pix, 256)), dim3(256), 0, ctx->stream, fb_flag,
                                (uint64_t)n_pix, 1u);
@@ -904,13 +884,40 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
         hipLaunchKernelGGL(nrc_fallback_list_k, dim3(nrc_blocks(n_pix, 256)), dim3(256), 0, ctx->stream, fb_flag, fb_off,
                            vis_list, n_vis, fb_list, qc + 1);
         TGX_LAUNCH_CHECK(ctx, "nrc_fallback_list_k");
-        if ((rc = tgx_gi_trace_list(ctx, gi, fb_list, qc + 1, n_pix, gcnt + GI_CNT_PLAIN, nullptr, nullptr,
-                                    radiance_dev, irradiance_dev, nullptr)))
-            return rc;
+    }
+    /* Training targets and fallback pixels share one bounce and one shadow
+     * launch; then the cache trains on this frame's targets. */
+    ctx->nrc_stats.train_samples = 0;
+    gi_trace_set sets[2];
+    uint32_t n_sets = 0;
+    if (n_train) {
+        hipLaunchKernelGGL(nrc_pick_k, dim3(nrc_blocks(n_train, 256)), dim3(256), 0, ctx->stream, vis_list, n_vis,
+                           n_train, (const uint32_t*)ctx->nrc_dev.ptr, train_list, qc);
+        TGX_LAUNCH_CHECK(ctx, "nrc_pick_k");
+        tgx_gi_params tp = *gi;
+        tp.spp = c.train_spp;
+        /* fresh bounce samples every frame (device frame counter), decorrelated
+         * from the displayed frame's */
+        tp.frame = gi->frame ^ 0x5BD1E995u;
+        sets[n_sets++] = {tp, train_list, qc, n_train, gcnt + GI_CNT_TRAIN, e_train, v_train, nullptr, nullptr,
+                          (const uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME};
+    }
+    if (fallback)
+        sets[n_sets++] = {*gi, fb_list, qc + 1, n_pix, gcnt + GI_CNT_PLAIN, nullptr, nullptr, radiance_dev,
+                          irradiance_dev, nullptr};
+    if (n_sets && (rc = tgx_gi_trace_sets(ctx, sets, n_sets))) return rc;
+    if (n_train) {
+        hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
+        TGX_LAUNCH_CHECK(ctx, "nrc_decay_k");
+        for (uint32_t first = 0; first < n_train; first += c.train_batch)
+            if ((rc = train_step(ctx, surf, train_list, qc, first, c.train_batch, c.train_batch, e_train, v_train,
+                                 cells, (float*)(qc + 8))))
+                return rc;
+        ctx->nrc_stats.train_samples = n_train;
     }
     hipLaunchKernelGGL(nrc_tick_k, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME);
     TGX_LAUNCH_CHECK(ctx, "nrc_tick_k");
-    ctx->nrc_fb_traced = use_cells || all_fb;
+    ctx->nrc_fb_traced = fallback;
     return TGX_OK;
 }
 
