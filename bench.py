@@ -59,11 +59,13 @@ def vis_alg_bytes(stats, n_objects: int, n_pixels: int) -> int:
 
 
 def trace_alg_bytes(gst, info) -> int:
-    """Both SVO trace launches of a GI frame: per ray its 32-B queue record
-    read and 24-B result written; per launch the SVO's nodes (32 B each) and
-    leaf bit blocks (4 KiB each) read once."""
-    rays = int(gst["bounce_rays"]) + int(gst["shadow_rays"])
-    return 56 * rays + 2 * (32 * int(info["n_nodes"]) + 4096 * int(info["n_leaves"]))
+    """The frame's SVO trace launch (bounce rays with their shadow rays
+    traced in the same launch): per bounce ray its 32-B queue record read;
+    per shadow ray the 2-B material of the bounce hit read and the 12-B
+    contribution written; the SVO's nodes (32 B each) and leaf bit blocks
+    (4 KiB each) read once."""
+    return (32 * int(gst["bounce_rays"]) + 14 * int(gst["shadow_rays"])
+            + 32 * int(info["n_nodes"]) + 4096 * int(info["n_leaves"]))
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -245,10 +247,12 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
         launches, total_ms = prof[dominant]
         avg_ms = total_ms / launches
         if dominant == "gi_trace_k":
-            # per ray: its 32-B queue record read and 24-B result written (the
-            # SVO reads, at most the 8 B/inner node + 4104 B/leaf structure
-            # per launch, are left out: a lower bound)
-            alg = 56 * traced_gi / launches
+            # per bounce ray its 32-B queue record read; per shadow ray (and
+            # per training ray, bounce or shadow) at least a 2-B material read
+            # and a 12-B contribution write; the SVO reads (at most 8 B per
+            # inner node + 4104 B per leaf per launch) are left out: a lower
+            # bound
+            alg = (32 * int(st["bounce_rays"]) + 14 * (int(st["shadow_rays"]) + int(st["train_rays"]))) / launches
         else:
             alg = None
         achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
@@ -629,8 +633,8 @@ def main() -> int:
         gst = ctx.gi_stats()
         rays_per_frame = primary + int(gst["bounce_rays"]) + int(gst["shadow_rays"])
         dominant = max(("gi_trace_k", "vis_traverse"), key=lambda k: kernels_ms.get(k, 0.0))
-        # per-launch bytes: the trace kernel runs twice per frame (bounce, shadow)
-        alg = (trace_alg_bytes(gst, info) // 2 if dominant == "gi_trace_k"
+        # per-launch bytes: one trace launch per frame (bounce + shadow rays)
+        alg = (trace_alg_bytes(gst, info) * args.steps // prof["gi_trace_k"][0] if dominant == "gi_trace_k"
                else vis_alg_bytes(vstats, scene.objects.size, primary))
     else:
         gst = None

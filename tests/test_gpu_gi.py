@@ -94,3 +94,38 @@ def test_gi_errors(ctx):
         assert e.value.code == 1
     finally:
         fresh.close()
+
+
+@pytest.mark.parametrize("sort", [2, 3])
+def test_queued_shadow_rays_equal_fused_launch(ctx, noise_setup, sort):
+    """The default frame traces each sun-facing bounce hit's shadow ray in
+    the bounce launch; with the shadow-sort tuning bit the shadow rays go
+    through a queue and a second launch. Both give the same bits and counts,
+    for plain GI and for the cache frame (two lists in one queue)."""
+    p, sc, mn, svo = noise_setup
+    ctx.upload(sc)
+    ctx.svo_build(mn, 11)
+    w, h = 480, 270
+    cam = T.generate_camera(p, 0, w, h)
+    prm = T.gi_params(T.seeded_sun(1), spp=4, frame=5)
+    cfg = T.nrc_config(origin=(0, 0, 0), scale=1 / 512, train_samples=1 << 14, train_batch=1 << 12,
+                       fallback_threshold=1.0, fallback_min_samples=1)
+    out = []
+    try:
+        for s in (0, sort):
+            ctx.set_tuning(trace_sort=s, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0)
+            say(f"plain GI and 3 cache frames, trace_sort {s}")
+            _, rad, irr = ctx.frame(cam, w, h, prm)
+            gst = ctx.gi_stats()
+            ctx.nrc_create(cfg)
+            for _ in range(3):
+                _, nrad, nirr = ctx.nrc_frame(cam, w, h, prm)
+            nst = ctx.nrc_stats()
+            out.append((rad, irr, tuple(gst), nrad, nirr, tuple(nst)))
+    finally:
+        ctx.set_tuning(trace_sort=0, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0)
+    a, b = out
+    for k in (0, 1, 3, 4):
+        assert a[k].tobytes() == b[k].tobytes(), f"output {k} differs"
+    assert a[2] == b[2] and a[5] == b[5]
+    assert a[5][1] > 0 and a[5][3] > 0 and a[5][4] > 0  # fallback pixels, training and fallback rays

@@ -35,13 +35,16 @@
  *
  * GI frame (estimator documented at tgx_gi_params in tgx.h):
  *   gi_gen_k     surface of each visible pixel, spp bounce rays appended
- *   gi_trace_k   bounce rays
- *   gi_shade_k   bounce hits facing the sun: material albedo and a shadow ray
- *   gi_trace_k   shadow rays
- *   gi_lit_k     unshadowed samples store their contribution
+ *   gi_trace_k   bounce rays; a lane whose bounce hit faces the sun stores
+ *                the albedo-weighted contribution and traces the shadow ray
+ *                next, clearing the contribution if it is occluded
  *   gi_resolve_k per pixel, contributions summed in sample order
- * Contributions are stored per (pixel, sample) and summed in sample order,
- * so the fp32 result equals a sequential per-pixel loop bit for bit.
+ * With sorted shadow rays (tuning trace_sort bit 1) the shadow rays go
+ * through a queue instead: gi_shade_k writes them, a second gi_trace_k
+ * traces them and gi_lit_k stores the unshadowed contributions. Both forms
+ * do the same fp32 operations per sample. Contributions are stored per
+ * (pixel, sample) and summed in sample order, so the fp32 result equals a
+ * sequential per-pixel loop bit for bit.
  */
 #include "tgx_gi_dev.h"
 #include "tgx_m4.h"
