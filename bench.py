@@ -69,6 +69,10 @@ def trace_alg_bytes(gst, info) -> int:
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# Cache fallback: trace a pixel when its cell's squared bias exceeds this
+# multiple of the plain estimator's variance (a margin for the noise of the
+# cells' estimates).
+NRC_FALLBACK_THRESHOLD = 4.0
 NRC_FWD_FLOPS = 2 * (78 * 64 + 3 * 64 * 64 + 64 * 3)   # per sample, unpadded
 NRC_TRAIN_FLOPS = 3 * NRC_FWD_FLOPS                     # forward + two backward products
 
@@ -102,26 +106,34 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     ctx.d2h(vis, vis_d)
     mask = vis != T.VISIBILITY_EMPTY
     plain = irr_frame(7)
-    cfg = T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16, train_batch=1 << 12,
-                       train_spp=4, fallback_threshold=1.0, fallback_min_samples=1, cell_size=8)
-    ctx.nrc_create(cfg)
     prm = T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=0)
-    for _ in range(pretrain):
-        ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
-    ctx.synchronize()
-    ctx.profile(True)
-    ctx.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = ctx.profile_stats()
-    ctx.profile(False)
-    st = ctx.nrc_stats()
-    irr = np.empty((H, W, 3), np.float32)
-    ctx.d2h(irr, irr_d)
-    ms = 1e3 * elapsed / steps
+
+    def run(flags: int, threshold: float):
+        """pretrain untimed frames, then `steps` timed frames of a fresh cache"""
+        ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16,
+                                    train_batch=1 << 12, train_spp=4, fallback_threshold=threshold,
+                                    fallback_min_samples=1, cell_size=8, flags=flags))
+        for _ in range(pretrain):
+            ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+        ctx.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = ctx.profile_stats()
+        ctx.profile(False)
+        st = ctx.nrc_stats()
+        irr = np.empty((H, W, 3), np.float32)
+        ctx.d2h(irr, irr_d)
+        return 1e3 * elapsed / steps, prof, st, irr
+
+    # reported beside: the t^2 rule and no fallback; then the bench's rule
+    ms_t2, _, st_t2, irr_t2 = run(0, 1.0)
+    ms_nf, _, _, irr_nf = run(T.NRC_FLAG_NO_FALLBACK, 1.0)
+    ms, prof, st, irr = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD)
     k_ms = {k: v[1] / steps for k, v in prof.items()}
     visible = int(st["visible_pixels"])
     infer_ms = k_ms.get("nrc_infer_k", 0.0)
@@ -137,7 +149,17 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "rays_per_s_equivalent": plain_rays / (ms * 1e-3),
         "rel_mse_vs_256spp": {"neural": rel_mse(irr[mask], ref[mask]), "plain_4spp": rel_mse(plain[mask], ref[mask]),
                               "reference_noise_floor": rel_mse(ref2[mask], ref[mask])},
+        # both 256-spp references averaged: half the reference noise
+        "rel_mse_vs_512spp": {"neural": rel_mse(irr[mask], (0.5 * (ref + ref2))[mask]),
+                              "plain_4spp": rel_mse(plain[mask], (0.5 * (ref + ref2))[mask])},
         "visible_pixels": visible, "fallback_pixels": int(st["fallback_pixels"]),
+        "fallback_rule": f"cell bias^2 > {NRC_FALLBACK_THRESHOLD} x the plain estimator's variance at 4 spp",
+        "fallback_rule_t2": {"rule": "cell bias^2 > 1.0 x t^2", "frame_ms": ms_t2,
+                             "fallback_pixels": int(st_t2["fallback_pixels"]),
+                             "rel_mse_vs_256spp": rel_mse(irr_t2[mask], ref[mask]),
+                             "rel_mse_vs_512spp": rel_mse(irr_t2[mask], (0.5 * (ref + ref2))[mask])},
+        "no_fallback": {"frame_ms": ms_nf, "rel_mse_vs_256spp": rel_mse(irr_nf[mask], ref[mask]),
+                        "rel_mse_vs_512spp": rel_mse(irr_nf[mask], (0.5 * (ref + ref2))[mask])},
         "train_samples": int(st["train_samples"]), "adam_steps_total": int(st["adam_steps"]),
         "train_loss": float(st["train_loss"]), "pretrain_frames": pretrain,
         "kernels_ms_per_step": {k: v for k, v in k_ms.items() if k.startswith("nrc_")},
@@ -196,8 +218,8 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
     for mode, flags in (("plain", 0), ("neural", T.FRAME_NEURAL)):
         if flags:
             ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / BATCH_WORLD, train_samples=1 << 16,
-                                        train_batch=1 << 12, train_spp=4, fallback_threshold=1.0,
-                                        fallback_min_samples=1, cell_size=8))
+                                        train_batch=1 << 12, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
+                                        fallback_min_samples=1, cell_size=8, flags=T.NRC_FLAG_FALLBACK_VS_TRACER))
         log(f"batch {mode}: autotune")
         t0 = time.perf_counter()
         best = ctx.autotune(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags, frames=1)

@@ -566,7 +566,8 @@ typedef struct tgx_nrc_config {
     uint32_t train_samples;         /* per frame of tgx_nrc_frame_device (65536) */
     uint32_t train_batch;           /* samples per Adam step, multiple of 512 (16384) */
     uint32_t train_spp;             /* SVO bounce samples per training target (4) */
-    float    fallback_threshold;    /* cell relative squared error above which pixels are traced */
+    float    fallback_threshold;    /* cell relative squared error above which pixels are traced
+                                       (relative to t^2, or to the tracer's variance) */
     uint32_t fallback_min_samples;  /* cells with fewer (decayed) training samples are traced */
     uint32_t cell_size;             /* edge of an error-estimate cell in voxels, power of two */
     uint32_t flags;                 /* TGX_NRC_FLAG_* */
@@ -575,6 +576,11 @@ typedef struct tgx_nrc_config {
 #define TGX_NRC_FLAG_NO_FALLBACK    1u  /* never trace: every visible pixel uses the cache */
 #define TGX_NRC_FLAG_ALL_FALLBACK   2u  /* trace every visible pixel (plain GI through this path) */
 #define TGX_NRC_FLAG_NO_TRAINING    4u  /* frames run inference only */
+/* Fallback where the cell's squared bias exceeds fallback_threshold times
+ * the variance the plain estimator would have at the frame's spp (measured
+ * on the training targets), instead of fallback_threshold times t^2: with
+ * threshold 1 a pixel is traced when tracing is expected to be closer. */
+#define TGX_NRC_FLAG_FALLBACK_VS_TRACER 8u
 
 typedef struct tgx_nrc_stats {
     uint64_t visible_pixels;

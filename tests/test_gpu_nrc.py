@@ -204,3 +204,33 @@ def test_fallback_follows_the_error_estimate(ctx, single_setup):
     st = ctx.nrc_stats()
     assert 0 < st["fallback_pixels"] < st["visible_pixels"]
     assert st["fallback_rays"] > 0
+
+
+def test_fallback_against_tracer_variance(ctx, single_setup):
+    """TGX_NRC_FLAG_FALLBACK_VS_TRACER: a pixel is traced when its cell's
+    squared bias exceeds the threshold times the plain estimator's variance.
+    An untrained cache traces every visible pixel (plain GI bit for bit);
+    after training, the traced set shrinks as the threshold grows (the
+    caches train identically, only the rule's weight differs)."""
+    p, sc, cam, mn = single_setup
+    ctx.upload(sc)
+    ctx.svo_build(mn, 11)
+    prm = T.gi_params(T.seeded_sun(1), spp=4)
+    c = cam[0]["camera"][:3]
+    counts = []
+    for th in (1e-3, 1.0, 1e6):
+        ctx.nrc_create(T.nrc_config(origin=tuple(np.floor(c) - 64), scale=1 / 128, train_samples=1 << 14,
+                                    train_batch=1 << 12, fallback_threshold=th, fallback_min_samples=1,
+                                    flags=T.NRC_FLAG_FALLBACK_VS_TRACER))
+        _, _, irr0 = ctx.nrc_frame(cam, 256, 256, prm)
+        st = ctx.nrc_stats()
+        assert st["fallback_pixels"] == st["visible_pixels"] > 0
+        if th == 1.0:
+            _, _, irr_g = ctx.frame(cam, 256, 256, prm)
+            assert np.array_equal(irr0.view(np.uint32), irr_g.view(np.uint32))
+        say(f"20 cache frames, tracer-variance fallback threshold {th}")
+        for _ in range(20):
+            ctx.nrc_frame(cam, 256, 256, prm)
+        counts.append(int(ctx.nrc_stats()["fallback_pixels"]))
+    assert counts[0] >= counts[1] >= counts[2]
+    assert counts[0] > counts[2]

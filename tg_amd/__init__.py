@@ -115,6 +115,7 @@ FRAME_STATS_DTYPE = np.dtype([("primary_rays", "<u8"), ("visible_pixels", "<u8")
 NRC_FLAG_NO_FALLBACK = 1
 NRC_FLAG_ALL_FALLBACK = 2
 NRC_FLAG_NO_TRAINING = 4
+NRC_FLAG_FALLBACK_VS_TRACER = 8
 
 
 def nrc_config(origin=(0.0, 0.0, 0.0), scale: float = 1.0 / 512, learning_rate: float = 1e-3, beta1: float = 0.9,

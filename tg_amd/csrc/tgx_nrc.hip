@@ -68,7 +68,7 @@ struct nrc_cell {
     int32_t  err;  /* sum of (y - t)^2 - var(t), fixed point */
     uint32_t ref;  /* sum of t^2, fixed point */
     uint32_t n;    /* samples * 256 */
-    uint32_t pad;
+    uint32_t var;  /* sum of var(t), fixed point */
 };
 
 struct nrc_enc {
@@ -293,11 +293,13 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
         loss[b] = se;
         if (cells) {
             const uint32_t c = cell_of(r, cell_shift);
-            const float bias2 = se - (tvar ? tvar[first + b] : 0.0f);
+            const float tv = tvar ? tvar[first + b] : 0.0f;
+            const float bias2 = se - tv;
             const float cl = fminf(fmaxf(bias2, -16.0f), 16.0f);
             atomicAdd(&cells[c].err, (int32_t)(cl * NRC_FIX));
             atomicAdd(&cells[c].ref, to_fix(t0 * t0 + t1 * t1 + t2 * t2));
             atomicAdd(&cells[c].n, 256u);
+            atomicAdd(&cells[c].var, to_fix(tv));
         }
     } else if (g == 0u) {
         loss[b] = 0.0f;
@@ -454,7 +456,9 @@ __global__ __launch_bounds__(256) void nrc_loss_k(const float* __restrict__ loss
 
 /* Inference over list entries b < *count (persistent waves, 16 samples a
  * tile, forward fragments in LDS). With cells, an entry whose cell is
- * unreliable gets fallback flag 1 instead of an output. */
+ * unreliable gets fallback flag 1 instead of an output: fewer than min_n
+ * samples, or a bias^2 sum above w_ref times the targets' t^2 sum plus
+ * w_var times their variance sum. */
 __global__ __launch_bounds__(NRC_BLOCK) void nrc_infer_k(const bf16x8* __restrict__ wf_g,
                                                          const gi_surf_rec* __restrict__ surf,
                                                          const uint32_t* __restrict__ list,
@@ -462,7 +466,8 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_infer_k(const bf16x8* __restric
                                                          float* __restrict__ out_entry, float* __restrict__ radiance,
                                                          float* __restrict__ irradiance,
                                                          const nrc_cell* __restrict__ cells, uint32_t cell_shift,
-                                                         float threshold, uint32_t min_n, uint32_t* __restrict__ fb_flag)
+                                                         float w_ref, float w_var, uint32_t min_n,
+                                                         uint32_t* __restrict__ fb_flag)
 {
     __shared__ bf16x8 wf[NRC_FWD_ELEMS / 8u];
     for (uint32_t i = threadIdx.x; i < NRC_FWD_ELEMS / 8u; i += NRC_BLOCK) wf[i] = wf_g[i];
@@ -485,7 +490,7 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_infer_k(const bf16x8* __restric
         bool fb = false;
         if (cells) {
             const nrc_cell c = cells[cell_of(r, cell_shift)];
-            fb = c.n < min_n || (float)c.err > threshold * (float)c.ref;
+            fb = c.n < min_n || (float)c.err > w_ref * (float)c.ref + w_var * (float)c.var;
             fb_flag[b] = fb ? 1u : 0u;
         }
         if (fb) continue;
@@ -513,15 +518,18 @@ __global__ void nrc_pick_k(const uint32_t* __restrict__ vis_list, const uint32_t
     out[j] = vis_list[h % nv];
 }
 
-/* Halves every cell of the error estimate (older frames fade out). */
+/* Scales every cell of the error estimate by 7/8 (older frames fade out
+ * over about eight frames: halving each frame left too few samples per cell
+ * and its noise sent many pixels to the fallback needlessly). */
 __global__ void nrc_decay_k(nrc_cell* __restrict__ cells)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NRC_CELLS) return;
     nrc_cell c = cells[i];
-    c.err >>= 1;
-    c.ref >>= 1;
-    c.n >>= 1;
+    c.err -= c.err >> 3;
+    c.ref -= c.ref >> 3;
+    c.n -= c.n >> 3;
+    c.var -= c.var >> 3;
     cells[i] = c;
 }
 
@@ -690,8 +698,9 @@ int upload_samples(tgx_context* ctx, const float* samples, uint32_t n, uint32_t 
 
 int launch_infer(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, const uint32_t* count_dev,
                  uint32_t max_count, float* out_entry, float* radiance, float* irradiance, const nrc_cell* cells,
-                 uint32_t* fb_flag)
+                 uint32_t* fb_flag, float w_var = -1.0f)
 {
+    /* w_var < 0: the t^2 rule with the configured threshold */
     if (max_count == 0) return TGX_OK;
     const tgx_nrc_config& c = ctx->nrc_cfg;
     uint32_t shift = 0;
@@ -704,7 +713,8 @@ int launch_infer(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list
         tgx_launch_scope s(ctx, "nrc_infer_k");
         hipLaunchKernelGGL(nrc_infer_k, dim3(want < cap ? want : cap), dim3(NRC_BLOCK), 0, ctx->stream,
                            (const bf16x8*)ctx->nrc_fwd.ptr, surf, list, count_dev, enc_of(c), out_entry, radiance,
-                           irradiance, cells, shift, c.fallback_threshold, c.fallback_min_samples * 256u, fb_flag);
+                           irradiance, cells, shift, w_var < 0.0f ? c.fallback_threshold : 0.0f,
+                           w_var < 0.0f ? 0.0f : w_var, c.fallback_min_samples * 256u, fb_flag);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_infer_k");
     return TGX_OK;
@@ -867,8 +877,12 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
     const bool all_fb = (c.flags & TGX_NRC_FLAG_ALL_FALLBACK) != 0;
     const bool use_cells = !(c.flags & TGX_NRC_FLAG_NO_FALLBACK) && !all_fb;
     if (!all_fb) {
+        /* against the tracer: a fallback pixel's error is the variance of a
+         * gi->spp-sample mean, train_spp / spp times a target's */
+        const float w_var = (c.flags & TGX_NRC_FLAG_FALLBACK_VS_TRACER)
+                                ? c.fallback_threshold * (float)c.train_spp / (float)gi->spp : -1.0f;
         if ((rc = launch_infer(ctx, surf, vis_list, n_vis, n_pix, nullptr, radiance_dev, irradiance_dev,
-                               use_cells ? cells : nullptr, fb_flag)))
+                               use_cells ? cells : nullptr, fb_flag, w_var)))
             return rc;
     }
     const bool fallback = use_cells || all_fb;
