@@ -8,6 +8,7 @@ in seconds: objects of 1..32 clusters per axis, value-noise solids, 64
 LUTs, several seeded cameras at the world centre."""
 from __future__ import annotations
 
+import os
 import sys
 
 import numpy as np
@@ -126,8 +127,11 @@ def test_frame_graph_plain_matches_uncaptured_and_oracle(ctx, batch_scene):
         buf.free()
 
 
-def test_graph_locks_context_buffers(ctx, batch_scene):
+def test_graph_locks_context_buffers(batch_scene):
+    """While a graph exists, calls that would grow a context buffer fail. A
+    context of its own, so no earlier test has grown the buffers already."""
     p, sc, cams, mn = batch_scene
+    ctx = T.Context(int(os.environ.get("TGX_DEVICE", "0")))
     setup_scene(ctx, p, mn)
     prm = T.gi_params(T.seeded_sun(4), spp=1)
     buf = Buffers(ctx, 2, W, H)
@@ -151,6 +155,7 @@ def test_graph_locks_context_buffers(ctx, batch_scene):
         ctx.svo_build(mn, DEPTH)  # unlocked again
     finally:
         buf.free()
+        ctx.close()
 
 
 def test_frame_graph_neural_matches_uncaptured(ctx, batch_scene):

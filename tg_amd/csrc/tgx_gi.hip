@@ -70,6 +70,8 @@ struct trace_params {
     uint32_t                  depth;
     uint32_t                  leaf_level;
     float                     mnx, mny, mnz;
+    float                     half;          /* 2^depth / 2 */
+    float                     cx, cy, cz;    /* root centre, min + half */
 };
 
 /* Ray record of the queues (32 B). */
@@ -178,12 +180,12 @@ __device__ static inline svo_res res_miss(uint32_t steps)
     return r;
 }
 
-/* Ray setup (tg_sparse_voxel_octree.c:570-594): false if the root box is
- * missed (the result is then a miss with 0 steps). */
-__device__ static inline bool lane_init(const trace_params& P, float half, const svo_ray& r, trace_lane& L)
+/* Ray setup (tg_sparse_voxel_octree.c:570-594) for the ray whose world-space
+ * origin and direction the lane holds in px, py, pz and dx, dy, dz: false
+ * if the root box is missed (the result is then a miss with 0 steps). */
+__device__ static inline bool lane_init_self(const trace_params& P, float half, trace_lane& L)
 {
-    const float ox = r.ox - (P.mnx + half), oy = r.oy - (P.mny + half), oz = r.oz - (P.mnz + half);
-    L.dx = r.dx; L.dy = r.dy; L.dz = r.dz;
+    const float ox = L.px - P.cx, oy = L.py - P.cy, oz = L.pz - P.cz;
     L.steps = 0;
     const float enter = fmax2(fmax2(axis_near(ox, L.dx, -half, half), axis_near(oy, L.dy, -half, half)),
                               axis_near(oz, L.dz, -half, half));
@@ -195,7 +197,6 @@ __device__ static inline bool lane_init(const trace_params& P, float half, const
     L.dist = enter > 0.0f ? enter : 0.0f;
     L.cur = 0;
     L.node = 0;
-    /* DDA step lengths 1/|d| (tg_amanatides_woo), fixed per ray */
     L.tdx = dda_len(1.0f, 1.0f, L.dx);
     L.tdy = dda_len(1.0f, 1.0f, L.dy);
     L.tdz = dda_len(1.0f, 1.0f, L.dz);
@@ -482,31 +483,46 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
     }
     const uint32_t n_rays = *n_rays_ptr;
     const uint32_t lane = threadIdx.x & 63u;
-    const float half = (float)(1u << P.depth) * 0.5f;
+    /* fused GI: each lane's bounce ray origin and id, behind the nodes; the
+     * wave's part of the stash is a uniform base indexed by lane */
+    float4* s_ray = (float4*)(s_inner + (kLds ? ((P.n_inner + P.n_leaves + 1u) & ~1u) : 0u)) +
+                    __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    const float half = P.half;
     trace_lane L = {};
     uint32_t slot = 0;
     /* wave totals: bounce hits, shadow rays and shadow hits per list */
     uint32_t n_hit0 = 0, n_hit1 = 0, n_sh0 = 0, n_sh1 = 0, n_shhit0 = 0, n_shhit1 = 0;
-    bool active = false, exhausted = false;
+    /* pending: the lane holds a ray (world origin in px..pz, direction in
+     * dx..dz) that is set up at the next iteration's start */
+    bool active = false, exhausted = false, pending = false;
     while (true) {
         /* wave-uniform refill once `refill` lanes are idle */
-        const unsigned long long idle = __ballot(!active);
+        const unsigned long long idle = __ballot(!active && !pending);
         const uint32_t want = (uint32_t)__popcll(idle);
         if (!exhausted && want >= refill) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(cursor, want);
-            base = __shfl(base, 0);
+            /* the whole wave runs this block, so lane 0 is the first active */
+            base = __builtin_amdgcn_readfirstlane(base);
             if (base + want >= n_rays) exhausted = true;
-            if (!active) {
+            if (!active && !pending) {
                 const uint32_t r = base + lane_rank(idle);
                 if (r < n_rays) {
                     slot = r;
-                    active = lane_init(P, half, rays[r], L);
-                    if (!active && !F.contrib) out[r] = res_miss(0u);
+                    const svo_ray q = rays[r];
+                    L.px = q.ox; L.py = q.oy; L.pz = q.oz;
+                    L.dx = q.dx; L.dy = q.dy; L.dz = q.dz;
+                    if (F.contrib) s_ray[lane] = make_float4(q.ox, q.oy, q.oz, __uint_as_float(q.id));
+                    pending = true;
                 }
             }
         }
-        if (!__any(active)) {
+        if (pending) {
+            pending = false;
+            active = lane_init_self(P, half, L);
+            if (!active && !F.contrib) out[slot] = res_miss(0u);
+        }
+        if (!__any(active || pending)) {
             if (exhausted) break;
             continue;
         }
@@ -533,14 +549,18 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             } else if (slot & TRACE_SHADOW) {
                 const uint32_t id = slot & ~TRACE_SHADOW;
                 if (res.flags & 1u) {
-                    /* occluded: the sample contributes nothing */
-                    float* c = F.contrib + 3ull * id;
+                    /* occluded: the sample contributes nothing (three single
+                     * stores: a merged store would hold three zero registers
+                     * across the loop) */
+                    volatile float* c = F.contrib + 3ull * id;
                     c[0] = 0.0f; c[1] = 0.0f; c[2] = 0.0f;
                     ev = id >= F.split ? 32u : 4u;
                 }
             } else {
-                const svo_ray b = rays[slot];
-                const uint32_t sh = b.id >= F.split ? 3u : 0u;
+                /* the ray's origin and id from LDS; its direction is L.dx..dz */
+                const float4 o = s_ray[lane];
+                const uint32_t id = __float_as_uint(o.w);
+                const uint32_t sh = id >= F.split ? 3u : 0u;
                 ev = (res.flags & 1u) << sh;
                 /* cosine of the hit face with the sun (gi_shade_k) */
                 const uint32_t ax = (res.flags >> 2) & 3u;
@@ -548,20 +568,23 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                 const float sc = ax == 0u ? F.sun0 : (ax == 1u ? F.sun1 : F.sun2);
                 const float cosv = ((res.flags & 1u) && !(res.distance < 0.0f)) ? sign * sc : 0.0f;
                 if (cosv > 0.0f) {
-                    float ox = b.ox + b.dx * res.distance, oy = b.oy + b.dy * res.distance,
-                          oz = b.oz + b.dz * res.distance;
+                    /* the shadow ray goes into the lane's own (dead) ray
+                     * registers and is set up with the next pending rays; a
+                     * shadow ray that misses the root box is unoccluded */
+                    L.px = o.x + L.dx * res.distance;
+                    L.py = o.y + L.dy * res.distance;
+                    L.pz = o.z + L.dz * res.distance;
                     const float off = sign * GI_SHADOW_OFFSET;
-                    if (ax == 0u) ox = ox + off; else if (ax == 1u) oy = oy + off; else oz = oz + off;
+                    if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
+                    L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
                     const uint16_t m = F.leaf_mats[(size_t)(res.node - P.n_inner) * TGX_SVO_BLOCK_VOXELS + res.voxel];
-                    const tgx_lut_entry e = F.luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
+                    const tgx_lut_entry* e = F.luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
                     const float wgt = cosv * F.sun_radiance * GI_INV_PI;
-                    float* c = F.contrib + 3ull * b.id;
-                    c[0] = e.albedo[0] * wgt; c[1] = e.albedo[1] * wgt; c[2] = e.albedo[2] * wgt;
-                    const svo_ray sr = {ox, oy, oz, F.sun0, F.sun1, F.sun2, b.id, 0u};
+                    float* c = F.contrib + 3ull * id;
+                    c[0] = e->albedo[0] * wgt; c[1] = e->albedo[1] * wgt; c[2] = e->albedo[2] * wgt;
                     ev |= 2u << sh;
-                    slot = b.id | TRACE_SHADOW;
-                    /* a shadow ray that misses the root box is unoccluded */
-                    active = lane_init(P, half, sr, L);
+                    slot = id | TRACE_SHADOW;
+                    pending = true;
                 }
             }
         }
@@ -1000,6 +1023,9 @@ static trace_params params_of(tgx_context* ctx)
     P.depth = ctx->svo_info.depth;
     P.leaf_level = ctx->svo_info.depth - 5u;
     P.mnx = ctx->svo_info.min[0]; P.mny = ctx->svo_info.min[1]; P.mnz = ctx->svo_info.min[2];
+    /* the kernels' former in-kernel expressions, in the same fp32 order */
+    P.half = (float)(1u << P.depth) * 0.5f;
+    P.cx = P.mnx + P.half; P.cy = P.mny + P.half; P.cz = P.mnz + P.half;
     return P;
 }
 
@@ -1021,12 +1047,15 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
     const uint32_t vote = ctx->tuning.trace_vote;
     {
         tgx_launch_scope s(ctx, "gi_trace_k");
+        /* dynamic LDS: nodes and brick masks (when they fit), then for the
+         * fused GI trace one 16-B ray stash per lane */
+        const uint32_t stash = F.contrib ? 16u * TRACE_BLOCK : 0u;
         if ((uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX)
-            hipLaunchKernelGGL(gi_trace_k<true>, grid, dim3(TRACE_BLOCK), 8u * (P.n_inner + P.n_leaves), ctx->stream, P,
-                               rays, n_dev, maxd, cursor, out, hit_counter, refill, vote, F);
+            hipLaunchKernelGGL(gi_trace_k<true>, grid, dim3(TRACE_BLOCK), 8u * ((P.n_inner + P.n_leaves + 1u) & ~1u) + stash,
+                               ctx->stream, P, rays, n_dev, maxd, cursor, out, hit_counter, refill, vote, F);
         else
-            hipLaunchKernelGGL(gi_trace_k<false>, grid, dim3(TRACE_BLOCK), 0, ctx->stream, P, rays, n_dev, maxd, cursor,
-                               out, hit_counter, refill, vote, F);
+            hipLaunchKernelGGL(gi_trace_k<false>, grid, dim3(TRACE_BLOCK), stash, ctx->stream, P, rays, n_dev, maxd,
+                               cursor, out, hit_counter, refill, vote, F);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
     return TGX_OK;
