@@ -467,7 +467,7 @@ struct trace_fuse {
 /* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd; the
  * result of ray i goes to out[i] (plain tracing). `cursor` must be 0 at
  * launch; hits (flag bit 0) are added to *hit_counter when it is not null. */
-template <bool kLds>
+template <bool kLds, bool kFused>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                     const svo_ray q = rays[r];
                     L.px = q.ox; L.py = q.oy; L.pz = q.oz;
                     L.dx = q.dx; L.dy = q.dy; L.dz = q.dz;
-                    if (F.contrib) s_ray[lane] = make_float4(q.ox, q.oy, q.oz, __uint_as_float(q.id));
+                    if constexpr (kFused) s_ray[lane] = make_float4(q.ox, q.oy, q.oz, __uint_as_float(q.id));
                     pending = true;
                 }
             }
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
         if (pending) {
             pending = false;
             active = lane_init_self(P, half, L);
-            if (!active && !F.contrib) out[slot] = res_miss(0u);
+            if (!kFused && !active) out[slot] = res_miss(0u);
         }
         if (!__any(active || pending)) {
             if (exhausted) break;
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
         uint32_t ev = 0u;
         if (lane_step<kLds>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda) && active) {
             active = false;
-            if (!F.contrib) {
+            if constexpr (!kFused) {
                 out[slot] = res;
                 ev = res.flags & 1u;
             } else if (slot & TRACE_SHADOW) {
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
         }
     }
     if (lane == 0) {
-        if (!F.contrib) {
+        if constexpr (!kFused) {
             if (hit_counter && n_hit0) atomicAdd(hit_counter, n_hit0);
         } else {
             if (n_hit0) atomicAdd(&F.stats0[2], n_hit0);
@@ -1049,13 +1049,14 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
         tgx_launch_scope s(ctx, "gi_trace_k");
         /* dynamic LDS: nodes and brick masks (when they fit), then for the
          * fused GI trace one 16-B ray stash per lane */
-        const uint32_t stash = F.contrib ? 16u * TRACE_BLOCK : 0u;
-        if ((uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX)
-            hipLaunchKernelGGL(gi_trace_k<true>, grid, dim3(TRACE_BLOCK), 8u * ((P.n_inner + P.n_leaves + 1u) & ~1u) + stash,
-                               ctx->stream, P, rays, n_dev, maxd, cursor, out, hit_counter, refill, vote, F);
-        else
-            hipLaunchKernelGGL(gi_trace_k<false>, grid, dim3(TRACE_BLOCK), stash, ctx->stream, P, rays, n_dev, maxd,
-                               cursor, out, hit_counter, refill, vote, F);
+        const bool fused = F.contrib != nullptr;
+        const uint32_t stash = fused ? 16u * TRACE_BLOCK : 0u;
+        const bool lds = (uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX;
+        const uint32_t shmem = (lds ? 8u * ((P.n_inner + P.n_leaves + 1u) & ~1u) : 0u) + stash;
+        auto kernel = lds ? (fused ? gi_trace_k<true, true> : gi_trace_k<true, false>)
+                          : (fused ? gi_trace_k<false, true> : gi_trace_k<false, false>);
+        hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor, out,
+                           hit_counter, refill, vote, F);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
     return TGX_OK;
