@@ -134,6 +134,25 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     ms_t2, _, st_t2, irr_t2 = run(0, 1.0)
     ms_nf, _, _, irr_nf = run(T.NRC_FLAG_NO_FALLBACK, 1.0)
     ms, prof, st, irr = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD)
+
+    def graph_ms(flags: int) -> float:
+        """one-camera frame captured as a graph (the cache keeps training on
+        every replay): 3 untimed replays, then `steps` timed ones"""
+        g = ctx.frame_graph(cam, W, H, prm, vis_d, rad_d, irr_d, flags)
+        try:
+            for _ in range(3):
+                g.launch()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.launch()
+            ctx.synchronize()
+            return 1e3 * (time.perf_counter() - t0) / steps
+        finally:
+            g.destroy()
+
+    captured = {"plain_frame_ms": graph_ms(0), "neural_frame_ms": graph_ms(T.FRAME_NEURAL)}
+    captured["speedup_vs_plain"] = captured["plain_frame_ms"] / captured["neural_frame_ms"]
     k_ms = {k: v[1] / steps for k, v in prof.items()}
     visible = int(st["visible_pixels"])
     infer_ms = k_ms.get("nrc_infer_k", 0.0)
@@ -167,6 +186,7 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
                  "infer_frac": infer_tflops / MFMA_BF16_PEAK_TFLOPS if infer_tflops else None,
                  "train_frac": train_tflops / MFMA_BF16_PEAK_TFLOPS if train_tflops else None,
                  "flops_per_sample": {"forward": NRC_FWD_FLOPS, "train": NRC_TRAIN_FLOPS}},
+        "captured_graphs": captured,
         "config": {"network": "96(79 used)->64x4->3 ReLU, bf16 MFMA, fp32 Adam lr 1e-3", "train_samples": 1 << 16,
                    "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512"},
     }

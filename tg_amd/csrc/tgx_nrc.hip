@@ -408,50 +408,73 @@ struct nrc_adam_args {
 /* device counters (nrc_dev): 0 frames run, 1 Adam steps taken */
 #define NRC_DEV_FRAME 0
 #define NRC_DEV_STEP  1
+#define NRC_DEV_DONE  2   /* blocks of the running Adam step that are done */
 
 __global__ void nrc_tick_k(uint32_t* counter) { *counter += 1u; }
 
-/* One Adam step; the step number t comes from the device counter (advanced
- * by nrc_tick_k before this launch), so replayed graphs keep counting. */
-__global__ void nrc_adam_k(const float* __restrict__ partial, uint32_t n_chunks, nrc_adam_args a,
-                           const uint32_t* __restrict__ dev, float* __restrict__ master, float* __restrict__ m,
-                           float* __restrict__ v, float* __restrict__ grad, __bf16* __restrict__ fwd,
-                           __bf16* __restrict__ bwd)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NRC_N) return;
-    const double t = (double)dev[NRC_DEV_STEP];
-    const float inv_bc1 = (float)(1.0 / (1.0 - pow((double)a.beta1, t)));
-    const float inv_bc2 = (float)(1.0 / (1.0 - pow((double)a.beta2, t)));
-    float gsum = 0.0f;
-    for (uint32_t c = 0; c < n_chunks; c++) gsum += partial[(size_t)c * NRC_N + i];
-    grad[i] = gsum;
-    const float mi = a.beta1 * m[i] + (1.0f - a.beta1) * gsum;
-    const float vi = a.beta2 * v[i] + (1.0f - a.beta2) * gsum * gsum;
-    m[i] = mi;
-    v[i] = vi;
-    const float w = master[i] - a.lr * (mi * inv_bc1) / (sqrtf(vi * inv_bc2) + a.eps);
-    master[i] = w;
-    int f, b;
-    param_slots(i, f, b);
-    const __bf16 h = (__bf16)w;
-    fwd[f] = h;
-    if (b >= 0) bwd[b] = h;
-}
-
-/* Sum of the per-sample squared errors of a batch in a fixed order. */
-__global__ __launch_bounds__(256) void nrc_loss_k(const float* __restrict__ loss, uint32_t n, float* __restrict__ out)
+/* One Adam step with step number t = the device step counter + 1. The last
+ * block to finish advances the counter (replayed graphs keep counting) and,
+ * with loss_out, adds the batch's per-sample squared errors to *loss_out in
+ * a fixed order. dev[NRC_DEV_DONE] counts finished blocks; it is 0 between
+ * launches. */
+__global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ partial, uint32_t n_chunks, nrc_adam_args a,
+                                                  uint32_t* __restrict__ dev, float* __restrict__ master,
+                                                  float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
+                                                  __bf16* __restrict__ fwd, __bf16* __restrict__ bwd,
+                                                  const float* __restrict__ loss, uint32_t n_loss,
+                                                  float* __restrict__ loss_out)
 {
     __shared__ float s[256];
-    float acc = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += 256u) acc += loss[i];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (uint32_t w = 128u; w > 0u; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
+    __shared__ uint32_t last;
+    __shared__ float bc[2];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t_u = dev[NRC_DEV_STEP] + 1u;
+    if (threadIdx.x == 0) {
+        /* bias corrections, once per block */
+        const double t = (double)t_u;
+        bc[0] = (float)(1.0 / (1.0 - pow((double)a.beta1, t)));
+        bc[1] = (float)(1.0 / (1.0 - pow((double)a.beta2, t)));
     }
-    if (threadIdx.x == 0) *out += s[0];
+    __syncthreads();
+    if (i < NRC_N) {
+        const float inv_bc1 = bc[0], inv_bc2 = bc[1];
+        float gsum = 0.0f;
+        for (uint32_t c = 0; c < n_chunks; c++) gsum += partial[(size_t)c * NRC_N + i];
+        grad[i] = gsum;
+        const float mi = a.beta1 * m[i] + (1.0f - a.beta1) * gsum;
+        const float vi = a.beta2 * v[i] + (1.0f - a.beta2) * gsum * gsum;
+        m[i] = mi;
+        v[i] = vi;
+        const float w = master[i] - a.lr * (mi * inv_bc1) / (sqrtf(vi * inv_bc2) + a.eps);
+        master[i] = w;
+        int f, b;
+        param_slots(i, f, b);
+        const __bf16 h = (__bf16)w;
+        fwd[f] = h;
+        if (b >= 0) bwd[b] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&dev[NRC_DEV_DONE], 1u) == gridDim.x - 1u ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!last) return;
+    if (loss_out) {
+        float acc = 0.0f;
+        for (uint32_t k = threadIdx.x; k < n_loss; k += 256u) acc += loss[k];
+        s[threadIdx.x] = acc;
+        __syncthreads();
+        for (uint32_t w = 128u; w > 0u; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *loss_out += s[0];
+    }
+    if (threadIdx.x == 0) {
+        dev[NRC_DEV_STEP] = t_u;
+        dev[NRC_DEV_DONE] = 0u;
+    }
 }
 
 /* Inference over list entries b < *count (persistent waves, 16 samples a
@@ -653,21 +676,15 @@ int train_step(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, 
     nrc_adam_args a;
     a.lr = c.learning_rate; a.beta1 = c.beta1; a.beta2 = c.beta2; a.eps = c.epsilon;
     uint32_t* dev = (uint32_t*)ctx->nrc_dev.ptr;
-    hipLaunchKernelGGL(nrc_tick_k, dim3(1), dim3(1), 0, ctx->stream, dev + NRC_DEV_STEP);
-    TGX_LAUNCH_CHECK(ctx, "nrc_tick_k");
     {
         tgx_launch_scope s(ctx, "nrc_adam_k");
         hipLaunchKernelGGL(nrc_adam_k, dim3(nrc_blocks(NRC_N, 256)), dim3(256), 0, ctx->stream,
-                           (const float*)ctx->nrc_partials.ptr, n_chunks, a, (const uint32_t*)dev,
-                           (float*)ctx->nrc_master.ptr,
+                           (const float*)ctx->nrc_partials.ptr, n_chunks, a, dev, (float*)ctx->nrc_master.ptr,
                            (float*)ctx->nrc_m.ptr, (float*)ctx->nrc_v.ptr, (float*)ctx->nrc_grad.ptr,
-                           (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr);
+                           (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr, (const float*)ctx->nrc_loss.ptr, Bc,
+                           loss_acc);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_adam_k");
-    if (loss_acc) {
-        hipLaunchKernelGGL(nrc_loss_k, dim3(1), dim3(256), 0, ctx->stream, (const float*)ctx->nrc_loss.ptr, Bc, loss_acc);
-        TGX_LAUNCH_CHECK(ctx, "nrc_loss_k");
-    }
     return TGX_OK;
 }
 
