@@ -163,3 +163,35 @@ def test_svo_update_equals_full_build(ctx):
     # an update right after a build reuses every leaf
     reused, voxelized = ctx.svo_update()
     assert voxelized == 0 and reused == int(ctx.svo_info()["n_leaves"])
+
+
+def test_svo_update_without_previous_dynamic_build(ctx):
+    """tgx_svo_update on an uploaded scene samples every leaf (nothing to
+    compare against) and equals tgx_svo_build; with a frame graph alive it is
+    refused like a build."""
+    p = T.noise_objects_params(seed=3)
+    sc = T.generate_scene(p)
+    ctx.upload(sc)
+    mn = np.array([-128.0, -128.0, -128.0], np.float32)
+    ctx.svo_build(mn, 9)
+    full = ctx.svo_export()
+    reused, voxelized = ctx.svo_update()
+    assert reused == 0 and voxelized == int(ctx.svo_info()["n_leaves"]) > 0
+    upd = ctx.svo_export()
+    for k in SVO_KEYS:
+        assert np.array_equal(upd[k], full[k]), k
+    w, h = 64, 36
+    cam = T.generate_camera(p, 0, w, h)
+    prm = T.gi_params(T.seeded_sun(1), spp=1)
+    bufs = [ctx.alloc(8 * w * h), ctx.alloc(12 * w * h)]
+    try:
+        g = ctx.frame_graph(cam, w, h, prm, bufs[0], bufs[1])
+        try:
+            with pytest.raises(T.TgxError):
+                ctx.svo_update()
+        finally:
+            g.destroy()
+        ctx.svo_update()
+    finally:
+        for b in bufs:
+            ctx.free(b)
