@@ -53,6 +53,20 @@ __global__ void gi_acc_k(const uint32_t* __restrict__ gcnt, const uint32_t* __re
     if ((mode & 4u) && nrc_q) acc[5] += nrc_q[1];
 }
 
+/* Frame totals of a multi-camera plain GI frame traced in one launch: the
+ * cameras' visible counts and the shared counters. */
+__global__ void gi_acc_all_k(const uint32_t* __restrict__ counts, uint32_t n, const uint32_t* __restrict__ gcnt,
+                             uint32_t* __restrict__ acc)
+{
+    if (threadIdx.x != 0) return;
+    uint32_t v = 0;
+    for (uint32_t c = 0; c < n; c++) v += counts[c];
+    acc[0] += v;
+    acc[1] += gcnt[GI_CNT_PLAIN + 1];
+    acc[2] += gcnt[GI_CNT_PLAIN + 3];
+    acc[3] += gcnt[GI_CNT_PLAIN + 2];
+}
+
 /* One frame as stream-ordered launches; `record` selects the capture-safe
  * visibility path. */
 static int frame_body(tgx_context* ctx, tgx_frame_graph* g, bool record)
@@ -70,6 +84,35 @@ static int frame_body(tgx_context* ctx, tgx_frame_graph* g, bool record)
     if (neural) {
         const uint32_t nf = ctx->nrc_cfg.flags;
         mode = ((nf & TGX_NRC_FLAG_NO_FALLBACK) ? 0u : 5u) | ((nf & TGX_NRC_FLAG_NO_TRAINING) ? 0u : 2u);
+    }
+    const uint64_t samples = (uint64_t)n * px * g->gi.spp;
+    if (!neural && n > 1 && samples < (1ull << 31)) {
+        /* Plain GI of every camera through one ray queue and one trace
+         * launch: each camera's surfaces and pixel list in its own slice. */
+        if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_CNT_TOTAL, "gi counters")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->gi_bsurf, sizeof(gi_surf_rec) * n * px, "gi surfaces per camera")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->gi_blist, 12ull * n * px, "gi pixel lists per camera")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->gi_bcount, 4ull * n, "gi visible counts per camera")))
+            return rc;
+        gi_surf_rec* surf = (gi_surf_rec*)ctx->gi_bsurf.ptr;
+        uint32_t* lists = (uint32_t*)ctx->gi_blist.ptr;
+        uint32_t* counts = (uint32_t*)ctx->gi_bcount.ptr;
+        uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
+        std::vector<gi_trace_set> sets(n);
+        for (uint32_t c = 0; c < n; c++) {
+            float* rad = g->rad + 3 * c * px;
+            float* irr = g->irr ? g->irr + 3 * c * px : nullptr;
+            if ((rc = tgx_gi_surfaces_at(ctx, &g->cams[c], g->w, g->h, g->vis + c * px, rad, irr, surf + c * px,
+                                         lists + 3 * c * px, counts + c)))
+                return rc;
+            sets[c] = {g->gi, lists + 3 * c * px, counts + c, (uint32_t)px, cnt + GI_CNT_PLAIN, nullptr, nullptr, rad,
+                       irr, nullptr, surf + c * px};
+        }
+        if ((rc = tgx_gi_trace_sets(ctx, sets.data(), n))) return rc;
+        hipLaunchKernelGGL(gi_acc_all_k, dim3(1), dim3(64), 0, ctx->stream, (const uint32_t*)counts, n,
+                           (const uint32_t*)cnt, g->acc);
+        TGX_LAUNCH_CHECK(ctx, "gi_acc_all_k");
+        return TGX_OK;
     }
     for (uint32_t c = 0; c < n; c++) {
         const uint64_t* vis = g->vis + c * px;

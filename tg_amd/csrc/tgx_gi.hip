@@ -1097,6 +1097,36 @@ int tgx_gi_validate(tgx_context* ctx, uint32_t w, uint32_t h, const tgx_gi_param
     return TGX_OK;
 }
 
+int tgx_gi_surfaces_at(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
+                       float* radiance_dev, float* irradiance_dev, gi_surf_rec* surf, uint32_t* list_scratch,
+                       uint32_t* count)
+{
+    const uint64_t n_pix = (uint64_t)w * h;
+    int rc;
+    gi_cam gc;
+    for (int i = 0; i < 3; i++) {
+        gc.pos[i] = cam->camera[i];
+        gc.bl[i] = cam->ray_bl[i]; gc.br[i] = cam->ray_br[i]; gc.tr[i] = cam->ray_tr[i]; gc.tl[i] = cam->ray_tl[i];
+    }
+    uint32_t* list = list_scratch;
+    uint32_t* flag = list + n_pix;
+    uint32_t* off = flag + n_pix;
+    {
+        tgx_launch_scope s(ctx, "gi_surface_k");
+        hipLaunchKernelGGL(gi_surface_k, dim3(blocks_for(n_pix)), dim3(GI_BLOCK), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
+                           (const tgx_lut_entry*)ctx->luts.ptr, gc, w, h, vis_dev, surf, flag, radiance_dev,
+                           irradiance_dev);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_surface_k");
+    if ((rc = tgx_prim_exclusive_scan_u32(ctx, flag, off, (uint32_t)n_pix))) return rc;
+    hipLaunchKernelGGL(gi_list_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream, flag, off, (uint32_t)n_pix, list,
+                       count);
+    TGX_LAUNCH_CHECK(ctx, "gi_list_k");
+    return TGX_OK;
+}
+
 int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
                     float* radiance_dev, float* irradiance_dev)
 {
@@ -1106,48 +1136,32 @@ int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_
         (rc = tgx_buffer_reserve(ctx, &ctx->gi_surf, sizeof(gi_surf_rec) * n_pix, "gi surfaces")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->gi_list, 12ull * n_pix, "gi pixel list")))
         return rc;
-    gi_cam gc;
-    for (int i = 0; i < 3; i++) {
-        gc.pos[i] = cam->camera[i];
-        gc.bl[i] = cam->ray_bl[i]; gc.br[i] = cam->ray_br[i]; gc.tr[i] = cam->ray_tr[i]; gc.tl[i] = cam->ray_tl[i];
-    }
-    uint32_t* list = (uint32_t*)ctx->gi_list.ptr;
-    uint32_t* flag = list + n_pix;
-    uint32_t* off = flag + n_pix;
-    {
-        tgx_launch_scope s(ctx, "gi_surface_k");
-        hipLaunchKernelGGL(gi_surface_k, dim3(blocks_for(n_pix)), dim3(GI_BLOCK), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
-                           (const tgx_lut_entry*)ctx->luts.ptr, gc, w, h, vis_dev, (gi_surf_rec*)ctx->gi_surf.ptr,
-                           flag, radiance_dev, irradiance_dev);
-    }
-    TGX_LAUNCH_CHECK(ctx, "gi_surface_k");
-    if ((rc = tgx_prim_exclusive_scan_u32(ctx, flag, off, (uint32_t)n_pix))) return rc;
-    hipLaunchKernelGGL(gi_list_k, dim3(blocks_for(n_pix)), dim3(256), 0, ctx->stream, flag, off, (uint32_t)n_pix, list,
-                       (uint32_t*)ctx->gi_counts.ptr + GI_CNT_VISIBLE);
-    TGX_LAUNCH_CHECK(ctx, "gi_list_k");
-    return TGX_OK;
+    return tgx_gi_surfaces_at(ctx, cam, w, h, vis_dev, radiance_dev, irradiance_dev, (gi_surf_rec*)ctx->gi_surf.ptr,
+                              (uint32_t*)ctx->gi_list.ptr, (uint32_t*)ctx->gi_counts.ptr + GI_CNT_VISIBLE);
 }
 
 int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_sets)
 {
-    if (n_sets == 0 || n_sets > 2) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "1 or 2 trace sets");
+    if (n_sets == 0 || n_sets > 1024) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "1 to 1024 trace sets");
     uint64_t n_samples = 0;
-    uint32_t id_base[2] = {0u, 0u};
+    std::vector<uint32_t> id_base(n_sets);
     for (uint32_t k = 0; k < n_sets; k++) {
         id_base[k] = (uint32_t)n_samples;
         n_samples += (uint64_t)sets[k].max_count * sets[k].prm.spp;
     }
     if (n_samples >= (1ull << 31)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%llu samples exceed 2^31",
                                                    (unsigned long long)n_samples);
+    const bool fused = !(ctx->tuning.trace_sort & 2u);
     int rc;
+    /* result records, shadow queues and weights only for the queued form */
     if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_rays, sizeof(svo_ray) * n_samples, "gi bounce rays")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_res, sizeof(svo_res) * n_samples, "gi bounce results")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow, sizeof(svo_ray) * n_samples, "gi shadow rays")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow_res, sizeof(svo_res) * n_samples, "gi shadow results")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->gi_weight, 12ull * n_samples, "gi weights")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->gi_contrib, 12ull * n_samples, "gi contributions")))
+        return rc;
+    if (!fused &&
+        ((rc = tgx_buffer_reserve(ctx, &ctx->gi_res, sizeof(svo_res) * n_samples, "gi bounce results")) ||
+         (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow, sizeof(svo_ray) * n_samples, "gi shadow rays")) ||
+         (rc = tgx_buffer_reserve(ctx, &ctx->gi_shadow_res, sizeof(svo_res) * n_samples, "gi shadow results")) ||
+         (rc = tgx_buffer_reserve(ctx, &ctx->gi_weight, 12ull * n_samples, "gi weights"))))
         return rc;
     svo_ray* rays = (svo_ray*)ctx->gi_rays.ptr;
     svo_res* res = (svo_res*)ctx->gi_res.ptr;
@@ -1155,16 +1169,19 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
     svo_res* sres = (svo_res*)ctx->gi_shadow_res.ptr;
     float* weight = (float*)ctx->gi_weight.ptr;
     float* contrib = (float*)ctx->gi_contrib.ptr;
-    const gi_surf_rec* surf = (const gi_surf_rec*)ctx->gi_surf.ptr;
-    /* one list: its own counters are the queues; two: the shared block */
+    /* two sets: per-set counters and the shared queue block; otherwise all
+     * in the first set's block */
     const bool two = n_sets == 2;
     uint32_t* cnt = two ? (uint32_t*)ctx->gi_counts.ptr + GI_CNT_QUEUE : sets[0].counters;
-    for (uint32_t k = 0; k < n_sets; k++)
-        hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, sets[k].counters, (uint64_t)GI_N_COUNTERS, 0u);
-    if (two) hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, sets[0].counters, (uint64_t)GI_N_COUNTERS, 0u);
+    if (two) {
+        hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, sets[1].counters, (uint64_t)GI_N_COUNTERS, 0u);
+        hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
+    }
     for (uint32_t k = 0; k < n_sets; k++) {
         const gi_trace_set& t = sets[k];
         if (t.max_count == 0) continue;
+        const gi_surf_rec* surf = t.surf ? t.surf : (const gi_surf_rec*)ctx->gi_surf.ptr;
         tgx_launch_scope s(ctx, "gi_gen_k");
         hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(t.max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, t.list,
                            t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k],
@@ -1199,7 +1216,8 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         hipLaunchKernelGGL(gi_shade_k, dim3((uint32_t)((n_samples + GI_BLOCK * SHADE_ITEMS - 1) / (GI_BLOCK * SHADE_ITEMS))),
                            dim3(GI_BLOCK), 0, ctx->stream, ctx->svo_n_inner, (const uint16_t*)ctx->svo_leaf_mats.ptr,
                            (const tgx_lut_entry*)ctx->luts.ptr, prm, rays, cnt + 1, res, shadow, weight, cnt,
-                           id_base[1], two ? sets[0].counters : nullptr, two ? sets[1].counters : nullptr);
+                           two ? id_base[1] : 0xFFFFFFFFu, two ? sets[0].counters : nullptr,
+                           two ? sets[1].counters : nullptr);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_shade_k");
     if (ctx->tuning.trace_sort & 2u) {
@@ -1219,6 +1237,7 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
     for (uint32_t k = 0; k < n_sets; k++) {
         const gi_trace_set& t = sets[k];
         if (t.max_count == 0) continue;
+        const gi_surf_rec* surf = t.surf ? t.surf : (const gi_surf_rec*)ctx->gi_surf.ptr;
         tgx_launch_scope s(ctx, "gi_resolve_k");
         hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(t.max_count)), dim3(256), 0, ctx->stream, surf, t.list,
                            t.count_dev, contrib + 3ull * id_base[k], t.prm.spp, t.e_entry, t.e_var, t.radiance_dev,
@@ -1233,7 +1252,7 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
                       float* irradiance_dev, const uint32_t* frame_salt)
 {
     const gi_trace_set t = {*prm, list, count_dev, max_count, cnt, e_entry, e_var, radiance_dev, irradiance_dev,
-                            frame_salt};
+                            frame_salt, nullptr};
     return tgx_gi_trace_sets(ctx, &t, 1);
 }
 

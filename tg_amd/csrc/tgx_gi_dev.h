@@ -74,6 +74,12 @@ int tgx_gi_validate(tgx_context* ctx, uint32_t w, uint32_t h, const tgx_gi_param
 int tgx_gi_surfaces(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
                     float* radiance_dev, float* irradiance_dev);
 
+/* The surface pass into caller buffers: surf[w h], list_scratch[3 w h]
+ * (list, flags, scan), *count = visible pixels. */
+int tgx_gi_surfaces_at(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis_dev,
+                       float* radiance_dev, float* irradiance_dev, gi_surf_rec* surf, uint32_t* list_scratch,
+                       uint32_t* count);
+
 /* Traces the estimator for the pixels list[0 .. *count_dev) (max_count
  * bounds the count) using the surfaces of the last surface pass. Per list
  * entry i, the irradiance E goes to e_entry[3 i] and the variance of that
@@ -99,16 +105,17 @@ struct gi_trace_set {
     float*          radiance_dev;
     float*          irradiance_dev;
     const uint32_t* frame_salt;
+    const gi_surf_rec* surf;        /* the list's surface records (null: the last surface pass's) */
 };
 
-/* Traces one or two lists through shared ray queues: one bounce launch and
- * one shadow launch for both, so a small list does not pay a launch tail of
- * its own. Each list's results equal those of tgx_gi_trace_list on it
- * alone (every sample is drawn from its own list entry, sample and frame,
- * and contributions are summed per entry in sample order). The lists share
- * prm except spp and frame. Per-list counters hold bounce rays (1), bounce
- * hits (2) and shadow rays (3); with two sets the queues use the block at
- * GI_CNT_QUEUE. */
+/* Traces several lists through one shared ray queue and one trace launch,
+ * so a small list does not pay a launch tail of its own. Each list's
+ * results equal those of tgx_gi_trace_list on it alone (every sample is
+ * drawn from its own list entry, sample and frame, and contributions are
+ * summed per entry in sample order). The lists share prm except spp and
+ * frame. Counters hold bounce rays (1), bounce hits (2) and shadow rays (3):
+ * with two sets per list, the queues then using the block at GI_CNT_QUEUE;
+ * with one set, or more than two, all in sets[0].counters (totals). */
 int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_sets);
 
 #endif

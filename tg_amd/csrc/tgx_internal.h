@@ -77,6 +77,9 @@ struct tgx_context {
     tgx_buffer   gi_counts;      /* GI queue counters and trace cursors     */
     tgx_buffer   gi_surf;        /* per-pixel surface records               */
     tgx_buffer   gi_list;        /* visible pixel list, flags, scan         */
+    tgx_buffer   gi_bsurf;       /* multi-camera frames: surfaces per camera */
+    tgx_buffer   gi_blist;       /* ... pixel lists, flags, scans per camera */
+    tgx_buffer   gi_bcount;      /* ... visible count per camera             */
     tgx_buffer   gi_rays;        /* bounce ray queue                        */
     tgx_buffer   gi_res;         /* bounce results                          */
     tgx_buffer   gi_shadow;      /* shadow ray queue                        */

@@ -529,7 +529,7 @@ void tgx_svo_release(tgx_context* ctx)
                           &ctx->svo_list_off, &ctx->svo_list, &ctx->svo_frames, &ctx->svo_rays,
                           &ctx->svo_hits, &ctx->prim_tmp, &ctx->gi_counts, &ctx->gi_surf,
                           &ctx->gi_rays, &ctx->gi_res, &ctx->gi_shadow, &ctx->gi_shadow_res,
-                          &ctx->gi_weight, &ctx->gi_contrib, &ctx->gi_list, &ctx->gi_rays2, &ctx->gi_sort, &ctx->svo_inner,
+                          &ctx->gi_weight, &ctx->gi_contrib, &ctx->gi_list, &ctx->gi_rays2, &ctx->gi_sort, &ctx->gi_bsurf, &ctx->gi_blist, &ctx->gi_bcount, &ctx->svo_inner,
                           &ctx->svo_bricks};
     for (tgx_buffer* b : bufs) tgx_buffer_release(b);
     tgx_buffer* prev[] = {&ctx->svo_prev_keys, &ctx->svo_prev_off, &ctx->svo_prev_cand, &ctx->svo_old_bits,
