@@ -106,7 +106,7 @@ static int frame_body(tgx_context* ctx, tgx_frame_graph* g, bool record)
                                          lists + 3 * c * px, counts + c)))
                 return rc;
             sets[c] = {g->gi, lists + 3 * c * px, counts + c, (uint32_t)px, cnt + GI_CNT_PLAIN, nullptr, nullptr, rad,
-                       irr, nullptr, surf + c * px};
+                       irr, nullptr, surf + c * px, 0u};
         }
         if ((rc = tgx_gi_trace_sets(ctx, sets.data(), n))) return rc;
         hipLaunchKernelGGL(gi_acc_all_k, dim3(1), dim3(64), 0, ctx->stream, (const uint32_t*)counts, n,
@@ -114,6 +114,8 @@ static int frame_body(tgx_context* ctx, tgx_frame_graph* g, bool record)
         TGX_LAUNCH_CHECK(ctx, "gi_acc_all_k");
         return TGX_OK;
     }
+    if (neural && n > 1 && !(ctx->nrc_cfg.flags & TGX_NRC_FLAG_NO_TRAINING))
+        return tgx_nrc_frames_from_vis(ctx, g->cams.data(), n, g->w, g->h, &g->gi, g->vis, g->rad, g->irr, g->acc);
     for (uint32_t c = 0; c < n; c++) {
         const uint64_t* vis = g->vis + c * px;
         float* rad = g->rad + 3 * c * px;

@@ -771,10 +771,10 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restri
                                                      const uint32_t* __restrict__ frame_salt,
                                                      svo_ray* __restrict__ rays, float* __restrict__ contrib,
                                                      uint32_t* __restrict__ counters, uint32_t id_base,
-                                                     uint32_t* __restrict__ stats)
+                                                     uint32_t* __restrict__ stats, uint32_t salt_add)
 {
     /* a device counter can advance the frame of replayed work */
-    if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + 1u);
+    if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + salt_add + 1u);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < *count_ptr;
     const uint32_t first = block_append<GI_BLOCK>(&counters[1], ok ? prm.spp : 0u);
@@ -1185,7 +1185,7 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         tgx_launch_scope s(ctx, "gi_gen_k");
         hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(t.max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, t.list,
                            t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k],
-                           two ? t.counters : nullptr);
+                           two ? t.counters : nullptr, t.salt_add);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
     /* shade, trace and lit use what the lists share: sun, radiance, range */
@@ -1252,7 +1252,7 @@ int tgx_gi_trace_list(tgx_context* ctx, const tgx_gi_params* prm, const uint32_t
                       float* irradiance_dev, const uint32_t* frame_salt)
 {
     const gi_trace_set t = {*prm, list, count_dev, max_count, cnt, e_entry, e_var, radiance_dev, irradiance_dev,
-                            frame_salt, nullptr};
+                            frame_salt, nullptr, 0u};
     return tgx_gi_trace_sets(ctx, &t, 1);
 }
 

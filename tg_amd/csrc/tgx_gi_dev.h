@@ -106,6 +106,7 @@ struct gi_trace_set {
     float*          irradiance_dev;
     const uint32_t* frame_salt;
     const gi_surf_rec* surf;        /* the list's surface records (null: the last surface pass's) */
+    uint32_t        salt_add;       /* added to *frame_salt (work drawn ahead of the counter's ticks) */
 };
 
 /* Traces several lists through one shared ray queue and one trace launch,

@@ -95,6 +95,7 @@ struct tgx_context {
     bool           nrc_ready = false;
     tgx_nrc_config nrc_cfg = {};
     bool           nrc_fb_traced = false; /* last frame traced a fallback list */
+    uint32_t       nrc_qc_off = 0;        /* u32 offset of the last camera's counters in nrc_queue */
     tgx_buffer     nrc_master, nrc_m, nrc_v;   /* fp32 [N_PARAMS]           */
     tgx_buffer     nrc_fwd, nrc_bwd; /* bf16 MFMA A-operand fragments       */
     tgx_buffer     nrc_partials;     /* fp32 [chunks][N_PARAMS]             */
@@ -159,6 +160,13 @@ int  tgx_visibility_record(tgx_context* ctx, const void* dev_cams, const tgx_cam
 /* The cache frame after visibility (vis_dev already holds the camera's words). */
 extern "C" int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32_t h, const tgx_gi_params* gi,
                             const uint64_t* vis_dev, float* radiance_dev, float* irradiance_dev);
+/* The cache frames of n_cams cameras after their visibility (camera k's
+ * words at vis_dev + k w h, outputs likewise): the same results as
+ * tgx_nrc_frame_from_vis camera after camera, with every camera's training
+ * targets traced in one launch. acc (may be null): frame totals. */
+extern "C" int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w,
+                                       uint32_t h, const tgx_gi_params* gi, const uint64_t* vis_dev,
+                                       float* radiance_dev, float* irradiance_dev, uint32_t* acc);
 int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
 void tgx_buffer_release(tgx_buffer* b);
 

@@ -530,9 +530,9 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_infer_k(const bf16x8* __restric
 /* Training pixels: entry j takes visible pixel hash(j, frame) % n_visible. */
 __global__ void nrc_pick_k(const uint32_t* __restrict__ vis_list, const uint32_t* __restrict__ n_vis_ptr,
                            uint32_t n_samples, const uint32_t* __restrict__ dev, uint32_t* __restrict__ out,
-                           uint32_t* __restrict__ count)
+                           uint32_t* __restrict__ count, uint32_t frame_add)
 {
-    const uint32_t frame = dev[NRC_DEV_FRAME];
+    const uint32_t frame = dev[NRC_DEV_FRAME] + frame_add;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nv = *n_vis_ptr;
     if (j == 0) *count = nv ? n_samples : 0u;
@@ -923,7 +923,7 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
     uint32_t n_sets = 0;
     if (n_train) {
         hipLaunchKernelGGL(nrc_pick_k, dim3(nrc_blocks(n_train, 256)), dim3(256), 0, ctx->stream, vis_list, n_vis,
-                           n_train, (const uint32_t*)ctx->nrc_dev.ptr, train_list, qc);
+                           n_train, (const uint32_t*)ctx->nrc_dev.ptr, train_list, qc, 0u);
         TGX_LAUNCH_CHECK(ctx, "nrc_pick_k");
         tgx_gi_params tp = *gi;
         tp.spp = c.train_spp;
@@ -931,11 +931,11 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
          * from the displayed frame's */
         tp.frame = gi->frame ^ 0x5BD1E995u;
         sets[n_sets++] = {tp, train_list, qc, n_train, gcnt + GI_CNT_TRAIN, e_train, v_train, nullptr, nullptr,
-                          (const uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME};
+                          (const uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME, nullptr, 0u};
     }
     if (fallback)
         sets[n_sets++] = {*gi, fb_list, qc + 1, n_pix, gcnt + GI_CNT_PLAIN, nullptr, nullptr, radiance_dev,
-                          irradiance_dev, nullptr};
+                          irradiance_dev, nullptr, nullptr, 0u};
     if (n_sets && (rc = tgx_gi_trace_sets(ctx, sets, n_sets))) return rc;
     if (n_train) {
         hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
@@ -949,6 +949,166 @@ int tgx_nrc_frame_from_vis(tgx_context* ctx, const tgx_camera* cam, uint32_t w, 
     hipLaunchKernelGGL(nrc_tick_k, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME);
     TGX_LAUNCH_CHECK(ctx, "nrc_tick_k");
     ctx->nrc_fb_traced = fallback;
+    ctx->nrc_qc_off = 0;
+    return TGX_OK;
+}
+
+/* Frame totals (tgx_frame_stats order: 0 visible, 1 bounce, 2 shadow,
+ * 3 hits, 4 training rays, 5 fallback pixels). With qc: the camera's
+ * visible pixels, and with mode 4 its fallback count; mode 2: the training
+ * block; mode 8: the fallback block. */
+__global__ void nrc_acc_k(const uint32_t* __restrict__ qc, const uint32_t* __restrict__ gcnt, uint32_t mode,
+                          uint32_t* __restrict__ acc)
+{
+    if (qc) acc[0] += gcnt[GI_CNT_VISIBLE];
+    if (mode & 8u) {
+        acc[1] += gcnt[GI_CNT_PLAIN + 1];
+        acc[2] += gcnt[GI_CNT_PLAIN + 3];
+        acc[3] += gcnt[GI_CNT_PLAIN + 2];
+    }
+    if (mode & 2u) acc[4] += gcnt[GI_CNT_TRAIN + 1] + gcnt[GI_CNT_TRAIN + 3];
+    if ((mode & 4u) && qc) acc[5] += qc[1];
+}
+
+/* d[0] = s[0] (one word between device buffers, capture-safe) */
+__global__ void nrc_copy1_k(const uint32_t* __restrict__ s, uint32_t* __restrict__ d) { *d = *s; }
+
+int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
+                            const tgx_gi_params* gi, const uint64_t* vis_dev, float* radiance_dev,
+                            float* irradiance_dev, uint32_t* acc)
+{
+    if (!ctx || !cams || !gi || !vis_dev || !radiance_dev || n_cams == 0) return TGX_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = check_ready(ctx))) return rc;
+    if ((rc = tgx_gi_validate(ctx, w, h, gi))) return rc;
+    const tgx_nrc_config& c = ctx->nrc_cfg;
+    const uint64_t px = (uint64_t)w * h;
+    const uint32_t n_train = (c.flags & TGX_NRC_FLAG_NO_TRAINING) ? 0u : c.train_samples;
+    if ((uint64_t)n_train * c.train_spp * n_cams >= (1ull << 31))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "train_samples * train_spp * cameras too large");
+    /* queue buffer: per camera 64 counters (0 train count, 1 fallback count,
+     * 8 loss), its training list and its fallback flags, scan and list */
+    const size_t q_words = (64ull + n_train + 3ull * px) * n_cams;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->nrc_queue, 4ull * q_words, "nrc queues")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_targets, 16ull * n_cams * (n_train ? n_train : 1) + 16, "nrc targets")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_CNT_TOTAL, "gi counters")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_bsurf, sizeof(gi_surf_rec) * n_cams * px, "gi surfaces per camera")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_blist, 12ull * n_cams * px, "gi pixel lists per camera")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->gi_bcount, 4ull * n_cams, "gi visible counts per camera")))
+        return rc;
+    uint32_t* q = (uint32_t*)ctx->nrc_queue.ptr;
+    uint32_t* train_lists = q + 64ull * n_cams;
+    uint32_t* fb_lists = train_lists + (size_t)n_train * n_cams;   /* camera k: flags, scan, list at 3 k px */
+    float* e_train = (float*)ctx->nrc_targets.ptr;
+    float* v_train = e_train + 3ull * n_train * n_cams;
+    nrc_cell* cells = (nrc_cell*)ctx->nrc_cells.ptr;
+    gi_surf_rec* surf = (gi_surf_rec*)ctx->gi_bsurf.ptr;
+    uint32_t* lists = (uint32_t*)ctx->gi_blist.ptr;
+    uint32_t* counts = (uint32_t*)ctx->gi_bcount.ptr;
+    uint32_t* gcnt = (uint32_t*)ctx->gi_counts.ptr;
+    const uint32_t* dev = (const uint32_t*)ctx->nrc_dev.ptr;
+    hipLaunchKernelGGL(nrc_fill_k, dim3(nrc_blocks(64ull * n_cams, 256)), dim3(256), 0, ctx->stream, q, 64ull * n_cams,
+                       0u);
+    TGX_LAUNCH_CHECK(ctx, "nrc_fill_k");
+
+    /* Every camera's surfaces, training pixels and training targets first:
+     * targets need no cache state, so one trace launch covers all cameras.
+     * Camera k draws what the k-th frame of a camera-by-camera sequence
+     * would: the frame counter is k ticks further there. */
+    std::vector<gi_trace_set> sets(n_train ? n_cams : 0u);
+    for (uint32_t k = 0; k < n_cams; k++) {
+        float* rad = radiance_dev + 3ull * k * px;
+        float* irr = irradiance_dev ? irradiance_dev + 3ull * k * px : nullptr;
+        if ((rc = tgx_gi_surfaces_at(ctx, &cams[k], w, h, vis_dev + k * px, rad, irr, surf + k * px,
+                                     lists + 3ull * k * px, counts + k)))
+            return rc;
+        if (!n_train) continue;
+        uint32_t* tl = train_lists + (size_t)n_train * k;
+        hipLaunchKernelGGL(nrc_pick_k, dim3(nrc_blocks(n_train, 256)), dim3(256), 0, ctx->stream, lists + 3ull * k * px,
+                           counts + k, n_train, dev, tl, q + 64ull * k, k);
+        TGX_LAUNCH_CHECK(ctx, "nrc_pick_k");
+        tgx_gi_params tp = *gi;
+        tp.spp = c.train_spp;
+        tp.frame = gi->frame ^ 0x5BD1E995u;
+        sets[k] = {tp, tl, q + 64ull * k, n_train, gcnt + GI_CNT_TRAIN, e_train + 3ull * n_train * k,
+                   v_train + (size_t)n_train * k, nullptr, nullptr, dev + NRC_DEV_FRAME, surf + k * px, k};
+    }
+    if (n_train && (rc = tgx_gi_trace_sets(ctx, sets.data(), n_cams))) return rc;
+    if (acc && n_train) {
+        hipLaunchKernelGGL(nrc_acc_k, dim3(1), dim3(1), 0, ctx->stream, (const uint32_t*)nullptr, (const uint32_t*)gcnt,
+                           2u, acc);
+        TGX_LAUNCH_CHECK(ctx, "nrc_acc_k");
+    }
+
+    /* Then camera by camera: inference with the cache as the previous
+     * camera left it, the fallback list, the cache's training step. The
+     * fallback pixels change no cache state, so all cameras' fallback lists
+     * are traced together at the end. */
+    const bool all_fb = (c.flags & TGX_NRC_FLAG_ALL_FALLBACK) != 0;
+    const bool use_cells = !(c.flags & TGX_NRC_FLAG_NO_FALLBACK) && !all_fb;
+    const bool fallback = use_cells || all_fb;
+    const float w_var = (c.flags & TGX_NRC_FLAG_FALLBACK_VS_TRACER)
+                            ? c.fallback_threshold * (float)c.train_spp / (float)gi->spp : -1.0f;
+    for (uint32_t k = 0; k < n_cams; k++) {
+        float* rad = radiance_dev + 3ull * k * px;
+        float* irr = irradiance_dev ? irradiance_dev + 3ull * k * px : nullptr;
+        const gi_surf_rec* sk = surf + k * px;
+        const uint32_t* vis_list = lists + 3ull * k * px;
+        const uint32_t* n_vis = counts + k;
+        uint32_t* qc = q + 64ull * k;
+        uint32_t* fb_flag = fb_lists + 3ull * k * px;
+        uint32_t* fb_off = fb_flag + px;
+        uint32_t* fb_list = fb_off + px;
+        if (!all_fb &&
+            (rc = launch_infer(ctx, sk, vis_list, n_vis, (uint32_t)px, nullptr, rad, irr, use_cells ? cells : nullptr,
+                               fb_flag, w_var)))
+            return rc;
+        if (fallback) {
+            if (all_fb) {
+                hipLaunchKernelGGL(nrc_fill_k, dim3(nrc_blocks(px, 256)), dim3(256), 0, ctx->stream, fb_flag, px, 1u);
+                TGX_LAUNCH_CHECK(ctx, "nrc_fill_k");
+            }
+            if ((rc = tgx_prim_exclusive_scan_u32(ctx, fb_flag, fb_off, (uint32_t)px))) return rc;
+            hipLaunchKernelGGL(nrc_fallback_list_k, dim3(nrc_blocks(px, 256)), dim3(256), 0, ctx->stream, fb_flag,
+                               fb_off, vis_list, n_vis, fb_list, qc + 1);
+            TGX_LAUNCH_CHECK(ctx, "nrc_fallback_list_k");
+        }
+        if (n_train) {
+            hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
+            TGX_LAUNCH_CHECK(ctx, "nrc_decay_k");
+            const uint32_t* tl = train_lists + (size_t)n_train * k;
+            for (uint32_t first = 0; first < n_train; first += c.train_batch)
+                if ((rc = train_step(ctx, sk, tl, qc, first, c.train_batch, c.train_batch, e_train + 3ull * n_train * k,
+                                     v_train + (size_t)n_train * k, cells, (float*)(qc + 8))))
+                    return rc;
+        }
+        hipLaunchKernelGGL(nrc_tick_k, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)ctx->nrc_dev.ptr + NRC_DEV_FRAME);
+        TGX_LAUNCH_CHECK(ctx, "nrc_tick_k");
+        /* the camera's visible count where the statistics look for it */
+        hipLaunchKernelGGL(nrc_copy1_k, dim3(1), dim3(1), 0, ctx->stream, n_vis, gcnt + GI_CNT_VISIBLE);
+        TGX_LAUNCH_CHECK(ctx, "nrc_copy1_k");
+        if (acc) {
+            hipLaunchKernelGGL(nrc_acc_k, dim3(1), dim3(1), 0, ctx->stream, (const uint32_t*)qc, (const uint32_t*)gcnt,
+                               fallback ? 4u : 0u, acc);
+            TGX_LAUNCH_CHECK(ctx, "nrc_acc_k");
+        }
+    }
+    if (fallback) {
+        std::vector<gi_trace_set> fs(n_cams);
+        for (uint32_t k = 0; k < n_cams; k++)
+            fs[k] = {*gi, fb_lists + 3ull * k * px + 2ull * px, q + 64ull * k + 1, (uint32_t)px, gcnt + GI_CNT_PLAIN,
+                     nullptr, nullptr, radiance_dev + 3ull * k * px,
+                     irradiance_dev ? irradiance_dev + 3ull * k * px : nullptr, nullptr, surf + k * px, 0u};
+        if ((rc = tgx_gi_trace_sets(ctx, fs.data(), n_cams))) return rc;
+        if (acc) {
+            hipLaunchKernelGGL(nrc_acc_k, dim3(1), dim3(1), 0, ctx->stream, (const uint32_t*)nullptr,
+                               (const uint32_t*)gcnt, 8u, acc);
+            TGX_LAUNCH_CHECK(ctx, "nrc_acc_k");
+        }
+    }
+    ctx->nrc_stats.train_samples = n_train;
+    ctx->nrc_fb_traced = fallback;
+    ctx->nrc_qc_off = 64u * (n_cams - 1u);
     return TGX_OK;
 }
 
@@ -969,7 +1129,8 @@ int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out)
     if (!ctx->nrc_queue.ptr || !ctx->gi_counts.ptr) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no cache frame yet");
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     uint32_t qc[64], gc[GI_CNT_TOTAL], dv[2];
-    TGX_HIP_CHECK(ctx, hipMemcpy(qc, ctx->nrc_queue.ptr, sizeof(qc), hipMemcpyDeviceToHost));
+    TGX_HIP_CHECK(ctx, hipMemcpy(qc, (const uint32_t*)ctx->nrc_queue.ptr + ctx->nrc_qc_off, sizeof(qc),
+                                 hipMemcpyDeviceToHost));
     TGX_HIP_CHECK(ctx, hipMemcpy(dv, ctx->nrc_dev.ptr, sizeof(dv), hipMemcpyDeviceToHost));
     TGX_HIP_CHECK(ctx, hipMemcpy(gc, ctx->gi_counts.ptr, sizeof(gc), hipMemcpyDeviceToHost));
     tgx_nrc_stats s = {};
