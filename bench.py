@@ -509,34 +509,44 @@ def counter_per_dispatch(db: Path, counter: str) -> dict:
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def measure_traffic(workload: str, kernel: str, steps: int = 2) -> dict | None:
-    """HBM bytes per launch of `kernel`: rocprofv3 FETCH_SIZE and WRITE_SIZE
-    passes (KiB per dispatch) over a probe run, each scaled by what the same
-    counter reads for a copy of COPY_BYTES (tgx_copy_k). Runs rocprofv3 as a
-    child process; returns None if it is unavailable or fails."""
+def pmc_pass(counters: list[str], workload: str, steps: int) -> dict | None:
+    """One rocprofv3 --pmc pass over a probe run: {counter: {kernel: mean
+    per-dispatch value}}, or None if rocprofv3 is unavailable or fails."""
     rp = shutil.which("rocprofv3")
     if rp is None:
         return None
+    log(f"rocprofv3 --pmc {' '.join(counters)} pass")
+    with tempfile.TemporaryDirectory(prefix="tgx_pmc_", dir="/tmp") as d:
+        cmd = [rp, "--pmc", *counters, "-d", d, "-o", "run", "--", sys.executable, str(ROOT / "bench.py"),
+               "--probe", "--workload", workload, "--steps", str(steps)]
+        proc = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                                stderr=subprocess.DEVNULL, start_new_session=True)
+        try:
+            proc.wait(timeout=150)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            log("pass timed out")
+            return None
+        dbs = sorted(Path(d).rglob("*.db"))
+        if not dbs:
+            log(f"pass left no results (exit {proc.returncode})")
+            return None
+        return {c: counter_per_dispatch(dbs[0], c) for c in counters}
+
+
+def measure_traffic(workload: str, kernel: str, steps: int = 2) -> dict | None:
+    """HBM bytes per launch of `kernel`: rocprofv3 FETCH_SIZE and WRITE_SIZE
+    passes (KiB per dispatch) over a probe run, each scaled by what the same
+    counter reads for a copy of COPY_BYTES (tgx_copy_k); then the kernel's L2
+    hit rate from TCC_HIT / TCC_MISS (one more pass). Runs rocprofv3 as a
+    child process; returns None if it is unavailable or fails."""
     got = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        log(f"rocprofv3 --pmc {counter} pass")
-        with tempfile.TemporaryDirectory(prefix="tgx_pmc_", dir="/tmp") as d:
-            cmd = [rp, "--pmc", counter, "-d", d, "-o", "run", "--", sys.executable, str(ROOT / "bench.py"),
-                   "--probe", "--workload", workload, "--steps", str(steps)]
-            proc = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
-                                    stderr=subprocess.DEVNULL, start_new_session=True)
-            try:
-                proc.wait(timeout=150)
-            except subprocess.TimeoutExpired:
-                os.killpg(proc.pid, signal.SIGKILL)
-                proc.wait()
-                log(f"{counter} pass timed out")
-                return None
-            dbs = sorted(Path(d).rglob("*.db"))
-            if not dbs:
-                log(f"{counter} pass left no results (exit {proc.returncode})")
-                return None
-            vals = counter_per_dispatch(dbs[0], counter)
+        r = pmc_pass([counter], workload, steps)
+        if r is None:
+            return None
+        vals = r[counter]
         if kernel not in vals or "tgx_copy_k" not in vals or vals["tgx_copy_k"] <= 0:
             log(f"{counter}: kernels {sorted(vals)} lack {kernel} or the copy")
             return None
@@ -545,9 +555,32 @@ def measure_traffic(workload: str, kernel: str, steps: int = 2) -> dict | None:
     w_k, w_c = got["WRITE_SIZE"]
     fetch_scale = COPY_BYTES / 1024.0 / f_c
     write_scale = COPY_BYTES / 1024.0 / w_c
-    return {"bytes": 1024.0 * (f_k * fetch_scale + w_k * write_scale),
-            "fetch_kib": f_k, "write_kib": w_k, "copy_fetch_kib": f_c, "copy_write_kib": w_c,
-            "copy_bytes": COPY_BYTES, "fetch_scale": fetch_scale, "write_scale": write_scale}
+    out = {"bytes": 1024.0 * (f_k * fetch_scale + w_k * write_scale),
+           "fetch_kib": f_k, "write_kib": w_k, "copy_fetch_kib": f_c, "copy_write_kib": w_c,
+           "copy_bytes": COPY_BYTES, "fetch_scale": fetch_scale, "write_scale": write_scale}
+    r = pmc_pass(["TCC_HIT_sum", "TCC_MISS_sum"], workload, steps)
+    if r is not None and kernel in r["TCC_HIT_sum"] and kernel in r["TCC_MISS_sum"]:
+        hit, miss = r["TCC_HIT_sum"][kernel], r["TCC_MISS_sum"][kernel]
+        out["l2_hits"], out["l2_misses"] = hit, miss
+        out["l2_hit_rate"] = hit / (hit + miss) if hit + miss > 0 else None
+    return out
+
+
+def copy_bandwidth(ctx, nbytes: int = COPY_BYTES, reps: int = 10) -> float:
+    """Attainable HBM bandwidth on this box: a device-to-device copy by the
+    library's copy kernel, read + written bytes over the wall time."""
+    a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    try:
+        ctx.d2d(b, a, nbytes)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.d2d(b, a, nbytes)
+        ctx.synchronize()
+        return 2.0 * nbytes * reps / (time.perf_counter() - t0) / 1e9
+    finally:
+        ctx.free(a)
+        ctx.free(b)
 
 
 def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 12.0) -> dict:
@@ -585,10 +618,24 @@ def cpu_baseline(scene, params, cam, mn, prm, workload: str, budget_s: float = 1
         t_gi += time.perf_counter() - t0
         rays += W * H + int(cnt[1] + cnt[3])
         frames += 1
+    # one thread, every 16th row of one frame (visibility and GI)
+    t0 = time.perf_counter()
+    vis1, _ = oracle.visibility_culled(oscene, cam, W, H, row_step=16, n_threads=1)
+    _, _, cnt1 = svo.gi(cam, W, H, vis1, prm, row_step=16)
+    t1 = time.perf_counter() - t0
+    rays1 = W * ((H + 15) // 16) + int(cnt1[1] + cnt1[3])
+    cpu_model = ""
+    try:
+        cpu_model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": rays / (t_vis + t_gi), "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{frames} whole 1920x1080 frames of the oracle (visibility {t_vis:.2f} s, GI at {SPP} spp "
                       f"{t_gi:.2f} s, {rays} primary+bounce+shadow rays) on {threads} threads; oracle SVO build "
-                      f"{t_build:.2f} s"}
+                      f"{t_build:.2f} s",
+            "single_thread": {"value": rays1 / t1, "unit": "rays/s",
+                              "sample": f"every 16th row of one frame, visibility and GI, {rays1} rays, {t1:.2f} s"},
+            "cpu_model": cpu_model, "host_cpus": os.cpu_count()}
 
 
 def main() -> int:
@@ -735,6 +782,8 @@ def main() -> int:
                                           args.neural_pretrain, ms_per_step, rays_per_frame)
     for p in (vis_d, rad_d, irr_d):
         ctx.free(p)
+    # the roofline's attainable counterpart: a streaming copy on this box
+    line["roofline"]["attainable_copy_gbs"] = copy_bandwidth(ctx)
     if args.workload == "frame":
         log("dynamic scene edits")
         line["dynamic"] = dynamic_bench(ctx, scene, mn, DEPTH)
@@ -748,6 +797,7 @@ def main() -> int:
         if tr is not None:
             line["roofline"]["traffic"] = tr["bytes"]
             line["roofline"]["traffic_detail"] = tr
+            line["roofline"]["l2_hit_rate"] = tr.get("l2_hit_rate")
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(scene, params, cam, mn, prm, args.workload)
