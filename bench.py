@@ -160,6 +160,22 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     infer_tflops = NRC_FWD_FLOPS * visible / (infer_ms * 1e-3) / 1e12 if infer_ms else None
     train_tflops = NRC_TRAIN_FLOPS * int(st["train_samples"]) / (train_ms * 1e-3) / 1e12 if train_ms else None
     traced = W * H + int(st["train_rays"]) + int(st["fallback_rays"])
+    # HBM roofline of the frame's trace launch, the same algorithmic bytes as
+    # the plain frame's (trace_alg_bytes): bounce rays are the listed entries
+    # times spp, the rest of each count are shadow rays
+    t_bounce = int(st["train_samples"]) * 4  # train_spp of the cache configuration above
+    f_bounce = int(st["fallback_pixels"]) * SPP
+    rays_b = t_bounce + f_bounce
+    rays_s = int(st["train_rays"]) + int(st["fallback_rays"]) - rays_b
+    info = ctx.svo_info()
+    n_launch, tr_ms = prof.get("gi_trace_k", (0, 0.0))
+    tr_avg = tr_ms / n_launch if n_launch else None
+    alg = ((32 * rays_b + 14 * rays_s) * steps / n_launch + 32 * int(info["n_nodes"]) + 4096 * int(info["n_leaves"])
+           if n_launch else None)
+    achieved = alg / (tr_avg * 1e-3) / 1e9 if alg and tr_avg else None
+    hbm = {"bound": "hbm", "kernel": "gi_trace_k", "kernel_avg_ms": tr_avg, "launches_per_frame": n_launch / steps,
+           "alg_bytes_per_launch": alg, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS if achieved else None}
     return {
         "frame_ms": ms,
         "plain_frame_ms": plain_ms,
@@ -187,6 +203,7 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
                  "train_frac": train_tflops / MFMA_BF16_PEAK_TFLOPS if train_tflops else None,
                  "flops_per_sample": {"forward": NRC_FWD_FLOPS, "train": NRC_TRAIN_FLOPS}},
         "captured_graphs": captured,
+        "roofline": hbm,
         "config": {"network": "96(79 used)->64x4->3 ReLU, bf16 MFMA, fp32 Adam lr 1e-3", "train_samples": 1 << 16,
                    "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512"},
     }
