@@ -12,6 +12,8 @@
  * device counters. While a graph exists the context neither grows nor frees
  * buffers, since the graph holds their addresses.
  */
+#include <algorithm>
+
 #include "tgx_gi_dev.h"
 
 #include <chrono>
@@ -85,37 +87,55 @@ static int frame_body(tgx_context* ctx, tgx_frame_graph* g, bool record)
         const uint32_t nf = ctx->nrc_cfg.flags;
         mode = ((nf & TGX_NRC_FLAG_NO_FALLBACK) ? 0u : 5u) | ((nf & TGX_NRC_FLAG_NO_TRAINING) ? 0u : 2u);
     }
-    const uint64_t samples = (uint64_t)n * px * g->gi.spp;
-    if (!neural && n > 1 && samples < (1ull << 31)) {
-        /* Plain GI of every camera through one ray queue and one trace
-         * launch: each camera's surfaces and pixel list in its own slice. */
+    /* Cameras go through the merged paths in groups whose per-camera state
+     * stays bounded: at most 2^26 pixels (32 cameras at 1080p) and 2^28
+     * bounce samples per group. */
+    const uint64_t by_px = (1ull << 26) / px, by_samples = (1ull << 28) / (px * g->gi.spp);
+    const uint32_t group = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>({(uint64_t)n, by_px, by_samples}));
+    if (!neural && n > 1 && group > 1) {
+        /* Plain GI of every camera of a group through one ray queue and one
+         * trace launch: each camera's surfaces and pixel list in its own
+         * slice. */
         if ((rc = tgx_buffer_reserve(ctx, &ctx->gi_counts, 4ull * GI_CNT_TOTAL, "gi counters")) ||
-            (rc = tgx_buffer_reserve(ctx, &ctx->gi_bsurf, sizeof(gi_surf_rec) * n * px, "gi surfaces per camera")) ||
-            (rc = tgx_buffer_reserve(ctx, &ctx->gi_blist, 12ull * n * px, "gi pixel lists per camera")) ||
-            (rc = tgx_buffer_reserve(ctx, &ctx->gi_bcount, 4ull * n, "gi visible counts per camera")))
+            (rc = tgx_buffer_reserve(ctx, &ctx->gi_bsurf, sizeof(gi_surf_rec) * group * px, "gi surfaces per camera")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->gi_blist, 12ull * group * px, "gi pixel lists per camera")) ||
+            (rc = tgx_buffer_reserve(ctx, &ctx->gi_bcount, 4ull * group, "gi visible counts per camera")))
             return rc;
         gi_surf_rec* surf = (gi_surf_rec*)ctx->gi_bsurf.ptr;
         uint32_t* lists = (uint32_t*)ctx->gi_blist.ptr;
         uint32_t* counts = (uint32_t*)ctx->gi_bcount.ptr;
         uint32_t* cnt = (uint32_t*)ctx->gi_counts.ptr;
-        std::vector<gi_trace_set> sets(n);
-        for (uint32_t c = 0; c < n; c++) {
-            float* rad = g->rad + 3 * c * px;
-            float* irr = g->irr ? g->irr + 3 * c * px : nullptr;
-            if ((rc = tgx_gi_surfaces_at(ctx, &g->cams[c], g->w, g->h, g->vis + c * px, rad, irr, surf + c * px,
-                                         lists + 3 * c * px, counts + c)))
-                return rc;
-            sets[c] = {g->gi, lists + 3 * c * px, counts + c, (uint32_t)px, cnt + GI_CNT_PLAIN, nullptr, nullptr, rad,
-                       irr, nullptr, surf + c * px, 0u};
+        for (uint32_t c0 = 0; c0 < n; c0 += group) {
+            const uint32_t m = std::min(group, n - c0);
+            std::vector<gi_trace_set> sets(m);
+            for (uint32_t k = 0; k < m; k++) {
+                const uint32_t c = c0 + k;
+                float* rad = g->rad + 3 * c * px;
+                float* irr = g->irr ? g->irr + 3 * c * px : nullptr;
+                if ((rc = tgx_gi_surfaces_at(ctx, &g->cams[c], g->w, g->h, g->vis + c * px, rad, irr, surf + k * px,
+                                             lists + 3 * k * px, counts + k)))
+                    return rc;
+                sets[k] = {g->gi, lists + 3 * k * px, counts + k, (uint32_t)px, cnt + GI_CNT_PLAIN, nullptr, nullptr,
+                           rad, irr, nullptr, surf + k * px, 0u};
+            }
+            if ((rc = tgx_gi_trace_sets(ctx, sets.data(), m))) return rc;
+            hipLaunchKernelGGL(gi_acc_all_k, dim3(1), dim3(64), 0, ctx->stream, (const uint32_t*)counts, m,
+                               (const uint32_t*)cnt, g->acc);
+            TGX_LAUNCH_CHECK(ctx, "gi_acc_all_k");
         }
-        if ((rc = tgx_gi_trace_sets(ctx, sets.data(), n))) return rc;
-        hipLaunchKernelGGL(gi_acc_all_k, dim3(1), dim3(64), 0, ctx->stream, (const uint32_t*)counts, n,
-                           (const uint32_t*)cnt, g->acc);
-        TGX_LAUNCH_CHECK(ctx, "gi_acc_all_k");
         return TGX_OK;
     }
-    if (neural && n > 1 && !(ctx->nrc_cfg.flags & TGX_NRC_FLAG_NO_TRAINING))
-        return tgx_nrc_frames_from_vis(ctx, g->cams.data(), n, g->w, g->h, &g->gi, g->vis, g->rad, g->irr, g->acc);
+    if (neural && n > 1 && !(ctx->nrc_cfg.flags & TGX_NRC_FLAG_NO_TRAINING)) {
+        /* groups run in camera order, so the result is the camera-by-camera
+         * sequence's either way */
+        for (uint32_t c0 = 0; c0 < n; c0 += group) {
+            const uint32_t m = std::min(group, n - c0);
+            if ((rc = tgx_nrc_frames_from_vis(ctx, g->cams.data() + c0, m, g->w, g->h, &g->gi, g->vis + c0 * px,
+                                              g->rad + 3 * c0 * px, g->irr ? g->irr + 3 * c0 * px : nullptr, g->acc)))
+                return rc;
+        }
+        return TGX_OK;
+    }
     for (uint32_t c = 0; c < n; c++) {
         const uint64_t* vis = g->vis + c * px;
         float* rad = g->rad + 3 * c * px;
