@@ -55,6 +55,7 @@
 
 #define TRACE_BLOCK 1024
 #define DDA_BURST 8u
+#define TRACE_ADV_MIN 16u   /* lanes of a wave that make it run the advances */
 #define TRACE_LDS_NODES_MAX 8000u   /* inner nodes + leaf brick masks in LDS (64000 B) */
 
 /* ------------------------------------------------------------------------ */
@@ -173,6 +174,7 @@ struct trace_lane {
 #define ND_LVL(n) (((n) >> 21) & 7u)
 #define ND_RC(n)  (((n) >> 24) & 7u)
 #define ND_DDA    (1u << 27)
+#define ND_ADV    (1u << 28)   /* an advance out of the child slot is pending */
 
 __device__ static inline svo_res res_miss(uint32_t steps)
 {
@@ -275,8 +277,10 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
                                         trace_lane& L, svo_res& res, bool active, bool allow_node, bool allow_dda)
 {
     bool adv = false, done = false;
-    allow_node = allow_node && active;
-    allow_dda = allow_dda && active;
+    /* a lane with a pending advance waits for the wave's next advance */
+    const bool pend = active && (L.node & ND_ADV);
+    allow_node = allow_node && active && !pend;
+    allow_dda = allow_dda && active && !pend;
     if (!(L.node & ND_DDA) && allow_node && (L.cur == SVO_NONE || L.steps >= TGX_SVO_MAX_STEPS || L.dist > maxd)) {
         res = res_miss(L.steps);
         done = true;
@@ -412,10 +416,18 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         res.pad = 0u;
         done = true;
     }
+    /* Advances run for many lanes at once: a lane's advance waits (ND_ADV)
+     * until enough of the wave's active lanes want one. */
+    const bool want = adv || pend;
+    const uint32_t n_want = (uint32_t)__popcll(__ballot(want));
+    const uint32_t n_act = (uint32_t)__popcll(__ballot(active));
+    const bool run_adv = n_want >= TRACE_ADV_MIN || 2u * n_want >= n_act;
+    if (!run_adv && adv) L.node |= ND_ADV;
+    adv = run_adv && want;
     if (__any(adv)) {
         /* leave the child box, then pop every node the position has left
          * (wave-uniform loop, lanes predicated) */
-        uint32_t n = L.node;
+        uint32_t n = L.node & ~ND_ADV;
         uint32_t cur = L.cur;
         if (adv) {
             const float cs = (float)(1u << (P.depth - ND_LVL(n) - 1u));
