@@ -455,14 +455,14 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
     return done;
 }
 
-/* Bounce queue traced with its shadow rays in the same launch (GI
- * estimator, one or two lists): when a lane's bounce ray hits a face that
+/* Fused GI tracing (gi_trace_k<*, true>): the bounce queue traced with its
+ * shadow rays in the same launch. When a lane's bounce ray hits a face that
  * faces the sun, the lane writes the sample's albedo-weighted contribution
  * and continues with the shadow ray itself; an occluded shadow ray clears
- * the contribution again. Same arithmetic as gi_shade_k + gi_lit_k.
- * contrib == nullptr: plain queue tracing. Ray ids at or above split belong
- * to the second list; per list, bounce hits go to stats[2], shadow rays to
- * stats[3] and shadow hits to stats[7]. */
+ * the contribution again. Same arithmetic as gi_shade_k + gi_lit_k. Ray
+ * ids at or above split count for the second counter block (stats1), the
+ * others for stats0 (both may be the same block): bounce hits go to
+ * stats[2], shadow rays to stats[3] and shadow hits to stats[7]. */
 struct trace_fuse {
     const uint16_t*      leaf_mats;
     const tgx_lut_entry* luts;
@@ -476,9 +476,11 @@ struct trace_fuse {
 
 #define TRACE_SHADOW 0x80000000u   /* slot of a lane tracing a shadow ray: id | TRACE_SHADOW */
 
-/* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd; the
- * result of ray i goes to out[i] (plain tracing). `cursor` must be 0 at
- * launch; hits (flag bit 0) are added to *hit_counter when it is not null. */
+/* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd.
+ * Plain tracing (kFused false): the result of ray i goes to out[i] and hits
+ * (flag bit 0) are added to *hit_counter when it is not null. Fused GI
+ * (kFused true): see trace_fuse; out and hit_counter are not used. `cursor`
+ * must be 0 at launch. */
 template <bool kLds, bool kFused>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
