@@ -160,6 +160,14 @@ int tgx_scene_export(tgx_context* ctx, tgx_object* objects, uint32_t* cluster_po
  * a dynamic scene: n_objects and n_clusters are the capacities (records of
  * free object slots are zero, tg_object_is_initialized at
  * tgvk_raytracer.c:1069-1077) and n_cluster_pointers the live pointers. */
+/* Sets one LUT entry of the scene (albedo and roughness, fp32): the
+ * counterpart of tg_raytracer_color_lut_set (tgvk_raytracer.c:1122-1142),
+ * which packs an RGBA8 colour into its single LUT. Entries are indexed
+ * (lut_idx, entry < 256); SVO leaves store indices, so no rebuild is
+ * needed, and frames (captured ones too) read the new value from their next
+ * launch on. TGX_ERR_INVALID_ARG outside the scene's LUTs. */
+int tgx_lut_set(tgx_context* ctx, uint32_t lut_idx, uint32_t entry, const tgx_lut_entry* value);
+
 typedef struct tgx_scene_counts {
     uint32_t n_objects;
     uint32_t n_cluster_pointers;

@@ -97,3 +97,31 @@ def test_pick_resolves_pointer_index_object_and_position(ctx):
             ctx.pick(vis_d, w, h, [[w, 0]])
     finally:
         ctx.free(vis_d)
+
+
+def test_lut_set_changes_one_entry(ctx):
+    """tgx_lut_set (tg_raytracer_color_lut_set) replaces one entry of the
+    device LUTs; the colour view then shows it for the voxels that use it."""
+    p = T.single_object_params(1)
+    sc = T.generate_scene(p)
+    ctx.upload(sc)
+    cam = T.generate_camera(p, 0, 96, 64)
+    vis = ctx.visibility(cam, 96, 64)[0]
+    before = ctx.shade(cam, 96, 64, vis, T.shade_params(T.SHADE_COLOR))
+    hit = vis != T.VISIBILITY_EMPTY
+    _, _, cptr, vox = T.decode(int(vis[hit][0]))
+    cidx = int(sc.cluster_pointers[cptr])
+    mat = int(sc.materials[512 * cidx + vox])
+    ctx.lut_set(0, mat, (0.25, 0.5, 0.75), 0.125)
+    luts = ctx.scene_arrays().luts
+    ref = sc.luts.copy()
+    ref[mat]["albedo"] = (0.25, 0.5, 0.75)
+    ref[mat]["roughness"] = 0.125
+    assert luts.tobytes() == ref.tobytes()
+    after = ctx.shade(cam, 96, 64, vis, T.shade_params(T.SHADE_COLOR))
+    changed = (after != before).any(-1)
+    assert changed.any() and changed[hit].sum() == changed.sum()
+    with pytest.raises(T.TgxError):
+        ctx.lut_set(1, 0, (0, 0, 0), 0)  # one LUT
+    with pytest.raises(T.TgxError):
+        ctx.lut_set(0, 256, (0, 0, 0), 0)

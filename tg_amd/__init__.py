@@ -221,6 +221,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_gen_stream_seed": (u64, [u64, u64]),
         "tgx_svo_build": (i32, [vp, vp, u32, u32]),
         "tgx_svo_update": (i32, [vp, vp, vp]),
+        "tgx_lut_set": (i32, [vp, u32, u32, vp]),
         "tgx_svo_get_info": (i32, [vp, vp]),
         "tgx_svo_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "tgx_svo_trace": (i32, [vp, vp, vp, u32, ctypes.c_float, vp]),
@@ -484,6 +485,13 @@ class Context:
     def svo_build(self, mn, depth: int, flags: int = 0) -> None:
         m = np.ascontiguousarray(mn, np.float32)
         self._check(self.lib.tgx_svo_build(self.h, _ptr(m), depth, flags), "tgx_svo_build")
+
+    def lut_set(self, lut_idx: int, entry: int, albedo, roughness: float) -> None:
+        """tgx_lut_set: one LUT entry (tg_raytracer_color_lut_set)."""
+        e = np.zeros(1, LUT_DTYPE)
+        e["albedo"] = albedo
+        e["roughness"] = roughness
+        self._check(self.lib.tgx_lut_set(self.h, lut_idx, entry, _ptr(e)), "tgx_lut_set")
 
     def svo_update(self) -> tuple[int, int]:
         """tgx_svo_update: rebuild after scene edits; (reused, voxelized) leaves."""

@@ -212,6 +212,23 @@ int tgx_object_destroy(tgx_context* ctx, uint32_t o)
     return TGX_OK;
 }
 
+int tgx_lut_set(tgx_context* ctx, uint32_t lut_idx, uint32_t entry, const tgx_lut_entry* value)
+{
+    if (!ctx || !value) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene");
+    if (lut_idx >= ctx->n_luts || entry >= TGX_LUT_ENTRIES)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "LUT %u entry %u outside %u LUTs of %u entries", lut_idx, entry,
+                        ctx->n_luts, TGX_LUT_ENTRIES);
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    /* stream-ordered: frames queued before see the old entry, later ones
+     * the new; the copy waits for its host source */
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync((tgx_lut_entry*)ctx->luts.ptr + (size_t)lut_idx * TGX_LUT_ENTRIES + entry, value,
+                                      sizeof(tgx_lut_entry), hipMemcpyHostToDevice, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TGX_OK;
+}
+
 int tgx_scene_get_counts(tgx_context* ctx, tgx_scene_counts* out)
 {
     if (!ctx || !out) return TGX_ERR_INVALID_ARG;
