@@ -189,6 +189,19 @@ __device__ static inline bool lane_init_self(const trace_params& P, float half, 
 {
     const float ox = L.px - P.cx, oy = L.py - P.cy, oz = L.pz - P.cz;
     L.steps = 0;
+    L.cur = 0;
+    L.node = 0;
+    L.tdx = dda_len(1.0f, 1.0f, L.dx);
+    L.tdy = dda_len(1.0f, 1.0f, L.dy);
+    L.tdz = dda_len(1.0f, 1.0f, L.dz);
+    if (__all(fabsf(ox) < half && fabsf(oy) < half && fabsf(oz) < half)) {
+        /* origins strictly inside the root box: every near-plane quotient is
+         * <= 0 and the far one > 0, so the slab test below would keep the
+         * origin with distance 0 */
+        L.px = ox; L.py = oy; L.pz = oz;
+        L.dist = 0.0f;
+        return true;
+    }
     const float enter = fmax2(fmax2(axis_near(ox, L.dx, -half, half), axis_near(oy, L.dy, -half, half)),
                               axis_near(oz, L.dz, -half, half));
     const float exit_ = fmin2(fmin2(axis_far(ox, L.dx, -half, half), axis_far(oy, L.dy, -half, half)),
@@ -197,11 +210,6 @@ __device__ static inline bool lane_init_self(const trace_params& P, float half, 
     L.px = ox; L.py = oy; L.pz = oz;
     if (enter > 0.0f) { L.px = ox + L.dx * enter; L.py = oy + L.dy * enter; L.pz = oz + L.dz * enter; }
     L.dist = enter > 0.0f ? enter : 0.0f;
-    L.cur = 0;
-    L.node = 0;
-    L.tdx = dda_len(1.0f, 1.0f, L.dx);
-    L.tdy = dda_len(1.0f, 1.0f, L.dy);
-    L.tdz = dda_len(1.0f, 1.0f, L.dz);
     return true;
 }
 
