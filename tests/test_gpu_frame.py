@@ -158,12 +158,19 @@ def test_graph_locks_context_buffers(batch_scene):
         ctx.close()
 
 
-def test_frame_graph_neural_matches_uncaptured(ctx, batch_scene):
+@pytest.mark.parametrize("nrc_flags", [0, T.NRC_FLAG_FALLBACK_VS_TRACER, T.NRC_FLAG_NO_FALLBACK,
+                                       T.NRC_FLAG_ALL_FALLBACK])
+def test_frame_graph_neural_matches_uncaptured(ctx, batch_scene, nrc_flags):
+    """A captured multi-camera cache frame (training targets of every camera
+    in one launch, fallback pixels of every camera in another) equals the
+    cache frames run camera by camera, outputs and weights bit for bit, for
+    each fallback mode."""
     p, sc, cams, mn = batch_scene
     setup_scene(ctx, p, mn)
     prm = T.gi_params(T.seeded_sun(4), spp=1, frame=5)
     n_cams, replays = 3, 2
-    cfg = T.nrc_config(origin=tuple(mn), scale=1.0 / (1 << DEPTH), train_samples=4096, train_batch=2048, seed=9)
+    cfg = T.nrc_config(origin=tuple(mn), scale=1.0 / (1 << DEPTH), train_samples=4096, train_batch=2048, seed=9,
+                       flags=nrc_flags)
     steps_per_frame = n_cams * 4096 // 2048
     buf = Buffers(ctx, n_cams, W, H)
     try:
