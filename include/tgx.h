@@ -204,8 +204,9 @@ int tgx_object_create(tgx_context* ctx, const uint32_t n_clusters_per_dim[3], co
  * object's cluster indices back in pointer order, shifts every later
  * cluster pointer down over the gap (on the device), lowers the first
  * pointer of every later object, clears the record and pushes the object
- * index back. The SVO is not updated: build it again before tracing. Both
- * calls are refused while a frame graph exists. */
+ * index back. Neither call touches the SVO: run tgx_svo_update (or
+ * tgx_svo_build) before tracing the edited scene. Both calls are refused
+ * while a frame graph exists. */
 int tgx_object_destroy(tgx_context* ctx, uint32_t object_idx);
 
 /* Scene files (host only; no context needed). tg keeps its scene in memory
@@ -335,6 +336,24 @@ int tgx_svo_get_info(tgx_context* ctx, tgx_svo_info* out);
 int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes, uint32_t* leaf_bits,
                    uint16_t* leaf_materials, uint32_t* leaf_n, uint32_t* list_offsets,
                    uint32_t* list);
+
+/* SVO files. tg builds its SVO in memory on the first frame and keeps no
+ * copy (tg_svo_create, tgvk_raytracer.c:1187-1196); a saved SVO loads in
+ * place of that build. save writes the nodes, leaf lists, per-leaf bits and
+ * the materials of solid voxels only, with an FNV-1a 64 checksum;
+ * TGX_ERR_NOT_BUILT without an SVO, TGX_ERR_IO if the file cannot be
+ * written. load needs the scene the SVO was built from (its cluster and LUT
+ * counts are checked) and no frame graph; it checks the checksum, the file
+ * size and every index the traversal follows (child ranges, parent links,
+ * node levels and coordinates, leaf lists, cluster and LUT indices) and
+ * fails with TGX_ERR_INVALID_ARG on any mismatch, TGX_ERR_IO if the file
+ * cannot be opened. Every check, the checksum included, runs before the
+ * device is touched, so a rejected file leaves the context's SVO as it was;
+ * only a device error or a file changed during the load leaves no SVO built.
+ * The loaded SVO exports and traces as the saved one did; the next
+ * tgx_svo_update samples every leaf. */
+int tgx_svo_save(tgx_context* ctx, const char* path);
+int tgx_svo_load(tgx_context* ctx, const char* path);
 
 /* Traces n rays (world-space origins and directions, 3 floats each)
  * through the SVO; a hit farther than max_distance is reported as a miss.

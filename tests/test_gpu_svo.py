@@ -150,3 +150,140 @@ def test_svo_visibility_matches_oracle(ctx, w, h):
     bad = got != ref
     assert not bad.any(), f"{int(bad.sum())} words differ, first at {np.argwhere(bad)[:3].tolist()}"
     assert (got != T.VISIBILITY_EMPTY).mean() > 0.3
+
+
+def fnv1a64(data: bytes) -> int:
+    h = 0xCBF29CE484222325
+    for b in data:
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def resealed(data: bytearray) -> bytes:
+    """The file with its trailing FNV-1a 64 checksum recomputed."""
+    body = bytes(data[:-8])
+    return body + np.uint64(fnv1a64(body)).tobytes()
+
+
+SVO_HEADER_BYTES = 64   # magic, version, n_inner, tgx_svo_info (32 B), n_clusters, n_luts, n_solid
+SVO_NODE_BYTES = 32
+
+
+def test_svo_file_round_trip(ctx, tmp_path):
+    """tgx_svo_save / tgx_svo_load: a loaded SVO exports, traces and renders
+    GI exactly as the built one, and equals the oracle's build."""
+    p = T.single_object_params(1)
+    sc = T.generate_scene(p)
+    cam = T.generate_camera(p, 0, 96, 64)
+    mn = np.array([-64.0, -64.0, -64.0], np.float32)
+    prm = T.gi_params(T.seeded_sun(1), spp=2)
+    ctx.upload(sc)
+    say("SVO file: build depth 7, save")
+    ctx.svo_build(mn, 7)
+    built = ctx.svo_export()
+    f1, f2 = tmp_path / "a.tgxsvo", tmp_path / "b.tgxsvo"
+    ctx.svo_save(f1)
+    vis, rad, irr = ctx.frame(cam, 96, 64, prm)
+    rng = np.random.default_rng(5)
+    o, d = rays_from(rng, 3000, -80, 80, centre=(0.0, 0.0, 0.0))
+    hits = ctx.svo_trace(o, d, 300.0)
+    with T.Context(0) as c2:
+        c2.upload(sc)
+        say("SVO file: load into a fresh context")
+        c2.svo_load(f1)
+        loaded = c2.svo_export()
+        ref = O.OracleSvo(O.OracleScene(sc), mn, 7).export()
+        assert_same_svo(loaded, ref, "loaded vs oracle")
+        for k in ("nodes", "bits", "materials", "leaf_n", "list_offsets", "list"):
+            assert np.array_equal(loaded[k], built[k]), f"loaded {k} differs from the build"
+        assert loaded["info"].tobytes() == built["info"].tobytes()
+        assert_same_hits(c2.svo_trace(o, d, 300.0), hits, "loaded trace")
+        v2, r2, i2 = c2.frame(cam, 96, 64, prm)
+        assert v2.tobytes() == vis.tobytes() and r2.tobytes() == rad.tobytes() and i2.tobytes() == irr.tobytes()
+        c2.svo_save(f2)
+        assert f1.read_bytes() == f2.read_bytes(), "save of a loaded SVO is not the same file"
+        say("SVO file: update after load samples every leaf")
+        reused, vox = c2.svo_update()
+        assert reused == 0 and vox == int(built["info"]["n_leaves"])
+        assert_same_svo(c2.svo_export(), ref, "update after load")
+    assert rad.any()
+
+
+def test_svo_file_rejects_bad_files(ctx, tmp_path):
+    """Corrupt, truncated, foreign and mismatched files fail with an error
+    before anything reaches the device, and the context keeps its SVO."""
+    objs = [{"n": (2, 2, 2), "translation": (40.0, 40.0, 40.0)},
+            {"n": (1, 3, 2), "translation": (90.0, 70.0, 60.0)}]
+    sc = S.make_scene(objs, seed=4)
+    ctx.upload(sc)
+    ctx.svo_build((0.0, 0.0, 0.0), 8)
+    good = ctx.svo_export()
+    path = tmp_path / "s.tgxsvo"
+    ctx.svo_save(path)
+    raw = bytearray(path.read_bytes())
+    info = good["info"]
+    n_nodes, nl = int(info["n_nodes"]), int(info["n_leaves"])
+    n_inner = n_nodes - nl
+    assert nl > 0 and int(info["n_list_entries"]) > 0
+
+    def expect(data: bytes, code: int, words: str, name: str) -> None:
+        q = tmp_path / name
+        q.write_bytes(data)
+        with pytest.raises(T.TgxError) as e:
+            ctx.svo_load(q)
+        assert e.value.code == code, (name, str(e.value))
+        assert words in str(e.value), (name, str(e.value))
+
+    say("SVO file: rejected files")
+    flipped = bytearray(raw)
+    flipped[len(raw) // 2] ^= 0x10
+    expect(bytes(flipped), 1, "", "flipped")
+    expect(bytes(raw[:-100]), 1, "", "truncated")
+    expect(bytes(raw) + b"\0", 1, "", "trailing")
+    expect(b"TGXSCN01" + bytes(raw[8:]), 1, "not an SVO file", "magic")
+    # child range past the node array, with a valid checksum
+    bad = bytearray(raw)
+    bad[SVO_HEADER_BYTES:SVO_HEADER_BYTES + 4] = np.uint32(n_nodes + 5).tobytes()
+    expect(resealed(bad), 1, "bad SVO nodes", "child_range")
+    # a leaf-level child that points back into the inner nodes
+    bad = bytearray(raw)
+    last_inner = SVO_HEADER_BYTES + SVO_NODE_BYTES * (n_inner - 1)
+    bad[last_inner:last_inner + 4] = np.uint32(1).tobytes()
+    expect(resealed(bad), 1, "bad SVO nodes", "leaf_child")
+    # a cluster index past the scene's clusters
+    lst = SVO_HEADER_BYTES + SVO_NODE_BYTES * n_nodes + 4 * nl + 4 * (nl + 1)
+    bad = bytearray(raw)
+    bad[lst:lst + 4] = np.uint32(0x7FFFFFFF).tobytes()
+    expect(resealed(bad), 1, "cluster index", "cluster")
+    # the root's level
+    bad = bytearray(raw)
+    bad[SVO_HEADER_BYTES + 12:SVO_HEADER_BYTES + 16] = np.uint32(int.from_bytes(
+        raw[SVO_HEADER_BYTES + 12:SVO_HEADER_BYTES + 16], "little") | (1 << 16)).tobytes()
+    expect(resealed(bad), 1, "bad SVO nodes", "root_level")
+    # a material whose LUT index is past the scene's LUTs (leaf data: no SVO left)
+    first_mat = lst + 4 * int(info["n_list_entries"])
+    for leaf_bits in good["bits"]:
+        first_mat += 4096
+        n_solid = int(np.unpackbits(leaf_bits.view(np.uint8)).sum())
+        if n_solid:
+            break
+        first_mat += 2 * n_solid
+    assert n_solid
+    bad = bytearray(raw)
+    bad[first_mat:first_mat + 2] = np.uint16(0xFF00).tobytes()
+    assert sc.luts.size // 256 < 255
+    expect(resealed(bad), 1, "bad leaf data", "material")
+    # every rejection above came before the device was touched
+    assert ctx.svo_info().tobytes() == info.tobytes()
+    for k in ("nodes", "bits", "materials", "list"):
+        assert np.array_equal(ctx.svo_export()[k], good[k])
+    with pytest.raises(T.TgxError) as e:
+        ctx.svo_load(tmp_path / "missing.tgxsvo")
+    assert e.value.code == 8
+    say("SVO file: scene mismatch")
+    ctx.upload(S.make_scene(objs[:1], seed=4))
+    expect(bytes(raw), 1, "clusters", "other_scene")
+    ctx.upload(sc)
+    ctx.svo_load(path)
+    for k in ("nodes", "bits", "materials", "list"):
+        assert np.array_equal(ctx.svo_export()[k], good[k])

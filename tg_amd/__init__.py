@@ -224,6 +224,8 @@ def load_library() -> ctypes.CDLL:
         "tgx_lut_set": (i32, [vp, u32, u32, vp]),
         "tgx_svo_get_info": (i32, [vp, vp]),
         "tgx_svo_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "tgx_svo_save": (i32, [vp, ctypes.c_char_p]),
+        "tgx_svo_load": (i32, [vp, ctypes.c_char_p]),
         "tgx_svo_trace": (i32, [vp, vp, vp, u32, ctypes.c_float, vp]),
         "tgx_gi_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
         "tgx_gi_last_stats": (i32, [vp, vp]),
@@ -520,6 +522,14 @@ class Context:
                                             _ptr(out["leaf_n"]), _ptr(out["list_offsets"]), _ptr(out["list"])),
                     "tgx_svo_export")
         return out
+
+    def svo_save(self, path) -> None:
+        """tgx_svo_save: the built SVO to a file."""
+        self._check(self.lib.tgx_svo_save(self.h, str(path).encode()), "tgx_svo_save")
+
+    def svo_load(self, path) -> None:
+        """tgx_svo_load: an SVO file saved from the uploaded scene, in place of a build."""
+        self._check(self.lib.tgx_svo_load(self.h, str(path).encode()), "tgx_svo_load")
 
     def svo_trace(self, origins: np.ndarray, dirs: np.ndarray, max_distance: float = 3.0e38) -> np.ndarray:
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)

@@ -29,6 +29,10 @@
 #include "tgx_m4.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
 
 /* Per-cluster-pointer transforms (rows 0..2 of each m4). */
 struct svo_frame {
@@ -893,6 +897,296 @@ int tgx_svo_get_info(tgx_context* ctx, tgx_svo_info* out)
     if (!ctx || !out) return TGX_ERR_INVALID_ARG;
     if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
     *out = ctx->svo_info;
+    return TGX_OK;
+}
+
+}  /* extern "C" */
+
+/* SVO files: "TGXSVO01", a header, the build's own node records, the leaf
+ * counts, list offsets and lists, then per leaf its 1024 bit words and the
+ * materials of its solid voxels only (in bit order), and an FNV-1a 64
+ * checksum of everything before it. */
+namespace {
+struct svo_file_header {
+    char         magic[8];
+    uint32_t     version;
+    uint32_t     n_inner;
+    tgx_svo_info info;
+    uint32_t     n_clusters;   /* of the scene the SVO was built from */
+    uint32_t     n_luts;
+    uint64_t     n_solid;      /* solid voxels over all leaves */
+};
+const char k_svo_magic[8] = {'T', 'G', 'X', 'S', 'V', 'O', '0', '1'};
+constexpr uint32_t k_svo_chunk = 2048u;   /* leaves per staged chunk */
+
+struct svo_stream {
+    FILE*    f = nullptr;
+    uint64_t h = 0xCBF29CE484222325ull;
+    bool     bad = false;
+    void hash(const void* p, size_t n)
+    {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 0x100000001B3ull; }
+    }
+    void put(const void* p, size_t n)
+    {
+        if (bad || (n && fwrite(p, 1, n, f) != n)) { bad = true; return; }
+        hash(p, n);
+    }
+    void get(void* p, size_t n)
+    {
+        if (bad || (n && fread(p, 1, n, f) != n)) { bad = true; return; }
+        hash(p, n);
+    }
+    ~svo_stream() { if (f) fclose(f); }
+};
+
+inline uint32_t popc32(uint32_t v) { return (uint32_t)__builtin_popcount(v); }
+
+/* The invariants tgx_svo_build's layout guarantees and the traversal relies
+ * on: every index a kernel follows stays inside its array. */
+const char* check_svo_nodes(const std::vector<svo_gnode>& g, uint32_t n_inner, uint32_t depth)
+{
+    const uint32_t n = (uint32_t)g.size(), leaf_level = depth - 5u;
+    if (n == 0 || n_inner == 0 || n_inner > n) return "node counts";
+    if (g[0].parent != SVO_NONE || ((g[0].masks >> 16) & 0xFFu) != 0u || g[0].cx || g[0].cy || g[0].cz) return "root";
+    uint64_t claimed = 0;   /* children named by some parent; each names its parent */
+    for (uint32_t i = 0; i < n; i++) {
+        const svo_gnode& a = g[i];
+        const uint32_t lvl = (a.masks >> 16) & 0xFFu, valid = a.masks & 0xFFu, leafm = (a.masks >> 8) & 0xFFu;
+        if (a.masks >> 24) return "node masks";
+        if (i >= n_inner) {
+            if (lvl != leaf_level || valid || leafm || a.first_child != SVO_NONE || a.leaf_idx != i - n_inner)
+                return "leaf node";
+            continue;
+        }
+        if (lvl >= leaf_level || a.leaf_idx != SVO_NONE) return "inner node";
+        const uint32_t kids = popc32(valid);
+        if (kids == 0) {
+            if (a.first_child != SVO_NONE || leafm) return "childless node";
+            continue;
+        }
+        claimed += kids;
+        const bool at_leaves = lvl + 1u == leaf_level;
+        if (leafm != (at_leaves ? valid : 0u)) return "leaf mask";
+        const uint64_t f = a.first_child, e = f + kids;
+        if (f <= i || e > (at_leaves ? n : n_inner) || (at_leaves && f < n_inner)) return "child range";
+        uint32_t slot = 0;
+        for (uint32_t rc = 0; rc < 8u; rc++) {
+            if (!(valid & (1u << rc))) continue;
+            const svo_gnode& c = g[f + slot++];
+            if (c.parent != i || ((c.masks >> 16) & 0xFFu) != lvl + 1u || c.cx != 2u * a.cx + (rc & 1u) ||
+                c.cy != 2u * a.cy + ((rc >> 1) & 1u) || c.cz != 2u * a.cz + (rc >> 2))
+                return "child record";
+        }
+    }
+    if (claimed != n - 1u) return "unreachable nodes";
+    return nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+int tgx_svo_save(tgx_context* ctx, const char* path)
+{
+    if (!ctx || !path) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const tgx_svo_info in = ctx->svo_info;
+    const uint32_t nl = in.n_leaves;
+    std::vector<svo_gnode> g(in.n_nodes);
+    std::vector<uint32_t> leaf_n(nl), off(nl + 1u), list(in.n_list_entries);
+    TGX_HIP_CHECK(ctx, hipMemcpy(g.data(), ctx->svo_nodes.ptr, sizeof(svo_gnode) * g.size(), hipMemcpyDeviceToHost));
+    if (nl) TGX_HIP_CHECK(ctx, hipMemcpy(leaf_n.data(), ctx->svo_leaf_n.ptr, 4ull * nl, hipMemcpyDeviceToHost));
+    TGX_HIP_CHECK(ctx, hipMemcpy(off.data(), ctx->svo_list_off.ptr, 4ull * (nl + 1u), hipMemcpyDeviceToHost));
+    if (!list.empty())
+        TGX_HIP_CHECK(ctx, hipMemcpy(list.data(), ctx->svo_list.ptr, 4ull * list.size(), hipMemcpyDeviceToHost));
+    /* solid voxel count first: the header carries it */
+    std::vector<uint32_t> bits((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS);
+    std::vector<uint16_t> mats((size_t)k_svo_chunk * TGX_SVO_BLOCK_VOXELS), packed;
+    uint64_t n_solid = 0;
+    for (uint32_t l0 = 0; l0 < nl; l0 += k_svo_chunk) {
+        const uint32_t m = std::min(k_svo_chunk, nl - l0);
+        TGX_HIP_CHECK(ctx, hipMemcpy(bits.data(), (const uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
+                                     4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyDeviceToHost));
+        for (size_t w = 0; w < (size_t)m * TGX_SVO_BLOCK_WORDS; w++) n_solid += popc32(bits[w]);
+    }
+    svo_stream s;
+    s.f = fopen(path, "wb");
+    if (!s.f) return tgx_fail(ctx, TGX_ERR_IO, "cannot write %s", path);
+    svo_file_header hd = {};
+    memcpy(hd.magic, k_svo_magic, 8);
+    hd.version = 1u;
+    hd.n_inner = ctx->svo_n_inner;
+    hd.info = in;
+    hd.n_clusters = ctx->n_clusters;
+    hd.n_luts = ctx->n_luts;
+    hd.n_solid = n_solid;
+    s.put(&hd, sizeof(hd));
+    s.put(g.data(), sizeof(svo_gnode) * g.size());
+    s.put(leaf_n.data(), 4ull * nl);
+    s.put(off.data(), 4ull * (nl + 1u));
+    s.put(list.data(), 4ull * list.size());
+    for (uint32_t l0 = 0; l0 < nl && !s.bad; l0 += k_svo_chunk) {
+        const uint32_t m = std::min(k_svo_chunk, nl - l0);
+        TGX_HIP_CHECK(ctx, hipMemcpy(bits.data(), (const uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
+                                     4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyDeviceToHost));
+        TGX_HIP_CHECK(ctx, hipMemcpy(mats.data(), (const uint16_t*)ctx->svo_leaf_mats.ptr + (size_t)l0 * TGX_SVO_BLOCK_VOXELS,
+                                     2ull * TGX_SVO_BLOCK_VOXELS * m, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < m; k++) {
+            const uint32_t* b = &bits[(size_t)k * TGX_SVO_BLOCK_WORDS];
+            const uint16_t* v = &mats[(size_t)k * TGX_SVO_BLOCK_VOXELS];
+            packed.clear();
+            for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++)
+                for (uint32_t word = b[w]; word; word &= word - 1u) packed.push_back(v[32u * w + (uint32_t)__builtin_ctz(word)]);
+            s.put(b, 4ull * TGX_SVO_BLOCK_WORDS);
+            s.put(packed.data(), 2ull * packed.size());
+        }
+    }
+    const uint64_t h = s.h;
+    s.put(&h, sizeof(h));
+    if (s.bad) return tgx_fail(ctx, TGX_ERR_IO, "write error on %s", path);
+    return TGX_OK;
+}
+
+int tgx_svo_load(tgx_context* ctx, const char* path)
+{
+    if (!ctx || !path) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (ctx->n_graphs > 0)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "a frame graph uses the SVO: destroy it before loading");
+    if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "load the SVO's scene first");
+    svo_stream s;
+    s.f = fopen(path, "rb");
+    if (!s.f) return tgx_fail(ctx, TGX_ERR_IO, "cannot open %s", path);
+    if (fseek(s.f, 0, SEEK_END) != 0) return tgx_fail(ctx, TGX_ERR_IO, "cannot seek in %s", path);
+    const long fsize = ftell(s.f);
+    rewind(s.f);
+    svo_file_header hd;
+    s.get(&hd, sizeof(hd));
+    if (s.bad || memcmp(hd.magic, k_svo_magic, 8) != 0 || hd.version != 1u)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s is not an SVO file", path);
+    const tgx_svo_info in = hd.info;
+    /* the fixed-size parts must fit in the file before anything is allocated */
+    const uint64_t fixed = sizeof(hd) + sizeof(svo_gnode) * (uint64_t)in.n_nodes + 4ull * (2ull * in.n_leaves + 1u) +
+                           4ull * in.n_list_entries + 4ull * TGX_SVO_BLOCK_WORDS * in.n_leaves + 2ull * hd.n_solid + 8u;
+    bool bad_min = false;
+    for (int a = 0; a < 3; a++)
+        bad_min |= !(in.min[a] == floorf(in.min[a])) ||
+                   !(fabsf(in.min[a]) + (float)(1u << (in.depth & 15u)) < 16777216.0f);
+    if (in.depth < TGX_SVO_MIN_DEPTH || in.depth > TGX_SVO_MAX_DEPTH || bad_min || in.n_nodes == 0 ||
+        in.n_nodes >= 0xFFFFFFu || hd.n_inner > in.n_nodes || in.n_leaves != in.n_nodes - hd.n_inner ||
+        hd.n_solid > (uint64_t)in.n_leaves * TGX_SVO_BLOCK_VOXELS || fsize < 0 || fixed != (uint64_t)fsize)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: inconsistent SVO header or file size", path);
+    if (hd.n_clusters != ctx->n_clusters || hd.n_luts != ctx->n_luts)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s was built from a scene of %u clusters and %u LUTs, not %u and %u",
+                        path, hd.n_clusters, hd.n_luts, ctx->n_clusters, ctx->n_luts);
+    const uint32_t nl = in.n_leaves;
+    std::vector<svo_gnode> g(in.n_nodes);
+    std::vector<uint32_t> leaf_n(nl), off(nl + 1u), list(in.n_list_entries);
+    s.get(g.data(), sizeof(svo_gnode) * g.size());
+    s.get(leaf_n.data(), 4ull * nl);
+    s.get(off.data(), 4ull * (nl + 1u));
+    s.get(list.data(), 4ull * list.size());
+    if (s.bad) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s is truncated", path);
+    if (const char* why = check_svo_nodes(g, hd.n_inner, in.depth))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: bad SVO nodes (%s)", path, why);
+    if (off[0] != 0u || off[nl] != in.n_list_entries) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: bad list offsets", path);
+    for (uint32_t l = 0; l < nl; l++)
+        if (off[l + 1] < off[l] || off[l + 1] - off[l] != leaf_n[l] ||
+            g[hd.n_inner + l].leaf_n != leaf_n[l])
+            return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: bad leaf lists", path);
+    for (uint32_t c : list)
+        if (c >= ctx->n_clusters) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: cluster index out of range", path);
+    for (const svo_gnode& a : g)
+        if (a.leaf_n && a.leaf_idx == SVO_NONE) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: bad node", path);
+
+    /* Leaf data in two passes over the file: the first checks every leaf
+     * and the checksum with nothing on the device touched, the second
+     * uploads (and hashes again, against a file changed in between). */
+    std::vector<uint32_t> bits((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS);
+    std::vector<uint16_t> mats((size_t)k_svo_chunk * TGX_SVO_BLOCK_VOXELS), packed;
+    const long leaf_data_at = ftell(s.f);
+    const uint64_t h_leaf = s.h;
+    auto leaf_pass = [&](bool upload) -> int {
+        uint64_t n_solid = 0;
+        for (uint32_t l0 = 0; l0 < nl; l0 += k_svo_chunk) {
+            const uint32_t m = std::min(k_svo_chunk, nl - l0);
+            if (upload) std::fill(mats.begin(), mats.end(), (uint16_t)0);
+            for (uint32_t k = 0; k < m && !s.bad; k++) {
+                uint32_t* b = &bits[(size_t)k * TGX_SVO_BLOCK_WORDS];
+                uint16_t* v = &mats[(size_t)k * TGX_SVO_BLOCK_VOXELS];
+                s.get(b, 4ull * TGX_SVO_BLOCK_WORDS);
+                uint32_t cnt = 0;
+                for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++) cnt += popc32(b[w]);
+                n_solid += cnt;
+                if (s.bad || n_solid > hd.n_solid) { s.bad = true; break; }
+                packed.resize(cnt);
+                s.get(packed.data(), 2ull * cnt);
+                uint32_t j = 0;
+                for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++)
+                    for (uint32_t word = b[w]; word; word &= word - 1u) {
+                        const uint16_t mv = packed[j++];
+                        if (ctx->n_luts && (uint32_t)(mv >> 8) >= ctx->n_luts) s.bad = true;
+                        v[32u * w + (uint32_t)__builtin_ctz(word)] = mv;
+                    }
+            }
+            if (s.bad) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: truncated or bad leaf data", path);
+            if (!upload) continue;
+            TGX_HIP_CHECK(ctx, hipMemcpy((uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
+                                         bits.data(), 4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyHostToDevice));
+            TGX_HIP_CHECK(ctx, hipMemcpy((uint16_t*)ctx->svo_leaf_mats.ptr + (size_t)l0 * TGX_SVO_BLOCK_VOXELS,
+                                         mats.data(), 2ull * TGX_SVO_BLOCK_VOXELS * m, hipMemcpyHostToDevice));
+        }
+        if (n_solid != hd.n_solid) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: solid voxel count mismatch", path);
+        return TGX_OK;
+    };
+    int rc;
+    if (leaf_data_at < 0) return tgx_fail(ctx, TGX_ERR_IO, "cannot tell the position in %s", path);
+    if ((rc = leaf_pass(false))) return rc;
+    const uint64_t h = s.h;
+    uint64_t stored = 0;
+    if (fread(&stored, sizeof(stored), 1, s.f) != 1 || stored != h)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: checksum mismatch", path);
+
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->svo_built = false;
+    ctx->svo_prev_valid = false;
+    ctx->svo_reused = 0;
+    const size_t nlr = nl ? nl : 1;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_nodes, sizeof(svo_gnode) * g.size(), "svo nodes")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_bits, 4ull * TGX_SVO_BLOCK_WORDS * nlr, "svo leaf bits")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_mats, 2ull * TGX_SVO_BLOCK_VOXELS * nlr, "svo leaf materials")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_n, 4ull * nlr, "svo leaf n")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_list_off, 4ull * (nlr + 1), "svo list offsets")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_list, 4ull * (list.empty() ? 1 : list.size()), "svo list")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_inner, 8ull * hd.n_inner, "svo inner nodes")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_bricks, 8ull * nlr, "svo leaf bricks")))
+        return rc;
+    if (fseek(s.f, leaf_data_at, SEEK_SET) != 0) return tgx_fail(ctx, TGX_ERR_IO, "cannot seek in %s", path);
+    s.h = h_leaf;
+    if ((rc = leaf_pass(true))) return rc;
+    if (s.h != h) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s changed while loading", path);
+    TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_nodes.ptr, g.data(), sizeof(svo_gnode) * g.size(), hipMemcpyHostToDevice));
+    if (nl) TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_leaf_n.ptr, leaf_n.data(), 4ull * nl, hipMemcpyHostToDevice));
+    TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_list_off.ptr, off.data(), 4ull * (nl + 1u), hipMemcpyHostToDevice));
+    if (!list.empty())
+        TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_list.ptr, list.data(), 4ull * list.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(svo_inner_k, dim3(blocks_for(hd.n_inner)), dim3(256), 0, ctx->stream,
+                       (const svo_gnode*)ctx->svo_nodes.ptr, hd.n_inner, (uint2*)ctx->svo_inner.ptr);
+    TGX_LAUNCH_CHECK(ctx, "svo_inner_k");
+    if (nl)
+        hipLaunchKernelGGL(svo_bricks_k, dim3(nl), dim3(64), 0, ctx->stream, (const uint32_t*)ctx->svo_leaf_bits.ptr,
+                           (unsigned long long*)ctx->svo_bricks.ptr);
+    TGX_LAUNCH_CHECK(ctx, "svo_bricks_k");
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->svo_info = in;
+    ctx->svo_n_inner = hd.n_inner;
+    ctx->svo_built = true;
+    ctx->svo_stale = false;
     return TGX_OK;
 }
 
