@@ -72,7 +72,10 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # Cache fallback: trace a pixel when its cell's squared bias exceeds this
 # multiple of the plain estimator's variance (a margin for the noise of the
 # cells' estimates).
-NRC_FALLBACK_THRESHOLD = 4.0
+NRC_FALLBACK_THRESHOLD = 16.0   # tracer rule: trace where bias^2 > 16 x the 4-spp variance
+NRC_FALLBACK_MIN_SAMPLES = 0    # no fallback for cells merely short of samples
+NRC_EMA_DECAY = 0.999       # inference weights: moving average over ~1000 Adam steps (~60 frames)
+NRC_REF_FRAMES = 1024       # 4-spp frames in the error reference (4096 spp)
 NRC_FWD_FLOPS = 2 * (78 * 64 + 3 * 64 * 64 + 64 * 3)   # per sample, unpadded
 NRC_TRAIN_FLOPS = 3 * NRC_FWD_FLOPS                     # forward + two backward products
 
@@ -85,36 +88,53 @@ def rel_mse(x, ref) -> float:
 
 def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, plain_ms: float,
                  plain_rays: int) -> dict:
-    """Config 3: the neural irradiance cache on the same view. Error of both
-    forms against a 256-spp FP64-accumulated reference (64 independent 4-spp
-    frames of the plain tracer); a second, independent 256-spp estimate
-    gives the reference's own noise floor."""
+    """Config 3: the neural irradiance cache on the same view, trained online
+    for `pretrain` frames of a static camera. Errors are relative MSE against
+    a 4096-spp FP64-accumulated reference (1024 independent 4-spp frames of
+    the plain tracer) in two independent 2048-spp halves: a quarter of the
+    halves' rel-MSE is the reference's own variance, which every measured
+    error carries and the debiased figures subtract (the cache is trained on
+    other samples, so its error and the reference's noise add)."""
     def irr_frame(frame: int) -> np.ndarray:
         ctx.frame_device(cam, W, H, T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=frame), vis_d, rad_d, irr_d)
         out = np.empty((H, W, 3), np.float32)
         ctx.d2h(out, irr_d)
         return out
 
-    refs = []
-    for base in (1000, 5000):
+    log(f"neural: {NRC_REF_FRAMES}-frame reference")
+    halves = []
+    for base in (100000, 200000):
         acc = np.zeros((H, W, 3))
-        for f in range(64):
+        for f in range(NRC_REF_FRAMES // 2):
             acc += irr_frame(base + f)
-        refs.append(acc / 64)
-    ref, ref2 = refs
+        halves.append(acc / (NRC_REF_FRAMES // 2))
     vis = np.empty((H, W), np.uint64)
     ctx.d2h(vis, vis_d)
     mask = vis != T.VISIBILITY_EMPTY
+    ref = 0.5 * (halves[0] + halves[1])
+    ref_var = rel_mse(halves[0][mask], halves[1][mask]) / 4.0
+
+    def err(x: np.ndarray) -> dict:
+        m = rel_mse(x[mask], ref[mask])
+        return {"rel_mse": m, "rel_mse_debiased": m - ref_var}
+
     plain = irr_frame(7)
     prm = T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=0)
 
-    def run(flags: int, threshold: float):
-        """pretrain untimed frames, then `steps` timed frames of a fresh cache"""
+    def run(flags: int, threshold: float, min_samples: int, ema: float, milestones=()):
+        """a fresh cache trained for `pretrain` untimed frames (errors read at
+        the milestones), then `steps` timed frames"""
         ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16,
                                     train_batch=1 << 12, train_spp=4, fallback_threshold=threshold,
-                                    fallback_min_samples=1, cell_size=8, flags=flags))
-        for _ in range(pretrain):
+                                    fallback_min_samples=min_samples, cell_size=8, flags=flags, ema_decay=ema))
+        log(f"neural: flags {flags} threshold {threshold} ema {ema}: {pretrain} training frames")
+        curve = []
+        for k in range(1, pretrain + 1):
             ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+            if k in milestones:
+                out = np.empty((H, W, 3), np.float32)
+                ctx.d2h(out, irr_d)
+                curve.append({"frames": k, **err(out), "fallback_pixels": int(ctx.nrc_stats()["fallback_pixels"])})
         ctx.synchronize()
         ctx.profile(True)
         ctx.profile_reset()
@@ -128,13 +148,21 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         st = ctx.nrc_stats()
         irr = np.empty((H, W, 3), np.float32)
         ctx.d2h(irr, irr_d)
-        return 1e3 * elapsed / steps, prof, st, irr
+        return 1e3 * elapsed / steps, prof, st, irr, curve
 
-    # reported beside: the t^2 rule and no fallback; then the bench's rule
-    ms_t2, _, st_t2, irr_t2 = run(0, 1.0)
-    ms_nf, _, _, irr_nf = run(T.NRC_FLAG_NO_FALLBACK, 1.0)
-    ms, prof, st, irr = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD)
-
+    # reported beside: the t^2 rule, no fallback, no weight average; then the
+    # bench's form, whose error is also read while it trains
+    ms_t2, _, st_t2, irr_t2, _ = run(0, 1.0, 1, NRC_EMA_DECAY)
+    ms_nf, _, _, irr_nf, _ = run(T.NRC_FLAG_NO_FALLBACK, 1.0, 1, NRC_EMA_DECAY)
+    ms_ne, _, st_ne, irr_ne, _ = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD, NRC_FALLBACK_MIN_SAMPLES,
+                                     0.0)
+    marks = sorted({m for m in (200, 400, 800, 1600, 3200, 6400, 12800) if m < pretrain} | {pretrain})
+    ms, prof, st, irr, curve = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD, NRC_FALLBACK_MIN_SAMPLES,
+                                   NRC_EMA_DECAY, marks)
+    e_neural, e_plain = err(irr), err(plain)
+    # the plain estimator's error falls as 1/spp: spp and frame time it
+    # needs for the cache's error (trace time taken as proportional to spp)
+    spp_eq = SPP * e_plain["rel_mse_debiased"] / e_neural["rel_mse_debiased"] if e_neural["rel_mse_debiased"] > 0 else None
     def graph_ms(flags: int) -> float:
         """one-camera frame captured as a graph (the cache keeps training on
         every replay): 3 untimed replays, then `steps` timed ones"""
@@ -182,19 +210,19 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "speedup_vs_plain": plain_ms / ms,
         "rays_traced_per_frame": traced,
         "rays_per_s_equivalent": plain_rays / (ms * 1e-3),
-        "rel_mse_vs_256spp": {"neural": rel_mse(irr[mask], ref[mask]), "plain_4spp": rel_mse(plain[mask], ref[mask]),
-                              "reference_noise_floor": rel_mse(ref2[mask], ref[mask])},
-        # both 256-spp references averaged: half the reference noise
-        "rel_mse_vs_512spp": {"neural": rel_mse(irr[mask], (0.5 * (ref + ref2))[mask]),
-                              "plain_4spp": rel_mse(plain[mask], (0.5 * (ref + ref2))[mask])},
+        "reference": {"spp": 4 * NRC_REF_FRAMES, "accumulation": "fp64", "rel_variance": ref_var},
+        "rel_mse_vs_4096spp": {"neural": e_neural, "plain_4spp": e_plain},
+        "meets_2pct_rel_mse": e_neural["rel_mse_debiased"] <= 0.02,
+        "plain_at_equal_error": {"spp": spp_eq, "frame_ms_est": plain_ms * spp_eq / SPP if spp_eq else None},
+        "convergence": curve,
         "visible_pixels": visible, "fallback_pixels": int(st["fallback_pixels"]),
-        "fallback_rule": f"cell bias^2 > {NRC_FALLBACK_THRESHOLD} x the plain estimator's variance at 4 spp",
-        "fallback_rule_t2": {"rule": "cell bias^2 > 1.0 x t^2", "frame_ms": ms_t2,
-                             "fallback_pixels": int(st_t2["fallback_pixels"]),
-                             "rel_mse_vs_256spp": rel_mse(irr_t2[mask], ref[mask]),
-                             "rel_mse_vs_512spp": rel_mse(irr_t2[mask], (0.5 * (ref + ref2))[mask])},
-        "no_fallback": {"frame_ms": ms_nf, "rel_mse_vs_256spp": rel_mse(irr_nf[mask], ref[mask]),
-                        "rel_mse_vs_512spp": rel_mse(irr_nf[mask], (0.5 * (ref + ref2))[mask])},
+        "fallback_rule": (f"cell bias^2 > {NRC_FALLBACK_THRESHOLD} x the plain estimator's variance at 4 spp, "
+                          f"min samples {NRC_FALLBACK_MIN_SAMPLES}"),
+        "weight_average": f"inference on the moving average of the weights, decay {NRC_EMA_DECAY} per Adam step",
+        "fallback_rule_t2": {"rule": "cell bias^2 > 1.0 x t^2, min samples 1", "frame_ms": ms_t2,
+                             "fallback_pixels": int(st_t2["fallback_pixels"]), **err(irr_t2)},
+        "no_fallback": {"frame_ms": ms_nf, **err(irr_nf)},
+        "no_weight_average": {"frame_ms": ms_ne, "fallback_pixels": int(st_ne["fallback_pixels"]), **err(irr_ne)},
         "train_samples": int(st["train_samples"]), "adam_steps_total": int(st["adam_steps"]),
         "train_loss": float(st["train_loss"]), "pretrain_frames": pretrain,
         "kernels_ms_per_step": {k: v for k, v in k_ms.items() if k.startswith("nrc_")},
@@ -205,7 +233,8 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "captured_graphs": captured,
         "roofline": hbm,
         "config": {"network": "96(79 used)->64x4->3 ReLU, bf16 MFMA, fp32 Adam lr 1e-3", "train_samples": 1 << 16,
-                   "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512"},
+                   "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512",
+                   "ema_decay": NRC_EMA_DECAY},
     }
 
 
@@ -256,7 +285,8 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
         if flags:
             ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / BATCH_WORLD, train_samples=1 << 16,
                                         train_batch=1 << 12, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
-                                        fallback_min_samples=1, cell_size=8, flags=T.NRC_FLAG_FALLBACK_VS_TRACER))
+                                        fallback_min_samples=NRC_FALLBACK_MIN_SAMPLES, cell_size=8,
+                                        flags=T.NRC_FLAG_FALLBACK_VS_TRACER, ema_decay=NRC_EMA_DECAY))
         log(f"batch {mode}: autotune")
         t0 = time.perf_counter()
         best = ctx.autotune(cams, W, H, prm, vis_d, rad_d, irr_d, flags=flags, frames=1)
@@ -669,7 +699,8 @@ def main() -> int:
                     help="time uncaptured batched frames (the same launches) instead of graph replays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-cache measurement")
-    ap.add_argument("--neural-pretrain", type=int, default=200, help="untimed cache training frames")
+    ap.add_argument("--neural-pretrain", type=int, default=6400,
+                    help="untimed cache training frames of the static view before the error is read")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 byte-counter passes")
     ap.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()

@@ -598,6 +598,9 @@ typedef struct tgx_nrc_config {
     uint32_t fallback_min_samples;  /* cells with fewer (decayed) training samples are traced */
     uint32_t cell_size;             /* edge of an error-estimate cell in voxels, power of two */
     uint32_t flags;                 /* TGX_NRC_FLAG_* */
+    float    ema_decay;             /* 0: inference uses the trained weights. In [0, 1): inference
+                                       uses their exponential moving average, e = d e + (1 - d) w
+                                       after every Adam step (the trained weights still train) */
 } tgx_nrc_config;
 
 #define TGX_NRC_FLAG_NO_FALLBACK    1u  /* never trace: every visible pixel uses the cache */
@@ -624,8 +627,13 @@ typedef struct tgx_nrc_stats {
 int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg);
 
 /* Master weights in the TGX_NRC_N_PARAMS layout (host arrays). set also
- * refreshes the BF16 copies; Adam state is kept. */
+ * refreshes the BF16 copies and sets the moving average (ema_decay) to the
+ * same weights; Adam state is kept. */
 int tgx_nrc_get_params(tgx_context* ctx, float* params);
+/* The fp32 weights inference packs to BF16: the moving average when
+ * ema_decay > 0, else the master weights. New: the reference has no
+ * learned part. */
+int tgx_nrc_get_inference_params(tgx_context* ctx, float* params);
 int tgx_nrc_set_params(tgx_context* ctx, const float* params);
 
 /* Inference on n host samples of 9 floats (position, normal, albedo) into

@@ -95,6 +95,43 @@ def test_gradient_and_adam_match_oracle(ctx):
     assert np.max(np.abs(w1 - expect)) <= 2e-7
 
 
+def test_weight_average_follows_adam_and_feeds_inference(ctx):
+    """ema_decay: after every Adam step the inference weights are
+    e = d e + (1 - d) w in fp32 (bit for bit), set_params resets them, and
+    inference runs on them; ema_decay 0 infers with the trained weights."""
+    d = np.float32(0.9)
+    ctx.nrc_create(T.nrc_config(ema_decay=float(d)))
+    w0 = oracle_params(5)
+    ctx.nrc_set_params(w0)
+    assert np.array_equal(ctx.nrc_inference_params(), w0)
+    p, nn, alb = samples(2000, 3)
+    t = target_fn(p, nn)
+    e = w0.copy()
+    for step in range(3):
+        say(f"nrc weight average step {step}")
+        ctx.nrc_train_batch(p, nn, alb, t)
+        w = ctx.nrc_params()
+        e = d * e + (np.float32(1.0) - d) * w
+        got = ctx.nrc_inference_params()
+        assert got.dtype == np.float32 and np.array_equal(got, e), step
+        assert not np.array_equal(got, w)
+    y = ctx.nrc_infer(p, nn, alb)
+    X = nrc.encode(p, nn, alb, (0, 0, 0), 1 / 512)
+    ye, _ = nrc.forward(e, X, bf=True)
+    yw, _ = nrc.forward(w, X, bf=True)
+    assert rel_l2(y, np.maximum(ye, 0)) <= INFER_TOL
+    assert rel_l2(y, np.maximum(ye, 0)) < rel_l2(y, np.maximum(yw, 0))
+    ctx.nrc_set_params(w0)
+    assert np.array_equal(ctx.nrc_inference_params(), w0)
+    # without the average, inference follows the trained weights
+    ctx.nrc_create(T.nrc_config())
+    ctx.nrc_set_params(w0)
+    ctx.nrc_train_batch(p, nn, alb, t)
+    assert np.array_equal(ctx.nrc_inference_params(), ctx.nrc_params())
+    with pytest.raises(T.TgxError):
+        ctx.nrc_create(T.nrc_config(ema_decay=1.0))
+
+
 def test_training_fits_a_smooth_field(ctx):
     ctx.nrc_create(T.nrc_config(seed=3))
     p, nn, alb = samples(1 << 15, 3)

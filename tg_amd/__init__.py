@@ -77,7 +77,7 @@ NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learni
                              ("beta1", "<f4"), ("beta2", "<f4"), ("epsilon", "<f4"), ("seed", "<u4"),
                              ("train_samples", "<u4"), ("train_batch", "<u4"), ("train_spp", "<u4"),
                              ("fallback_threshold", "<f4"), ("fallback_min_samples", "<u4"),
-                             ("cell_size", "<u4"), ("flags", "<u4")])
+                             ("cell_size", "<u4"), ("flags", "<u4"), ("ema_decay", "<f4")])
 NRC_STATS_DTYPE = np.dtype([("visible_pixels", "<u8"), ("fallback_pixels", "<u8"), ("train_samples", "<u8"),
                             ("train_rays", "<u8"), ("fallback_rays", "<u8"), ("adam_steps", "<u4"),
                             ("train_loss", "<f4")])
@@ -121,13 +121,15 @@ NRC_FLAG_FALLBACK_VS_TRACER = 8
 def nrc_config(origin=(0.0, 0.0, 0.0), scale: float = 1.0 / 512, learning_rate: float = 1e-3, beta1: float = 0.9,
                beta2: float = 0.99, epsilon: float = 1e-8, seed: int = 1, train_samples: int = 1 << 16,
                train_batch: int = 1 << 12, train_spp: int = 4, fallback_threshold: float = 1.0,
-               fallback_min_samples: int = 1, cell_size: int = 8, flags: int = 0) -> np.ndarray:
+               fallback_min_samples: int = 1, cell_size: int = 8, flags: int = 0,
+               ema_decay: float = 0.0) -> np.ndarray:
     c = np.zeros(1, NRC_CONFIG_DTYPE)
     c["origin"] = origin
     for k, v in (("scale", scale), ("learning_rate", learning_rate), ("beta1", beta1), ("beta2", beta2),
                  ("epsilon", epsilon), ("seed", seed), ("train_samples", train_samples),
                  ("train_batch", train_batch), ("train_spp", train_spp), ("fallback_threshold", fallback_threshold),
-                 ("fallback_min_samples", fallback_min_samples), ("cell_size", cell_size), ("flags", flags)):
+                 ("fallback_min_samples", fallback_min_samples), ("cell_size", cell_size), ("flags", flags),
+                 ("ema_decay", ema_decay)):
         c[k] = v
     return c
 
@@ -232,6 +234,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_frame_device": (i32, [vp, vp, u32, u32, vp, vp, vp, vp]),
         "tgx_nrc_create": (i32, [vp, vp]),
         "tgx_nrc_get_params": (i32, [vp, vp]),
+        "tgx_nrc_get_inference_params": (i32, [vp, vp]),
         "tgx_nrc_set_params": (i32, [vp, vp]),
         "tgx_nrc_infer": (i32, [vp, vp, u32, vp]),
         "tgx_nrc_train_batch": (i32, [vp, vp, vp, u32, vp, vp]),
@@ -569,6 +572,12 @@ class Context:
     def nrc_params(self) -> np.ndarray:
         p = np.zeros(NRC_N_PARAMS, np.float32)
         self._check(self.lib.tgx_nrc_get_params(self.h, _ptr(p)), "tgx_nrc_get_params")
+        return p
+
+    def nrc_inference_params(self) -> np.ndarray:
+        """tgx_nrc_get_inference_params: the weights inference uses (the moving average with ema_decay)."""
+        p = np.zeros(NRC_N_PARAMS, np.float32)
+        self._check(self.lib.tgx_nrc_get_inference_params(self.h, _ptr(p)), "tgx_nrc_get_inference_params")
         return p
 
     def nrc_set_params(self, params: np.ndarray) -> None:

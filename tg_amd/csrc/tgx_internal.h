@@ -98,6 +98,8 @@ struct tgx_context {
     uint32_t       nrc_qc_off = 0;        /* u32 offset of the last camera's counters in nrc_queue */
     tgx_buffer     nrc_master, nrc_m, nrc_v;   /* fp32 [N_PARAMS]           */
     tgx_buffer     nrc_fwd, nrc_bwd; /* bf16 MFMA A-operand fragments       */
+    tgx_buffer     nrc_ema;          /* fp32 moving average of the weights  */
+    tgx_buffer     nrc_efwd;         /* its bf16 forward fragments          */
     tgx_buffer     nrc_partials;     /* fp32 [chunks][N_PARAMS]             */
     tgx_buffer     nrc_grad;         /* fp32 [N_PARAMS], last step          */
     tgx_buffer     nrc_train;        /* bf16 [feature][batch] activations   */

@@ -398,7 +398,7 @@ __global__ void nrc_pack_k(const float* __restrict__ master, __bf16* __restrict_
     param_slots(i, f, b);
     const __bf16 v = (__bf16)master[i];
     fwd[f] = v;
-    if (b >= 0) bwd[b] = v;
+    if (b >= 0 && bwd) bwd[b] = v;
 }
 
 struct nrc_adam_args {
@@ -412,7 +412,9 @@ struct nrc_adam_args {
 
 __global__ void nrc_tick_k(uint32_t* counter) { *counter += 1u; }
 
-/* One Adam step with step number t = the device step counter + 1. The last
+/* One Adam step with step number t = the device step counter + 1; with ema,
+ * the moving average e = ema_d e + (1 - ema_d) w of the new weights and its
+ * forward fragments follow. The last
  * block to finish advances the counter (replayed graphs keep counting) and,
  * with loss_out, adds the batch's per-sample squared errors to *loss_out in
  * a fixed order. dev[NRC_DEV_DONE] counts finished blocks; it is 0 between
@@ -422,7 +424,8 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
                                                   float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
                                                   __bf16* __restrict__ fwd, __bf16* __restrict__ bwd,
                                                   const float* __restrict__ loss, uint32_t n_loss,
-                                                  float* __restrict__ loss_out)
+                                                  float* __restrict__ loss_out, float ema_d,
+                                                  float* __restrict__ ema, __bf16* __restrict__ efwd)
 {
     __shared__ float s[256];
     __shared__ uint32_t last;
@@ -452,6 +455,12 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
         const __bf16 h = (__bf16)w;
         fwd[f] = h;
         if (b >= 0) bwd[b] = h;
+        if (ema) {
+            /* the inference weights: moving average of the trained ones */
+            const float e = ema_d * ema[i] + (1.0f - ema_d) * w;
+            ema[i] = e;
+            efwd[f] = (__bf16)e;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -583,7 +592,8 @@ static inline uint32_t nrc_blocks(uint64_t n, uint32_t per) { return (uint32_t)(
 
 void tgx_nrc_release(tgx_context* ctx)
 {
-    tgx_buffer* bufs[] = {&ctx->nrc_master, &ctx->nrc_m, &ctx->nrc_v, &ctx->nrc_fwd, &ctx->nrc_bwd,
+    tgx_buffer* bufs[] = {&ctx->nrc_master, &ctx->nrc_m, &ctx->nrc_v, &ctx->nrc_fwd, &ctx->nrc_bwd, &ctx->nrc_ema,
+                          &ctx->nrc_efwd,
                           &ctx->nrc_partials, &ctx->nrc_grad, &ctx->nrc_train, &ctx->nrc_loss,
                           &ctx->nrc_samples, &ctx->nrc_targets, &ctx->nrc_out, &ctx->nrc_cells, &ctx->nrc_queue,
                           &ctx->nrc_dev};
@@ -638,6 +648,13 @@ int pack(tgx_context* ctx)
     hipLaunchKernelGGL(nrc_pack_k, dim3(nrc_blocks(NRC_N, 256)), dim3(256), 0, ctx->stream,
                        (const float*)ctx->nrc_master.ptr, (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr);
     TGX_LAUNCH_CHECK(ctx, "nrc_pack_k");
+    /* the moving average starts from the same weights */
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->nrc_ema.ptr, ctx->nrc_master.ptr, 4ull * NRC_N, hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+    TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_efwd.ptr, 0, 2ull * NRC_FWD_ELEMS, ctx->stream));
+    hipLaunchKernelGGL(nrc_pack_k, dim3(nrc_blocks(NRC_N, 256)), dim3(256), 0, ctx->stream,
+                       (const float*)ctx->nrc_ema.ptr, (__bf16*)ctx->nrc_efwd.ptr, (__bf16*)nullptr);
+    TGX_LAUNCH_CHECK(ctx, "nrc_pack_k");
     return TGX_OK;
 }
 
@@ -682,7 +699,8 @@ int train_step(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, 
                            (const float*)ctx->nrc_partials.ptr, n_chunks, a, dev, (float*)ctx->nrc_master.ptr,
                            (float*)ctx->nrc_m.ptr, (float*)ctx->nrc_v.ptr, (float*)ctx->nrc_grad.ptr,
                            (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr, (const float*)ctx->nrc_loss.ptr, Bc,
-                           loss_acc);
+                           loss_acc, c.ema_decay, c.ema_decay > 0.0f ? (float*)ctx->nrc_ema.ptr : nullptr,
+                           (__bf16*)ctx->nrc_efwd.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_adam_k");
     return TGX_OK;
@@ -729,7 +747,8 @@ int launch_infer(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list
     {
         tgx_launch_scope s(ctx, "nrc_infer_k");
         hipLaunchKernelGGL(nrc_infer_k, dim3(want < cap ? want : cap), dim3(NRC_BLOCK), 0, ctx->stream,
-                           (const bf16x8*)ctx->nrc_fwd.ptr, surf, list, count_dev, enc_of(c), out_entry, radiance,
+                           (const bf16x8*)(c.ema_decay > 0.0f ? ctx->nrc_efwd.ptr : ctx->nrc_fwd.ptr), surf, list,
+                           count_dev, enc_of(c), out_entry, radiance,
                            irradiance, cells, shift, w_var < 0.0f ? c.fallback_threshold : 0.0f,
                            w_var < 0.0f ? 0.0f : w_var, c.fallback_min_samples * 256u, fb_flag);
     }
@@ -747,8 +766,8 @@ int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg)
     ctx->last_error.clear();
     const tgx_nrc_config& c = *cfg;
     if (!(c.scale > 0.0f) || !(c.learning_rate > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) ||
-        !(c.beta2 >= 0.0f && c.beta2 < 1.0f) || !(c.epsilon >= 0.0f))
-        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad encoding scale or Adam parameters");
+        !(c.beta2 >= 0.0f && c.beta2 < 1.0f) || !(c.epsilon >= 0.0f) || !(c.ema_decay >= 0.0f && c.ema_decay < 1.0f))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "bad encoding scale, Adam parameters or ema_decay");
     if (c.train_batch == 0 || c.train_batch % TGX_NRC_BATCH_ALIGN || c.train_samples % c.train_batch ||
         c.train_samples > (1u << 24))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG,
@@ -767,6 +786,8 @@ int tgx_nrc_create(tgx_context* ctx, const tgx_nrc_config* cfg)
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_grad, 4ull * NRC_N, "nrc gradient")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_fwd, 2ull * NRC_FWD_ELEMS, "nrc forward fragments")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_bwd, 2ull * NRC_BWD_ELEMS, "nrc backward fragments")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_ema, 4ull * NRC_N, "nrc weight average")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_efwd, 2ull * NRC_FWD_ELEMS, "nrc average fragments")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_cells, sizeof(nrc_cell) * (size_t)NRC_CELLS, "nrc error cells")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_dev, 64, "nrc counters")))
         return rc;
@@ -793,6 +814,17 @@ int tgx_nrc_get_params(tgx_context* ctx, float* params)
     if ((rc = check_ready(ctx))) return rc;
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     TGX_HIP_CHECK(ctx, hipMemcpy(params, ctx->nrc_master.ptr, 4ull * NRC_N, hipMemcpyDeviceToHost));
+    return TGX_OK;
+}
+
+int tgx_nrc_get_inference_params(tgx_context* ctx, float* params)
+{
+    if (!ctx || !params) return TGX_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = check_ready(ctx))) return rc;
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const tgx_buffer& src = ctx->nrc_cfg.ema_decay > 0.0f ? ctx->nrc_ema : ctx->nrc_master;
+    TGX_HIP_CHECK(ctx, hipMemcpy(params, src.ptr, 4ull * NRC_N, hipMemcpyDeviceToHost));
     return TGX_OK;
 }
 
