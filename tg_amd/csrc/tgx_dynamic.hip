@@ -67,6 +67,7 @@ int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t clust
 {
     if (!ctx) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
+    ctx->scene_version++;
     if (object_capacity == 0 || cluster_capacity == 0 || !luts || n_luts == 0)
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "capacities, LUTs and n_luts must be non-zero");
     if ((uint64_t)cluster_capacity >= (1ull << TGX_CLUSTER_POINTER_BITS))
@@ -115,6 +116,7 @@ int tgx_object_create(tgx_context* ctx, const uint32_t n_clusters_per_dim[3], co
     ctx->last_error.clear();
     if (!ctx->dynamic) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no dynamic scene: call tgx_scene_reserve first");
     if (ctx->n_graphs > 0) return tgx_fail(ctx, TGX_ERR_CAPACITY, "a frame graph holds the scene: destroy it first");
+    ctx->scene_version++;
     if (!n_clusters_per_dim || !translation || !rotation || !out_object_idx)
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "null argument");
     const uint64_t n64 = (uint64_t)n_clusters_per_dim[0] * n_clusters_per_dim[1] * n_clusters_per_dim[2];

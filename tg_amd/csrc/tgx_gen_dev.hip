@@ -63,6 +63,7 @@ int tgx_gen_scene_device(tgx_context* ctx, const tgx_gen_params* p)
 {
     if (!ctx || !p) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
+    ctx->scene_version++;
     tgx_gen_sizes sz;
     int rc = tgx_gen_scene_sizes(p, &sz);
     if (rc) return tgx_fail(ctx, rc, "bad generator parameters or more than 2^31 clusters");
@@ -134,7 +135,7 @@ int tgx_gen_scene_device(tgx_context* ctx, const tgx_gen_params* p)
     ctx->n_clusters = sz.n_clusters;
     ctx->n_luts = sz.n_luts;
     ctx->has_scene = true;
-    return TGX_OK;
+    return tgx_vis_solid_refresh(ctx);
 }
 
 int tgx_scene_export(tgx_context* ctx, tgx_object* objects, uint32_t* cluster_pointers, uint32_t* cluster_idx_to_object,

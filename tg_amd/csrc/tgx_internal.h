@@ -42,6 +42,9 @@ struct tgx_context {
 
     /* Visibility work buffers. */
     tgx_buffer   vis_clusters;   /* compacted per-(camera, cluster) records */
+    tgx_buffer   vis_solid;      /* u8 per cluster index: has a solid voxel */
+    uint64_t     scene_version = 0;     /* bumped by every write of cluster bits */
+    uint64_t     vis_solid_version = ~0ull; /* scene_version vis_solid was made for */
     tgx_buffer   vis_rec_flags;  /* 1 if a record is in the tile lists      */
     tgx_buffer   vis_global_keys;/* records with screen-sized rectangles    */
     tgx_buffer   vis_counters;   /* per-camera record counts + scalars      */
@@ -170,6 +173,9 @@ extern "C" int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams,
                                        uint32_t h, const tgx_gi_params* gi, const uint64_t* vis_dev,
                                        float* radiance_dev, float* irradiance_dev, uint32_t* acc);
 int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
+/* Brings the per-cluster solid flags of the visibility cull up to date with
+ * the scene's bits (a kernel over every cluster index when they changed). */
+int  tgx_vis_solid_refresh(tgx_context* ctx);
 void tgx_buffer_release(tgx_buffer* b);
 
 /* Kernel launch bracket: records HIP events on the context stream around a

@@ -119,7 +119,7 @@ int tgx_destroy(tgx_context* ctx)
         (void)hipEventDestroy(p.stop);
     }
     tgx_buffer* bufs[] = {&ctx->objects, &ctx->cluster_pointers, &ctx->cluster_idx_to_object,
-                          &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters,
+                          &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters, &ctx->vis_solid,
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,
@@ -158,6 +158,7 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
 {
     if (!ctx || !s) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
+    ctx->scene_version++;
     if (!s->objects || !s->cluster_pointers || !s->cluster_idx_to_object || !s->bits)
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "scene arrays must not be NULL");
     if (s->n_objects == 0 || s->n_cluster_pointers == 0 || s->n_clusters == 0)
@@ -206,7 +207,7 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
     ctx->n_clusters = s->n_clusters;
     ctx->n_luts = s->luts ? s->n_luts : 0;
     ctx->has_scene = true;
-    return TGX_OK;
+    return tgx_vis_solid_refresh(ctx);
 }
 
 int tgx_device_alloc(tgx_context* ctx, size_t bytes, void** out_dev)

@@ -245,15 +245,11 @@ __device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, ui
 /* Builds the record of one (camera, cluster pointer); false if the cluster
  * is empty, out of every pixel ray's reach or beyond the far plane. */
 __device__ static bool setup_one(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
-                                 const uint32_t* __restrict__ c2o, const uint32_t* __restrict__ bits,
+                                 const uint32_t* __restrict__ c2o, const uint8_t* __restrict__ solid,
                                  uint32_t cptr, const vis_cam& cam, uint32_t W, uint32_t H, vis_cluster& rec)
 {
     const uint32_t cidx = cptrs[cptr];
-    const uint4* b4 = reinterpret_cast<const uint4*>(bits + (size_t)cidx * 16u);
-    uint32_t any = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) { const uint4 q = b4[i]; any |= q.x | q.y | q.z | q.w; }
-    if (!any) return false; /* no solid voxel: the walk can never hit */
+    if (!solid[cidx]) return false; /* no solid voxel: the walk can never hit */
 
     const tgx_object ob = objects[c2o[cidx]];
     const uint32_t rel = cptr - ob.first_cluster_pointer;
@@ -310,10 +306,25 @@ __device__ static inline void for_each_rect_tile(bool has, uint32_t rect_lo, uin
     }
 }
 
+/* One byte per cluster index: 1 when its 8^3 grid holds a solid voxel.
+ * The cull tests it for every (camera, cluster pointer) instead of the
+ * cluster's 64 B of bits. */
+__global__ __launch_bounds__(256) void vis_solid_k(const uint4* __restrict__ bits, uint32_t n_clusters,
+                                                   uint8_t* __restrict__ solid)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_clusters) return;
+    const uint4* b4 = bits + 4ull * c;
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { const uint4 q = b4[i]; any |= q.x | q.y | q.z | q.w; }
+    solid[c] = any ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
                                                  const uint32_t* __restrict__ cptrs,
                                                  const uint32_t* __restrict__ c2o,
-                                                 const uint32_t* __restrict__ bits, uint32_t n_cptr,
+                                                 const uint8_t* __restrict__ solid, uint32_t n_cptr,
                                                  const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                  uint32_t tiles_x, vis_cluster* __restrict__ recs,
                                                  uint32_t* __restrict__ rec_flags,
@@ -327,7 +338,7 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
     const vis_cam& cam = cams[ci];
     vis_cluster rec;
     bool has = false;
-    if (cptr < n_cptr) has = setup_one(objects, cptrs, c2o, bits, cptr, cam, W, H, rec);
+    if (cptr < n_cptr) has = setup_one(objects, cptrs, c2o, solid, cptr, cam, W, H, rec);
     bool binned = false;
     if (has) {
         const uint32_t k = atomicAdd(&rec_count[ci], 1u);
@@ -584,6 +595,18 @@ static bool make_cam(const tgx_camera& c, vis_cam& v)
     return true;
 }
 
+int tgx_vis_solid_refresh(tgx_context* ctx)
+{
+    if (ctx->vis_solid_version == ctx->scene_version || ctx->n_clusters == 0) return TGX_OK;
+    int rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_solid, ctx->n_clusters, "cluster solid flags"))) return rc;
+    hipLaunchKernelGGL(vis_solid_k, dim3((ctx->n_clusters + 255u) / 256u), dim3(256), 0, ctx->stream,
+                       (const uint4*)ctx->bits.ptr, ctx->n_clusters, (uint8_t*)ctx->vis_solid.ptr);
+    TGX_LAUNCH_CHECK(ctx, "vis_solid_k");
+    ctx->vis_solid_version = ctx->scene_version;
+    return TGX_OK;
+}
+
 static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t W,
                             uint32_t H, uint64_t* out_dev, const vis_cam* resident)
 {
@@ -606,6 +629,7 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     const uint32_t n_tiles = (uint32_t)n_tiles64;
 
     int rc;
+    if ((rc = tgx_vis_solid_refresh(ctx))) return rc;
     std::vector<vis_cam> vc;
     if (!resident) {
         vc.resize(n_cams);
@@ -646,7 +670,7 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
         tgx_launch_scope s(ctx, "vis_setup");
         hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
                            (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint32_t*)ctx->bits.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->vis_solid.ptr,
                            n_cptr, d_cams, W, H, tiles_x, d_recs, d_flags, d_counters, d_global_count, d_gkeys, d_count);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_setup");
