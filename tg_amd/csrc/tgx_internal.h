@@ -43,6 +43,7 @@ struct tgx_context {
     /* Visibility work buffers. */
     tgx_buffer   vis_clusters;   /* compacted per-(camera, cluster) records */
     tgx_buffer   vis_solid;      /* u8 per cluster index: has a solid voxel */
+    tgx_buffer   vis_objvis;     /* u8 per (camera, object): not culled     */
     uint64_t     scene_version = 0;     /* bumped by every write of cluster bits */
     uint64_t     vis_solid_version = ~0ull; /* scene_version vis_solid was made for */
     tgx_buffer   vis_rec_flags;  /* 1 if a record is in the tile lists      */

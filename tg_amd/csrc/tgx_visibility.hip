@@ -242,16 +242,79 @@ __device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, ui
     return true;
 }
 
+/* Object-level cull, one byte per (camera, object): 0 only when no
+ * cluster of the object can get a rectangle from cluster_rect. The
+ * object's box, grown by one voxel, goes through cluster_rect's transform
+ * into the camera's corner-ray basis w (fx = w1 / w0, fy = w2 / w0). A
+ * half-space test that holds at all eight corners holds on the whole box,
+ * so on every cluster box in it and every clipped point cluster_rect
+ * projects: all corners behind the near plane, or beyond one screen edge by
+ * 3 pixels (the rectangle's padding is 1.5). A box that reaches the cube
+ * around the camera, where cluster_rect accepts clusters as full-screen, is
+ * kept. The one-voxel growth covers the clusters' own growth and the
+ * rounding of both computations. */
+__global__ __launch_bounds__(256) void vis_objcull_k(const tgx_object* __restrict__ objects, uint32_t n_objects,
+                                                     const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
+                                                     uint8_t* __restrict__ objvis)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ci = blockIdx.y;
+    if (o >= n_objects) return;
+    const vis_cam& cam = cams[ci];
+    uint8_t keep = 1u;
+    if (!cam.full_screen) {
+        const tgx_object ob = objects[o];
+        const float ex = 4.0f * (float)ob.n_cluster_pointers_per_dim[0] + 1.0f;
+        const float ey = 4.0f * (float)ob.n_cluster_pointers_per_dim[1] + 1.0f;
+        const float ez = 4.0f * (float)ob.n_cluster_pointers_per_dim[2] + 1.0f;
+        const float tcx = ob.translation[0] - cam.pos[0];
+        const float tcy = ob.translation[1] - cam.pos[1];
+        const float tcz = ob.translation[2] - cam.pos[2];
+        const float mx = 3.0f / (float)W, my = 3.0f / (float)H;
+        float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+        bool left = true, right = true, below = true, above = true;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float lx = (k & 1) ? ex : -ex, ly = (k & 2) ? ey : -ey, lz = (k & 4) ? ez : -ez;
+            const float px = ob.rotation[0] * lx + ob.rotation[4] * ly + ob.rotation[8] * lz + tcx;
+            const float py = ob.rotation[1] * lx + ob.rotation[5] * ly + ob.rotation[9] * lz + tcy;
+            const float pz = ob.rotation[2] * lx + ob.rotation[6] * ly + ob.rotation[10] * lz + tcz;
+            float w[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                w[i] = cam.inv[3 * i + 0] * px + cam.inv[3 * i + 1] * py + cam.inv[3 * i + 2] * pz;
+                lo[i] = fminf(lo[i], w[i]);
+                hi[i] = fmaxf(hi[i], w[i]);
+            }
+            left = left && w[1] + mx * w[0] < 0.0f;
+            right = right && w[1] - (1.0f + mx) * w[0] > 0.0f;
+            below = below && w[2] + my * w[0] < 0.0f;
+            above = above && w[2] - (1.0f + my) * w[0] > 0.0f;
+        }
+        bool near_cube = true;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if (hi[i] < -VIS_RECT_DELTA - 1.0f || lo[i] > VIS_RECT_EPS + VIS_RECT_DELTA + 1.0f) near_cube = false;
+        const bool behind = hi[0] < VIS_RECT_EPS - 1.0f;
+        if (!near_cube && (behind || left || right || below || above)) keep = 0u;
+    }
+    objvis[(size_t)ci * n_objects + o] = keep;
+}
+
 /* Builds the record of one (camera, cluster pointer); false if the cluster
- * is empty, out of every pixel ray's reach or beyond the far plane. */
+ * is empty, its object is culled, or it is out of every pixel ray's reach
+ * or beyond the far plane. */
 __device__ static bool setup_one(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
                                  const uint32_t* __restrict__ c2o, const uint8_t* __restrict__ solid,
-                                 uint32_t cptr, const vis_cam& cam, uint32_t W, uint32_t H, vis_cluster& rec)
+                                 const uint8_t* __restrict__ objvis, uint32_t cptr, const vis_cam& cam, uint32_t W,
+                                 uint32_t H, vis_cluster& rec)
 {
     const uint32_t cidx = cptrs[cptr];
     if (!solid[cidx]) return false; /* no solid voxel: the walk can never hit */
+    const uint32_t oi = c2o[cidx];
+    if (!objvis[oi]) return false;  /* no cluster of the object reaches a pixel */
 
-    const tgx_object ob = objects[c2o[cidx]];
+    const tgx_object ob = objects[oi];
     const uint32_t rel = cptr - ob.first_cluster_pointer;
     const uint32_t nx = ob.n_cluster_pointers_per_dim[0], ny = ob.n_cluster_pointers_per_dim[1];
     const uint32_t rx = rel % nx, ry = (rel / nx) % ny, rz = rel / (nx * ny);
@@ -324,7 +387,9 @@ __global__ __launch_bounds__(256) void vis_solid_k(const uint4* __restrict__ bit
 __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
                                                  const uint32_t* __restrict__ cptrs,
                                                  const uint32_t* __restrict__ c2o,
-                                                 const uint8_t* __restrict__ solid, uint32_t n_cptr,
+                                                 const uint8_t* __restrict__ solid,
+                                                 const uint8_t* __restrict__ objvis, uint32_t n_objects,
+                                                 uint32_t n_cptr,
                                                  const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                  uint32_t tiles_x, vis_cluster* __restrict__ recs,
                                                  uint32_t* __restrict__ rec_flags,
@@ -338,7 +403,8 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
     const vis_cam& cam = cams[ci];
     vis_cluster rec;
     bool has = false;
-    if (cptr < n_cptr) has = setup_one(objects, cptrs, c2o, solid, cptr, cam, W, H, rec);
+    if (cptr < n_cptr)
+        has = setup_one(objects, cptrs, c2o, solid, objvis + (size_t)ci * n_objects, cptr, cam, W, H, rec);
     bool binned = false;
     if (has) {
         const uint32_t k = atomicAdd(&rec_count[ci], 1u);
@@ -646,6 +712,8 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (2 * n_cams + 8), "vis counters"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_count, 4ull * n_tiles, "tile counts"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_objvis, (size_t)ctx->n_objects * n_cams, "object cull flags")))
+        return rc;
 
     const vis_cam* d_cams = resident ? resident : (const vis_cam*)ctx->vis_cams.ptr;
     /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts */
@@ -667,11 +735,19 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     }
     TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
     {
+        tgx_launch_scope s(ctx, "vis_objcull_k");
+        hipLaunchKernelGGL(vis_objcull_k, dim3((ctx->n_objects + 255u) / 256u, n_cams), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, d_cams, W, H,
+                           (uint8_t*)ctx->vis_objvis.ptr);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_objcull_k");
+    {
         tgx_launch_scope s(ctx, "vis_setup");
         hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
                            (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
                            (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->vis_solid.ptr,
-                           n_cptr, d_cams, W, H, tiles_x, d_recs, d_flags, d_counters, d_global_count, d_gkeys, d_count);
+                           (const uint8_t*)ctx->vis_objvis.ptr, ctx->n_objects, n_cptr, d_cams, W, H, tiles_x, d_recs,
+                           d_flags, d_counters, d_global_count, d_gkeys, d_count);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_setup");
     {
