@@ -161,3 +161,44 @@ def test_error_paths(ctx):
         assert e.value.code == 4
     finally:
         fresh.close()
+
+
+def test_object_cull_at_frustum_edges(ctx):
+    """Objects astride each screen edge (inside, on and just beyond it), behind
+    the camera and around the near plane: the per-object cull must drop only
+    objects none of whose clusters can cover a pixel, so the frame equals the
+    oracle's brute-force visibility word for word."""
+    rng = np.random.default_rng(23)
+    cam = T.make_camera((3.0, -2.0, 5.0), pitch=0.2, yaw=0.7, roll=0.1, fov_y=1.0, aspect=1.25, far=400.0)
+    pos = cam["camera"][0, :3].astype(np.float64)
+    c = {k: cam[k][0, :3].astype(np.float64) for k in ("ray_bl", "ray_br", "ray_tr", "ray_tl")}
+    fwd = sum(c.values()) / 4.0
+    fwd /= np.linalg.norm(fwd)
+    edges = [(c["ray_bl"], c["ray_tl"], c["ray_bl"] - c["ray_br"]),   # left
+             (c["ray_br"], c["ray_tr"], c["ray_br"] - c["ray_bl"]),   # right
+             (c["ray_bl"], c["ray_br"], c["ray_bl"] - c["ray_tl"]),   # bottom
+             (c["ray_tl"], c["ray_tr"], c["ray_tl"] - c["ray_bl"])]   # top
+    objs = []
+    for a, b, out in edges:
+        out = out / np.linalg.norm(out)
+        for d in (40.0, 90.0, 200.0):
+            for s in (0.0, 0.5, 1.0, 1.08, 1.4):
+                n = tuple(int(v) for v in rng.integers(1, 3, 3))
+                h = 4.0 * np.sqrt(3.0) * max(n)
+                ray = a + rng.uniform(0.1, 0.9) * (b - a)
+                centre = pos + d * ray / np.linalg.norm(ray) + s * h * out
+                objs.append({"n": n, "translation": tuple(centre), "rotation": S.random_rotation(rng)})
+    near = []
+    for d in (-30.0, -9.0, -2.0, 0.5, 1.5, 6.0):
+        lat = rng.normal(size=3)
+        near.append({"n": (1, 1, 2), "translation": tuple(pos + d * fwd + (2.0 + abs(d)) * lat),
+                     "rotation": S.random_rotation(rng)})
+    w, h_ = 128, 96
+    for name, group, lo, hi in (("frustum edges", objs, 0.02, 0.9), ("behind and near", near, 0.0, 1.0)):
+        sc = S.make_scene(group, seed=12)
+        ctx.upload(sc)
+        say(f"object cull, {name}: {len(group)} objects, {w}x{h_}")
+        got = ctx.visibility(cam, w, h_)[0]
+        ref = O.visibility_brute(O.OracleScene(sc), cam, w, h_)
+        assert_same(got, ref, name)
+        assert lo <= (ref != T.VISIBILITY_EMPTY).mean() <= hi, name
