@@ -34,7 +34,9 @@
  * atomic (ballot + lane rank).
  *
  * GI frame (estimator documented at tgx_gi_params in tgx.h):
- *   gi_gen_k     surface of each visible pixel, spp bounce rays appended
+ *   gi_gen_k     surface of each visible pixel, spp bounce rays at
+ *                slots fixed by the list entry (gi_gen_tail_k then moves
+ *                the queue's tail)
  *   gi_trace_k   bounce rays; a lane whose bounce hit faces the sun stores
  *                the albedo-weighted contribution and traces the shadow ray
  *                next, clearing the contribution if it is occluded
@@ -786,22 +788,24 @@ __global__ void gi_list_k(const uint32_t* __restrict__ flag, const uint32_t* __r
 
 /* spp cosine bounce rays per listed pixel, from p + sqrt(3) n; ray id =
  * id_base + list entry * spp + sample (id_base separates the lists that
- * share a queue). stats (may be null) counts the list's bounce rays. */
+ * share a queue). Entry i's rays take the queue slots after the tail
+ * counters[1] had at launch, so the launch needs no atomics;
+ * gi_gen_tail_k then moves the tail and the list's ray count. */
 __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restrict__ surf,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count_ptr, tgx_gi_params prm,
                                                      const uint32_t* __restrict__ frame_salt,
                                                      svo_ray* __restrict__ rays, float* __restrict__ contrib,
-                                                     uint32_t* __restrict__ counters, uint32_t id_base,
-                                                     uint32_t* __restrict__ stats, uint32_t salt_add)
+                                                     const uint32_t* __restrict__ counters, uint32_t id_base,
+                                                     uint32_t salt_add)
 {
     /* a device counter can advance the frame of replayed work */
     if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + salt_add + 1u);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool ok = i < *count_ptr;
-    const uint32_t first = block_append<GI_BLOCK>(&counters[1], ok ? prm.spp : 0u);
-    if (stats) (void)block_append<GI_BLOCK>(&stats[1], ok ? prm.spp : 0u);
-    if (!ok) return;
+    if (i >= *count_ptr) return;
+    /* entry i's samples take slots [tail + i spp, + spp) after the queue's
+     * tail at launch; gi_gen_tail_k advances the tail afterwards */
+    const uint32_t first = counters[1] + i * prm.spp;
     const uint32_t pix = list[i];
     const gi_surf_rec sr = surf[pix];
     const f3 n = {sr.n[0], sr.n[1], sr.n[2]};
@@ -815,6 +819,17 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restri
         const size_t c = 3ull * ((size_t)id_base + (size_t)i * prm.spp + k);
         contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
     }
+}
+
+/* Advances the bounce queue's tail (and the list's generated-ray count)
+ * past one gi_gen_k launch: min(count, max_count) entries of spp rays. */
+__global__ void gi_gen_tail_k(const uint32_t* __restrict__ count_ptr, uint32_t max_count, uint32_t spp,
+                              uint32_t* __restrict__ counters, uint32_t* __restrict__ stats)
+{
+    const uint32_t c = *count_ptr;
+    const uint32_t n = (c < max_count ? c : max_count) * spp;
+    counters[1] += n;
+    if (stats) stats[1] += n;
 }
 
 /* Cosine of a bounce hit's entry face with the sun; > 0 when the hit takes
@@ -1204,10 +1219,13 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         const gi_trace_set& t = sets[k];
         if (t.max_count == 0) continue;
         const gi_surf_rec* surf = t.surf ? t.surf : (const gi_surf_rec*)ctx->gi_surf.ptr;
-        tgx_launch_scope s(ctx, "gi_gen_k");
-        hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(t.max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, t.list,
-                           t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k],
-                           two ? t.counters : nullptr, t.salt_add);
+        {
+            tgx_launch_scope s(ctx, "gi_gen_k");
+            hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(t.max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, t.list,
+                               t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k], t.salt_add);
+        }
+        hipLaunchKernelGGL(gi_gen_tail_k, dim3(1), dim3(1), 0, ctx->stream, t.count_dev, t.max_count, t.prm.spp, cnt,
+                           two ? t.counters : nullptr);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_gen_k");
     /* shade, trace and lit use what the lists share: sun, radiance, range */
