@@ -445,6 +445,32 @@ def svo_visibility_bench(ctx, cam, steps: int) -> dict:
     return {"ms": ms, "rays_per_s": W * H / (ms * 1e-3), "hit_pixels": int((vis != T.VISIBILITY_EMPTY).sum())}
 
 
+def svo_file_bench(ctx, cam, prm, vis_d, rad_d, irr_d) -> dict:
+    """The frame's SVO saved to a file and loaded back in place of the build:
+    file size, wall times (synchronous calls), and one GI frame after the
+    load compared byte for byte with one before it."""
+    def frame() -> bytes:
+        ctx.frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+        out = np.empty((H, W, 3), np.float32)
+        ctx.d2h(out, irr_d)
+        return out.tobytes()
+
+    before = frame()
+    with tempfile.TemporaryDirectory(prefix="tgx_svo_", dir="/tmp") as d:
+        path = Path(d) / "frame.tgxsvo"
+        t0 = time.perf_counter()
+        ctx.svo_save(path)
+        save_ms = 1e3 * (time.perf_counter() - t0)
+        size = path.stat().st_size
+        t0 = time.perf_counter()
+        ctx.svo_load(path)
+        load_ms = 1e3 * (time.perf_counter() - t0)
+    info = ctx.svo_info()
+    return {"bytes": size, "save_ms": save_ms, "load_ms": load_ms, "nodes": int(info["n_nodes"]),
+            "leaves": int(info["n_leaves"]), "leaf_bytes_in_hbm": (4096 + 2 * 32768) * int(info["n_leaves"]),
+            "frame_after_load_identical": frame() == before}
+
+
 def dynamic_bench(ctx, scene, mn, depth: int) -> dict:
     """The frame's scene rebuilt as a dynamic scene, object by object, then
     the first object destroyed (every later pointer shifts) and created
@@ -822,6 +848,9 @@ def main() -> int:
     line["shading"] = shading_bench(ctx, cam, vis_d, args.steps)
     line["svo_visibility"] = svo_visibility_bench(ctx, cam, args.steps)
     line["pick"] = pick_bench(ctx, cam, vis_d)
+    if args.workload == "frame":
+        log("svo file save / load")
+        line["svo_file"] = svo_file_bench(ctx, cam, prm, vis_d, rad_d, irr_d)
     if gst is not None:
         line["gi"] = {k: int(gst[k]) for k in gst.dtype.names}
         if not args.no_neural:
