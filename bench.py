@@ -89,8 +89,8 @@ def rel_mse(x, ref) -> float:
 def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, plain_ms: float,
                  plain_rays: int) -> dict:
     """The neural irradiance cache (512-object noise scene, 1920x1080) on the
-    same view, trained online
-    for `pretrain` frames of a static camera. Errors are relative MSE against
+    same view, trained online for `pretrain` frames of a static camera.
+    Errors are relative MSE against
     a 4096-spp FP64-accumulated reference (1024 independent 4-spp frames of
     the plain tracer) in two independent 2048-spp halves: a quarter of the
     halves' rel-MSE is the reference's own variance, which every measured
