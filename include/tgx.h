@@ -143,6 +143,11 @@ int         tgx_synchronize(tgx_context* ctx);
 
 /* Uploads the whole scene (objects, cluster maps, solid bits, materials,
  * LUTs) into library-owned device buffers, replacing any previous scene.
+ * materials may be NULL (every voxel then has material 0); luts may be
+ * NULL (GI, cache and shading calls then refuse). Replacing the scene
+ * marks the SVO stale: tracing calls return TGX_ERR_NOT_BUILT until
+ * tgx_svo_update or tgx_svo_build. Refused (TGX_ERR_CAPACITY) while a
+ * frame graph exists, as are tgx_scene_reserve and tgx_gen_scene_device.
  * Replaces the staging pushes of tg_raytracer_create_object
  * (tgvk_raytracer.c:805-992) and tg_raytracer_color_lut_set (:1122-1142).
  * Errors: TGX_ERR_CAPACITY if n_cluster_pointers >= 2^31 (the 31-bit
@@ -214,13 +219,15 @@ int tgx_object_destroy(tgx_context* ctx, uint32_t object_idx);
  * byte with an FNV-1a 64 checksum. save: TGX_ERR_IO if the file cannot be
  * written. counts / load: TGX_ERR_IO if it cannot be opened,
  * TGX_ERR_INVALID_ARG if it is not a scene file, is truncated or fails the
- * checksum. load fills caller arrays sized by tgx_scene_file_counts
- * (materials may be NULL; a file without materials loads zeros). */
+ * checksum. load fills caller arrays sized by `counts` (as
+ * tgx_scene_file_counts returned them; materials may be NULL, a file
+ * without materials loads zeros) and refuses, with TGX_ERR_INVALID_ARG and
+ * before writing any array, a file whose header counts differ from them. */
 int tgx_scene_save(const char* path, const tgx_scene_desc* scene);
 int tgx_scene_file_counts(const char* path, tgx_scene_counts* out);
-int tgx_scene_load(const char* path, tgx_object* objects, uint32_t* cluster_pointers,
-                   uint32_t* cluster_idx_to_object, uint32_t* bits, uint8_t* materials,
-                   tgx_lut_entry* luts);
+int tgx_scene_load(const char* path, const tgx_scene_counts* counts, tgx_object* objects,
+                   uint32_t* cluster_pointers, uint32_t* cluster_idx_to_object, uint32_t* bits,
+                   uint8_t* materials, tgx_lut_entry* luts);
 
 /* ------------------------------------------------------------------------ */
 /* Primary visibility                                                       */
@@ -397,8 +404,13 @@ int tgx_svo_visibility_device(tgx_context* ctx, const tgx_camera* cam, uint32_t 
  *   - at a bounce hit facing the sun: LUT albedo of the voxel's material
  *     and one shadow ray towards the sun, started 1/64 voxel off the hit
  *     face, same range;
- *   - irradiance E = (1/spp) sum_k albedo_k * max(0, n_k . sun) * radiance / pi
+ *   - E = (1/spp) sum_k albedo_k * max(0, n_k . sun) * radiance / pi
  *     over lit bounce hits; radiance L = albedo_p * E (per RGB channel).
+ *     Each lit hit returns the Lambertian radiance albedo_k (n_k . sun)
+ *     radiance / pi, and under cosine-weighted sampling the mean of the
+ *     returned radiance is the irradiance at the pixel divided by pi: E is
+ *     that quantity (called "irradiance" in the outputs below), and L is
+ *     the pixel's outgoing Lambertian radiance.
  * RNG state of (pixel i, sample s): murmur3_fmix32 (util.inc:47-56) of
  * i * spp + s + 0x9E3779B9 * (frame + 1), with 0 replaced by 0x6D2B79F5. */
 typedef struct tgx_gi_params {
@@ -418,7 +430,7 @@ typedef struct tgx_gi_stats {
     uint64_t shadow_rays;
 } tgx_gi_stats;
 
-/* Computes per-pixel RGB radiance (and optionally irradiance) of one camera
+/* Computes per-pixel RGB radiance (and optionally E, the irradiance / pi) of one camera
  * from its visibility buffer vis_dev (w * h words, as written by
  * tgx_visibility_device) with the SVO of tgx_svo_build. Outputs are device
  * buffers of 3 * w * h floats, row-major RGB; pixels without a hit get 0.

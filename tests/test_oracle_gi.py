@@ -4,7 +4,10 @@ tgx_gi_params in include/tgx.h; tg has no GI of its own).
 Closed-form scenes: a closed box (every bounce ray hits, no sunlight: zero
 radiance), an open floor (every bounce ray escapes), a floor with a wall
 lit from one side (light only on the lit side), plus exact linearity in
-the sun radiance and the cosine lobe's first moment.
+the sun radiance and the cosine lobe's first moment. The magnitude is fixed
+by a floor beside a sunlit wall, against the float64 configuration factor
+(Lambert's contour formula, itself checked against the parallel-rectangle
+closed form, the infinite-wall limit 1/2 and cosine-weighted sampling).
 """
 from __future__ import annotations
 
@@ -150,3 +153,99 @@ def test_cosine_lobe_first_moment():
     assert c.min() >= 0.0
     assert abs(c.mean() - 2.0 / 3.0) < 0.01          # E[cos] = 2/3 for a cosine lobe
     assert abs((c ** 2).mean() - 0.5) < 0.01         # E[cos^2] = 1/2
+
+
+def config_factor(p, n, verts) -> float:
+    """Configuration factor from a differential area at p (unit normal n) to
+    a planar polygon lying in its upper hemisphere, by Lambert's contour
+    formula F = 1/(2 pi) sum_i gamma_i n . (R_i x R_i+1) / |R_i x R_i+1|,
+    gamma_i the angle between consecutive vertex vectors. float64."""
+    p, n = np.asarray(p, np.float64), np.asarray(n, np.float64)
+    v = [np.asarray(q, np.float64) - p for q in verts]
+    f = 0.0
+    for i in range(len(v)):
+        a, b = v[i], v[(i + 1) % len(v)]
+        c = np.cross(a, b)
+        nc = np.linalg.norm(c)
+        if nc == 0.0:
+            continue
+        f += np.arctan2(nc, a @ b) * (n @ c) / nc
+    return abs(f) / (2.0 * np.pi)
+
+
+def test_config_factor_matches_closed_forms_and_sampling():
+    # element below the corner of a parallel a x b rectangle at height c
+    # (textbook closed form)
+    a, b, c = 3.0, 5.0, 2.0
+    A, B = a / c, b / c
+    closed = (A / np.sqrt(1 + A * A) * np.arctan(B / np.sqrt(1 + A * A))
+              + B / np.sqrt(1 + B * B) * np.arctan(A / np.sqrt(1 + B * B))) / (2 * np.pi)
+    got = config_factor((0, 0, 0), (0, 0, 1), [(0, 0, c), (a, 0, c), (a, b, c), (0, b, c)])
+    assert abs(got - closed) < 1e-12
+    # a huge wall standing on the element's horizon takes half the cosine lobe
+    big = 1e7
+    half = config_factor((0, 0, 0), (0, 1, 0), [(1, 0, -big), (1, big, -big), (1, big, big), (1, 0, big)])
+    assert abs(half - 0.5) < 1e-6
+    # a finite perpendicular wall against cosine-weighted sampling (numpy)
+    rng = np.random.default_rng(3)
+    u1, u2 = rng.random(2_000_000), rng.random(2_000_000)
+    r, phi = np.sqrt(u1), 2 * np.pi * u2
+    d = np.stack([r * np.cos(phi), np.sqrt(1 - u1), r * np.sin(phi)], 1)  # cosine lobe about +y
+    t = 4.0 / d[:, 0]
+    hit = (d[:, 0] > 0) & (t * d[:, 1] <= 6.0) & (np.abs(t * d[:, 2] - 1.0) <= 3.0)
+    f = config_factor((0, 0, 0), (0, 1, 0), [(4, 0, -2), (4, 6, -2), (4, 6, 4), (4, 0, 4)])
+    assert abs(hit.mean() - f) < 4 * np.sqrt(f * (1 - f) / hit.size)
+
+
+def test_irradiance_magnitude_beside_a_sunlit_wall():
+    """Closed-form magnitude of the estimator: floor pixels beside a tall,
+    wide wall whose -x face alone is sunlit. A cosine-weighted bounce ray
+    from the floor's origin P (the hit point plus sqrt(3) along +y) sees
+    only that face, so E(P) = F(P) a_wall (n_wall . s) L / pi with F the
+    float64 configuration factor from P to the face above P's horizon, and
+    the radiance is a_floor * E per channel. The oracle's mean over 16
+    frames x 64 spp per pixel must agree within 3 sigma of the binomial
+    sampling error; removing 1/pi, the wall albedo or the cosine changes it
+    by far more, and removing the primary albedo fails the per-pixel
+    product."""
+    full = np.full(16, 0xFFFFFFFF, np.uint32)
+    objs = [{"n": (4, 1, 4), "translation": (-8.0, -4.0, 0.0)},     # floor: x -24..8, y -8..0, z -16..16
+            {"n": (1, 12, 12), "translation": (20.0, 40.0, 0.0)}]   # wall: x 16..24, y -8..88, z -48..48
+    n_floor, n_wall = 16, 144
+    sc = S.make_scene(objs, bits_per_cluster=[full] * (n_floor + n_wall), seed=9)
+    sc.materials[:512 * n_floor] = 1
+    sc.materials[512 * n_floor:] = 2
+    a_floor = np.array([0.5, 0.7, 0.9], np.float32)
+    a_wall = np.array([0.8, 0.6, 0.3], np.float32)
+    sc.luts["albedo"][1], sc.luts["albedo"][2] = a_floor, a_wall
+    cam = T.make_camera((-8.0, 40.0, 0.0), pitch=-1.5, yaw=0.0, roll=0.0, fov_y=0.5, aspect=1.0)
+    sun = np.array([-0.6, 0.7, 0.3])
+    w = h = 16
+    spp, frames = 64, 16
+    os_ = O.OracleScene(sc)
+    vis, _ = O.visibility_culled(os_, cam, w, h)
+    svo = O.OracleSvo(os_, (-64.0, -64.0, -64.0), 8)
+    irr_sum = np.zeros((h, w, 3), np.float64)
+    for f in range(frames):
+        prm = T.gi_params(sun, spp=spp, frame=f)
+        rad, irr, cnt = svo.gi(cam, w, h, vis, prm)
+        assert cnt[0] == w * h
+        np.testing.assert_array_equal(rad, irr * a_floor)  # L = a_p E, per channel, in fp32
+        irr_sum += irr
+    s32 = prm["sun_dir"][0].astype(np.float64)
+    unit = a_wall.astype(np.float64) * (-s32[0]) * 1.0 / np.pi  # contribution of one lit sample
+    expect = np.zeros(3)
+    var = f_sum = 0.0
+    for y in range(h):
+        for x in range(w):
+            ok, p, n, a = _surface(os_, cam, w, h, x, y, int(vis[y, x]))
+            assert ok and n[1] == 1.0 and np.array_equal(a, a_floor)
+            o = p.astype(np.float64) + np.sqrt(3.0) * n.astype(np.float64)
+            F = config_factor(o, (0, 1, 0), [(16, o[1], -48), (16, 88, -48), (16, 88, 48), (16, o[1], 48)])
+            expect += F * unit
+            f_sum += F
+            var += F * (1 - F) / (spp * frames)
+    got = irr_sum.sum(axis=(0, 1)) / frames
+    sigma = np.sqrt(var) * unit
+    assert np.all(np.abs(got - expect) < 3 * sigma + 1e-4 * expect), (got, expect, sigma)
+    assert 0.2 < f_sum / (w * h) < 0.5  # the wall takes a sizeable share of the lobe

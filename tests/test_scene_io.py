@@ -64,3 +64,24 @@ def test_corrupt_truncated_and_foreign_files_are_rejected(tmp_path):
     with pytest.raises(T.TgxError) as e:
         T.load_scene(tmp_path / "missing.tgxs")
     assert e.value.code == 8  # TGX_ERR_IO
+
+
+def test_load_refuses_counts_that_differ_from_the_header(tmp_path):
+    """A file replaced between tgx_scene_file_counts and tgx_scene_load (here:
+    counts of a smaller scene) is refused before any caller array is written."""
+    small = T.generate_scene(T.single_object_params(1))
+    big = T.generate_scene(T.noise_objects_params(seed=5, n_objects=30, max_extent=6, world=128.0, n_luts=3))
+    path = tmp_path / "s.tgxs"
+    T.save_scene(path, big)
+    lib = T.load_library()
+    counts = np.zeros(1, T.SCENE_COUNTS_DTYPE)
+    counts["n_objects"], counts["n_cluster_pointers"] = small.objects.size, small.n_cluster_pointers
+    counts["n_clusters"], counts["n_luts"] = small.n_clusters, small.luts.size // 256
+    guard = 4096
+    arrays = [np.zeros(small.objects.size + guard, T.OBJECT_DTYPE),
+              np.zeros(small.n_cluster_pointers + guard, np.uint32), np.zeros(small.n_clusters + guard, np.uint32),
+              np.zeros(16 * small.n_clusters + guard, np.uint32), np.zeros(512 * small.n_clusters + guard, np.uint8),
+              np.zeros(small.luts.size + guard, T.LUT_DTYPE)]
+    rc = lib.tgx_scene_load(str(path).encode(), counts.ctypes.data, *[a.ctypes.data for a in arrays])
+    assert rc == 1  # TGX_ERR_INVALID_ARG
+    assert not any(a.view(np.uint8).any() for a in arrays)

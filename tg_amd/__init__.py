@@ -251,7 +251,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_object_destroy": (i32, [vp, u32]),
         "tgx_scene_save": (i32, [ctypes.c_char_p, ctypes.POINTER(_SceneDesc)]),
         "tgx_scene_file_counts": (i32, [ctypes.c_char_p, vp]),
-        "tgx_scene_load": (i32, [ctypes.c_char_p, vp, vp, vp, vp, vp, vp]),
+        "tgx_scene_load": (i32, [ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp]),
         "tgx_frame_graph_launch": (i32, [vp, vp]),
         "tgx_frame_graph_stats": (i32, [vp, vp, vp]),
         "tgx_frame_graph_destroy": (i32, [vp, vp]),
@@ -347,18 +347,18 @@ def save_scene(path, scene: "Scene") -> None:
 def load_scene(path) -> "Scene":
     """Reads a scene file (tgx_scene_file_counts + tgx_scene_load)."""
     lib = load_library()
-    c = np.zeros(1, SCENE_COUNTS_DTYPE)
-    rc = lib.tgx_scene_file_counts(str(path).encode(), _ptr(c))
+    counts = np.zeros(1, SCENE_COUNTS_DTYPE)
+    rc = lib.tgx_scene_file_counts(str(path).encode(), _ptr(counts))
     if rc:
         raise TgxError(rc, f"tgx_scene_file_counts({path})")
-    c = c[0]
+    c = counts[0]
     s = Scene(objects=np.zeros(int(c["n_objects"]), OBJECT_DTYPE),
               cluster_pointers=np.zeros(int(c["n_cluster_pointers"]), np.uint32),
               cluster_idx_to_object=np.zeros(int(c["n_clusters"]), np.uint32),
               bits=np.zeros(16 * int(c["n_clusters"]), np.uint32),
               materials=np.zeros(512 * int(c["n_clusters"]), np.uint8),
               luts=np.zeros(256 * int(c["n_luts"]), LUT_DTYPE))
-    rc = lib.tgx_scene_load(str(path).encode(), _ptr(s.objects), _ptr(s.cluster_pointers),
+    rc = lib.tgx_scene_load(str(path).encode(), _ptr(counts), _ptr(s.objects), _ptr(s.cluster_pointers),
                             _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials), _ptr(s.luts))
     if rc:
         raise TgxError(rc, f"tgx_scene_load({path})")

@@ -72,11 +72,13 @@ int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t clust
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "capacities, LUTs and n_luts must be non-zero");
     if ((uint64_t)cluster_capacity >= (1ull << TGX_CLUSTER_POINTER_BITS))
         return tgx_fail(ctx, TGX_ERR_CAPACITY, "%u clusters exceed the 31-bit pointer field", cluster_capacity);
+    int rc;
+    if ((rc = tgx_scene_replace_begin(ctx, "tgx_scene_reserve"))) return rc;
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     ctx->has_scene = false;
+    ctx->has_materials = false;
     ctx->dynamic = false;
     ctx->svo_prev_valid = false;
-    int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->objects, sizeof(tgx_object) * (size_t)object_capacity, "objects")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->cluster_pointers, 4ull * cluster_capacity, "cluster pointers")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->cluster_idx_to_object, 4ull * cluster_capacity, "cluster->object")) ||
@@ -103,6 +105,7 @@ int tgx_scene_reserve(tgx_context* ctx, uint32_t object_capacity, uint32_t clust
     ctx->n_clusters = cluster_capacity;
     ctx->n_cluster_pointers = 0;
     ctx->n_luts = n_luts;
+    ctx->has_materials = true;
     ctx->has_scene = true;
     ctx->dynamic = true;
     return TGX_OK;
@@ -240,7 +243,7 @@ int tgx_scene_get_counts(tgx_context* ctx, tgx_scene_counts* out)
     out->n_cluster_pointers = ctx->n_cluster_pointers;
     out->n_clusters = ctx->n_clusters;
     out->n_luts = ctx->n_luts;
-    out->has_materials = ctx->materials.ptr ? 1u : 0u;
+    out->has_materials = ctx->has_materials ? 1u : 0u;
     return TGX_OK;
 }
 

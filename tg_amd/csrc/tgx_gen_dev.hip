@@ -89,8 +89,10 @@ int tgx_gen_scene_device(tgx_context* ctx, const tgx_gen_params* p)
         go[o] = {objs[o].first_cluster_pointer, objs[o].n_cluster_pointers_per_dim[0],
                  objs[o].n_cluster_pointers_per_dim[1], objs[o].n_cluster_pointers_per_dim[2], off[3ull * o],
                  off[3ull * o + 1], off[3ull * o + 2], 0.0};
+    if ((rc = tgx_scene_replace_begin(ctx, "tgx_gen_scene_device"))) return rc;
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     ctx->has_scene = false;
+    ctx->has_materials = false;
     ctx->dynamic = false;
     ctx->svo_prev_valid = false;
     ctx->dyn_objects.clear();
@@ -134,6 +136,7 @@ int tgx_gen_scene_device(tgx_context* ctx, const tgx_gen_params* p)
     ctx->n_cluster_pointers = sz.n_cluster_pointers;
     ctx->n_clusters = sz.n_clusters;
     ctx->n_luts = sz.n_luts;
+    ctx->has_materials = true;
     ctx->has_scene = true;
     return tgx_vis_solid_refresh(ctx);
 }
@@ -150,8 +153,10 @@ int tgx_scene_export(tgx_context* ctx, tgx_object* objects, uint32_t* cluster_po
     if (cluster_idx_to_object)
         TGX_HIP_CHECK(ctx, hipMemcpy(cluster_idx_to_object, ctx->cluster_idx_to_object.ptr, 4ull * ctx->n_clusters, hipMemcpyDeviceToHost));
     if (bits) TGX_HIP_CHECK(ctx, hipMemcpy(bits, ctx->bits.ptr, 64ull * ctx->n_clusters, hipMemcpyDeviceToHost));
-    if (materials && ctx->materials.ptr)
+    if (materials && ctx->has_materials)
         TGX_HIP_CHECK(ctx, hipMemcpy(materials, ctx->materials.ptr, 512ull * ctx->n_clusters, hipMemcpyDeviceToHost));
+    else if (materials)
+        memset(materials, 0, 512ull * ctx->n_clusters);
     if (luts && ctx->luts.ptr)
         TGX_HIP_CHECK(ctx, hipMemcpy(luts, ctx->luts.ptr, sizeof(tgx_lut_entry) * TGX_LUT_ENTRIES * ctx->n_luts, hipMemcpyDeviceToHost));
     return TGX_OK;

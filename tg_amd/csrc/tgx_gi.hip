@@ -1125,7 +1125,8 @@ static int sort_rays(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_de
 int tgx_gi_validate(tgx_context* ctx, uint32_t w, uint32_t h, const tgx_gi_params* prm)
 {
     if (!ctx->has_scene) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no scene uploaded");
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    int rc;
+    if ((rc = tgx_svo_check_current(ctx))) return rc;
     if (!ctx->luts.ptr || ctx->n_luts == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "the scene has no LUTs");
     if (prm->spp < 1 || prm->spp > 64 || !(prm->max_distance > 0.0f))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "spp must be in [1, 64] and max_distance > 0");
@@ -1152,7 +1153,7 @@ int tgx_gi_surfaces_at(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint
         tgx_launch_scope s(ctx, "gi_surface_k");
         hipLaunchKernelGGL(gi_surface_k, dim3(blocks_for(n_pix)), dim3(GI_BLOCK), 0, ctx->stream,
                            (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, tgx_scene_materials(ctx),
                            (const tgx_lut_entry*)ctx->luts.ptr, gc, w, h, vis_dev, surf, flag, radiance_dev,
                            irradiance_dev);
     }
@@ -1303,7 +1304,7 @@ int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* direction
 {
     if (!ctx || (n && (!origins || !directions || !out_host))) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (int rc0 = tgx_svo_check_current(ctx)) return rc0;
     if (n == 0) return TGX_OK;
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_rays, sizeof(svo_ray) * (size_t)n, "svo rays")) ||
@@ -1335,7 +1336,7 @@ int tgx_svo_visibility_device(tgx_context* ctx, const tgx_camera* cam, uint32_t 
     if (!ctx) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
     if (!cam || !vis_dev) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "null argument");
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (int rc0 = tgx_svo_check_current(ctx)) return rc0;
     if (w == 0 || h == 0 || w > 65535 || h > 65535 || (uint64_t)w * h >= (1ull << 31))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "invalid resolution %ux%u", w, h);
     if (!(cam->far_plane > 0.0f)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "far plane must be > 0");

@@ -38,6 +38,7 @@ struct tgx_context {
     uint32_t     n_cluster_pointers = 0;
     uint32_t     n_clusters = 0;
     uint32_t     n_luts = 0;
+    bool         has_materials = false; /* the current scene carries materials (else all 0) */
     tgx_buffer   objects, cluster_pointers, cluster_idx_to_object, bits, materials, luts;
 
     /* Visibility work buffers. */
@@ -174,6 +175,22 @@ extern "C" int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams,
                                        uint32_t h, const tgx_gi_params* gi, const uint64_t* vis_dev,
                                        float* radiance_dev, float* irradiance_dev, uint32_t* acc);
 int  tgx_buffer_reserve(tgx_context* ctx, tgx_buffer* b, size_t bytes, const char* what);
+/* Material bytes of the current scene for the kernels: null when the scene
+ * was uploaded without materials (every voxel then has material 0), never
+ * a buffer left over from an earlier scene. */
+static inline const uint8_t* tgx_scene_materials(const tgx_context* ctx)
+{
+    return ctx->has_materials ? (const uint8_t*)ctx->materials.ptr : nullptr;
+}
+/* Called by every call that replaces the scene (upload, reserve, device
+ * generation) once its arguments are checked: refused while a frame graph
+ * holds the scene's buffers; otherwise the SVO of the previous scene no
+ * longer describes the scene and is marked stale. */
+int  tgx_scene_replace_begin(tgx_context* ctx, const char* what);
+/* TGX_OK if an SVO is built and describes the current scene: tracing calls
+ * refuse an SVO made stale by scene replacement or object edits
+ * (TGX_ERR_NOT_BUILT until tgx_svo_update or tgx_svo_build). */
+int  tgx_svo_check_current(tgx_context* ctx);
 /* Brings the per-cluster solid flags of the visibility cull up to date with
  * the scene's bits (a kernel over every cluster index when they changed). */
 int  tgx_vis_solid_refresh(tgx_context* ctx);

@@ -118,15 +118,24 @@ int tgx_scene_file_counts(const char* path, tgx_scene_counts* out)
     return TGX_OK;
 }
 
-int tgx_scene_load(const char* path, tgx_object* objects, uint32_t* cluster_pointers, uint32_t* cluster_idx_to_object,
-                   uint32_t* bits, uint8_t* materials, tgx_lut_entry* luts)
+int tgx_scene_load(const char* path, const tgx_scene_counts* counts, tgx_object* objects, uint32_t* cluster_pointers,
+                   uint32_t* cluster_idx_to_object, uint32_t* bits, uint8_t* materials, tgx_lut_entry* luts)
 {
-    if (!path || !objects || !cluster_pointers || !cluster_idx_to_object || !bits || !luts) return TGX_ERR_INVALID_ARG;
+    if (!path || !counts || !objects || !cluster_pointers || !cluster_idx_to_object || !bits || !luts)
+        return TGX_ERR_INVALID_ARG;
     io_stream s = {fopen(path, "rb"), 0xCBF29CE484222325ull, 0};
     if (!s.f) return TGX_ERR_IO;
     file_header hd;
     int rc = read_header(&s, &hd);
     if (rc) { fclose(s.f); return rc; }
+    /* the caller's arrays hold exactly `counts`: a header that differs (the
+     * file was replaced since tgx_scene_file_counts) is refused before any
+     * array is written */
+    if (hd.n_objects != counts->n_objects || hd.n_cluster_pointers != counts->n_cluster_pointers ||
+        hd.n_clusters != counts->n_clusters || hd.n_luts != counts->n_luts) {
+        fclose(s.f);
+        return TGX_ERR_INVALID_ARG;
+    }
     get(&s, objects, sizeof(tgx_object) * (size_t)hd.n_objects);
     get(&s, cluster_pointers, 4u * (size_t)hd.n_cluster_pointers);
     get(&s, cluster_idx_to_object, 4u * (size_t)hd.n_clusters);

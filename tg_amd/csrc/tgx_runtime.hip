@@ -77,6 +77,23 @@ tgx_launch_scope::~tgx_launch_scope()
     ctx->pending.push_back({stat_idx, start, stop});
 }
 
+int tgx_scene_replace_begin(tgx_context* ctx, const char* what)
+{
+    if (ctx->n_graphs > 0)
+        return tgx_fail(ctx, TGX_ERR_CAPACITY, "%s: a frame graph holds the scene's buffers: destroy it first", what);
+    ctx->svo_stale = true;
+    return TGX_OK;
+}
+
+int tgx_svo_check_current(tgx_context* ctx)
+{
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    if (ctx->svo_stale)
+        return tgx_fail(ctx, TGX_ERR_NOT_BUILT,
+                        "the scene changed since the SVO was built: call tgx_svo_update or tgx_svo_build");
+    return TGX_OK;
+}
+
 /* Grid-stride 16-byte copy. */
 __global__ __launch_bounds__(256) void tgx_copy_k(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n)
 {
@@ -186,8 +203,11 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
             return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "object %u lut_idx %u >= %u", o, ob->lut_idx,
                             s->n_luts);
     }
-    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     int rc;
+    if ((rc = tgx_scene_replace_begin(ctx, "tgx_scene_upload"))) return rc;
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    ctx->has_scene = false;
+    ctx->has_materials = false;
     ctx->dynamic = false;
     ctx->svo_prev_valid = false;
     ctx->dyn_objects.clear();
@@ -206,6 +226,7 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
     ctx->n_cluster_pointers = s->n_cluster_pointers;
     ctx->n_clusters = s->n_clusters;
     ctx->n_luts = s->luts ? s->n_luts : 0;
+    ctx->has_materials = s->materials != nullptr;
     ctx->has_scene = true;
     return tgx_vis_solid_refresh(ctx);
 }

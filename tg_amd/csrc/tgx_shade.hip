@@ -284,6 +284,8 @@ int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
     if (w == 0 || h == 0 || w > 65535 || h > 65535 || (uint64_t)w * h >= (1ull << 31))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "invalid resolution %ux%u", w, h);
     if (prm->view > TGX_SHADE_SHADING) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "unknown view %u", prm->view);
+    /* every hit pixel reads its LUT entry (shading.frag:132-135) */
+    if (!ctx->luts.ptr || ctx->n_luts == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "the scene has no LUTs");
     shade_cam sc;
     cam_of(cam, sc);
     shade_consts k;
@@ -304,7 +306,7 @@ int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
         tgx_launch_scope s(ctx, "shade_k");
         hipLaunchKernelGGL(shade_k, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
                            (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->materials.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, tgx_scene_materials(ctx),
                            (const tgx_lut_entry*)ctx->luts.ptr, sc, w, h, vis_dev, k, (float4*)rgba_dev);
     }
     TGX_LAUNCH_CHECK(ctx, "shade_k");

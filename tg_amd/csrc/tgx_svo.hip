@@ -817,7 +817,7 @@ static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32
         tgx_launch_scope s(ctx, "svo_leaf_k");
         hipLaunchKernelGGL(svo_leaf_k, dim3(n_leaves), dim3(256), 0, ctx->stream, nodes, lvl_base[leaf_level],
                            lvl_scptr[leaf_level].as<uint32_t>(), lvl_first[leaf_level].as<uint32_t>(), n_leaf_pairs,
-                           n_leaves, frames, (const uint32_t*)ctx->bits.ptr, (const uint8_t*)ctx->materials.ptr,
+                           n_leaves, frames, (const uint32_t*)ctx->bits.ptr, tgx_scene_materials(ctx),
                            mn[0], mn[1], mn[2], (flags & TGX_SVO_FLAG_REFERENCE_CORNER_QUIRK) ? 1u : 0u,
                            (uint32_t*)ctx->svo_leaf_bits.ptr, (uint16_t*)ctx->svo_leaf_mats.ptr,
                            (uint32_t*)ctx->svo_leaf_n.ptr, list_tmp.as<uint32_t>(), nodes,
@@ -991,7 +991,8 @@ int tgx_svo_save(tgx_context* ctx, const char* path)
 {
     if (!ctx || !path) return TGX_ERR_INVALID_ARG;
     ctx->last_error.clear();
-    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    /* the file records the scene's counts: a stale SVO would not match them */
+    if (int rc0 = tgx_svo_check_current(ctx)) return rc0;
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const tgx_svo_info in = ctx->svo_info;
