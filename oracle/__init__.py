@@ -63,6 +63,7 @@ def load() -> ctypes.CDLL:
         "tgxo_gi_surface": (i32, [vp, vp, u32, u32, u32, u32, u64, vp, vp, vp]),
         "tgxo_gi_seed": (u32, [u32, u32, u32, u32]),
         "tgxo_gi_direction": (None, [vp, u32, vp]),
+        "tgxo_gi_alg_bytes": (i32, [vp, vp, u32, u32, vp, vp, u32, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -229,6 +230,19 @@ class OracleSvo:
         cnt = np.zeros(4, np.uint64)
         load().tgxo_gi(self.h, _p(cam), w, h, _p(vis), _p(params), row_step, _p(rad), _p(irr), _p(cnt))
         return rad, irr, cnt
+
+    def gi_alg_bytes(self, cam: np.ndarray, w: int, h: int, vis: np.ndarray, params: np.ndarray,
+                     with_irradiance: bool = False) -> dict:
+        """Algorithmic bytes of the GI frame (tgxo_gi_alg_bytes): unique 64-B
+        lines the oracle's execution reads, per part, plus bytes written."""
+        vis = np.ascontiguousarray(vis, np.uint64)
+        out = np.zeros(16, np.uint64)
+        if load().tgxo_gi_alg_bytes(self.h, _p(cam), w, h, _p(vis), _p(params), int(with_irradiance), _p(out)):
+            raise RuntimeError("tgxo_gi_alg_bytes: allocation failed")
+        names = ("nodes", "leaf_data", "voxel_pool", "leaf_materials", "luts_hits", "cluster_pointers",
+                 "cluster_to_object", "objects", "scene_materials", "luts_surfaces", "visibility")
+        return {"surface": int(out[0]), "trace": int(out[1]), "written": int(out[2]), "total": int(out[3]),
+                "lines": {k: int(v) for k, v in zip(names, out[4:15])}}
 
     def trace(self, origins: np.ndarray, dirs: np.ndarray, max_distance: float = 3.0e38) -> np.ndarray:
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)

@@ -92,6 +92,23 @@ void      tgxo_svo_trace(const tgxo_svo* svo, const float* origins, const float*
 void      tgxo_gi(const tgxo_svo* svo, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
                   const tgx_gi_params* params, uint32_t row_step, float* radiance, float* irradiance,
                   uint64_t counts[4]);
+/* Algorithmic bytes of one GI frame (the frame of tgxo_gi at row_step 1):
+ * 64 x the unique 64-B lines the oracle's execution reads, per array in
+ * the reference's layouts, plus the bytes it writes. out[0] = surfaces:
+ * the visibility words and the scene lines of visible pixels (pointer map,
+ * cluster map, object record, material); out[1] = traversals: SVO node
+ * (u32 each, build order), leaf data (260 B) and voxel-pool (4 KiB per
+ * leaf) lines of every bounce and shadow ray, the leaf-material (2 B per
+ * voxel) and LUT lines of lit bounce hits; out[2] =
+ * radiance written (12 B per pixel, 24 with irradiance); out[3] = total.
+ * out[4..14] = the line counts: nodes, leaf data, voxel pool, leaf
+ * materials, LUT (hits), pointer map, cluster map, objects, scene
+ * materials, LUT (surfaces), visibility words. A line two kernels read
+ * counts in both parts. Returns 0, or -1 if a bitmap could not be
+ * allocated. */
+int       tgxo_gi_alg_bytes(const tgxo_svo* svo, const tgx_camera* cam, uint32_t w, uint32_t h,
+                            const uint64_t* vis, const tgx_gi_params* params, uint32_t with_irradiance,
+                            uint64_t out[16]);
 int       tgxo_gi_surface(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h, uint32_t px,
                           uint32_t py, uint64_t word, float out_p[3], float out_n[3], float out_albedo[3]);
 uint32_t  tgxo_gi_seed(uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t frame);

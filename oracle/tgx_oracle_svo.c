@@ -222,6 +222,32 @@ struct tgxo_svo {
     int       failed;
 };
 
+/* Line recorder of the algorithmic-bytes count (tgxo_gi_alg_bytes): one
+ * bit per 64-B line of each array the GI frame reads, in the reference's
+ * layouts. NULL in every parity path. */
+typedef struct line_rec {
+    uint64_t* node;   /* nodes, u32 each in build order (tg_sparse_voxel_octree.h:6-11) */
+    uint64_t* leafd;  /* leaf data, 260 B each: n and the cluster list (:13-17) */
+    uint64_t* bits;   /* voxel pool, 4096 B per leaf (:121-126) */
+    uint64_t* mats;   /* leaf materials, 2 B per voxel */
+    uint64_t* luts;   /* LUT entries of lit bounce hits, 16 B */
+    uint64_t* lutsp;  /* LUT entries of primary surfaces */
+    uint64_t* cptr;   /* cluster pointers, 4 B */
+    uint64_t* c2o;    /* cluster -> object, 4 B */
+    uint64_t* obj;    /* object records, 96 B */
+    uint64_t* smat;   /* scene materials, 1 B per voxel */
+} line_rec;
+
+static void mark_line(uint64_t* bm, uint64_t line)
+{
+    if (!bm) return;
+    const uint64_t bit = 1ull << (line & 63u);
+    /* read first: most lines are marked already, and a shared read keeps
+     * the cache line from bouncing between threads */
+    if (!(__atomic_load_n(&bm[line >> 6], __ATOMIC_RELAXED) & bit))
+        __atomic_fetch_or(&bm[line >> 6], bit, __ATOMIC_RELAXED);
+}
+
 static int grow(void** p, uint32_t* cap, uint32_t need, size_t elem)
 {
     if (need <= *cap) return 1;
@@ -554,7 +580,8 @@ static int intersect_ray_aabb(v3 o, v3 d, v3 mn, v3 mx, float* enter, float* exi
 
 /* tg_amanatides_woo over a 32^3 block; the entry voxel is clamped into the
  * block on both sides. Returns 1 with the voxel coordinates on a hit. */
-static int block_dda(v3 h, v3 d, const uint32_t* grid, int* ox, int* oy, int* oz)
+static int block_dda(v3 h, v3 d, const uint32_t* grid, int* ox, int* oy, int* oz, uint64_t* rec_bits,
+                     uint64_t rec_base)
 {
     const float e = 32.0f;
     const float em1 = e - 1.0f;
@@ -573,6 +600,7 @@ static int block_dda(v3 h, v3 d, const uint32_t* grid, int* ox, int* oy, int* oz
     else if (d.z < 0.0f) { sz = -1; tmz = (h.z - (float)z) / -d.z; tdz = 1.0f / -d.z; }
     for (uint32_t it = 0; it < 4u * TGX_DDA_MAX_ITERATIONS; it++) {
         const uint32_t v = 1024u * (uint32_t)z + 32u * (uint32_t)y + (uint32_t)x;
+        mark_line(rec_bits, rec_base + (v >> 9)); /* 16 words per line */
         if (grid[v >> 5] & (1u << (v & 31u))) { *ox = x; *oy = y; *oz = z; return 1; }
         if (tmx < tmy) {
             if (tmx < tmz) { tmx += tdx; x += sx; if (x < 0 || x >= 32) break; }
@@ -601,8 +629,17 @@ static void entry_normal(v3 o, v3 d, v3 mn, v3 mx, int32_t nrm[3])
     nrm[ax] = dc > 0.0f ? -1 : 1;
 }
 
+static void traverse_rec(const tgxo_svo* s, const float origin[3], const float dir[3], float max_distance,
+                         tgx_svo_hit* out, const line_rec* rec);
+
 void tgxo_svo_traverse(const tgxo_svo* s, const float origin[3], const float dir[3], float max_distance,
                        tgx_svo_hit* out)
+{
+    traverse_rec(s, origin, dir, max_distance, out, NULL);
+}
+
+static void traverse_rec(const tgxo_svo* s, const float origin[3], const float dir[3], float max_distance,
+                         tgx_svo_hit* out, const line_rec* rec)
 {
     memset(out, 0, sizeof(*out));
     out->distance = F32_MAX;
@@ -632,6 +669,7 @@ void tgxo_svo_traverse(const tgxo_svo* s, const float origin[3], const float dir
         const uint32_t pi = st_idx[ss - 1];
         const v3 pmn = st_min[ss - 1], pmx = st_max[ss - 1];
         const onode* pn = &s->nodes[pi];
+        if (rec) mark_line(rec->node, (uint64_t)pi * 4u / 64u);
         const v3 ce = v3_mulf(v3_sub(pmx, pmn), 0.5f);
         uint32_t rc = 0;
         v3 cmin = pmn;
@@ -645,10 +683,15 @@ void tgxo_svo_traverse(const tgxo_svo* s, const float origin[3], const float dir
             const uint32_t ci = pn->first_child + off;
             if (pn->leaf & (1u << rc)) {
                 const uint32_t li = s->nodes[ci].leaf_idx;
+                if (rec) {
+                    mark_line(rec->node, (uint64_t)ci * 4u / 64u);
+                    mark_line(rec->leafd, (uint64_t)li * 260u / 64u);
+                }
                 if (s->leaf_n[li] != 0) {
                     const v3 hg = v3_sub(pos, cmin);
                     int vx, vy, vz;
-                    if (block_dda(hg, d, &s->bits[(size_t)li * TGX_SVO_BLOCK_WORDS], &vx, &vy, &vz)) {
+                    if (block_dda(hg, d, &s->bits[(size_t)li * TGX_SVO_BLOCK_WORDS], &vx, &vy, &vz,
+                                  rec ? rec->bits : NULL, (uint64_t)li * 64u)) {
                         const v3 vmn = v3_add(cmin, (v3){(float)vx, (float)vy, (float)vz});
                         const v3 vmx = v3_add(vmn, (v3){1.0f, 1.0f, 1.0f});
                         (void)intersect_ray_aabb(pos, d, vmn, vmx, &enter, &exit_);
@@ -829,9 +872,9 @@ int tgxo_gi_surface(const tgxo_scene* sc, const tgx_camera* cam, uint32_t w, uin
 /* Leaf index of the hit's block (binary search over the canonical order
  * is not needed: the traversal below records it). */
 static void traverse_leaf(const tgxo_svo* s, const float o[3], const float d[3], float maxd, tgx_svo_hit* h,
-                          uint32_t* leaf)
+                          uint32_t* leaf, const line_rec* rec)
 {
-    tgxo_svo_traverse(s, o, d, maxd, h);
+    traverse_rec(s, o, d, maxd, h, rec);
     *leaf = 0xFFFFFFFFu;
     if (!h->hit) return;
     /* locate the leaf node of the hit block by descending from the root */
@@ -846,8 +889,35 @@ static void traverse_leaf(const tgxo_svo* s, const float o[3], const float d[3],
     *leaf = s->nodes[ni].leaf_idx;
 }
 
+static void gi_impl(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
+                    const tgx_gi_params* prm, uint32_t row_step, float* radiance, float* irradiance,
+                    uint64_t counts[4], const line_rec* rec);
+
 void tgxo_gi(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
              const tgx_gi_params* prm, uint32_t row_step, float* radiance, float* irradiance, uint64_t counts[4])
+{
+    gi_impl(s, cam, w, h, vis, prm, row_step, radiance, irradiance, counts, NULL);
+}
+
+/* Marks the scene lines tgxo_gi_surface reads for the visibility word of a
+ * visible pixel (decode, maps, object record, material, LUT entry). */
+static void mark_surface(const tgxo_scene* sc, uint64_t word, const line_rec* rec)
+{
+    const uint32_t cptr = (uint32_t)(word >> 9) & 0x7FFFFFFFu, v = (uint32_t)word & 511u;
+    const uint32_t cidx = sc->cluster_pointers[cptr];
+    const uint32_t obj = sc->cluster_idx_to_object[cidx];
+    mark_line(rec->cptr, (uint64_t)cptr * 4u / 64u);
+    mark_line(rec->c2o, (uint64_t)cidx * 4u / 64u);
+    mark_line(rec->obj, (uint64_t)obj * 96u / 64u);
+    mark_line(rec->obj, ((uint64_t)obj * 96u + 95u) / 64u);
+    const uint32_t mat = sc->materials ? sc->materials[(size_t)cidx * 512u + v] : 0u;
+    if (sc->materials) mark_line(rec->smat, ((uint64_t)cidx * 512u + v) / 64u);
+    mark_line(rec->lutsp, ((uint64_t)sc->objects[obj].lut_idx * TGX_LUT_ENTRIES + mat) * 16u / 64u);
+}
+
+static void gi_impl(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
+                    const tgx_gi_params* prm, uint32_t row_step, float* radiance, float* irradiance,
+                    uint64_t counts[4], const line_rec* rec)
 {
     const tgxo_scene* sc = s->scene;
     const v3 sun = {prm->sun_dir[0], prm->sun_dir[1], prm->sun_dir[2]};
@@ -863,6 +933,7 @@ void tgxo_gi(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, c
             radiance[3 * pix] = radiance[3 * pix + 1] = radiance[3 * pix + 2] = 0.0f;
             if (irradiance) irradiance[3 * pix] = irradiance[3 * pix + 1] = irradiance[3 * pix + 2] = 0.0f;
             if (!tgxo_gi_surface(sc, cam, w, h, x, y, vis[pix], p, nrm, alb)) continue;
+            if (rec) mark_surface(sc, vis[pix], rec);
             n_vis++;
             const v3 n = {nrm[0], nrm[1], nrm[2]};
             for (uint32_t k = 0; k < prm->spp; k++) {
@@ -872,7 +943,7 @@ void tgxo_gi(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, c
                 const float dd[3] = {d.x, d.y, d.z};
                 tgx_svo_hit hb;
                 uint32_t leaf;
-                traverse_leaf(s, ob, dd, prm->max_distance, &hb, &leaf);
+                traverse_leaf(s, ob, dd, prm->max_distance, &hb, &leaf, rec);
                 n_b++;
                 if (!hb.hit) continue;
                 n_bh++;
@@ -884,10 +955,14 @@ void tgxo_gi(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, c
                 os[ax] = os[ax] + (float)hb.normal[ax] * SHADOW_OFFSET;
                 const float sd[3] = {sun.x, sun.y, sun.z};
                 tgx_svo_hit hs;
-                tgxo_svo_traverse(s, os, sd, prm->max_distance, &hs);
+                traverse_rec(s, os, sd, prm->max_distance, &hs, rec);
                 n_sh++;
                 if (hs.hit) continue;
                 const uint16_t m = s->mats[(size_t)leaf * TGX_SVO_BLOCK_VOXELS + hb.voxel];
+                if (rec) {
+                    mark_line(rec->mats, ((uint64_t)leaf * TGX_SVO_BLOCK_VOXELS + hb.voxel) * 2u / 64u);
+                    mark_line(rec->luts, ((uint64_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)) * 16u / 64u);
+                }
                 const tgx_lut_entry* e = &sc->luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
                 const float wgt = cosv * prm->sun_radiance * INV_PI;
                 acc[0] = acc[0] + e->albedo[0] * wgt;
@@ -903,4 +978,61 @@ void tgxo_gi(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, c
         }
     }
     counts[0] = n_vis; counts[1] = n_b; counts[2] = n_bh; counts[3] = n_sh;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Algorithmic bytes of a GI frame                                          */
+/* ------------------------------------------------------------------------ */
+
+static uint64_t* line_bitmap(uint64_t bytes)
+{
+    return (uint64_t*)calloc((size_t)((bytes + 63u) / 64u / 64u + 1u), sizeof(uint64_t));
+}
+
+static uint64_t count_lines(const uint64_t* bm, uint64_t bytes)
+{
+    uint64_t n = 0;
+    for (uint64_t i = 0; i < (bytes + 63u) / 64u / 64u + 1u; i++) n += (uint64_t)__builtin_popcountll(bm[i]);
+    return n;
+}
+
+int tgxo_gi_alg_bytes(const tgxo_svo* s, const tgx_camera* cam, uint32_t w, uint32_t h, const uint64_t* vis,
+                      const tgx_gi_params* prm, uint32_t with_irradiance, uint64_t out[16])
+{
+    const tgxo_scene* sc = s->scene;
+    const uint64_t b_node = 4ull * s->n_nodes, b_leafd = 260ull * s->n_leaves;
+    const uint64_t b_bits = 4096ull * s->n_leaves, b_mats = 2ull * TGX_SVO_BLOCK_VOXELS * s->n_leaves;
+    const uint64_t b_luts = 16ull * TGX_LUT_ENTRIES * (sc->n_luts ? sc->n_luts : 1u);
+    const uint64_t b_cptr = 4ull * sc->n_cluster_pointers, b_c2o = 4ull * sc->n_clusters;
+    const uint64_t b_obj = 96ull * sc->n_objects, b_smat = 512ull * sc->n_clusters;
+    line_rec r = {line_bitmap(b_node), line_bitmap(b_leafd), line_bitmap(b_bits), line_bitmap(b_mats),
+                  line_bitmap(b_luts), line_bitmap(b_luts), line_bitmap(b_cptr), line_bitmap(b_c2o),
+                  line_bitmap(b_obj), line_bitmap(b_smat)};
+    float* rad = (float*)malloc(sizeof(float) * 3u * (size_t)w * h);
+    int ok = r.node && r.leafd && r.bits && r.mats && r.luts && r.lutsp && r.cptr && r.c2o && r.obj && r.smat && rad;
+    if (ok) {
+        uint64_t counts[4];
+        gi_impl(s, cam, w, h, vis, prm, 1u, rad, NULL, counts, &r);
+        const uint64_t P = (uint64_t)w * h;
+        uint64_t* n = out + 4;
+        n[0] = count_lines(r.node, b_node);
+        n[1] = count_lines(r.leafd, b_leafd);
+        n[2] = count_lines(r.bits, b_bits);
+        n[3] = count_lines(r.mats, b_mats);
+        n[4] = count_lines(r.luts, b_luts);
+        n[5] = count_lines(r.cptr, b_cptr);
+        n[6] = count_lines(r.c2o, b_c2o);
+        n[7] = count_lines(r.obj, b_obj);
+        n[8] = count_lines(r.smat, b_smat);
+        n[9] = count_lines(r.lutsp, b_luts);
+        n[10] = (8u * P + 63u) / 64u; /* every visibility word is read once */
+        n[11] = 0;
+        out[0] = 64u * (n[10] + n[5] + n[6] + n[7] + n[8] + n[9]);
+        out[1] = 64u * (n[0] + n[1] + n[2] + n[3] + n[4]);
+        out[2] = (with_irradiance ? 24u : 12u) * P;
+        out[3] = out[0] + out[1] + out[2];
+    }
+    free(r.node); free(r.leafd); free(r.bits); free(r.mats); free(r.luts); free(r.lutsp);
+    free(r.cptr); free(r.c2o); free(r.obj); free(r.smat); free(rad);
+    return ok ? 0 : -1;
 }

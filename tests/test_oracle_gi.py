@@ -249,3 +249,72 @@ def test_irradiance_magnitude_beside_a_sunlit_wall():
     sigma = np.sqrt(var) * unit
     assert np.all(np.abs(got - expect) < 3 * sigma + 1e-4 * expect), (got, expect, sigma)
     assert 0.2 < f_sum / (w * h) < 0.5  # the wall takes a sizeable share of the lobe
+
+
+def test_alg_bytes_surface_part_and_written_bytes_exact():
+    """The surface part of tgxo_gi_alg_bytes recounted in numpy: every
+    visibility word, and for each visible pixel the lines of its cluster
+    pointer, cluster map entry, object record (96 B, up to two lines), voxel
+    material and LUT entry."""
+    p = T.noise_objects_params(seed=6, n_objects=30, max_extent=5, world=96.0, n_luts=3)
+    sc = T.generate_scene(p)
+    w, h = 64, 48
+    cam = T.generate_camera(p, 0, w, h)
+    os_ = O.OracleScene(sc)
+    vis, _ = O.visibility_culled(os_, cam, w, h)
+    svo = O.OracleSvo(os_, np.floor(cam[0]["camera"][:3]) - 64.0, 7)
+    prm = T.gi_params(T.seeded_sun(6), spp=2)
+    b = svo.gi_alg_bytes(cam, w, h, vis, prm)
+    lines = {k: set() for k in ("cptr", "c2o", "obj", "mat", "lut")}
+    for y in range(h):
+        for x in range(w):
+            word = int(vis[y, x])
+            if not _surface(os_, cam, w, h, x, y, word)[0]:
+                continue
+            cptr, v = (word >> 9) & 0x7FFFFFFF, word & 511
+            cidx = int(sc.cluster_pointers[cptr])
+            obj = int(sc.cluster_idx_to_object[cidx])
+            lines["cptr"].add(cptr * 4 // 64)
+            lines["c2o"].add(cidx * 4 // 64)
+            lines["obj"] |= {obj * 96 // 64, (obj * 96 + 95) // 64}
+            lines["mat"].add((cidx * 512 + v) // 64)
+            m = int(sc.materials[cidx * 512 + v])
+            lines["lut"].add((int(sc.objects[obj]["lut_idx"]) * 256 + m) * 16 // 64)
+    assert len(lines["cptr"]) > 3
+    expect = 64 * (-(-8 * w * h // 64) + sum(len(s) for s in lines.values()))
+    assert b["surface"] == expect
+    assert b["lines"]["luts_surfaces"] == len(lines["lut"]) and b["lines"]["objects"] == len(lines["obj"])
+    assert b["written"] == 12 * w * h
+    assert svo.gi_alg_bytes(cam, w, h, vis, prm, with_irradiance=True)["written"] == 24 * w * h
+    info = svo.info()
+    n = b["lines"]
+    assert 1 <= n["nodes"] <= -(-4 * int(info["n_nodes"]) // 64)
+    assert 1 <= n["voxel_pool"] <= 64 * int(info["n_leaves"])
+    assert n["leaf_data"] <= int(info["n_leaves"]) and n["leaf_materials"] <= 1024 * int(info["n_leaves"])
+    assert b["trace"] == 64 * (n["nodes"] + n["leaf_data"] + n["voxel_pool"] + n["leaf_materials"] + n["luts_hits"])
+    assert b["total"] == b["surface"] + b["trace"] + b["written"]
+
+
+def test_alg_bytes_hit_shading_lines():
+    """Traversal part: in the closed box no bounce hit is lit, so no leaf
+    material or LUT line is read for hits while the voxel pool is; beside the
+    sunlit wall every lit hit has the wall's one material, one LUT line."""
+    sc = scene_from_grid(np.pad(np.zeros((28, 28, 28), bool), 2, constant_values=True), (0.0, 0.0, 0.0))
+    cam = T.make_camera((0.5, 0.5, 0.5), pitch=0.2, yaw=0.7, roll=0.0, fov_y=1.4, aspect=1.0)
+    os_ = O.OracleScene(sc)
+    vis, _ = O.visibility_culled(os_, cam, 32, 32)
+    b = O.OracleSvo(os_, (-64.0, -64.0, -64.0), 7).gi_alg_bytes(cam, 32, 32, vis, T.gi_params(spp=4))
+    assert b["lines"]["leaf_materials"] == 0 and b["lines"]["luts_hits"] == 0
+    assert b["lines"]["voxel_pool"] > 0 and b["lines"]["leaf_data"] > 0
+    full = np.full(16, 0xFFFFFFFF, np.uint32)
+    objs = [{"n": (4, 1, 4), "translation": (-8.0, -4.0, 0.0)}, {"n": (1, 12, 12), "translation": (20.0, 40.0, 0.0)}]
+    sc = S.make_scene(objs, bits_per_cluster=[full] * 160, seed=9)
+    sc.materials[:512 * 16] = 1
+    sc.materials[512 * 16:] = 2
+    cam = T.make_camera((-8.0, 40.0, 0.0), pitch=-1.5, yaw=0.0, roll=0.0, fov_y=0.5, aspect=1.0)
+    os_ = O.OracleScene(sc)
+    vis, _ = O.visibility_culled(os_, cam, 16, 16)
+    b = O.OracleSvo(os_, (-64.0, -64.0, -64.0), 8).gi_alg_bytes(cam, 16, 16, vis,
+                                                                T.gi_params((-0.6, 0.7, 0.3), spp=16))
+    assert b["lines"]["luts_hits"] == 1 and b["lines"]["luts_surfaces"] == 1
+    assert b["lines"]["leaf_materials"] > 0
