@@ -287,3 +287,63 @@ def test_svo_file_rejects_bad_files(ctx, tmp_path):
     ctx.svo_load(path)
     for k in ("nodes", "bits", "materials", "list"):
         assert np.array_equal(ctx.svo_export()[k], good[k])
+
+
+def test_svo_depth13_far_corner_bit_exact(ctx):
+    """Depth 13 (root side 8192): inner nodes sit at levels up to 7, whose
+    coordinates reach 127, the largest value the traversal's 7-bit node
+    fields hold. Objects at the root's far corner, its centre and its min
+    corner; rays around each."""
+    rng = np.random.default_rng(13)
+    objs = []
+    for base in (8150.0, 4090.0, 40.0):
+        objs += [{"n": tuple(rng.integers(1, 4, 3)), "translation": tuple(base + rng.uniform(-24, 24, 3)),
+                  "rotation": S.random_rotation(rng)} for _ in range(5)]
+    objs.append({"n": (2, 2, 2), "translation": (8180.0, 8180.0, 8180.0)})
+    sc = S.make_scene(objs, seed=13)
+    ctx.upload(sc)
+    mn = (0.0, 0.0, 0.0)
+    say("depth-13 SVO, objects at the far corner")
+    ctx.svo_build(mn, 13)
+    ref = O.OracleSvo(O.OracleScene(sc), mn, 13)
+    e = ref.export()
+    assert_same_svo(ctx.svo_export(), e, "depth 13")
+    inner = e.nodes[e.nodes["level"] == 7]
+    assert inner.size and int(inner["coord"].max()) == 127
+    for base in (8150.0, 4090.0, 40.0):
+        o, d = rays_from(rng, 3000, base - 60, base + 60, centre=(base, base, base))
+        o = np.clip(o, 0.5, 8191.5).astype(np.float32)
+        say(f"depth-13 trace around {base}")
+        assert_same_hits(ctx.svo_trace(o, d, 200.0), ref.trace(o, d, 200.0), f"depth 13 at {base}")
+
+
+def test_svo_depth12_wide_world_bit_exact(ctx):
+    """Depth 12 over the batched workload's world shape: objects of 1-32
+    clusters per axis in a 4096^3 world (24 of them, so the oracle builds it
+    in seconds), the root centred on the first camera. SVO, 20000 traced
+    rays and a GI frame bit for bit."""
+    p = T.noise_objects_params(seed=12, n_objects=24, max_extent=32, world=4096.0, n_luts=64)
+    ctx.generate_scene_device(p)  # equals the host generator's scene (test_gpu_frame)
+    sc = ctx.scene_export(p)
+    cam = T.generate_camera(p, 0, 480, 270)
+    mn = np.floor(cam[0]["camera"][:3]) - 2048.0
+    say("24-object 4096^3 world, SVO depth 12")
+    ctx.svo_build(mn, 12)
+    osc = O.OracleScene(sc)
+    ref = O.OracleSvo(osc, mn, 12)
+    assert_same_svo(ctx.svo_export(), ref.export(), "depth 12")
+    rng = np.random.default_rng(12)
+    centres = sc.objects["translation"][:8]
+    o = np.concatenate([c + rng.uniform(-150, 150, (2500, 3)) for c in centres]).astype(np.float32)
+    d = rng.normal(size=o.shape)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    say("depth-12 trace 20000 rays")
+    g = ctx.svo_trace(o, d, 1024.0)
+    assert_same_hits(g, ref.trace(o, d, 1024.0), "depth 12 trace")
+    assert g["hit"].mean() > 0.05
+    prm = T.gi_params(T.seeded_sun(12), spp=2)
+    say("depth-12 GI frame 480x270")
+    vis, rad, irr = ctx.frame(cam, 480, 270, prm)
+    rr, ri, _ = ref.gi(cam, 480, 270, vis, prm)
+    assert (vis != T.VISIBILITY_EMPTY).mean() > 0.02
+    assert rad.tobytes() == rr.tobytes() and irr.tobytes() == ri.tobytes()
