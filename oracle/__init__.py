@@ -61,6 +61,7 @@ def load() -> ctypes.CDLL:
         "tgxo_shade": (None, [vp, vp, u32, u32, vp, vp, vp]),
         "tgxo_shade_pixel": (None, [vp, vp, u32, u32, u32, u32, u64, vp, vp]),
         "tgxo_gi_surface": (i32, [vp, vp, u32, u32, u32, u32, u64, vp, vp, vp]),
+        "tgxo_gi_surface_count": (u64, [vp, vp, u32, u32, vp, i32]),
         "tgxo_gi_seed": (u32, [u32, u32, u32, u32]),
         "tgxo_gi_direction": (None, [vp, u32, vp]),
         "tgxo_gi_alg_bytes": (i32, [vp, vp, u32, u32, vp, vp, u32, vp]),
@@ -106,6 +107,15 @@ def visibility_culled(oscene: OracleScene, cam: np.ndarray, w: int, h: int, row_
     out = np.full((h, w), 0xFFFFFFFFFFFFFFFF, np.uint64)
     pairs = load().tgxo_visibility_culled(oscene.ref, _p(cam), w, h, row_step, n_threads, _p(out))
     return out, int(pairs)
+
+
+def gi_surface_count(oscene: OracleScene, cam: np.ndarray, w: int, h: int, vis: np.ndarray,
+                     n_threads: int = 0) -> int:
+    """Pixels of one camera's visibility words that have a primary surface
+    (the visible pixels a GI frame lists)."""
+    v = np.ascontiguousarray(vis, np.uint64)
+    assert v.size == w * h
+    return int(load().tgxo_gi_surface_count(oscene.ref, _p(cam), w, h, _p(v), n_threads))
 
 
 def shade(oscene: OracleScene, cam: np.ndarray, w: int, h: int, vis: np.ndarray, params: np.ndarray) -> np.ndarray:

@@ -111,6 +111,11 @@ int       tgxo_gi_alg_bytes(const tgxo_svo* svo, const tgx_camera* cam, uint32_t
                             uint64_t out[16]);
 int       tgxo_gi_surface(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h, uint32_t px,
                           uint32_t py, uint64_t word, float out_p[3], float out_n[3], float out_albedo[3]);
+/* Number of pixels of vis (w * h words of camera cam) that have a primary
+ * surface by tgxo_gi_surface, i.e. the visible pixels a GI frame lists
+ * (OpenMP over rows; n_threads <= 0 keeps the default). */
+uint64_t  tgxo_gi_surface_count(const tgxo_scene* s, const tgx_camera* cam, uint32_t w, uint32_t h,
+                                const uint64_t* vis, int n_threads);
 uint32_t  tgxo_gi_seed(uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t frame);
 /* Cosine-weighted bounce direction about unit n from one RNG seed. */
 void      tgxo_gi_direction(const float n[3], uint32_t seed, float out[3]);

@@ -31,6 +31,8 @@
  */
 #include "tgx_oracle.h"
 
+#include <omp.h>
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -867,6 +869,21 @@ int tgxo_gi_surface(const tgxo_scene* sc, const tgx_camera* cam, uint32_t w, uin
     out_n[0] = nws.x; out_n[1] = nws.y; out_n[2] = nws.z;
     out_albedo[0] = e->albedo[0]; out_albedo[1] = e->albedo[1]; out_albedo[2] = e->albedo[2];
     return 1;
+}
+
+uint64_t tgxo_gi_surface_count(const tgxo_scene* sc, const tgx_camera* cam, uint32_t w, uint32_t h,
+                               const uint64_t* vis, int n_threads)
+{
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    uint64_t n = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : n)
+    for (int64_t y = 0; y < (int64_t)h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            const uint64_t word = vis[(size_t)y * w + x];
+            float p[3], nn[3], a[3];
+            if (word != ~0ull && tgxo_gi_surface(sc, cam, w, h, x, (uint32_t)y, word, p, nn, a)) n++;
+        }
+    return n;
 }
 
 /* Leaf index of the hit's block (binary search over the canonical order

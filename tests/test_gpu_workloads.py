@@ -63,7 +63,7 @@ def test_batched_frame_16_cameras_depth12_visibility_every_8th_row():
         ctx.d2h(vis, bufs[0])
         ctx.d2h(rad, bufs[1])
         assert int(st["primary_rays"]) == n_px
-        assert int(st["visible_pixels"]) == int((vis != T.VISIBILITY_EMPTY).sum()) > 0.3 * n_px
+        assert int(st["visible_pixels"]) > 0.3 * n_px
         assert int(st["bounce_rays"]) == 4 * int(st["visible_pixels"]) and int(st["shadow_rays"]) > 0
         assert np.isfinite(rad).all() and rad.min() >= 0.0 and rad.sum() > 0
         say("batched workload: neural-cache frame graph with fallback, 30 replays")
@@ -90,6 +90,13 @@ def test_batched_frame_16_cameras_depth12_visibility_every_8th_row():
         ctx.close()
     oscene = O.OracleScene(scene)
     threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1), 16)
+    # Visible pixels are words with depth < 1 whose hit voxel the surface
+    # pass re-intersects; words at depth exactly 1 are hits of the buffer but
+    # not surfaces.
+    n_surf = sum(O.gi_surface_count(oscene, cams[c:c + 1], W, H, vis[c], threads) for c in range(n_cams))
+    say(f"batched workload: {int(st['visible_pixels'])} visible pixels, oracle surfaces {n_surf}, "
+        f"non-empty words {int((vis != T.VISIBILITY_EMPTY).sum())}")
+    assert int(st["visible_pixels"]) == n_surf
     for c in range(n_cams):
         say(f"batched workload: oracle visibility, camera {c}, every {row_step}th row")
         ref, _ = O.visibility_culled(oscene, cams[c:c + 1], W, H, row_step=row_step, n_threads=threads)

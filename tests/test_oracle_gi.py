@@ -318,3 +318,23 @@ def test_alg_bytes_hit_shading_lines():
                                                                 T.gi_params((-0.6, 0.7, 0.3), spp=16))
     assert b["lines"]["luts_hits"] == 1 and b["lines"]["luts_surfaces"] == 1
     assert b["lines"]["leaf_materials"] > 0
+
+
+def test_surface_count_equals_per_pixel_surfaces():
+    """tgxo_gi_surface_count counts the pixels tgxo_gi_surface accepts: a
+    word at depth exactly 1 (depth field all ones) is a buffer hit but no
+    surface, and empty words are skipped."""
+    p = T.noise_objects_params(seed=2, n_objects=20, max_extent=4, world=80.0)
+    sc = T.generate_scene(p)
+    cam = T.generate_camera(p, 0, 40, 30)
+    os_ = O.OracleScene(sc)
+    vis, _ = O.visibility_culled(os_, cam, 40, 30)
+    hits = np.argwhere(vis != T.VISIBILITY_EMPTY)
+    assert len(hits) > 50
+    y0, x0 = hits[0]
+    vis[y0, x0] = np.uint64((0xFFFFFF << 40) | (int(vis[y0, x0]) & ((1 << 40) - 1)))
+    want = sum(_surface(os_, cam, 40, 30, x, y, int(vis[y, x]))[0]
+               for y in range(30) for x in range(40) if vis[y, x] != T.VISIBILITY_EMPTY)
+    assert want == len(hits) - 1
+    assert O.gi_surface_count(os_, cam, 40, 30, vis, 1) == want
+    assert O.gi_surface_count(os_, cam, 40, 30, vis, 3) == want
