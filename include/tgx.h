@@ -252,6 +252,27 @@ int tgx_visibility(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams,
 int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams,
                           uint32_t w, uint32_t h, uint64_t* out_dev);
 
+/* Scratch of tgx_visibility_device_noalloc: one array of this many bytes
+ * per GPU, held from the library's first context on that GPU until the
+ * process ends. */
+#define TGX_VIS_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* As tgx_visibility_device, but allocates nothing: no hipMalloc, no
+ * hipFree, no allocating library routine, no HIP objects. No counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops). Every work buffer
+ * (camera records, per-(camera, cluster pointer) records, tile counts and
+ * lists, cull flags) comes from a fixed device array of
+ * TGX_VIS_NOALLOC_SCRATCH_BYTES per GPU. Cameras run in batches of at most
+ * 16 that fit it with at least 64 tile-list entries per tile; a call where
+ * even one camera does not fit (116 B + 92 B per cluster pointer + 520 B
+ * per 16x16 tile + 1 B per object + 1 B per cluster, 256-B aligned parts)
+ * fails with TGX_ERR_CAPACITY. The results equal tgx_visibility_device's.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two contexts, two threads) race. */
+int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams,
+                                  uint32_t w, uint32_t h, uint64_t* out_dev);
+
 /* Host-side decode of one visibility word, replacing
  * tg_raytracer_get_hovered_voxel's unpack (tgvk_raytracer.c:1639-1653).
  * The 31-bit field is returned as what the GPU wrote: a cluster POINTER
@@ -668,6 +689,7 @@ int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, ui
                          const tgx_gi_params* gi, uint64_t* vis_dev, float* radiance_dev,
                          float* irradiance_dev);
 int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out);
+
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and profiling helpers (for callers without their own       */

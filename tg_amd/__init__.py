@@ -199,6 +199,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_scene_upload": (i32, [vp, ctypes.POINTER(_SceneDesc)]),
         "tgx_visibility": (i32, [vp, vp, u32, u32, u32, vp]),
         "tgx_visibility_device": (i32, [vp, vp, u32, u32, u32, vp]),
+        "tgx_visibility_device_noalloc": (i32, [vp, vp, u32, u32, u32, vp]),
         "tgx_visibility_clear_device": (i32, [vp, vp, u64]),
         "tgx_visibility_decode": (i32, [u64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "tgx_visibility_last_stats": (i32, [vp, ctypes.POINTER(VisibilityStats)]),
@@ -476,6 +477,13 @@ class Context:
         cams = np.ascontiguousarray(cams, dtype=CAMERA_DTYPE)
         self._check(self.lib.tgx_visibility_device(self.h, _ptr(cams), cams.size, w, h, out_dev),
                     "tgx_visibility_device")
+
+    def visibility_device_noalloc(self, cams: np.ndarray, w: int, h: int, out_dev: int) -> None:
+        """tgx_visibility_device_noalloc: the same words, work buffers in the
+        library's fixed per-GPU scratch array (no device allocation)."""
+        cams = np.ascontiguousarray(cams, dtype=CAMERA_DTYPE)
+        self._check(self.lib.tgx_visibility_device_noalloc(self.h, _ptr(cams), cams.size, w, h, out_dev),
+                    "tgx_visibility_device_noalloc")
 
     def last_stats(self) -> VisibilityStats:
         st = VisibilityStats()

@@ -195,14 +195,18 @@ int  tgx_svo_check_current(tgx_context* ctx);
  * the scene's bits (a kernel over every cluster index when they changed). */
 int  tgx_vis_solid_refresh(tgx_context* ctx);
 void tgx_buffer_release(tgx_buffer* b);
+/* Looks up, once per device, the device address of the scratch array of
+ * tgx_visibility_device_noalloc (the context's device must be current). */
+int  tgx_vis_scratch_init(tgx_context* ctx);
 
 /* Kernel launch bracket: records HIP events on the context stream around a
- * launch when profiling is enabled. */
+ * launch when profiling is enabled and `on` is true (paths that must not
+ * create HIP objects pass false). */
 struct tgx_launch_scope {
     tgx_context* ctx;
     int          stat_idx = -1;
     hipEvent_t   start = nullptr, stop = nullptr;
-    tgx_launch_scope(tgx_context* c, const char* name);
+    tgx_launch_scope(tgx_context* c, const char* name, bool on = true);
     ~tgx_launch_scope();
 };
 

@@ -52,9 +52,9 @@ void tgx_buffer_release(tgx_buffer* b)
     b->bytes = 0;
 }
 
-tgx_launch_scope::tgx_launch_scope(tgx_context* c, const char* name) : ctx(c)
+tgx_launch_scope::tgx_launch_scope(tgx_context* c, const char* name, bool on) : ctx(c)
 {
-    if (!ctx->profiling) return;
+    if (!on || !ctx->profiling) return;
     for (size_t i = 0; i < ctx->stats.size(); i++)
         if (strncmp(ctx->stats[i].name, name, sizeof(ctx->stats[i].name)) == 0) stat_idx = (int)i;
     if (stat_idx < 0) {
@@ -118,6 +118,12 @@ int tgx_create(int device, tgx_context** out_ctx)
         return TGX_ERR_HIP;
     }
     if (hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return TGX_ERR_HIP;
+    }
+    if (tgx_vis_scratch_init(ctx) != TGX_OK) {
+        (void)hipHostFree(ctx->host_pinned);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return TGX_ERR_HIP;

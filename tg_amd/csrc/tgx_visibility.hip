@@ -32,6 +32,8 @@
 
 #include <math.h>
 
+#include <mutex>
+
 #define VIS_TILE 16u
 #define VIS_GLOBAL_TILES 256u
 #define VIS_F32_MAX 3.402823466e+38f
@@ -437,6 +439,19 @@ __global__ void vis_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t valu
     if (i < n) p[i] = value;
 }
 
+/* Camera records passed by value (kernel arguments), so that a call can
+ * place them in device memory without a copy from host memory. */
+#define VIS_ARG_CAMS 16u
+struct vis_cam_pack {
+    vis_cam c[VIS_ARG_CAMS];
+};
+
+__global__ void vis_cams_k(vis_cam_pack pack, uint32_t n, vis_cam* __restrict__ out)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < n) out[i] = pack.c[i];
+}
+
 /* Single-workgroup exclusive scan: off[i] = sum(count[0..i)), and
  * count[i] becomes the scatter cursor (= off[i]); the total goes to
  * *total. The last thread also accumulates the frame statistics:
@@ -673,26 +688,114 @@ int tgx_vis_solid_refresh(tgx_context* ctx)
     return TGX_OK;
 }
 
+/* Device buffers of one visibility batch. */
+struct vis_bufs {
+    const vis_cam*      cams;
+    const uint8_t*      solid;      /* u8 per cluster index                */
+    uint8_t*            objvis;     /* u8 per (camera, object)             */
+    vis_cluster*        recs;       /* per (camera, cluster pointer)       */
+    uint32_t*           rec_flags;
+    uint64_t*           gkeys;
+    uint32_t*           counters;   /* 2 n_cams + 8                        */
+    uint32_t*           tile_count;
+    uint32_t*           tile_off;
+    uint64_t*           list;
+    uint32_t            capacity;   /* tile-list entries                   */
+    unsigned long long* stats;      /* 8 u32, cleared by the caller        */
+    bool                timed;      /* profiling scopes allowed            */
+};
+
+/* The passes of one batch of n_cams cameras over buffers that hold them. */
+static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cams, uint32_t W, uint32_t H,
+                             uint64_t* out_dev)
+{
+    const uint32_t tiles_x = (W + VIS_TILE - 1) / VIS_TILE, tiles_y = (H + VIS_TILE - 1) / VIS_TILE;
+    const uint32_t tiles_per_cam = tiles_x * tiles_y;
+    const uint32_t n_tiles = tiles_per_cam * n_cams;
+    const uint32_t n_cptr = ctx->n_cluster_pointers;
+    /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts */
+    uint32_t* d_counters = b.counters;
+    uint32_t* d_global_count = d_counters + n_cams + 8;
+    {
+        tgx_launch_scope s(ctx, "vis_fill_u32", b.timed);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((n_tiles + 255) / 256), dim3(256), 0, ctx->stream, b.tile_count, n_tiles, 0u);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((2 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters,
+                           2 * n_cams + 8, 0u);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    {
+        tgx_launch_scope s(ctx, "vis_objcull_k", b.timed);
+        hipLaunchKernelGGL(vis_objcull_k, dim3((ctx->n_objects + 255u) / 256u, n_cams), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, b.cams, W, H, b.objvis);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_objcull_k");
+    {
+        tgx_launch_scope s(ctx, "vis_setup", b.timed);
+        hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
+                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, b.solid, (const uint8_t*)b.objvis,
+                           ctx->n_objects, n_cptr, b.cams, W, H, tiles_x, b.recs, b.rec_flags, d_counters,
+                           d_global_count, b.gkeys, b.tile_count);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_setup");
+    {
+        tgx_launch_scope s(ctx, "vis_scan", b.timed);
+        hipLaunchKernelGGL(vis_scan, dim3(1), dim3(1024), 0, ctx->stream, b.tile_count, b.tile_off, n_tiles,
+                           d_counters + n_cams, d_counters, n_cams, b.stats);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_scan");
+    {
+        tgx_launch_scope s(ctx, "vis_scatter", b.timed);
+        const uint64_t threads = (uint64_t)n_cptr * VIS_SCATTER_LANES;
+        hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)((threads + 255) / 256), n_cams), dim3(256), 0, ctx->stream,
+                           b.recs, b.rec_flags, d_counters, b.cams, tiles_x, b.tile_count, b.list, b.capacity);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_scatter");
+    {
+        /* After the scatter, tile_count[t] is the end of tile t's list. */
+        tgx_launch_scope s(ctx, "vis_traverse", b.timed);
+        hipLaunchKernelGGL(vis_traverse, dim3(n_tiles), dim3(256), 0, ctx->stream, b.cams, W, H, tiles_x,
+                           tiles_per_cam, b.tile_off, b.tile_count, b.list, b.capacity, d_counters, d_global_count,
+                           b.gkeys, b.recs, (const uint32_t*)ctx->bits.ptr, out_dev);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_traverse");
+    ctx->vis_stats.clusters_total += (uint64_t)n_cptr * n_cams;
+    ctx->vis_stats.tiles = tiles_per_cam;
+    ctx->vis_stats.tile_size = VIS_TILE;
+    return TGX_OK;
+}
+
+/* Writes every visibility word of a batch of n_cams cameras as empty (a
+ * scene without cluster pointers). */
+static int visibility_empty(tgx_context* ctx, uint32_t n_cams, uint32_t W, uint32_t H, uint64_t* out_dev, bool timed)
+{
+    const uint64_t words = 2ull * n_cams * W * H;
+    if (words >= (1ull << 32)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
+    {
+        tgx_launch_scope s(ctx, "vis_fill_u32", timed);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (uint32_t*)out_dev, (uint32_t)words, 0xFFFFFFFFu);
+    }
+    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    return TGX_OK;
+}
+
+static bool batch_fits_grid(uint32_t n_cams, uint32_t W, uint32_t H, uint32_t n_cptr)
+{
+    const uint64_t tiles = (uint64_t)((W + VIS_TILE - 1) / VIS_TILE) * ((H + VIS_TILE - 1) / VIS_TILE) * n_cams;
+    return tiles < (1ull << 31) && (uint64_t)n_cptr * n_cams < (1ull << 32);
+}
+
 static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t W,
                             uint32_t H, uint64_t* out_dev, const vis_cam* resident)
 {
     const uint32_t tiles_x = (W + VIS_TILE - 1) / VIS_TILE, tiles_y = (H + VIS_TILE - 1) / VIS_TILE;
     const uint32_t tiles_per_cam = tiles_x * tiles_y;
-    const uint64_t n_tiles64 = (uint64_t)tiles_per_cam * n_cams;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
-    if (n_cptr == 0) {
-        /* a dynamic scene without objects: every pixel is empty */
-        const uint64_t words = 2ull * n_cams * W * H;
-        if (words >= (1ull << 32)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
-        hipLaunchKernelGGL(vis_fill_u32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (uint32_t*)out_dev, (uint32_t)words, 0xFFFFFFFFu);
-        TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
-        return TGX_OK;
-    }
+    if (n_cptr == 0) return visibility_empty(ctx, n_cams, W, H, out_dev, true);
+    if (!batch_fits_grid(n_cams, W, H, n_cptr)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
     const uint64_t n_slots = (uint64_t)n_cptr * n_cams;
-    if (n_tiles64 >= (1ull << 31) || n_slots >= (1ull << 32))
-        return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
-    const uint32_t n_tiles = (uint32_t)n_tiles64;
+    const uint32_t n_tiles = tiles_per_cam * n_cams;
 
     int rc;
     if ((rc = tgx_vis_solid_refresh(ctx))) return rc;
@@ -714,73 +817,26 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_objvis, (size_t)ctx->n_objects * n_cams, "object cull flags")))
         return rc;
-
-    const vis_cam* d_cams = resident ? resident : (const vis_cam*)ctx->vis_cams.ptr;
-    /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts */
-    uint32_t* d_counters = (uint32_t*)ctx->vis_counters.ptr;
-    uint32_t* d_global_count = d_counters + n_cams + 8;
-    uint32_t* d_flags = (uint32_t*)ctx->vis_rec_flags.ptr;
-    uint64_t* d_gkeys = (uint64_t*)ctx->vis_global_keys.ptr;
-    uint32_t* d_count = (uint32_t*)ctx->vis_tile_count.ptr;
-    uint32_t* d_off = (uint32_t*)ctx->vis_tile_off.ptr;
-    vis_cluster* d_recs = (vis_cluster*)ctx->vis_clusters.ptr;
     if (!resident)
         TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->vis_cams.ptr, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice,
                                           ctx->stream));
-
-    {
-        tgx_launch_scope s(ctx, "vis_fill_u32");
-        hipLaunchKernelGGL(vis_fill_u32, dim3((n_tiles + 255) / 256), dim3(256), 0, ctx->stream, d_count, n_tiles, 0u);
-        hipLaunchKernelGGL(vis_fill_u32, dim3((2 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters, 2 * n_cams + 8, 0u);
-    }
-    TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
-    {
-        tgx_launch_scope s(ctx, "vis_objcull_k");
-        hipLaunchKernelGGL(vis_objcull_k, dim3((ctx->n_objects + 255u) / 256u, n_cams), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, d_cams, W, H,
-                           (uint8_t*)ctx->vis_objvis.ptr);
-    }
-    TGX_LAUNCH_CHECK(ctx, "vis_objcull_k");
-    {
-        tgx_launch_scope s(ctx, "vis_setup");
-        hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, (const uint8_t*)ctx->vis_solid.ptr,
-                           (const uint8_t*)ctx->vis_objvis.ptr, ctx->n_objects, n_cptr, d_cams, W, H, tiles_x, d_recs,
-                           d_flags, d_counters, d_global_count, d_gkeys, d_count);
-    }
-    TGX_LAUNCH_CHECK(ctx, "vis_setup");
-    {
-        tgx_launch_scope s(ctx, "vis_scan");
-        hipLaunchKernelGGL(vis_scan, dim3(1), dim3(1024), 0, ctx->stream, d_count, d_off, n_tiles, d_counters + n_cams,
-                           d_counters, n_cams, (unsigned long long*)ctx->vis_stats_dev.ptr);
-    }
-    TGX_LAUNCH_CHECK(ctx, "vis_scan");
-
+    vis_bufs b;
+    b.cams = resident ? resident : (const vis_cam*)ctx->vis_cams.ptr;
+    b.solid = (const uint8_t*)ctx->vis_solid.ptr;
+    b.objvis = (uint8_t*)ctx->vis_objvis.ptr;
+    b.recs = (vis_cluster*)ctx->vis_clusters.ptr;
+    b.rec_flags = (uint32_t*)ctx->vis_rec_flags.ptr;
+    b.gkeys = (uint64_t*)ctx->vis_global_keys.ptr;
+    b.counters = (uint32_t*)ctx->vis_counters.ptr;
+    b.tile_count = (uint32_t*)ctx->vis_tile_count.ptr;
+    b.tile_off = (uint32_t*)ctx->vis_tile_off.ptr;
     /* The tile lists go into a buffer of fixed capacity (grown between
      * frames from the largest total seen); no host round trip mid-frame. */
-    const uint32_t capacity = ctx->vis_list_capacity;
-    uint64_t* d_list = (uint64_t*)ctx->vis_tile_list.ptr;
-    {
-        tgx_launch_scope s(ctx, "vis_scatter");
-        const uint64_t threads = (uint64_t)n_cptr * VIS_SCATTER_LANES;
-        hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)((threads + 255) / 256), n_cams), dim3(256), 0, ctx->stream,
-                           d_recs, d_flags, d_counters, d_cams, tiles_x, d_count, d_list, capacity);
-    }
-    TGX_LAUNCH_CHECK(ctx, "vis_scatter");
-    {
-        /* After the scatter, d_count[t] is the end of tile t's list. */
-        tgx_launch_scope s(ctx, "vis_traverse");
-        hipLaunchKernelGGL(vis_traverse, dim3(n_tiles), dim3(256), 0, ctx->stream, d_cams, W, H, tiles_x,
-                           tiles_per_cam, d_off, d_count, d_list, capacity, d_counters, d_global_count, d_gkeys,
-                           d_recs, (const uint32_t*)ctx->bits.ptr, out_dev);
-    }
-    TGX_LAUNCH_CHECK(ctx, "vis_traverse");
-
-    ctx->vis_stats.clusters_total += n_slots;
-    ctx->vis_stats.tiles = tiles_per_cam;
-    ctx->vis_stats.tile_size = VIS_TILE;
-    return TGX_OK;
+    b.list = (uint64_t*)ctx->vis_tile_list.ptr;
+    b.capacity = ctx->vis_list_capacity;
+    b.stats = (unsigned long long*)ctx->vis_stats_dev.ptr;
+    b.timed = true;
+    return visibility_launch(ctx, b, n_cams, W, H, out_dev);
 }
 
 static int check_args(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h)
@@ -952,6 +1008,131 @@ int tgx_visibility_last_stats(tgx_context* ctx, tgx_visibility_stats* out)
     *out = ctx->vis_stats;
     out->clusters_binned = hs[0];
     out->tile_entries = hs[1];
+    return TGX_OK;
+}
+
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Visibility without allocation                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Every work buffer of tgx_visibility_device_noalloc lives in this array
+ * (one copy per GPU, held until the process ends). Host code never passes
+ * the array's name to a kernel: its device address comes from
+ * hipGetSymbolAddress, once per device, into vis_scratch_addr. */
+static __device__ __align__(256) unsigned char scratch[TGX_VIS_NOALLOC_SCRATCH_BYTES];
+
+#define VIS_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* vis_scratch_addr[VIS_MAX_DEVICES];
+
+int tgx_vis_scratch_init(tgx_context* ctx)
+{
+    if (ctx->device < 0 || ctx->device >= VIS_MAX_DEVICES)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "device %d beyond the scratch table", ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (vis_scratch_addr[ctx->device]) return TGX_OK;
+    void* p = nullptr;
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    TGX_HIP_CHECK(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return tgx_fail(ctx, TGX_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    vis_scratch_addr[ctx->device] = p;
+    return TGX_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255u) & ~(size_t)255u; }
+
+/* Scratch layout of one batch of n_cams cameras; returns the bytes before
+ * the tile list. */
+static size_t noalloc_fixed_bytes(const tgx_context* ctx, uint32_t n_cams, uint32_t tiles_per_cam)
+{
+    const uint64_t slots = (uint64_t)ctx->n_cluster_pointers * n_cams;
+    return align256(sizeof(vis_cam) * n_cams) + align256(ctx->n_clusters) +
+           align256((size_t)ctx->n_objects * n_cams) + align256(sizeof(vis_cluster) * slots) + align256(4 * slots) +
+           align256(8 * slots) + align256(4ull * (2 * n_cams + 8)) + 2 * align256(4ull * tiles_per_cam * n_cams) +
+           align256(64);
+}
+
+extern "C" {
+
+int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w,
+                                  uint32_t h, uint64_t* out_dev)
+{
+    int rc = check_args(ctx, cams, n_cams, w, h);
+    if (rc) return rc;
+    if (!out_dev) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "out_dev is NULL");
+    if (ctx->device < 0 || ctx->device >= VIS_MAX_DEVICES || !vis_scratch_addr[ctx->device])
+        return tgx_fail(ctx, TGX_ERR_HIP, "no scratch address for device %d", ctx->device);
+    unsigned char* base = (unsigned char*)vis_scratch_addr[ctx->device];
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    memset(&ctx->vis_stats, 0, sizeof(ctx->vis_stats));
+    const uint32_t tiles_x = (w + VIS_TILE - 1) / VIS_TILE, tiles_y = (h + VIS_TILE - 1) / VIS_TILE;
+    const uint32_t tiles_per_cam = tiles_x * tiles_y;
+    const uint32_t n_cptr = ctx->n_cluster_pointers;
+    /* Cameras per batch: as many as fit with at least 64 tile-list entries
+     * per tile (a frame whose lists overflow stays exact, only slower). */
+    uint32_t per = n_cams < VIS_ARG_CAMS ? n_cams : VIS_ARG_CAMS;
+    while (per > 0 && noalloc_fixed_bytes(ctx, per, tiles_per_cam) + 8ull * 64u * tiles_per_cam * per >
+                          (size_t)TGX_VIS_NOALLOC_SCRATCH_BYTES)
+        per--;
+    if (per == 0)
+        return tgx_fail(ctx, TGX_ERR_CAPACITY,
+                        "one camera at %ux%u over %u cluster pointers needs %zu bytes of scratch; the maximum is %llu",
+                        w, h, n_cptr, noalloc_fixed_bytes(ctx, 1, tiles_per_cam) + 8ull * 64u * tiles_per_cam,
+                        (unsigned long long)TGX_VIS_NOALLOC_SCRATCH_BYTES);
+    if (n_cptr && !batch_fits_grid(per, w, h, n_cptr))
+        return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
+    for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
+        const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
+        uint64_t* out = out_dev + (size_t)c0 * w * h;
+        if (n_cptr == 0) {
+            if ((rc = visibility_empty(ctx, n, w, h, out, false))) return rc;
+            continue;
+        }
+        const uint64_t slots = (uint64_t)n_cptr * n;
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            unsigned char* p = base + off;
+            off += align256(bytes);
+            return (void*)p;
+        };
+        vis_cam_pack pack;
+        memset(&pack, 0, sizeof(pack));
+        for (uint32_t i = 0; i < n; i++) {
+            make_cam(cams[c0 + i], pack.c[i]);
+            pack.c[i].rec_base = i * n_cptr;
+            pack.c[i].tile_base = i * tiles_per_cam;
+        }
+        vis_bufs b;
+        vis_cam* d_cams = (vis_cam*)carve(sizeof(vis_cam) * n);
+        uint8_t* d_solid = (uint8_t*)carve(ctx->n_clusters);
+        b.cams = d_cams;
+        b.solid = d_solid;
+        b.objvis = (uint8_t*)carve((size_t)ctx->n_objects * n);
+        b.recs = (vis_cluster*)carve(sizeof(vis_cluster) * slots);
+        b.rec_flags = (uint32_t*)carve(4 * slots);
+        b.gkeys = (uint64_t*)carve(8 * slots);
+        b.counters = (uint32_t*)carve(4ull * (2 * n + 8));
+        b.tile_count = (uint32_t*)carve(4ull * tiles_per_cam * n);
+        b.tile_off = (uint32_t*)carve(4ull * tiles_per_cam * n);
+        b.stats = (unsigned long long*)carve(64);
+        const size_t left = (size_t)TGX_VIS_NOALLOC_SCRATCH_BYTES - off;
+        const size_t entries = left / 8u;
+        b.list = (uint64_t*)(base + off);
+        b.capacity = entries > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)entries;
+        b.timed = false;
+        hipLaunchKernelGGL(vis_cams_k, dim3(1), dim3(64), 0, ctx->stream, pack, n, d_cams);
+        TGX_LAUNCH_CHECK(ctx, "vis_cams_k");
+        /* The solid flags are derived from the scene's bits on every call:
+         * the array keeps whatever the last call on this GPU left. */
+        hipLaunchKernelGGL(vis_solid_k, dim3((ctx->n_clusters + 255u) / 256u), dim3(256), 0, ctx->stream,
+                           (const uint4*)ctx->bits.ptr, ctx->n_clusters, d_solid);
+        TGX_LAUNCH_CHECK(ctx, "vis_solid_k");
+        hipLaunchKernelGGL(vis_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)b.stats, 16u, 0u);
+        TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+        if ((rc = visibility_launch(ctx, b, n, w, h, out))) return rc;
+    }
     return TGX_OK;
 }
 
