@@ -537,6 +537,13 @@ typedef struct tgx_tuning {
                                      lanes needing them; k in 1..64: a wave runs only DDA
                                      bursts once >= k/64 of its active lanes are in a leaf,
                                      else only node steps (0) */
+    uint32_t trace_regroup;       /* fused GI launch: 0 runs each lane's phase in its own wave
+                                     slot; k in 1..8 regroups the rays of a 1024-lane workgroup
+                                     by phase after every step (one workgroup per CU), node-walk
+                                     waves taking up to k node steps per step (0) */
+    uint32_t trace_skip;          /* regrouped launch: 1 crosses 8^3 bricks without solid voxels
+                                     in one exact step of the leaf DDA instead of voxel by
+                                     voxel (0) */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);
@@ -690,6 +697,18 @@ int tgx_nrc_frame_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, ui
                          float* irradiance_dev);
 int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out);
 
+/* The cache's training state in a file: its configuration, the frame and
+ * Adam-step counters, the fp32 master weights, both Adam moments, the
+ * moving average and the error-estimate table, sealed by an FNV-1a 64
+ * checksum. tgx_nrc_load re-creates the cache from it (TGX_ERR_IO if the
+ * file cannot be opened; TGX_ERR_INVALID_ARG for a wrong size, magic,
+ * layout, checksum, configuration or non-finite value, leaving the current
+ * cache as it was; refused while a frame graph exists). Frames after a load
+ * equal the frames the saved cache would have run next, bit for bit. New:
+ * the reference has no learned part; this is the resume point of online
+ * training. */
+int tgx_nrc_save(tgx_context* ctx, const char* path);
+int tgx_nrc_load(tgx_context* ctx, const char* path);
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and profiling helpers (for callers without their own       */

@@ -129,3 +129,47 @@ def test_queued_shadow_rays_equal_fused_launch(ctx, noise_setup, sort):
         assert a[k].tobytes() == b[k].tobytes(), f"output {k} differs"
     assert a[2] == b[2] and a[5] == b[5]
     assert a[5][1] > 0 and a[5][3] > 0 and a[5][4] > 0  # fallback pixels, training and fallback rays
+
+
+@pytest.mark.parametrize("regroup", [1, 3])
+def test_regrouped_trace_equals_oracle_and_lane_machine(ctx, noise_setup, regroup):
+    """The fused GI launch with the workgroup's rays regrouped by phase
+    (tuning trace_regroup) gives the oracle's bits and counts at 480x270 and
+    on every 8th row at 1920x1080, and the same cache frames (training
+    targets and fallback pixels in one launch) as the lane state machine."""
+    p, sc, mn, svo = noise_setup
+    ctx.upload(sc)
+    ctx.svo_build(mn, 11)
+    prm = T.gi_params(T.seeded_sun(1), spp=4, frame=5)
+    cfg = T.nrc_config(origin=(0, 0, 0), scale=1 / 512, train_samples=1 << 14, train_batch=1 << 12,
+                       fallback_threshold=1.0, fallback_min_samples=1)
+    w, h = 480, 270
+    cam = T.generate_camera(p, 0, w, h)
+    out = []
+    try:
+        for m in (0, regroup):
+            ctx.set_tuning(trace_regroup=m)
+            say(f"trace_regroup {m}: plain GI 480x270 and 3 cache frames")
+            vis, rad, irr = ctx.frame(cam, w, h, prm)
+            gst = tuple(int(x) for x in ctx.gi_stats())
+            ctx.nrc_create(cfg)
+            for _ in range(3):
+                _, nrad, nirr = ctx.nrc_frame(cam, w, h, prm)
+            out.append((rad, irr, gst, nrad, nirr, tuple(ctx.nrc_stats())))
+        say(f"trace_regroup {regroup}: GI 1920x1080")
+        cam2 = T.generate_camera(p, 0, 1920, 1080)
+        vis2, rad2, irr2 = ctx.frame(cam2, 1920, 1080, prm)
+    finally:
+        ctx.set_tuning(trace_regroup=0)
+    a, b = out
+    for k in (0, 1, 3, 4):
+        assert a[k].tobytes() == b[k].tobytes(), f"output {k} differs"
+    assert a[2] == b[2] and a[5] == b[5]
+    rr, ri, cnt = svo.gi(cam, w, h, vis, prm)
+    assert np.array_equal(b[0].view(np.uint32), rr.view(np.uint32))
+    assert np.array_equal(b[1].view(np.uint32), ri.view(np.uint32))
+    assert b[2] == tuple(int(c) for c in cnt)
+    rr2, ri2, _ = svo.gi(cam2, 1920, 1080, vis2, prm, row_step=8)
+    assert np.array_equal(rad2[::8].view(np.uint32), rr2[::8].view(np.uint32))
+    assert np.array_equal(irr2[::8].view(np.uint32), ri2[::8].view(np.uint32))
+    assert rad2.sum() > 0

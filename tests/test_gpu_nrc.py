@@ -271,3 +271,64 @@ def test_fallback_against_tracer_variance(ctx, single_setup):
         counts.append(int(ctx.nrc_stats()["fallback_pixels"]))
     assert counts[0] >= counts[1] >= counts[2]
     assert counts[0] > counts[2]
+
+
+def test_training_state_file_resumes_bit_for_bit(ctx, tmp_path):
+    """tgx_nrc_save / tgx_nrc_load: a fresh context that loads the state of
+    a cache trained for 12 frames runs the next 5 frames exactly as the
+    saved cache does (outputs, master and averaged weights, Adam steps);
+    corrupted, truncated and missing files are refused and leave the cache
+    as it was."""
+    import os
+    w, h = 240, 135
+    p = T.noise_objects_params(seed=1)
+    sc = T.generate_scene(p)
+    cam = T.generate_camera(p, 0, w, h)
+    mn = np.floor(cam[0]["camera"][:3]) - 1024.0
+    cfg = T.nrc_config(origin=(0, 0, 0), scale=1 / 512, train_samples=1 << 14, train_batch=1 << 12,
+                       fallback_threshold=4.0, fallback_min_samples=1, flags=T.NRC_FLAG_FALLBACK_VS_TRACER,
+                       ema_decay=0.99)
+    prm = T.gi_params(T.seeded_sun(1), spp=4, frame=0)
+    ctx.upload(sc)
+    ctx.svo_build(mn, 11)
+    ctx.nrc_create(cfg)
+    for _ in range(12):
+        ctx.nrc_frame(cam, w, h, prm)
+    path = tmp_path / "cache.nrc"
+    ctx.nrc_save(path)
+    say("cache state: 5 frames after the save")
+    outs = [ctx.nrc_frame(cam, w, h, prm)[1:] for _ in range(5)]
+    params, iparams, steps = ctx.nrc_params(), ctx.nrc_inference_params(), int(ctx.nrc_stats()["adam_steps"])
+    fb = int(ctx.nrc_stats()["fallback_pixels"])
+    assert steps == 17 * 4
+
+    c2 = T.Context(int(os.environ.get("TGX_DEVICE", "0")))
+    try:
+        c2.upload(sc)
+        c2.svo_build(mn, 11)
+        c2.nrc_load(path)
+        say("cache state: 5 frames after the load in a fresh context")
+        outs2 = [c2.nrc_frame(cam, w, h, prm)[1:] for _ in range(5)]
+        for (r1, i1), (r2, i2) in zip(outs, outs2):
+            assert r1.tobytes() == r2.tobytes() and i1.tobytes() == i2.tobytes()
+        assert c2.nrc_params().tobytes() == params.tobytes()
+        assert c2.nrc_inference_params().tobytes() == iparams.tobytes()
+        st2 = c2.nrc_stats()
+        assert int(st2["adam_steps"]) == steps and int(st2["fallback_pixels"]) == fb
+
+        before = c2.nrc_params()
+        data = bytearray(path.read_bytes())
+        bad = tmp_path / "bad.nrc"
+        data[len(data) // 2] ^= 0x40
+        bad.write_bytes(bytes(data))
+        for f, code in ((bad, 1), (tmp_path / "missing.nrc", 8)):
+            with pytest.raises(T.TgxError) as e:
+                c2.nrc_load(f)
+            assert e.value.code == code
+        bad.write_bytes(path.read_bytes()[:-100])
+        with pytest.raises(T.TgxError) as e:
+            c2.nrc_load(bad)
+        assert e.value.code == 1
+        assert c2.nrc_params().tobytes() == before.tobytes()
+    finally:
+        c2.close()

@@ -133,7 +133,10 @@ def test_neural_cache_1080p_with_fallback():
         prm = T.gi_params(sun, spp=4, frame=0)
         ctx.nrc_frame(cam, W, H, prm)
         first = ctx.nrc_stats()
-        assert first["fallback_pixels"] == first["visible_pixels"] == int(mask.sum())
+        # visible pixels: words with a surface (a word at depth exactly 1 is none)
+        n_surf = O.gi_surface_count(O.OracleScene(sc), cam, W, H, vis)
+        assert first["fallback_pixels"] == first["visible_pixels"] == n_surf
+        assert 0 <= int(mask.sum()) - n_surf < 100
         say("cache 1080p: 200 training frames with fallback")
         for _ in range(200):
             _, _, irr = ctx.nrc_frame(cam, W, H, prm)

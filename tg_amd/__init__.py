@@ -70,7 +70,7 @@ def seeded_sun(seed: int) -> np.ndarray:
     return np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
 
 TUNING_DTYPE = np.dtype([("trace_sort", "<u4"), ("trace_refill", "<u4"), ("trace_blocks_per_cu", "<u4"),
-                         ("trace_vote", "<u4")])
+                         ("trace_vote", "<u4"), ("trace_regroup", "<u4"), ("trace_skip", "<u4")])
 
 NRC_N_PARAMS = 18624
 NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learning_rate", "<f4"),
@@ -200,6 +200,8 @@ def load_library() -> ctypes.CDLL:
         "tgx_visibility": (i32, [vp, vp, u32, u32, u32, vp]),
         "tgx_visibility_device": (i32, [vp, vp, u32, u32, u32, vp]),
         "tgx_visibility_device_noalloc": (i32, [vp, vp, u32, u32, u32, vp]),
+        "tgx_nrc_save": (i32, [vp, ctypes.c_char_p]),
+        "tgx_nrc_load": (i32, [vp, ctypes.c_char_p]),
         "tgx_visibility_clear_device": (i32, [vp, vp, u64]),
         "tgx_visibility_decode": (i32, [u64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "tgx_visibility_last_stats": (i32, [vp, ctypes.POINTER(VisibilityStats)]),
@@ -622,6 +624,14 @@ class Context:
         params = np.ascontiguousarray(params, dtype=GI_PARAMS_DTYPE)
         self._check(self.lib.tgx_nrc_frame_device(self.h, _ptr(cam), w, h, _ptr(params), vis_dev, radiance_dev,
                                                   irradiance_dev), "tgx_nrc_frame_device")
+
+    def nrc_save(self, path) -> None:
+        """Training state of the cache to a file (tgx_nrc_save)."""
+        self._check(self.lib.tgx_nrc_save(self.h, str(path).encode()), "tgx_nrc_save")
+
+    def nrc_load(self, path) -> None:
+        """Re-creates the cache from a training-state file (tgx_nrc_load)."""
+        self._check(self.lib.tgx_nrc_load(self.h, str(path).encode()), "tgx_nrc_load")
 
     def nrc_stats(self):
         st = np.zeros(1, NRC_STATS_DTYPE)

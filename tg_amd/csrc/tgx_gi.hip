@@ -52,6 +52,7 @@
 #include "tgx_m4.h"
 
 #define GI_SQRT3 1.73205080757f
+#define SVO_INF __builtin_inff()
 #define GI_SHADOW_OFFSET 0.015625f
 #define GI_INV_PI 0.318309886f
 
@@ -279,12 +280,77 @@ __device__ static inline void pop_walk(const trace_params& P, const uint2* s_inn
     }
 }
 
+/* Exact skip of an 8^3 brick holding no solid voxel: the state of
+ * tg_amanatides_woo's 32^3 walk (tg_amanatides_woo.c:3-114) when its voxel
+ * first leaves the brick of (x, y, z), with the reference's additions in
+ * the reference's order. The walk steps the axis with the smallest t_max,
+ * ties going to z, then y (tmx < tmy ? (tmx < tmz ? x : z) : (tmy < tmz ?
+ * y : z)); so the steps inside the brick are the merge of the three axes'
+ * t_max sequences V_a(j) = fl(V_a(j-1) + t_delta_a) ordered by (value,
+ * z < y < x). Axis a leaves the brick at its N_a-th step, taken at
+ * V_a(N_a - 1); the brick is left by the axis e whose exit event comes
+ * first, and every other axis b has taken the steps whose events precede
+ * it: those with V_b(j) < V_e(N_e - 1), or equal where b wins the tie. No
+ * voxel of the brick is solid, so no step between stops the walk. The
+ * caller guarantees |t_delta| < 2^100 on moving axes, so no t_max
+ * overflows and a non-moving axis (t_max = F32_MAX) never steps. `moves`
+ * gets the steps that stay inside the 32^3 block. */
+__device__ static inline void brick_skip(int& x, int& y, int& z, float& tmx, float& tmy, float& tmz, float tdx,
+                                         float tdy, float tdz, int sx, int sy, int sz, uint32_t& moves)
+{
+    const int nx = sx > 0 ? 8 - (x & 7) : (x & 7) + 1;
+    const int ny = sy > 0 ? 8 - (y & 7) : (y & 7) + 1;
+    const int nz = sz > 0 ? 8 - (z & 7) : (z & 7) + 1;
+    float ex_x = tmx, ex_y = tmy, ex_z = tmz;
+#pragma unroll
+    for (int j = 1; j < 8; j++) {
+        ex_x = j < nx ? ex_x + tdx : ex_x;
+        ex_y = j < ny ? ex_y + tdy : ex_y;
+        ex_z = j < nz ? ex_z + tdz : ex_z;
+    }
+    ex_x = sx ? ex_x : SVO_INF;
+    ex_y = sy ? ex_y : SVO_INF;
+    ex_z = sz ? ex_z : SVO_INF;
+    const bool ez = ex_z <= ex_y && ex_z <= ex_x;
+    const bool ey = !ez && ex_y <= ex_x;
+    const bool ex = !ez && !ey;
+    const float te = ez ? ex_z : (ey ? ex_y : ex_x);
+    /* events of the other axes before the exit event; x wins no tie, y
+     * wins a tie against x, z against x and y */
+    float ux = tmx, uy = tmy, uz = tmz;
+    uint32_t kx = 0, ky = 0, kz = 0;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+        const bool px = ux < te;
+        const bool py = uy < te || (ex && uy == te);
+        const bool pz = uz < te || (!ez && uz == te);
+        ux = px ? ux + tdx : ux;
+        uy = py ? uy + tdy : uy;
+        uz = pz ? uz + tdz : uz;
+        kx += px ? 1u : 0u;
+        ky += py ? 1u : 0u;
+        kz += pz ? 1u : 0u;
+    }
+    kx = ex ? (uint32_t)nx : kx;
+    ky = ey ? (uint32_t)ny : ky;
+    kz = ez ? (uint32_t)nz : kz;
+    tmx = ex ? te + tdx : ux;
+    tmy = ey ? te + tdy : uy;
+    tmz = ez ? te + tdz : uz;
+    x += sx * (int)kx;
+    y += sy * (int)ky;
+    z += sz * (int)kz;
+    const uint32_t outside = (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
+    moves = kx + ky + kz - outside;
+}
+
 /* One step of a lane: one iteration of the reference's node loop, or up to
  * DDA_BURST iterations of the leaf DDA. Returns true when the ray is done;
  * res is then its result. */
-template <bool kLds>
+template <bool kLds, bool kSkip = false>
 __device__ static inline bool lane_step(const trace_params& P, const uint2* s_inner, float half, float maxd,
-                                        trace_lane& L, svo_res& res, bool active, bool allow_node, bool allow_dda)
+                                        trace_lane& L, svo_res& res, bool active, bool allow_node, bool allow_dda,
+                                        bool defer_leave = false)
 {
     bool adv = false, done = false;
     /* a lane with a pending advance waits for the wave's next advance */
@@ -367,10 +433,27 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         const float tdx = L.tdx, tdy = L.tdy, tdz = L.tdz;
         const unsigned long long bm = L.bm;
         uint32_t wi_last = L.wi, wv = L.wv;
+        /* brick skips need every moving axis' t_delta below 2^100 */
+        const uint32_t skip_ok = (!kSkip || ((sx == 0 || tdx < 0x1p100f) && (sy == 0 || tdy < 0x1p100f) &&
+                                             (sz == 0 || tdz < 0x1p100f))) ? 1u : 0u;
 #pragma unroll 1
         for (uint32_t k = 0; k < DDA_BURST; k++) {
             const uint32_t cap = it >> 8;
-            const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
+            uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
+            if constexpr (kSkip) {
+                /* a lane in a brick without solid voxels crosses it in one
+                 * go; it continues in the next brick with the step below */
+                const uint32_t emp = live & (cap ^ 1u) & skip_ok & ((uint32_t)(bm >> b) ^ 1u) & 1u;
+                if (emp) {
+                    uint32_t mv;
+                    brick_skip(x, y, z, tmx, tmy, tmz, tdx, tdy, tdz, sx, sy, sz, mv);
+                    it += mv;
+                    const uint32_t outside = (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
+                    leave_i |= outside;
+                    live &= outside ^ 1u;
+                    b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
+                }
+            }
             const uint32_t inb = live & (cap ^ 1u) & (uint32_t)(bm >> b);
             const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
             if ((inb & 1u) && wi != wi_last) { wi_last = wi; wv = grid[wi]; }
@@ -403,6 +486,12 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         if (leave) {
             L.node &= ~ND_DDA;
             adv = true;
+        }
+        /* a lane that left its leaf may leave the advance to a later step
+         * (ND_ADV), taken with other lanes in the same phase */
+        if (defer_leave && leave) {
+            L.node |= ND_ADV;
+            adv = false;
         }
     }
     if (hit) {
@@ -632,6 +721,215 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             if (n_sh1) atomicAdd(&F.stats1[3], n_sh1);
             if (n_shhit1) atomicAdd(&F.stats1[7], n_shhit1);
         }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Regrouped fused GI tracing                                               */
+/* ------------------------------------------------------------------------ */
+
+/* gi_trace_rg_k runs the fused GI launch of gi_trace_k<*, true> with the
+ * same per-ray arithmetic, but keeps the waves of a workgroup in one phase:
+ * after every step the workgroup sorts its rays by phase (leaf DDA first,
+ * then node walk, then idle lanes) through LDS, so a wave runs DDA bursts
+ * for (nearly) all of its lanes or node steps and advances for (nearly) all
+ * of them, instead of both at partial width. A lane leaving its leaf leaves
+ * the advance to the node phase (ND_ADV). Idle lanes collect in the last
+ * waves, which refill from the queue 64 rays per atomic. One 1024-lane
+ * workgroup per CU; LDS holds the traversal layout (when it fits) and six
+ * float4 planes of lane state. */
+#define RG_BLOCK 1024u
+#define RG_WAVES (RG_BLOCK / 64u)
+#define RG_PLANES 6u
+
+struct rg_ctl {
+    uint32_t cnt[RG_WAVES][2];   /* per wave: DDA lanes, node-walk lanes */
+    uint32_t exhausted;
+    uint32_t pad[3];
+};
+
+/* Result of a lane's finished ray in the fused GI trace (same operations as
+ * gi_trace_k<*, true>): a sun-facing bounce hit stores its contribution and
+ * turns the lane into its shadow ray (pending set-up); an occluded shadow
+ * ray clears its contribution. Returns the lane's event bits (1 bounce hit,
+ * 2 shadow ray, 4 shadow hit; << 3 for the second list). */
+__device__ static inline uint32_t fused_result(const trace_params& P, const trace_fuse& F, trace_lane& L,
+                                               const svo_res& res, uint32_t& slot, float ox, float oy, float oz,
+                                               uint32_t sid, bool& pending)
+{
+    uint32_t ev = 0u;
+    if (slot & TRACE_SHADOW) {
+        const uint32_t id = slot & ~TRACE_SHADOW;
+        if (res.flags & 1u) {
+            volatile float* c = F.contrib + 3ull * id;
+            c[0] = 0.0f; c[1] = 0.0f; c[2] = 0.0f;
+            ev = id >= F.split ? 32u : 4u;
+        }
+        return ev;
+    }
+    const uint32_t sh = sid >= F.split ? 3u : 0u;
+    ev = (res.flags & 1u) << sh;
+    const uint32_t ax = (res.flags >> 2) & 3u;
+    const float sign = (res.flags & 16u) ? 1.0f : -1.0f;
+    const float sc = ax == 0u ? F.sun0 : (ax == 1u ? F.sun1 : F.sun2);
+    const float cosv = ((res.flags & 1u) && !(res.distance < 0.0f)) ? sign * sc : 0.0f;
+    if (cosv > 0.0f) {
+        L.px = ox + L.dx * res.distance;
+        L.py = oy + L.dy * res.distance;
+        L.pz = oz + L.dz * res.distance;
+        const float off = sign * GI_SHADOW_OFFSET;
+        if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
+        L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
+        const uint16_t m = F.leaf_mats[(size_t)(res.node - P.n_inner) * TGX_SVO_BLOCK_VOXELS + res.voxel];
+        const tgx_lut_entry* e = F.luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
+        const float wgt = cosv * F.sun_radiance * GI_INV_PI;
+        float* c = F.contrib + 3ull * sid;
+        c[0] = e->albedo[0] * wgt; c[1] = e->albedo[1] * wgt; c[2] = e->albedo[2] * wgt;
+        ev |= 2u << sh;
+        slot = sid | TRACE_SHADOW;
+        pending = true;
+    }
+    return ev;
+}
+
+template <bool kLds, bool kSkip>
+__global__ __launch_bounds__(RG_BLOCK, 1) void gi_trace_rg_k(trace_params P, const svo_ray* __restrict__ rays,
+                                                             const uint32_t* __restrict__ n_rays_ptr, float maxd,
+                                                             uint32_t* __restrict__ cursor, uint32_t refill,
+                                                             uint32_t node_iters, const trace_fuse F)
+{
+    extern __shared__ uint2 s_inner[];
+    const uint32_t lds_nodes = kLds ? ((P.n_inner + P.n_leaves + 1u) & ~1u) : 0u;
+    float4* planes = (float4*)(s_inner + lds_nodes);
+    rg_ctl* ctl = (rg_ctl*)(planes + RG_PLANES * RG_BLOCK);
+    if constexpr (kLds) {
+        for (uint32_t i = threadIdx.x; i < P.n_inner; i += blockDim.x) s_inner[i] = P.inner[i];
+        const uint2* gb = (const uint2*)P.bricks;
+        for (uint32_t i = threadIdx.x; i < P.n_leaves; i += blockDim.x) s_inner[P.n_inner + i] = gb[i];
+    }
+    if (threadIdx.x == 0) ctl->exhausted = 0u;
+    __syncthreads();
+    const uint32_t n_rays = *n_rays_ptr;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const float half = P.half;
+    trace_lane L = {};
+    uint32_t slot = 0, sid = 0;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    uint32_t n_hit0 = 0, n_hit1 = 0, n_sh0 = 0, n_sh1 = 0, n_shhit0 = 0, n_shhit1 = 0;
+    bool active = false, pending = false;
+    while (true) {
+        /* refill: idle lanes sit in the last waves after a regroup */
+        const unsigned long long idle = __ballot(!active && !pending);
+        const uint32_t want = (uint32_t)__popcll(idle);
+        if (want >= refill && *(volatile uint32_t*)&ctl->exhausted == 0u) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(cursor, want);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base + want >= n_rays && lane == 0) *(volatile uint32_t*)&ctl->exhausted = 1u;
+            if (!active && !pending) {
+                const uint32_t r = base + lane_rank(idle);
+                if (r < n_rays) {
+                    slot = r;
+                    const svo_ray q = rays[r];
+                    L.px = q.ox; L.py = q.oy; L.pz = q.oz;
+                    L.dx = q.dx; L.dy = q.dy; L.dz = q.dz;
+                    ox = q.ox; oy = q.oy; oz = q.oz; sid = q.id;
+                    pending = true;
+                }
+            }
+        }
+        if (pending) {
+            pending = false;
+            active = lane_init_self(P, half, L);
+        }
+        /* one step of each lane's phase; node-walk lanes take up to
+         * node_iters steps while the wave has node work */
+        const bool was_dda = active && (L.node & ND_DDA);
+        uint32_t ev = 0u;
+        for (uint32_t k = 0; k < node_iters; k++) {
+            const bool step_dda = k == 0 && was_dda;
+            const bool step_node = active && !was_dda && !(L.node & ND_DDA);
+            if (!__any(step_dda || step_node)) break;
+            svo_res res;
+            if (lane_step<kLds, kSkip>(P, s_inner, half, maxd, L, res, active && (step_dda || step_node), step_node,
+                                       step_dda, true) &&
+                active && (step_dda || step_node)) {
+                active = false;
+                ev |= fused_result(P, F, L, res, slot, ox, oy, oz, sid, pending);
+            }
+        }
+        if (__any(ev != 0u)) {
+            n_hit0 += (uint32_t)__popcll(__ballot(ev & 1u));
+            n_sh0 += (uint32_t)__popcll(__ballot(ev & 2u));
+            n_shhit0 += (uint32_t)__popcll(__ballot(ev & 4u));
+            n_hit1 += (uint32_t)__popcll(__ballot(ev & 8u));
+            n_sh1 += (uint32_t)__popcll(__ballot(ev & 16u));
+            n_shhit1 += (uint32_t)__popcll(__ballot(ev & 32u));
+        }
+        /* regroup: DDA lanes first, then node-walk lanes (pending shadow
+         * rays included), then idle lanes */
+        const bool key_d = active && (L.node & ND_DDA);
+        const bool key_n = (active || pending) && !key_d;
+        const unsigned long long bd = __ballot(key_d), bn = __ballot(key_n);
+        if (lane == 0) {
+            ctl->cnt[wave][0] = (uint32_t)__popcll(bd);
+            ctl->cnt[wave][1] = (uint32_t)__popcll(bn);
+        }
+        __syncthreads();
+        uint32_t td = 0, tn = 0, pd = 0, pn = 0;
+        for (uint32_t w = 0; w < RG_WAVES; w++) {
+            const uint32_t cd = ctl->cnt[w][0], cn = ctl->cnt[w][1];
+            pd += w < wave ? cd : 0u;
+            pn += w < wave ? cn : 0u;
+            td += cd;
+            tn += cn;
+        }
+        const bool done_all = ctl->exhausted != 0u && td + tn == 0u;
+        if (done_all) break;
+        if (key_d || key_n) {
+            const uint32_t dst = key_d ? pd + lane_rank(bd) : td + pn + lane_rank(bn);
+            const uint32_t flags = (active ? 1u : 0u) | (pending ? 2u : 0u);
+            planes[0 * RG_BLOCK + dst] = make_float4(L.px, L.py, L.pz, L.dist);
+            planes[1 * RG_BLOCK + dst] = make_float4(L.dx, L.dy, L.dz, __uint_as_float(L.steps));
+            planes[2 * RG_BLOCK + dst] = make_float4(__uint_as_float(L.cur), __uint_as_float(L.node),
+                                                     __uint_as_float(L.leaf), __uint_as_float(L.vox));
+            planes[3 * RG_BLOCK + dst] = make_float4(L.tmx, L.tmy, L.tmz, __uint_as_float(slot));
+            planes[4 * RG_BLOCK + dst] = make_float4(L.tdx, L.tdy, L.tdz, __uint_as_float(flags));
+            planes[5 * RG_BLOCK + dst] = make_float4(ox, oy, oz, __uint_as_float(sid));
+        }
+        __syncthreads();
+        const uint32_t me = threadIdx.x;
+        if (me < td + tn) {
+            const float4 a = planes[0 * RG_BLOCK + me], b = planes[1 * RG_BLOCK + me];
+            const float4 c = planes[2 * RG_BLOCK + me], d = planes[3 * RG_BLOCK + me];
+            const float4 e = planes[4 * RG_BLOCK + me], f = planes[5 * RG_BLOCK + me];
+            L.px = a.x; L.py = a.y; L.pz = a.z; L.dist = a.w;
+            L.dx = b.x; L.dy = b.y; L.dz = b.z; L.steps = __float_as_uint(b.w);
+            L.cur = __float_as_uint(c.x); L.node = __float_as_uint(c.y);
+            L.leaf = __float_as_uint(c.z); L.vox = __float_as_uint(c.w);
+            L.tmx = d.x; L.tmy = d.y; L.tmz = d.z; slot = __float_as_uint(d.w);
+            L.tdx = e.x; L.tdy = e.y; L.tdz = e.z;
+            const uint32_t flags = __float_as_uint(e.w);
+            active = (flags & 1u) != 0u;
+            pending = (flags & 2u) != 0u;
+            ox = f.x; oy = f.y; oz = f.z; sid = __float_as_uint(f.w);
+            /* the leaf's brick mask and an empty bit-word cache */
+            if (active && (L.node & ND_DDA)) L.bm = bricks_at<kLds>(P.bricks, s_inner, P.n_inner, L.leaf);
+            L.wi = 0xFFFFFFFFu;
+        } else {
+            active = false;
+            pending = false;
+        }
+        /* the planes and counts are written again only after the next
+         * round's first barrier */
+    }
+    if (lane == 0) {
+        if (n_hit0) atomicAdd(&F.stats0[2], n_hit0);
+        if (n_sh0) atomicAdd(&F.stats0[3], n_sh0);
+        if (n_shhit0) atomicAdd(&F.stats0[7], n_shhit0);
+        if (n_hit1) atomicAdd(&F.stats1[2], n_hit1);
+        if (n_sh1) atomicAdd(&F.stats1[3], n_sh1);
+        if (n_shhit1) atomicAdd(&F.stats1[7], n_shhit1);
     }
 }
 
@@ -1099,6 +1397,38 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
     return TGX_OK;
 }
 
+/* The fused GI launch as regrouped waves (gi_trace_rg_k): one 1024-lane
+ * workgroup per CU, node_iters node steps per round. */
+static int launch_trace_rg(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t n_max, float maxd,
+                           uint32_t* cursor, const trace_fuse& F, uint32_t node_iters)
+{
+    if (n_max == 0) return TGX_OK;
+    int cus = 256, max_lds = 65536;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    const uint32_t want = (uint32_t)((n_max + RG_BLOCK - 1) / RG_BLOCK);
+    const dim3 grid(want < (uint32_t)cus ? want : (uint32_t)cus);
+    const trace_params P = params_of(ctx);
+    const size_t state = sizeof(float4) * RG_PLANES * RG_BLOCK + sizeof(rg_ctl);
+    const size_t nodes = 8u * (size_t)((P.n_inner + P.n_leaves + 1u) & ~1u);
+    const bool lds = (uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX && nodes + state <= (size_t)max_lds;
+    if (state > (size_t)max_lds)
+        return tgx_fail(ctx, TGX_ERR_CAPACITY, "regrouped trace needs %zu B of LDS per workgroup, the device has %d",
+                        state, max_lds);
+    const size_t shmem = (lds ? nodes : 0u) + state;
+    const bool skip = ctx->tuning.trace_skip != 0u;
+    auto kernel = lds ? (skip ? gi_trace_rg_k<true, true> : gi_trace_rg_k<true, false>)
+                      : (skip ? gi_trace_rg_k<false, true> : gi_trace_rg_k<false, false>);
+    TGX_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    {
+        tgx_launch_scope s(ctx, "gi_trace_k");
+        hipLaunchKernelGGL(kernel, grid, dim3(RG_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor,
+                           ctx->tuning.trace_refill, node_iters, F);
+    }
+    TGX_LAUNCH_CHECK(ctx, "gi_trace_rg_k");
+    return TGX_OK;
+}
+
 /* Reorders the ray queue rays[0 .. *n_dev) into `sorted` by gi_ray_keys_k
  * (cap bounds the count). */
 static int sort_rays(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t cap, svo_ray* sorted)
@@ -1247,8 +1577,14 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         F.split = two ? id_base[1] : 0xFFFFFFFFu;
         F.sun0 = prm.sun_dir[0]; F.sun1 = prm.sun_dir[1]; F.sun2 = prm.sun_dir[2];
         F.sun_radiance = prm.sun_radiance;
-        if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, nullptr, &F)))
+        if (ctx->tuning.trace_regroup) {
+            if ((rc = launch_trace_rg(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, F,
+                                      ctx->tuning.trace_regroup)))
+                return rc;
+        } else if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, nullptr,
+                                      &F))) {
             return rc;
+        }
     } else {
     if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, two ? nullptr : cnt + 2)))
         return rc;

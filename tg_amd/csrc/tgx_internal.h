@@ -94,7 +94,7 @@ struct tgx_context {
     tgx_buffer   gi_rays2;       /* sorted bounce rays                      */
     tgx_buffer   gi_sort;        /* sort keys and indices                   */
     uint32_t     gi_spp = 0;     /* spp of the last GI call                 */
-    tgx_tuning   tuning = {0u, 16u, 2u, 0u};
+    tgx_tuning   tuning = {0u, 16u, 2u, 0u, 0u, 0u};
 
     /* Neural irradiance cache (tgx_nrc.hip). */
     bool           nrc_ready = false;

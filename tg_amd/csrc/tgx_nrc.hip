@@ -30,6 +30,7 @@
 #include "tgx_gi_dev.h"
 
 #include <cmath>
+#include <cstdio>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -1177,6 +1178,137 @@ int tgx_nrc_last_stats(tgx_context* ctx, tgx_nrc_stats* out)
     memcpy(&lsum, &qc[8], 4);
     s.train_loss = s.train_samples ? lsum / (3.0f * (float)s.train_samples) : 0.0f;
     *out = s;
+    return TGX_OK;
+}
+
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Training-state files                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* "TGXNRC01" | nrc_file_header | config | device counters (64 B) | master,
+ * Adam m, Adam v, moving average (fp32 N each) | error-estimate cells |
+ * u64 FNV-1a 64 of every byte before it. */
+struct nrc_file_header {
+    char     magic[8];
+    uint32_t version;
+    uint32_t n_params;
+    uint32_t n_cells;
+    uint32_t cell_bytes;
+    uint32_t config_bytes;
+    uint32_t pad;
+};
+static const char k_nrc_magic[8] = {'T', 'G', 'X', 'N', 'R', 'C', '0', '1'};
+
+static uint64_t nrc_fnv(const unsigned char* p, size_t n)
+{
+    uint64_t h = 0xCBF29CE484222325ull;
+    for (size_t i = 0; i < n; i++) { h ^= p[i]; h *= 0x100000001B3ull; }
+    return h;
+}
+
+static size_t nrc_file_bytes()
+{
+    return sizeof(nrc_file_header) + sizeof(tgx_nrc_config) + 64u + 4u * 4ull * NRC_N +
+           sizeof(nrc_cell) * (size_t)NRC_CELLS + 8u;
+}
+
+extern "C" {
+
+int tgx_nrc_save(tgx_context* ctx, const char* path)
+{
+    if (!ctx || !path) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    int rc;
+    if ((rc = check_ready(ctx))) return rc;
+    TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<unsigned char> buf(nrc_file_bytes());
+    unsigned char* q = buf.data();
+    nrc_file_header hd = {};
+    memcpy(hd.magic, k_nrc_magic, 8);
+    hd.version = 1u;
+    hd.n_params = NRC_N;
+    hd.n_cells = NRC_CELLS;
+    hd.cell_bytes = (uint32_t)sizeof(nrc_cell);
+    hd.config_bytes = (uint32_t)sizeof(tgx_nrc_config);
+    memcpy(q, &hd, sizeof(hd));
+    q += sizeof(hd);
+    memcpy(q, &ctx->nrc_cfg, sizeof(tgx_nrc_config));
+    q += sizeof(tgx_nrc_config);
+    TGX_HIP_CHECK(ctx, hipMemcpy(q, ctx->nrc_dev.ptr, 64, hipMemcpyDeviceToHost));
+    q += 64;
+    const tgx_buffer* arrays[4] = {&ctx->nrc_master, &ctx->nrc_m, &ctx->nrc_v, &ctx->nrc_ema};
+    for (const tgx_buffer* b : arrays) {
+        TGX_HIP_CHECK(ctx, hipMemcpy(q, b->ptr, 4ull * NRC_N, hipMemcpyDeviceToHost));
+        q += 4ull * NRC_N;
+    }
+    TGX_HIP_CHECK(ctx, hipMemcpy(q, ctx->nrc_cells.ptr, sizeof(nrc_cell) * (size_t)NRC_CELLS, hipMemcpyDeviceToHost));
+    q += sizeof(nrc_cell) * (size_t)NRC_CELLS;
+    const uint64_t h = nrc_fnv(buf.data(), (size_t)(q - buf.data()));
+    memcpy(q, &h, 8);
+    FILE* f = fopen(path, "wb");
+    if (!f) return tgx_fail(ctx, TGX_ERR_IO, "cannot open %s for writing", path);
+    const bool ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if (fclose(f) != 0 || !ok) return tgx_fail(ctx, TGX_ERR_IO, "write of %s failed", path);
+    return TGX_OK;
+}
+
+int tgx_nrc_load(tgx_context* ctx, const char* path)
+{
+    if (!ctx || !path) return TGX_ERR_INVALID_ARG;
+    ctx->last_error.clear();
+    if (ctx->n_graphs > 0)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "a frame graph uses the cache: destroy it before loading");
+    FILE* f = fopen(path, "rb");
+    if (!f) return tgx_fail(ctx, TGX_ERR_IO, "cannot open %s", path);
+    const size_t want = nrc_file_bytes();
+    std::vector<unsigned char> buf(want + 1);
+    const size_t got = fread(buf.data(), 1, want + 1, f);
+    fclose(f);
+    if (got != want) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: %zu bytes, a cache state file has %zu", path, got, want);
+    nrc_file_header hd;
+    memcpy(&hd, buf.data(), sizeof(hd));
+    if (memcmp(hd.magic, k_nrc_magic, 8) != 0 || hd.version != 1u)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: not a cache state file", path);
+    if (hd.n_params != NRC_N || hd.n_cells != NRC_CELLS || hd.cell_bytes != sizeof(nrc_cell) ||
+        hd.config_bytes != sizeof(tgx_nrc_config))
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: network or table sizes differ from this library's", path);
+    uint64_t h;
+    memcpy(&h, buf.data() + want - 8, 8);
+    if (nrc_fnv(buf.data(), want - 8) != h) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: checksum mismatch", path);
+    const unsigned char* q = buf.data() + sizeof(hd);
+    tgx_nrc_config cfg;
+    memcpy(&cfg, q, sizeof(cfg));
+    q += sizeof(cfg);
+    /* the stored configuration passes the same checks as a new cache's;
+     * a rejected file leaves the current cache as it was */
+    std::vector<float> arr(4ull * NRC_N);
+    memcpy(arr.data(), q + 64, 4ull * 4 * NRC_N);
+    for (float v : arr)
+        if (!std::isfinite(v)) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: non-finite weights or moments", path);
+    int rc = tgx_nrc_create(ctx, &cfg);
+    if (rc) return rc;
+    TGX_HIP_CHECK(ctx, hipMemcpy(ctx->nrc_dev.ptr, q, 64, hipMemcpyHostToDevice));
+    q += 64;
+    tgx_buffer* arrays[4] = {&ctx->nrc_master, &ctx->nrc_m, &ctx->nrc_v, &ctx->nrc_ema};
+    for (tgx_buffer* b : arrays) {
+        TGX_HIP_CHECK(ctx, hipMemcpy(b->ptr, q, 4ull * NRC_N, hipMemcpyHostToDevice));
+        q += 4ull * NRC_N;
+    }
+    TGX_HIP_CHECK(ctx, hipMemcpy(ctx->nrc_cells.ptr, q, sizeof(nrc_cell) * (size_t)NRC_CELLS, hipMemcpyHostToDevice));
+    /* BF16 fragments of the master weights and of the moving average */
+    TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_fwd.ptr, 0, 2ull * NRC_FWD_ELEMS, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_bwd.ptr, 0, 2ull * NRC_BWD_ELEMS, ctx->stream));
+    hipLaunchKernelGGL(nrc_pack_k, dim3(nrc_blocks(NRC_N, 256)), dim3(256), 0, ctx->stream,
+                       (const float*)ctx->nrc_master.ptr, (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr);
+    TGX_LAUNCH_CHECK(ctx, "nrc_pack_k");
+    TGX_HIP_CHECK(ctx, hipMemsetAsync(ctx->nrc_efwd.ptr, 0, 2ull * NRC_FWD_ELEMS, ctx->stream));
+    hipLaunchKernelGGL(nrc_pack_k, dim3(nrc_blocks(NRC_N, 256)), dim3(256), 0, ctx->stream,
+                       (const float*)ctx->nrc_ema.ptr, (__bf16*)ctx->nrc_efwd.ptr, (__bf16*)nullptr);
+    TGX_LAUNCH_CHECK(ctx, "nrc_pack_k");
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return TGX_OK;
 }
 
