@@ -380,6 +380,13 @@ int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes, uint32_t* leaf_bits,
  * only a device error or a file changed during the load leaves no SVO built.
  * The loaded SVO exports and traces as the saved one did; the next
  * tgx_svo_update samples every leaf. */
+/* Device memory the built SVO's data takes (node records, leaf bits,
+ * material offsets and word prefixes, the solid voxels' materials, cluster
+ * lists and the traversal layout) and its solid voxel count. A leaf keeps
+ * materials for its solid voxels only (2 B each, in bit order) plus 2 KiB of
+ * per-word prefix counts, where a dense grid would take 64 KiB. */
+int tgx_svo_device_bytes(tgx_context* ctx, uint64_t* bytes, uint64_t* solid_voxels);
+
 int tgx_svo_save(tgx_context* ctx, const char* path);
 int tgx_svo_load(tgx_context* ctx, const char* path);
 
@@ -544,6 +551,8 @@ typedef struct tgx_tuning {
     uint32_t trace_skip;          /* regrouped launch: 1 crosses 8^3 bricks without solid voxels
                                      in one exact step of the leaf DDA instead of voxel by
                                      voxel (0) */
+    uint32_t vis_setup_blocks;    /* visibility: workgroups per camera of the grid-stride cull
+                                     and tile-list scatter passes, 64..16384 (2048) */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);
@@ -651,6 +660,14 @@ typedef struct tgx_nrc_config {
  * on the training targets), instead of fallback_threshold times t^2: with
  * threshold 1 a pixel is traced when tracing is expected to be closer. */
 #define TGX_NRC_FLAG_FALLBACK_VS_TRACER 8u
+/* Frames of several cameras (frame graphs, tgx_frame_batch): every camera
+ * renders with the cache the previous frame left, then one training pass
+ * takes train_samples / train_batch Adam steps, each on train_batch
+ * samples of every camera, and the error estimate decays once per frame.
+ * Without the flag the cameras run as consecutive single-camera frames
+ * (train_samples of training per camera). One camera: the same frame
+ * either way. */
+#define TGX_NRC_FLAG_JOINT_TRAINING 16u
 
 typedef struct tgx_nrc_stats {
     uint64_t visible_pixels;

@@ -218,11 +218,23 @@ def test_autotune_keeps_results(ctx, batch_scene):
         g.launch()
         before = buf.read()
         g.destroy()
-        say("autotune over 10 candidates, 2 cameras")
+        say("autotune (coordinate descent over trace form, skip, workgroups, refill, cull blocks), 2 cameras")
         best = ctx.autotune(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr, frames=1)
         assert tuple(ctx.tuning()[0]) == tuple(best[0])
         assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 8, 16, 24, 32)
+        assert best["trace_regroup"][0] in (0, 1, 2, 4) and best["trace_skip"][0] in (0, 1)
+        assert best["vis_setup_blocks"][0] in (256, 1024, 2048, 4096)
         assert best["trace_sort"][0] == 0 and best["trace_vote"][0] == 0
+        # every form the tuner tries gives the same frame
+        for form in ((1, 0), (2, 1), (0, 0)):
+            ctx.set_tuning(trace_regroup=form[0], trace_skip=form[1], vis_setup_blocks=256)
+            g = ctx.frame_graph(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr)
+            g.launch()
+            got = buf.read()
+            g.destroy()
+            for a, b in zip(before, got):
+                assert a.tobytes() == b.tobytes(), f"form {form}"
+        ctx.set_tuning(**{k: int(best[k][0]) for k in best.dtype.names})
         g = ctx.frame_graph(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr)
         g.launch()
         after = buf.read()
@@ -230,5 +242,84 @@ def test_autotune_keeps_results(ctx, batch_scene):
         for a, b in zip(before, after):
             assert a.tobytes() == b.tobytes()
     finally:
-        ctx.set_tuning(trace_sort=0, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0)
+        ctx.set_tuning(trace_sort=0, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0, trace_regroup=0,
+                       trace_skip=0, vis_setup_blocks=2048)
         buf.free()
+
+
+def test_frame_graph_joint_training(ctx, batch_scene):
+    """Joint training (TGX_NRC_FLAG_JOINT_TRAINING): a captured 3-camera
+    cache frame equals the same frames run uncaptured with tgx_frame_batch,
+    outputs and weights bit for bit; a frame takes train_samples /
+    train_batch Adam steps, each on train_batch samples of every camera."""
+    p, sc, cams, mn = batch_scene
+    setup_scene(ctx, p, mn)
+    prm = T.gi_params(T.seeded_sun(4), spp=1, frame=5)
+    n_cams, replays = 3, 2
+    cfg = T.nrc_config(origin=tuple(mn), scale=1.0 / (1 << DEPTH), train_samples=4096, train_batch=2048, seed=9,
+                       flags=T.NRC_FLAG_FALLBACK_VS_TRACER | T.NRC_FLAG_JOINT_TRAINING)
+    steps_per_frame = 4096 // 2048
+    buf = Buffers(ctx, n_cams, W, H)
+    try:
+        ctx.nrc_create(cfg)
+        say(f"joint training: {n_cams} cameras {W}x{H}, captured, {replays} replays")
+        g = ctx.frame_graph(cams[:n_cams], W, H, prm, buf.vis, buf.rad, buf.irr, flags=T.FRAME_NEURAL)
+        try:
+            for _ in range(replays):
+                g.launch()
+            st = g.stats()
+            out_g = buf.read()
+            w_g = ctx.nrc_params()
+            assert ctx.nrc_stats()["adam_steps"] == (1 + replays) * steps_per_frame
+        finally:
+            g.destroy()
+        assert st["train_rays"] > 0
+        say("joint training: the same frames uncaptured")
+        ctx.nrc_create(cfg)
+        for _ in range(1 + replays):
+            ctx.frame_batch(cams[:n_cams], W, H, prm, buf.vis, buf.rad, buf.irr, flags=T.FRAME_NEURAL)
+        out_b = buf.read()
+        for a, b in zip(out_g, out_b):
+            assert a.tobytes() == b.tobytes()
+        assert ctx.nrc_params().tobytes() == w_g.tobytes()
+        assert ctx.nrc_stats()["adam_steps"] == (1 + replays) * steps_per_frame
+    finally:
+        buf.free()
+
+
+def test_joint_training_learns_every_camera(ctx):
+    """Two cameras trained jointly (2^16 targets per camera and frame, 16
+    Adam steps of 2 x 4096 samples): after 200 frames each camera's cache
+    irradiance is far closer to a 256-spp reference than a 4-spp frame."""
+    from oracle import nrc as N
+    w, h = 240, 135
+    p = T.noise_objects_params(seed=1)
+    sc = T.generate_scene(p)
+    cams = np.concatenate([T.generate_camera(p, i, w, h) for i in range(2)])
+    ctx.upload(sc)
+    ctx.svo_build(np.floor(cams[0]["camera"][:3]) - 1024.0, 11)
+    sun = T.seeded_sun(1)
+    refs, plains = [], []
+    for c in range(2):
+        acc = np.zeros((h, w, 3), np.float64)
+        for f in range(64):
+            acc += ctx.frame(cams[c:c + 1], w, h, T.gi_params(sun, spp=4, frame=1000 + f))[2]
+        refs.append(acc / 64)
+        plains.append(ctx.frame(cams[c:c + 1], w, h, T.gi_params(sun, spp=4, frame=7)))
+    ctx.nrc_create(T.nrc_config(origin=(0, 0, 0), scale=1 / 512, train_samples=1 << 16, train_batch=1 << 12,
+                                flags=T.NRC_FLAG_NO_FALLBACK | T.NRC_FLAG_JOINT_TRAINING))
+    buf = Buffers(ctx, 2, w, h)
+    try:
+        say("joint training: 2 cameras, 200 frames")
+        for _ in range(200):
+            ctx.frame_batch(cams, w, h, T.gi_params(sun, spp=4), buf.vis, buf.rad, buf.irr, flags=T.FRAME_NEURAL)
+        vis, _, irr = buf.read()
+    finally:
+        buf.free()
+    assert ctx.nrc_stats()["adam_steps"] == 200 * 16
+    for c in range(2):
+        m = vis[c] != T.VISIBILITY_EMPTY
+        e_plain = N.rel_mse(plains[c][2][m], refs[c][m])
+        e_nrc = N.rel_mse(irr[c][m], refs[c][m])
+        say(f"camera {c}: rel-MSE plain 4 spp {e_plain:.3f}, jointly trained cache {e_nrc:.3f}")
+        assert e_nrc < 0.15 * e_plain

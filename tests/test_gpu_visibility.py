@@ -202,3 +202,35 @@ def test_object_cull_at_frustum_edges(ctx):
         ref = O.visibility_brute(O.OracleScene(sc), cam, w, h_)
         assert_same(got, ref, name)
         assert lo <= (ref != T.VISIBILITY_EMPTY).mean() <= hi, name
+
+
+def test_object_cull_at_the_far_plane(ctx):
+    """Objects around the far plane (nearest voxel just inside, on and just
+    beyond it, and whole objects beyond it): the per-object far cull drops
+    only objects that cannot hold a word with depth <= 1, so the frame equals
+    the oracle's brute-force visibility word for word."""
+    rng = np.random.default_rng(31)
+    far = 60.0
+    cam = T.make_camera((1.0, 2.0, -3.0), pitch=-0.1, yaw=0.4, roll=0.0, fov_y=1.1, aspect=1.5, far=far)
+    pos = cam["camera"][0, :3].astype(np.float64)
+    c = [cam[k][0, :3].astype(np.float64) for k in ("ray_bl", "ray_br", "ray_tr", "ray_tl")]
+    objs = []
+    for k in range(40):
+        n = tuple(int(v) for v in rng.integers(1, 4, 3))
+        ray = c[0] + rng.uniform(0.1, 0.9) * (c[1] - c[0]) + rng.uniform(0.1, 0.9) * (c[3] - c[0])
+        ray /= np.linalg.norm(ray)
+        half = 4.0 * np.array(n, np.float64)
+        # the centre's distance puts the object's near side around the far plane
+        d = far + rng.uniform(-1.0, 1.0) * float(np.linalg.norm(half)) + rng.uniform(-3.0, 30.0)
+        objs.append({"n": n, "translation": tuple(pos + d * ray), "rotation": S.random_rotation(rng)})
+    sc = S.make_scene(objs, seed=13)
+    ctx.upload(sc)
+    w, h = 120, 80
+    say(f"object cull at the far plane: {len(objs)} objects, {w}x{h}")
+    got = ctx.visibility(cam, w, h)[0]
+    ref = O.visibility_brute(O.OracleScene(sc), cam, w, h)
+    assert_same(got, ref, "far plane")
+    hits = ref[ref != T.VISIBILITY_EMPTY]
+    assert hits.size > 50
+    # some hits lie in the last percent of the depth range
+    assert ((hits >> np.uint64(40)) > np.uint64(int(0.97 * 16777215))).any()

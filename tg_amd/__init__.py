@@ -70,7 +70,8 @@ def seeded_sun(seed: int) -> np.ndarray:
     return np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
 
 TUNING_DTYPE = np.dtype([("trace_sort", "<u4"), ("trace_refill", "<u4"), ("trace_blocks_per_cu", "<u4"),
-                         ("trace_vote", "<u4"), ("trace_regroup", "<u4"), ("trace_skip", "<u4")])
+                         ("trace_vote", "<u4"), ("trace_regroup", "<u4"), ("trace_skip", "<u4"),
+                         ("vis_setup_blocks", "<u4")])
 
 NRC_N_PARAMS = 18624
 NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learning_rate", "<f4"),
@@ -116,6 +117,7 @@ NRC_FLAG_NO_FALLBACK = 1
 NRC_FLAG_ALL_FALLBACK = 2
 NRC_FLAG_NO_TRAINING = 4
 NRC_FLAG_FALLBACK_VS_TRACER = 8
+NRC_FLAG_JOINT_TRAINING = 16
 
 
 def nrc_config(origin=(0.0, 0.0, 0.0), scale: float = 1.0 / 512, learning_rate: float = 1e-3, beta1: float = 0.9,
@@ -201,6 +203,7 @@ def load_library() -> ctypes.CDLL:
         "tgx_visibility_device": (i32, [vp, vp, u32, u32, u32, vp]),
         "tgx_visibility_device_noalloc": (i32, [vp, vp, u32, u32, u32, vp]),
         "tgx_nrc_save": (i32, [vp, ctypes.c_char_p]),
+        "tgx_svo_device_bytes": (i32, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "tgx_nrc_load": (i32, [vp, ctypes.c_char_p]),
         "tgx_visibility_clear_device": (i32, [vp, vp, u64]),
         "tgx_visibility_decode": (i32, [u64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
@@ -714,6 +717,12 @@ class Context:
                                               _ptr(s.cluster_idx_to_object), _ptr(s.bits), _ptr(s.materials),
                                               _ptr(s.luts)), "tgx_scene_export")
         return s
+
+    def svo_device_bytes(self) -> tuple[int, int]:
+        """(device bytes of the SVO's data, solid voxels) (tgx_svo_device_bytes)."""
+        b, n = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.lib.tgx_svo_device_bytes(self.h, ctypes.byref(b), ctypes.byref(n)), "tgx_svo_device_bytes")
+        return int(b.value), int(n.value)
 
     def svo_visibility_device(self, cam: np.ndarray, w: int, h: int, vis_dev: int) -> None:
         cam = np.ascontiguousarray(cam, dtype=CAMERA_DTYPE)

@@ -324,29 +324,56 @@ int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint
     hipEvent_t t0 = nullptr, t1 = nullptr;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
-    /* the candidates: persistent trace workgroups per CU and the refill
-     * threshold, with unsorted rays as captured frames trace them; the frame
-     * is run once untimed per candidate, then timed */
-    const uint32_t bpc[2] = {1u, 2u}, refill[5] = {4u, 8u, 16u, 24u, 32u};
-    for (uint32_t k = 0; k < 10 && !rc; k++) {
-        tgx_tuning t = start;
-        t.trace_blocks_per_cu = bpc[k / 5];
-        t.trace_refill = refill[k % 5];
-        t.trace_sort = 0u;
-        t.trace_vote = 0u;
+    /* Coordinate descent over the launch shapes and queue policies, with
+     * unsorted rays as captured frames trace them: each knob in turn takes
+     * each of its values with the others at their best so far; a candidate
+     * runs one untimed frame, then `frames` timed ones. No knob changes a
+     * result. */
+    struct knob {
+        uint32_t tgx_tuning::*field;
+        uint32_t n;
+        uint32_t values[6];
+    };
+    const knob knobs[] = {
+        {&tgx_tuning::trace_regroup, 4u, {0u, 1u, 2u, 4u}},          /* trace launch form, node steps per round */
+        {&tgx_tuning::trace_skip, 2u, {0u, 1u}},                      /* exact empty-brick skip (regrouped form) */
+        {&tgx_tuning::trace_blocks_per_cu, 2u, {1u, 2u}},             /* lane-machine workgroups per CU */
+        {&tgx_tuning::trace_refill, 5u, {4u, 8u, 16u, 24u, 32u}},     /* idle lanes that trigger a queue refill */
+        {&tgx_tuning::vis_setup_blocks, 4u, {256u, 1024u, 2048u, 4096u}}, /* cull / scatter workgroups per camera */
+    };
+    best.trace_sort = 0u;
+    best.trace_vote = 0u;
+    auto time_candidate = [&](const tgx_tuning& t, double& ms_out) -> int {
         ctx->tuning = t;
-        if ((rc = frame_body(ctx, &g, false))) break;
+        int r = frame_body(ctx, &g, false);
+        if (r) return r;
         (void)hipEventRecord(t0, ctx->stream);
-        for (uint32_t f = 0; f < frames && !rc; f++) rc = frame_body(ctx, &g, false);
+        for (uint32_t f = 0; f < frames && !r; f++) r = frame_body(ctx, &g, false);
         (void)hipEventRecord(t1, ctx->stream);
-        if (rc) break;
-        if ((e = hipEventSynchronize(t1)) != hipSuccess) {
-            rc = tgx_fail(ctx, TGX_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
-            break;
-        }
+        if (r) return r;
+        if ((e = hipEventSynchronize(t1)) != hipSuccess)
+            return tgx_fail(ctx, TGX_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, t0, t1);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+        ms_out = ms;
+        return TGX_OK;
+    };
+    rc = time_candidate(best, best_ms);
+    for (const knob& kb : knobs) {
+        if (rc) break;
+        /* the skip exists in the regrouped form only, the workgroups per CU
+         * in the lane-machine form only */
+        if (kb.field == &tgx_tuning::trace_skip && best.trace_regroup == 0u) continue;
+        if (kb.field == &tgx_tuning::trace_blocks_per_cu && best.trace_regroup != 0u) continue;
+        const uint32_t cur = best.*kb.field;
+        for (uint32_t i = 0; i < kb.n && !rc; i++) {
+            if (kb.values[i] == cur) continue;
+            tgx_tuning t = best;
+            t.*kb.field = kb.values[i];
+            double ms = 0.0;
+            if ((rc = time_candidate(t, ms))) break;
+            if (ms < best_ms) { best_ms = ms; best = t; }
+        }
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

@@ -562,8 +562,25 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
  * ids at or above split count for the second counter block (stats1), the
  * others for stats0 (both may be the same block): bounce hits go to
  * stats[2], shadow rays to stats[3] and shadow hits to stats[7]. */
+/* Leaf materials are stored for solid voxels only, leaf by leaf in bit
+ * order: solid voxel v of leaf l is entry mat_off[l] + (solid voxels of the
+ * leaf's earlier words, wpre) + (solid voxels below v in its word). */
+struct leaf_mat_view {
+    const uint16_t* pool;
+    const uint32_t* mat_off;
+    const uint16_t* wpre;
+    const uint32_t* bits;
+};
+
+__device__ static inline uint16_t leaf_material(const leaf_mat_view& M, uint32_t leaf, uint32_t voxel)
+{
+    const size_t w = (size_t)leaf * TGX_SVO_BLOCK_WORDS + (voxel >> 5);
+    const uint32_t below = M.bits[w] & ((1u << (voxel & 31u)) - 1u);
+    return M.pool[M.mat_off[leaf] + M.wpre[w] + (uint32_t)__popc(below)];
+}
+
 struct trace_fuse {
-    const uint16_t*      leaf_mats;
+    leaf_mat_view        mats;
     const tgx_lut_entry* luts;
     float*               contrib;
     uint32_t*            stats0;
@@ -690,7 +707,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                     const float off = sign * GI_SHADOW_OFFSET;
                     if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
                     L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
-                    const uint16_t m = F.leaf_mats[(size_t)(res.node - P.n_inner) * TGX_SVO_BLOCK_VOXELS + res.voxel];
+                    const uint16_t m = leaf_material(F.mats, res.node - P.n_inner, res.voxel);
                     const tgx_lut_entry* e = F.luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
                     const float wgt = cosv * F.sun_radiance * GI_INV_PI;
                     float* c = F.contrib + 3ull * id;
@@ -780,7 +797,7 @@ __device__ static inline uint32_t fused_result(const trace_params& P, const trac
         const float off = sign * GI_SHADOW_OFFSET;
         if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
         L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
-        const uint16_t m = F.leaf_mats[(size_t)(res.node - P.n_inner) * TGX_SVO_BLOCK_VOXELS + res.voxel];
+        const uint16_t m = leaf_material(F.mats, res.node - P.n_inner, res.voxel);
         const tgx_lut_entry* e = F.luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
         const float wgt = cosv * F.sun_radiance * GI_INV_PI;
         float* c = F.contrib + 3ull * sid;
@@ -1147,7 +1164,7 @@ __device__ static inline float shade_cos(const svo_res& r, const tgx_gi_params& 
  * With two lists in the queue (stats0 and stats1 not null), each list's
  * bounce hits and shadow rays are counted apart; ids below split belong to
  * the first. */
-__global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, const uint16_t* __restrict__ leaf_mats,
+__global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, leaf_mat_view mats,
                                                        const tgx_lut_entry* __restrict__ luts, tgx_gi_params prm,
                                                        const svo_ray* __restrict__ rays,
                                                        const uint32_t* __restrict__ n_rays_ptr,
@@ -1191,7 +1208,7 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, const u
         if (ax == 0u) ox = ox + off; else if (ax == 1u) oy = oy + off; else oz = oz + off;
         const svo_ray sr = {ox, oy, oz, prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2], b.id, slot};
         shadow[slot] = sr;
-        const uint16_t m = leaf_mats[(size_t)(r.node - n_inner) * TGX_SVO_BLOCK_VOXELS + r.voxel];
+        const uint16_t m = leaf_material(mats, r.node - n_inner, r.voxel);
         const tgx_lut_entry e = luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
         const float wgt = cosv * prm.sun_radiance * GI_INV_PI;
         weight[3ull * slot] = e.albedo[0] * wgt;
@@ -1397,6 +1414,12 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
     return TGX_OK;
 }
 
+static leaf_mat_view leaf_mats_of(const tgx_context* ctx)
+{
+    return {(const uint16_t*)ctx->svo_leaf_mats.ptr, (const uint32_t*)ctx->svo_leaf_moff.ptr,
+            (const uint16_t*)ctx->svo_leaf_wpre.ptr, (const uint32_t*)ctx->svo_leaf_bits.ptr};
+}
+
 /* The fused GI launch as regrouped waves (gi_trace_rg_k): one 1024-lane
  * workgroup per CU, node_iters node steps per round. */
 static int launch_trace_rg(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t n_max, float maxd,
@@ -1419,7 +1442,13 @@ static int launch_trace_rg(tgx_context* ctx, const svo_ray* rays, const uint32_t
     const bool skip = ctx->tuning.trace_skip != 0u;
     auto kernel = lds ? (skip ? gi_trace_rg_k<true, true> : gi_trace_rg_k<true, false>)
                       : (skip ? gi_trace_rg_k<false, true> : gi_trace_rg_k<false, false>);
-    TGX_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    /* the LDS limit is raised once per variant, before any capture */
+    const uint32_t variant = 1u << ((lds ? 2u : 0u) + (skip ? 1u : 0u));
+    if (!(ctx->rg_lds_set & variant)) {
+        TGX_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               max_lds));
+        ctx->rg_lds_set |= variant;
+    }
     {
         tgx_launch_scope s(ctx, "gi_trace_k");
         hipLaunchKernelGGL(kernel, grid, dim3(RG_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor,
@@ -1569,7 +1598,7 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
     if (!(ctx->tuning.trace_sort & 2u)) {
         /* bounce and shadow rays in one launch */
         trace_fuse F = {};
-        F.leaf_mats = (const uint16_t*)ctx->svo_leaf_mats.ptr;
+        F.mats = leaf_mats_of(ctx);
         F.luts = (const tgx_lut_entry*)ctx->luts.ptr;
         F.contrib = contrib;
         F.stats0 = two ? sets[0].counters : cnt;
@@ -1591,7 +1620,7 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
     {
         tgx_launch_scope s(ctx, "gi_shade_k");
         hipLaunchKernelGGL(gi_shade_k, dim3((uint32_t)((n_samples + GI_BLOCK * SHADE_ITEMS - 1) / (GI_BLOCK * SHADE_ITEMS))),
-                           dim3(GI_BLOCK), 0, ctx->stream, ctx->svo_n_inner, (const uint16_t*)ctx->svo_leaf_mats.ptr,
+                           dim3(GI_BLOCK), 0, ctx->stream, ctx->svo_n_inner, leaf_mats_of(ctx),
                            (const tgx_lut_entry*)ctx->luts.ptr, prm, rays, cnt + 1, res, shadow, weight, cnt,
                            two ? id_base[1] : 0xFFFFFFFFu, two ? sets[0].counters : nullptr,
                            two ? sets[1].counters : nullptr);

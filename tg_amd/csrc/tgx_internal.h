@@ -44,7 +44,7 @@ struct tgx_context {
     /* Visibility work buffers. */
     tgx_buffer   vis_clusters;   /* compacted per-(camera, cluster) records */
     tgx_buffer   vis_solid;      /* u8 per cluster index: has a solid voxel */
-    tgx_buffer   vis_objvis;     /* u8 per (camera, object): not culled     */
+    tgx_buffer   vis_objcnt;     /* u32 per (camera, object): kept counts   */
     uint64_t     scene_version = 0;     /* bumped by every write of cluster bits */
     uint64_t     vis_solid_version = ~0ull; /* scene_version vis_solid was made for */
     tgx_buffer   vis_rec_flags;  /* 1 if a record is in the tile lists      */
@@ -68,7 +68,10 @@ struct tgx_context {
     tgx_svo_info svo_info = {};
     tgx_buffer   svo_nodes;      /* svo_gnode[n_nodes], canonical order     */
     tgx_buffer   svo_leaf_bits;  /* 1024 u32 per leaf                       */
-    tgx_buffer   svo_leaf_mats;  /* 32768 u16 per leaf                      */
+    tgx_buffer   svo_leaf_mats;  /* u16 per solid voxel, leaf by leaf in bit order */
+    tgx_buffer   svo_leaf_moff;  /* n_leaves + 1: a leaf's first material   */
+    tgx_buffer   svo_leaf_wpre;  /* 1024 u16 per leaf: solid voxels in earlier words */
+    uint64_t     svo_n_solid = 0;/* solid voxels over all leaves            */
     tgx_buffer   svo_leaf_n;     /* clusters per leaf that sampled a voxel  */
     tgx_buffer   svo_list_off;   /* n_leaves + 1                            */
     tgx_buffer   svo_list;       /* cluster idx lists                       */
@@ -94,7 +97,8 @@ struct tgx_context {
     tgx_buffer   gi_rays2;       /* sorted bounce rays                      */
     tgx_buffer   gi_sort;        /* sort keys and indices                   */
     uint32_t     gi_spp = 0;     /* spp of the last GI call                 */
-    tgx_tuning   tuning = {0u, 16u, 2u, 0u, 0u, 0u};
+    uint32_t     rg_lds_set = 0; /* regrouped trace variants whose LDS limit is raised */
+    tgx_tuning   tuning = {0u, 16u, 2u, 0u, 0u, 0u, 2048u};
 
     /* Neural irradiance cache (tgx_nrc.hip). */
     bool           nrc_ready = false;
@@ -137,7 +141,7 @@ struct tgx_context {
     uint32_t     svo_prev_leaves = 0;
     uint32_t     svo_reused = 0;
     tgx_buffer   svo_prev_keys, svo_prev_off, svo_prev_cand;
-    tgx_buffer   svo_old_bits, svo_old_mats, svo_old_n, svo_old_list_off, svo_old_list;
+    tgx_buffer   svo_old_bits, svo_old_mats, svo_old_moff, svo_old_wpre, svo_old_n, svo_old_list_off, svo_old_list;
     /* Temporary arrays of a build: one arena, plus spills freed at its end. */
     tgx_buffer          svo_arena;
     size_t              svo_arena_used = 0, svo_arena_want = 0, svo_arena_spill_bytes = 0;

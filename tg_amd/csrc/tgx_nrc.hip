@@ -42,6 +42,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define NRC_WTILES     76u      /* 16 x 16 weight-gradient tiles */
 #define NRC_ROWS       624u     /* [feature] rows of the training scratch */
 #define NRC_CELLS      (1u << 18)
+#define NRC_ADAM_BLOCKS ((NRC_N + 255u) / 256u)   /* workgroups of nrc_adam_k */
 #define NRC_FIX        1048576.0f /* fixed point of the error estimate, 2^20 */
 #define NRC_BLOCK      256u
 
@@ -70,6 +71,18 @@ struct nrc_cell {
     uint32_t ref;  /* sum of t^2, fixed point */
     uint32_t n;    /* samples * 256 */
     uint32_t var;  /* sum of var(t), fixed point */
+};
+
+/* A training batch drawn from several cameras: `len` consecutive samples
+ * per camera, camera k's surfaces, list, count and targets at k times the
+ * strides (in records, entries, words and samples). One camera: len = the
+ * batch, strides 0. */
+struct nrc_segs {
+    uint32_t len;
+    uint32_t surf;
+    uint32_t list;
+    uint32_t count;
+    uint32_t tgt;
 };
 
 struct nrc_enc {
@@ -265,23 +278,29 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
                                                           const float* __restrict__ tvar, nrc_enc enc, float two_inv_n,
                                                           uint32_t Bc, __bf16* __restrict__ train,
                                                           float* __restrict__ loss, nrc_cell* __restrict__ cells,
-                                                          uint32_t cell_shift)
+                                                          uint32_t cell_shift, nrc_segs sg)
 {
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, n = lane & 15u;
     const uint32_t tile = (blockIdx.x * NRC_BLOCK + threadIdx.x) >> 6;
     const uint32_t b = tile * 16u + n;          /* sample within the batch */
     if (tile * 16u >= Bc) return;               /* whole wave */
-    const uint32_t count = *count_ptr;
-    const bool valid = first + b < count;
+    /* the batch is sg.len samples from each of Bc / sg.len cameras */
+    const uint32_t k = b / sg.len, bl = b - k * sg.len;
+    surf += (size_t)k * sg.surf;
+    if (list) list += (size_t)k * sg.list;
+    targets += 3ull * k * sg.tgt;
+    if (tvar) tvar += (size_t)k * sg.tgt;
+    const uint32_t count = count_ptr[(size_t)k * sg.count];
+    const bool valid = first + bl < count;
     gi_surf_rec r = {};
-    if (valid) r = surf[list ? list[first + b] : first + b];
+    if (valid) r = surf[list ? list[first + bl] : first + bl];
     bf16x8 x[3], h[4][2];
     encode(r, enc, g, x);
     f32x4 y;
     forward_tile(wf, lane, x, h, y);
     /* loss gradient in lanes g == 0 (features 0..2 of the output tile) */
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
-    if (valid && g == 0u) { t0 = targets[3ull * (first + b)]; t1 = targets[3ull * (first + b) + 1]; t2 = targets[3ull * (first + b) + 2]; }
+    if (valid && g == 0u) { t0 = targets[3ull * (first + bl)]; t1 = targets[3ull * (first + bl) + 1]; t2 = targets[3ull * (first + bl) + 2]; }
     const float d0 = y[0] - t0, d1 = y[1] - t1, d2 = y[2] - t2;
     bf16x8 dz5;
 #pragma unroll
@@ -294,7 +313,7 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
         loss[b] = se;
         if (cells) {
             const uint32_t c = cell_of(r, cell_shift);
-            const float tv = tvar ? tvar[first + b] : 0.0f;
+            const float tv = tvar ? tvar[first + bl] : 0.0f;
             const float bias2 = se - tv;
             const float cl = fminf(fmaxf(bias2, -16.0f), 16.0f);
             atomicAdd(&cells[c].err, (int32_t)(cl * NRC_FIX));
@@ -327,10 +346,11 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
         for (uint32_t c = 0; c < 3u; c++) train[(size_t)(R_DZ5 + c) * Bc + b] = dz5[c];
 }
 
-/* One 16 x 16 tile of dW_L = A_L^T dZ_(L+1) over NRC_CHUNK samples per
- * wave; partial[chunk][param]. */
+/* One 16 x 16 tile of dW_L = A_L^T dZ_(L+1) over `per` consecutive
+ * chunks of NRC_CHUNK samples per wave (in sample order); partial[group]
+ * [param] for the n_chunks groups. */
 __global__ __launch_bounds__(NRC_BLOCK) void nrc_wgrad_k(const __bf16* __restrict__ train, uint32_t Bc,
-                                                         uint32_t n_chunks, float* __restrict__ partial)
+                                                         uint32_t n_chunks, uint32_t per, float* __restrict__ partial)
 {
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, n = lane & 15u;
     const uint32_t wave = (blockIdx.x * NRC_BLOCK + threadIdx.x) >> 6;
@@ -343,13 +363,13 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_wgrad_k(const __bf16* __restric
     const uint32_t arow = (L == 0u ? R_ACT0 : R_ACT1 + 64u * (L - 1u)) + 16u * ti + n;
     const uint32_t brow = (L == 4u ? R_DZ5 : R_DZ1 + 64u * L) + 16u * to + n;
     const bool bvalid = L != 4u || n < 3u;
-    const __bf16* ap = train + (size_t)arow * Bc + chunk * NRC_CHUNK + 8u * g;
-    const __bf16* bp = train + (size_t)brow * Bc + chunk * NRC_CHUNK + 8u * g;
+    const __bf16* ap = train + (size_t)arow * Bc + (size_t)chunk * per * NRC_CHUNK + 8u * g;
+    const __bf16* bp = train + (size_t)brow * Bc + (size_t)chunk * per * NRC_CHUNK + 8u * g;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     bf16x8 zero;
 #pragma unroll
     for (uint32_t j = 0; j < 8u; j++) zero[j] = (__bf16)0.0f;
-    for (uint32_t k = 0; k < NRC_CHUNK; k += 32u) {
+    for (uint32_t k = 0; k < per * NRC_CHUNK; k += 32u) {
         const bf16x8 a = *(const bf16x8*)(ap + k);
         const bf16x8 bq = bvalid ? *(const bf16x8*)(bp + k) : zero;
         acc = mfma16(a, bq, acc);
@@ -425,8 +445,8 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
                                                   float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
                                                   __bf16* __restrict__ fwd, __bf16* __restrict__ bwd,
                                                   const float* __restrict__ loss, uint32_t n_loss,
-                                                  float* __restrict__ loss_out, float ema_d,
-                                                  float* __restrict__ ema, __bf16* __restrict__ efwd)
+                                                  float* __restrict__ loss_part, float* __restrict__ loss_out,
+                                                  float ema_d, float* __restrict__ ema, __bf16* __restrict__ efwd)
 {
     __shared__ float s[256];
     __shared__ uint32_t last;
@@ -463,6 +483,19 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
             efwd[f] = (__bf16)e;
         }
     }
+    /* the batch's loss: every block sums a strided slice in a fixed order,
+     * the last block adds the block sums in block order */
+    if (loss_out) {
+        float acc = 0.0f;
+        for (uint32_t k = i; k < n_loss; k += gridDim.x * blockDim.x) acc += loss[k];
+        s[threadIdx.x] = acc;
+        __syncthreads();
+        for (uint32_t w = 128u; w > 0u; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) loss_part[blockIdx.x] = s[0];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -470,16 +503,11 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
     }
     __syncthreads();
     if (!last) return;
-    if (loss_out) {
+    if (loss_out && threadIdx.x == 0) {
+        __threadfence();
         float acc = 0.0f;
-        for (uint32_t k = threadIdx.x; k < n_loss; k += 256u) acc += loss[k];
-        s[threadIdx.x] = acc;
-        __syncthreads();
-        for (uint32_t w = 128u; w > 0u; w >>= 1) {
-            if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) *loss_out += s[0];
+        for (uint32_t b = 0; b < gridDim.x; b++) acc += ((volatile const float*)loss_part)[b];
+        *loss_out += acc;
     }
     if (threadIdx.x == 0) {
         dev[NRC_DEV_STEP] = t_u;
@@ -665,14 +693,19 @@ int pack(tgx_context* ctx)
  * squared errors. */
 int train_step(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, const uint32_t* count_dev,
                uint32_t first, uint32_t Bc, uint32_t n_norm, const float* targets, const float* tvar,
-               nrc_cell* cells, float* loss_acc)
+               nrc_cell* cells, float* loss_acc, const nrc_segs* segs = nullptr)
 {
+    const nrc_segs sg = segs ? *segs : nrc_segs{Bc, 0u, 0u, 0u, 0u};
     const tgx_nrc_config& c = ctx->nrc_cfg;
-    const uint32_t n_chunks = Bc / NRC_CHUNK;
+    /* weight-gradient partials: one per group of `per` chunks, at most 16
+     * groups (a large batch is summed in fewer, longer MFMA runs) */
+    uint32_t per = 1;
+    while (Bc / NRC_CHUNK / per > 16u && (Bc / NRC_CHUNK) % (2u * per) == 0u) per *= 2u;
+    const uint32_t n_chunks = Bc / NRC_CHUNK / per;
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->nrc_train, 2ull * NRC_ROWS * Bc, "nrc training scratch")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->nrc_partials, 4ull * NRC_N * n_chunks, "nrc gradient partials")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_loss, 4ull * Bc, "nrc losses")))
+        (rc = tgx_buffer_reserve(ctx, &ctx->nrc_loss, 4ull * Bc + 4ull * NRC_ADAM_BLOCKS, "nrc losses")))
         return rc;
     uint32_t shift = 0;
     while ((1u << shift) < c.cell_size) shift++;
@@ -681,13 +714,13 @@ int train_step(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, 
         hipLaunchKernelGGL(nrc_fwdbwd_k, dim3(nrc_blocks(Bc / 16u, NRC_BLOCK / 64u)), dim3(NRC_BLOCK), 0, ctx->stream,
                            (const bf16x8*)ctx->nrc_fwd.ptr, (const bf16x8*)ctx->nrc_bwd.ptr, surf, list, count_dev,
                            first, targets, tvar, enc_of(c), 2.0f / (3.0f * (float)n_norm), Bc,
-                           (__bf16*)ctx->nrc_train.ptr, (float*)ctx->nrc_loss.ptr, cells, shift);
+                           (__bf16*)ctx->nrc_train.ptr, (float*)ctx->nrc_loss.ptr, cells, shift, sg);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_fwdbwd_k");
     {
         tgx_launch_scope s(ctx, "nrc_wgrad_k");
         hipLaunchKernelGGL(nrc_wgrad_k, dim3(nrc_blocks((uint64_t)NRC_WTILES * n_chunks, NRC_BLOCK / 64u)),
-                           dim3(NRC_BLOCK), 0, ctx->stream, (const __bf16*)ctx->nrc_train.ptr, Bc, n_chunks,
+                           dim3(NRC_BLOCK), 0, ctx->stream, (const __bf16*)ctx->nrc_train.ptr, Bc, n_chunks, per,
                            (float*)ctx->nrc_partials.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_wgrad_k");
@@ -700,7 +733,7 @@ int train_step(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list, 
                            (const float*)ctx->nrc_partials.ptr, n_chunks, a, dev, (float*)ctx->nrc_master.ptr,
                            (float*)ctx->nrc_m.ptr, (float*)ctx->nrc_v.ptr, (float*)ctx->nrc_grad.ptr,
                            (__bf16*)ctx->nrc_fwd.ptr, (__bf16*)ctx->nrc_bwd.ptr, (const float*)ctx->nrc_loss.ptr, Bc,
-                           loss_acc, c.ema_decay, c.ema_decay > 0.0f ? (float*)ctx->nrc_ema.ptr : nullptr,
+                           (float*)ctx->nrc_loss.ptr + Bc, loss_acc, c.ema_decay, c.ema_decay > 0.0f ? (float*)ctx->nrc_ema.ptr : nullptr,
                            (__bf16*)ctx->nrc_efwd.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "nrc_adam_k");
@@ -1082,6 +1115,9 @@ int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     const bool fallback = use_cells || all_fb;
     const float w_var = (c.flags & TGX_NRC_FLAG_FALLBACK_VS_TRACER)
                             ? c.fallback_threshold * (float)c.train_spp / (float)gi->spp : -1.0f;
+    /* joint training: every camera renders with the cache the previous frame
+     * left; one training pass over all cameras' targets follows */
+    const bool joint = (c.flags & TGX_NRC_FLAG_JOINT_TRAINING) && n_cams > 1;
     for (uint32_t k = 0; k < n_cams; k++) {
         float* rad = radiance_dev + 3ull * k * px;
         float* irr = irradiance_dev ? irradiance_dev + 3ull * k * px : nullptr;
@@ -1106,7 +1142,7 @@ int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams, uint32_t n
                                fb_off, vis_list, n_vis, fb_list, qc + 1);
             TGX_LAUNCH_CHECK(ctx, "nrc_fallback_list_k");
         }
-        if (n_train) {
+        if (n_train && !joint) {
             hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
             TGX_LAUNCH_CHECK(ctx, "nrc_decay_k");
             const uint32_t* tl = train_lists + (size_t)n_train * k;
@@ -1126,6 +1162,19 @@ int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams, uint32_t n
             TGX_LAUNCH_CHECK(ctx, "nrc_acc_k");
         }
     }
+    if (n_train && joint) {
+        /* one decay of the error estimate per frame, then train_samples /
+         * train_batch Adam steps, each on train_batch samples of every
+         * camera (camera-major within the batch) */
+        hipLaunchKernelGGL(nrc_decay_k, dim3(nrc_blocks(NRC_CELLS, 256)), dim3(256), 0, ctx->stream, cells);
+        TGX_LAUNCH_CHECK(ctx, "nrc_decay_k");
+        const nrc_segs sg = {c.train_batch, (uint32_t)px, n_train, 64u, n_train};
+        const uint32_t bj = c.train_batch * n_cams;
+        for (uint32_t first = 0; first < n_train; first += c.train_batch)
+            if ((rc = train_step(ctx, surf, train_lists, q, first, bj, bj, e_train, v_train, cells,
+                                 (float*)(q + 64ull * (n_cams - 1u) + 8), &sg)))
+                return rc;
+    }
     if (fallback) {
         std::vector<gi_trace_set> fs(n_cams);
         for (uint32_t k = 0; k < n_cams; k++)
@@ -1139,7 +1188,7 @@ int tgx_nrc_frames_from_vis(tgx_context* ctx, const tgx_camera* cams, uint32_t n
             TGX_LAUNCH_CHECK(ctx, "nrc_acc_k");
         }
     }
-    ctx->nrc_stats.train_samples = n_train;
+    ctx->nrc_stats.train_samples = joint ? n_train * n_cams : n_train;
     ctx->nrc_fb_traced = fallback;
     ctx->nrc_qc_off = 64u * (n_cams - 1u);
     return TGX_OK;

@@ -142,7 +142,7 @@ int tgx_destroy(tgx_context* ctx)
         (void)hipEventDestroy(p.stop);
     }
     tgx_buffer* bufs[] = {&ctx->objects, &ctx->cluster_pointers, &ctx->cluster_idx_to_object,
-                          &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters, &ctx->vis_solid, &ctx->vis_objvis,
+                          &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters, &ctx->vis_solid, &ctx->vis_objcnt,
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,
@@ -199,6 +199,7 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
         if (s->cluster_idx_to_object[c] >= s->n_objects)
             return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "cluster %u -> object %u >= %u", c,
                             s->cluster_idx_to_object[c], s->n_objects);
+    uint64_t covered = 0;
     for (uint32_t o = 0; o < s->n_objects; o++) {
         const tgx_object* ob = &s->objects[o];
         const uint64_t n = (uint64_t)ob->n_cluster_pointers_per_dim[0] * ob->n_cluster_pointers_per_dim[1] *
@@ -208,7 +209,17 @@ int tgx_scene_upload(tgx_context* ctx, const tgx_scene_desc* s)
         if (s->luts && ob->lut_idx >= s->n_luts)
             return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "object %u lut_idx %u >= %u", o, ob->lut_idx,
                             s->n_luts);
+        /* tg's model: an object's pointers map to clusters of that object */
+        for (uint64_t c = ob->first_cluster_pointer; c < ob->first_cluster_pointer + n; c++)
+            if (s->cluster_idx_to_object[s->cluster_pointers[c]] != o)
+                return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "cluster pointer %llu of object %u maps to object %u",
+                                (unsigned long long)c, o, s->cluster_idx_to_object[s->cluster_pointers[c]]);
+        covered += n;
     }
+    /* ... and the objects' ranges (disjoint by the check above) cover every pointer */
+    if (covered != s->n_cluster_pointers)
+        return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "objects cover %llu of %u cluster pointers",
+                        (unsigned long long)covered, s->n_cluster_pointers);
     int rc;
     if ((rc = tgx_scene_replace_begin(ctx, "tgx_scene_upload"))) return rc;
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -296,7 +307,8 @@ int tgx_tuning_set(tgx_context* ctx, const tgx_tuning* t)
 {
     if (!ctx || !t) return TGX_ERR_INVALID_ARG;
     if (t->trace_sort > 3u || t->trace_refill < 1u || t->trace_refill > 64u || t->trace_blocks_per_cu < 1u ||
-        t->trace_blocks_per_cu > 2u || t->trace_vote > 64u || t->trace_regroup > 8u || t->trace_skip > 1u)
+        t->trace_blocks_per_cu > 2u || t->trace_vote > 64u || t->trace_regroup > 8u || t->trace_skip > 1u ||
+        t->vis_setup_blocks < 64u || t->vis_setup_blocks > 16384u)
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "tuning out of range");
     ctx->tuning = *t;
     return TGX_OK;

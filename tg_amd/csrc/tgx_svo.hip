@@ -320,6 +320,14 @@ __device__ static inline void sample_center(const f3* bc, uint32_t x, uint32_t y
     cz = omtx * py0z + tx * py1z;
 }
 
+/* Leaf sampling, in two passes over the same leaves with the same
+ * arithmetic. kMats false: the leaf's bits, its cluster list and its solid
+ * voxel count. kMats true (after the counts are scanned into mat_off): the
+ * materials of its solid voxels only, in bit order, at mat_off[leaf] in the
+ * pool, and the per-word prefix counts that index them (wpre[w] = solid
+ * voxels in the leaf's words before w). A voxel's material is that of the
+ * first cluster in list order that samples it solid. */
+template <bool kMats>
 __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ nodes, uint32_t leaf_base,
                                                   const uint32_t* __restrict__ pair_cptr,
                                                   const uint32_t* __restrict__ leaf_first_pair, uint32_t n_pairs,
@@ -327,14 +335,20 @@ __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ 
                                                   const uint32_t* __restrict__ bits,
                                                   const uint8_t* __restrict__ materials, float mnx, float mny,
                                                   float mnz, uint32_t quirk, uint32_t* __restrict__ leaf_bits,
-                                                  uint16_t* __restrict__ leaf_mats, uint32_t* __restrict__ leaf_n,
+                                                  uint32_t* __restrict__ leaf_solid, const uint32_t* __restrict__ mat_off,
+                                                  uint16_t* __restrict__ wpre, uint16_t* __restrict__ pool,
+                                                  uint32_t* __restrict__ leaf_n,
                                                   uint32_t* __restrict__ list_tmp, svo_gnode* __restrict__ nodes_rw,
                                                   const uint32_t* __restrict__ reuse, const uint32_t* __restrict__ old_bits,
-                                                  const uint16_t* __restrict__ old_mats, const uint32_t* __restrict__ old_n,
+                                                  const uint32_t* __restrict__ old_off, const uint16_t* __restrict__ old_wpre,
+                                                  const uint16_t* __restrict__ old_pool, const uint32_t* __restrict__ old_n,
                                                   const uint32_t* __restrict__ old_list_off,
                                                   const uint32_t* __restrict__ old_list)
 {
     __shared__ uint32_t lbits[TGX_SVO_BLOCK_WORDS];
+    __shared__ uint32_t fbits[kMats ? TGX_SVO_BLOCK_WORDS : 1];
+    __shared__ uint16_t lpre[kMats ? TGX_SVO_BLOCK_WORDS : 1];
+    __shared__ uint32_t wsum[4];
     __shared__ int any_flag;
     __shared__ uint32_t n_contrib, last_cidx;
     const uint32_t leaf = blockIdx.x;
@@ -343,17 +357,23 @@ __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ 
         /* an update: same candidate clusters in the same order, none of
          * them new, so the previous build's leaf is this leaf */
         const uint32_t j = reuse[leaf];
-        const uint4* sb = (const uint4*)(old_bits + (size_t)j * TGX_SVO_BLOCK_WORDS);
-        uint4* db = (uint4*)(leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS);
-        for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS / 4u; w += blockDim.x) db[w] = sb[w];
-        const uint4* sm = (const uint4*)(old_mats + (size_t)j * TGX_SVO_BLOCK_VOXELS);
-        uint4* dm = (uint4*)(leaf_mats + (size_t)leaf * TGX_SVO_BLOCK_VOXELS);
-        for (uint32_t w = tid; w < TGX_SVO_BLOCK_VOXELS / 8u; w += blockDim.x) dm[w] = sm[w];
-        const uint32_t n = old_n[j], b = leaf_first_pair[leaf], o = old_list_off[j];
-        for (uint32_t i = tid; i < n; i += blockDim.x) list_tmp[b + i] = old_list[o + i];
-        if (tid == 0) {
-            leaf_n[leaf] = n;
-            nodes_rw[leaf_base + leaf].leaf_n = n;
+        if constexpr (!kMats) {
+            const uint4* sb = (const uint4*)(old_bits + (size_t)j * TGX_SVO_BLOCK_WORDS);
+            uint4* db = (uint4*)(leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS);
+            for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS / 4u; w += blockDim.x) db[w] = sb[w];
+            const uint32_t n = old_n[j], b = leaf_first_pair[leaf], o = old_list_off[j];
+            for (uint32_t i = tid; i < n; i += blockDim.x) list_tmp[b + i] = old_list[o + i];
+            if (tid == 0) {
+                leaf_n[leaf] = n;
+                nodes_rw[leaf_base + leaf].leaf_n = n;
+                leaf_solid[leaf] = old_off[j + 1] - old_off[j];
+            }
+        } else {
+            const uint2* sw = (const uint2*)(old_wpre + (size_t)j * TGX_SVO_BLOCK_WORDS);
+            uint2* dw = (uint2*)(wpre + (size_t)leaf * TGX_SVO_BLOCK_WORDS);
+            for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS / 4u; w += blockDim.x) dw[w] = sw[w];
+            const uint32_t o = old_off[j], n = old_off[j + 1] - o, d = mat_off[leaf];
+            for (uint32_t i = tid; i < n; i += blockDim.x) pool[d + i] = old_pool[o + i];
         }
         return;
     }
@@ -362,10 +382,35 @@ __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ 
     const float pmaxx = pminx + 32.0f, pmaxy = pminy + 32.0f, pmaxz = pminz + 32.0f;
     for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS; w += blockDim.x) lbits[w] = 0u;
     if (tid == 0) { n_contrib = 0u; last_cidx = SVO_NONE; }
-    uint16_t* mats = leaf_mats + (size_t)leaf * TGX_SVO_BLOCK_VOXELS;
-    /* materials of empty voxels are 0; the barrier below orders these
-     * stores before the sampled ones */
-    for (uint32_t w = tid; w < TGX_SVO_BLOCK_VOXELS / 8u; w += blockDim.x) ((uint4*)mats)[w] = make_uint4(0u, 0u, 0u, 0u);
+    uint16_t* mats = nullptr;
+    if constexpr (kMats) {
+        /* the leaf's final bits (first pass) and their per-word prefix */
+        const uint32_t* fb = leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS;
+        uint32_t c[4], tot = 0;
+        for (uint32_t i = 0; i < 4u; i++) {
+            const uint32_t w = 4u * tid + i;
+            fbits[w] = fb[w];
+            c[i] = (uint32_t)__popc(fbits[w]);
+            tot += c[i];
+        }
+        /* exclusive scan of the 256 threads' totals: wave scan, then waves */
+        const uint32_t lane = tid & 63u, wv = tid >> 6;
+        uint32_t incl = tot;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t u = __shfl_up(incl, d);
+            if (lane >= d) incl += u;
+        }
+        if (lane == 63u) wsum[wv] = incl;
+        __syncthreads();
+        uint32_t run = incl - tot;
+        for (uint32_t k = 0; k < wv; k++) run += wsum[k];
+        for (uint32_t i = 0; i < 4u; i++) {
+            lpre[4u * tid + i] = (uint16_t)run;
+            wpre[(size_t)leaf * TGX_SVO_BLOCK_WORDS + 4u * tid + i] = (uint16_t)run;
+            run += c[i];
+        }
+        mats = pool + mat_off[leaf];
+    }
     const uint32_t b = leaf_first_pair[leaf];
     const uint32_t e = leaf + 1 < n_leaves ? leaf_first_pair[leaf + 1] : n_pairs;
     __syncthreads();
@@ -418,23 +463,37 @@ __global__ __launch_bounds__(256) void svo_leaf_k(const svo_gnode* __restrict__ 
              * list order writes the material. */
             if (!(lbits[bv >> 5] & m)) {
                 atomicOr(&lbits[bv >> 5], m);
-                mats[bv] = (uint16_t)(f.lut | (materials ? (uint32_t)materials[cv] : 0u));
+                if constexpr (kMats)
+                    mats[lpre[bv >> 5] + (uint32_t)__popc(fbits[bv >> 5] & (m - 1u))] =
+                        (uint16_t)(f.lut | (materials ? (uint32_t)materials[cv] : 0u));
             }
         }
-        if (any) any_flag = 1;
-        __syncthreads();
-        if (tid == 0 && any_flag && last_cidx != f.cidx) {
-            list_tmp[b + n_contrib] = f.cidx;
-            n_contrib++;
-            last_cidx = f.cidx;
+        if constexpr (!kMats) {
+            if (any) any_flag = 1;
+            __syncthreads();
+            if (tid == 0 && any_flag && last_cidx != f.cidx) {
+                list_tmp[b + n_contrib] = f.cidx;
+                n_contrib++;
+                last_cidx = f.cidx;
+            }
         }
         __syncthreads();
     }
-    uint32_t* out = leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS;
-    for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS; w += blockDim.x) out[w] = lbits[w];
-    if (tid == 0) {
-        leaf_n[leaf] = n_contrib;
-        nodes_rw[leaf_base + leaf].leaf_n = n_contrib;
+    if constexpr (!kMats) {
+        uint32_t* out = leaf_bits + (size_t)leaf * TGX_SVO_BLOCK_WORDS;
+        uint32_t cnt = 0;
+        for (uint32_t w = tid; w < TGX_SVO_BLOCK_WORDS; w += blockDim.x) {
+            out[w] = lbits[w];
+            cnt += (uint32_t)__popc(lbits[w]);
+        }
+        for (uint32_t d = 32; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d);
+        if ((tid & 63u) == 0u) wsum[tid >> 6] = cnt;
+        __syncthreads();
+        if (tid == 0) {
+            leaf_n[leaf] = n_contrib;
+            nodes_rw[leaf_base + leaf].leaf_n = n_contrib;
+            leaf_solid[leaf] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        }
     }
 }
 
@@ -529,7 +588,8 @@ __global__ __launch_bounds__(64) void svo_bricks_k(const uint32_t* __restrict__ 
 
 void tgx_svo_release(tgx_context* ctx)
 {
-    tgx_buffer* bufs[] = {&ctx->svo_nodes, &ctx->svo_leaf_bits, &ctx->svo_leaf_mats, &ctx->svo_leaf_n,
+    tgx_buffer* bufs[] = {&ctx->svo_nodes, &ctx->svo_leaf_bits, &ctx->svo_leaf_mats, &ctx->svo_leaf_moff,
+                          &ctx->svo_leaf_wpre, &ctx->svo_old_moff, &ctx->svo_old_wpre, &ctx->svo_leaf_n,
                           &ctx->svo_list_off, &ctx->svo_list, &ctx->svo_frames, &ctx->svo_rays,
                           &ctx->svo_hits, &ctx->prim_tmp, &ctx->gi_counts, &ctx->gi_surf,
                           &ctx->gi_rays, &ctx->gi_res, &ctx->gi_shadow, &ctx->gi_shadow_res,
@@ -664,6 +724,8 @@ static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32
     if (update) {
         std::swap(ctx->svo_leaf_bits, ctx->svo_old_bits);
         std::swap(ctx->svo_leaf_mats, ctx->svo_old_mats);
+        std::swap(ctx->svo_leaf_moff, ctx->svo_old_moff);
+        std::swap(ctx->svo_leaf_wpre, ctx->svo_old_wpre);
         std::swap(ctx->svo_leaf_n, ctx->svo_old_n);
         std::swap(ctx->svo_list_off, ctx->svo_old_list_off);
         std::swap(ctx->svo_list, ctx->svo_old_list);
@@ -793,13 +855,16 @@ static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32
     /* Leaves. */
     const size_t nl = n_leaves ? n_leaves : 1;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_bits, 4ull * TGX_SVO_BLOCK_WORDS * nl, "svo leaf bits")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_mats, 2ull * TGX_SVO_BLOCK_VOXELS * nl, "svo leaf materials")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_moff, 4ull * (nl + 1), "svo material offsets")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_wpre, 2ull * TGX_SVO_BLOCK_WORDS * nl, "svo material prefixes")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_n, 4ull * nl, "svo leaf n")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->svo_list_off, 4ull * (nl + 1), "svo list offsets")))
         return rc;
     const uint32_t n_leaf_pairs = lvl_n[leaf_level];
-    dbuf list_tmp;
-    if ((rc = dalloc(ctx, list_tmp, 4ull * (n_leaf_pairs ? n_leaf_pairs : 1), "svo list"))) return rc;
+    dbuf list_tmp, solid;
+    if ((rc = dalloc(ctx, list_tmp, 4ull * (n_leaf_pairs ? n_leaf_pairs : 1), "svo list")) ||
+        (rc = dalloc(ctx, solid, 4ull * nl, "svo solid counts")))
+        return rc;
     dbuf reuse;
     if (update && n_leaves) {
         if ((rc = dalloc(ctx, reuse, 4ull * n_leaves + 4, "svo reuse"))) return rc;
@@ -813,17 +878,45 @@ static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32
         TGX_LAUNCH_CHECK(ctx, "svo_match_k");
         if ((rc = read_u32(ctx, n_reused, &ctx->svo_reused))) return rc;
     }
+    /* Leaf sampling in two passes: bits, lists and solid counts; then,
+     * with the counts scanned into material offsets, the materials of the
+     * solid voxels only. */
+    const uint32_t quirk = (flags & TGX_SVO_FLAG_REFERENCE_CORNER_QUIRK) ? 1u : 0u;
+    const uint32_t* rz = update ? reuse.as<uint32_t>() : nullptr;
     if (n_leaves) {
         tgx_launch_scope s(ctx, "svo_leaf_k");
-        hipLaunchKernelGGL(svo_leaf_k, dim3(n_leaves), dim3(256), 0, ctx->stream, nodes, lvl_base[leaf_level],
+        hipLaunchKernelGGL(svo_leaf_k<false>, dim3(n_leaves), dim3(256), 0, ctx->stream, nodes, lvl_base[leaf_level],
                            lvl_scptr[leaf_level].as<uint32_t>(), lvl_first[leaf_level].as<uint32_t>(), n_leaf_pairs,
                            n_leaves, frames, (const uint32_t*)ctx->bits.ptr, tgx_scene_materials(ctx),
-                           mn[0], mn[1], mn[2], (flags & TGX_SVO_FLAG_REFERENCE_CORNER_QUIRK) ? 1u : 0u,
-                           (uint32_t*)ctx->svo_leaf_bits.ptr, (uint16_t*)ctx->svo_leaf_mats.ptr,
-                           (uint32_t*)ctx->svo_leaf_n.ptr, list_tmp.as<uint32_t>(), nodes,
-                           update ? reuse.as<uint32_t>() : nullptr, (const uint32_t*)ctx->svo_old_bits.ptr,
-                           (const uint16_t*)ctx->svo_old_mats.ptr, (const uint32_t*)ctx->svo_old_n.ptr,
-                           (const uint32_t*)ctx->svo_old_list_off.ptr, (const uint32_t*)ctx->svo_old_list.ptr);
+                           mn[0], mn[1], mn[2], quirk, (uint32_t*)ctx->svo_leaf_bits.ptr, solid.as<uint32_t>(),
+                           (const uint32_t*)nullptr, (uint16_t*)nullptr, (uint16_t*)nullptr,
+                           (uint32_t*)ctx->svo_leaf_n.ptr, list_tmp.as<uint32_t>(), nodes, rz,
+                           (const uint32_t*)ctx->svo_old_bits.ptr, (const uint32_t*)ctx->svo_old_moff.ptr,
+                           (const uint16_t*)ctx->svo_old_wpre.ptr, (const uint16_t*)ctx->svo_old_mats.ptr,
+                           (const uint32_t*)ctx->svo_old_n.ptr, (const uint32_t*)ctx->svo_old_list_off.ptr,
+                           (const uint32_t*)ctx->svo_old_list.ptr);
+    }
+    TGX_LAUNCH_CHECK(ctx, "svo_leaf_k");
+    if ((rc = tgx_prim_exclusive_scan_u32(ctx, solid.as<uint32_t>(), (uint32_t*)ctx->svo_leaf_moff.ptr, n_leaves))) return rc;
+    uint32_t n_solid = 0;
+    if ((rc = scan_total(ctx, solid.as<uint32_t>(), (const uint32_t*)ctx->svo_leaf_moff.ptr, n_leaves, &n_solid))) return rc;
+    *(uint32_t*)((char*)ctx->host_pinned + 336) = n_solid;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync((uint32_t*)ctx->svo_leaf_moff.ptr + n_leaves, (char*)ctx->host_pinned + 336, 4,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_mats, 2ull * (n_solid ? n_solid : 1), "svo leaf materials"))) return rc;
+    ctx->svo_n_solid = n_solid;
+    if (n_leaves) {
+        tgx_launch_scope s(ctx, "svo_leaf_mats_k");
+        hipLaunchKernelGGL(svo_leaf_k<true>, dim3(n_leaves), dim3(256), 0, ctx->stream, nodes, lvl_base[leaf_level],
+                           lvl_scptr[leaf_level].as<uint32_t>(), lvl_first[leaf_level].as<uint32_t>(), n_leaf_pairs,
+                           n_leaves, frames, (const uint32_t*)ctx->bits.ptr, tgx_scene_materials(ctx),
+                           mn[0], mn[1], mn[2], quirk, (uint32_t*)ctx->svo_leaf_bits.ptr, (uint32_t*)nullptr,
+                           (const uint32_t*)ctx->svo_leaf_moff.ptr, (uint16_t*)ctx->svo_leaf_wpre.ptr,
+                           (uint16_t*)ctx->svo_leaf_mats.ptr, (uint32_t*)nullptr, (uint32_t*)nullptr, nodes, rz,
+                           (const uint32_t*)ctx->svo_old_bits.ptr, (const uint32_t*)ctx->svo_old_moff.ptr,
+                           (const uint16_t*)ctx->svo_old_wpre.ptr, (const uint16_t*)ctx->svo_old_mats.ptr,
+                           (const uint32_t*)ctx->svo_old_n.ptr, (const uint32_t*)ctx->svo_old_list_off.ptr,
+                           (const uint32_t*)ctx->svo_old_list.ptr);
     }
     TGX_LAUNCH_CHECK(ctx, "svo_leaf_k");
     if ((rc = tgx_prim_exclusive_scan_u32(ctx, (const uint32_t*)ctx->svo_leaf_n.ptr, (uint32_t*)ctx->svo_list_off.ptr, n_leaves))) return rc;
@@ -889,6 +982,21 @@ static int svo_build(tgx_context* ctx, const float mn[3], uint32_t depth, uint32
     ctx->svo_info.flags = flags;
     ctx->svo_built = true;
     ctx->svo_stale = false;
+    return TGX_OK;
+}
+
+int tgx_svo_device_bytes(tgx_context* ctx, uint64_t* bytes, uint64_t* solid_voxels)
+{
+    if (!ctx || !bytes) return TGX_ERR_INVALID_ARG;
+    if (!ctx->svo_built) return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "no SVO built");
+    const tgx_svo_info& in = ctx->svo_info;
+    const uint64_t nl = in.n_leaves;
+    *bytes = (uint64_t)sizeof(svo_gnode) * in.n_nodes                  /* node records           */
+             + (4ull * TGX_SVO_BLOCK_WORDS + 2ull * TGX_SVO_BLOCK_WORDS) * nl /* bits, word prefixes */
+             + 4ull * (nl + 1) + 2ull * ctx->svo_n_solid               /* material offsets, pool */
+             + 4ull * nl + 4ull * (nl + 1) + 4ull * in.n_list_entries  /* cluster lists          */
+             + 8ull * ctx->svo_n_inner + 8ull * nl;                    /* traversal layout       */
+    if (solid_voxels) *solid_voxels = ctx->svo_n_solid;
     return TGX_OK;
 }
 
@@ -1004,16 +1112,13 @@ int tgx_svo_save(tgx_context* ctx, const char* path)
     TGX_HIP_CHECK(ctx, hipMemcpy(off.data(), ctx->svo_list_off.ptr, 4ull * (nl + 1u), hipMemcpyDeviceToHost));
     if (!list.empty())
         TGX_HIP_CHECK(ctx, hipMemcpy(list.data(), ctx->svo_list.ptr, 4ull * list.size(), hipMemcpyDeviceToHost));
-    /* solid voxel count first: the header carries it */
+    /* the leaves' materials are stored as in the file: solid voxels only,
+     * in bit order, from the leaf's material offset */
+    std::vector<uint32_t> moff(nl + 1u);
+    TGX_HIP_CHECK(ctx, hipMemcpy(moff.data(), ctx->svo_leaf_moff.ptr, 4ull * (nl + 1u), hipMemcpyDeviceToHost));
     std::vector<uint32_t> bits((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS);
-    std::vector<uint16_t> mats((size_t)k_svo_chunk * TGX_SVO_BLOCK_VOXELS), packed;
-    uint64_t n_solid = 0;
-    for (uint32_t l0 = 0; l0 < nl; l0 += k_svo_chunk) {
-        const uint32_t m = std::min(k_svo_chunk, nl - l0);
-        TGX_HIP_CHECK(ctx, hipMemcpy(bits.data(), (const uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
-                                     4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyDeviceToHost));
-        for (size_t w = 0; w < (size_t)m * TGX_SVO_BLOCK_WORDS; w++) n_solid += popc32(bits[w]);
-    }
+    std::vector<uint16_t> mats;
+    const uint64_t n_solid = moff[nl];
     svo_stream s;
     s.f = fopen(path, "wb");
     if (!s.f) return tgx_fail(ctx, TGX_ERR_IO, "cannot write %s", path);
@@ -1034,16 +1139,13 @@ int tgx_svo_save(tgx_context* ctx, const char* path)
         const uint32_t m = std::min(k_svo_chunk, nl - l0);
         TGX_HIP_CHECK(ctx, hipMemcpy(bits.data(), (const uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
                                      4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyDeviceToHost));
-        TGX_HIP_CHECK(ctx, hipMemcpy(mats.data(), (const uint16_t*)ctx->svo_leaf_mats.ptr + (size_t)l0 * TGX_SVO_BLOCK_VOXELS,
-                                     2ull * TGX_SVO_BLOCK_VOXELS * m, hipMemcpyDeviceToHost));
+        mats.resize((size_t)moff[l0 + m] - moff[l0]);
+        if (!mats.empty())
+            TGX_HIP_CHECK(ctx, hipMemcpy(mats.data(), (const uint16_t*)ctx->svo_leaf_mats.ptr + moff[l0],
+                                         2ull * mats.size(), hipMemcpyDeviceToHost));
         for (uint32_t k = 0; k < m; k++) {
-            const uint32_t* b = &bits[(size_t)k * TGX_SVO_BLOCK_WORDS];
-            const uint16_t* v = &mats[(size_t)k * TGX_SVO_BLOCK_VOXELS];
-            packed.clear();
-            for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++)
-                for (uint32_t word = b[w]; word; word &= word - 1u) packed.push_back(v[32u * w + (uint32_t)__builtin_ctz(word)]);
-            s.put(b, 4ull * TGX_SVO_BLOCK_WORDS);
-            s.put(packed.data(), 2ull * packed.size());
+            s.put(&bits[(size_t)k * TGX_SVO_BLOCK_WORDS], 4ull * TGX_SVO_BLOCK_WORDS);
+            s.put(mats.data() + (moff[l0 + k] - moff[l0]), 2ull * (moff[l0 + k + 1] - moff[l0 + k]));
         }
     }
     const uint64_t h = s.h;
@@ -1107,41 +1209,45 @@ int tgx_svo_load(tgx_context* ctx, const char* path)
     /* Leaf data in two passes over the file: the first checks every leaf
      * and the checksum with nothing on the device touched, the second
      * uploads (and hashes again, against a file changed in between). */
-    std::vector<uint32_t> bits((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS);
-    std::vector<uint16_t> mats((size_t)k_svo_chunk * TGX_SVO_BLOCK_VOXELS), packed;
+    std::vector<uint32_t> bits((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS), moff(nl + 1u);
+    std::vector<uint16_t> mats, wpre((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS);
     const long leaf_data_at = ftell(s.f);
     const uint64_t h_leaf = s.h;
     auto leaf_pass = [&](bool upload) -> int {
         uint64_t n_solid = 0;
         for (uint32_t l0 = 0; l0 < nl; l0 += k_svo_chunk) {
             const uint32_t m = std::min(k_svo_chunk, nl - l0);
-            if (upload) std::fill(mats.begin(), mats.end(), (uint16_t)0);
+            const uint64_t chunk_first = n_solid;
+            mats.clear();
             for (uint32_t k = 0; k < m && !s.bad; k++) {
                 uint32_t* b = &bits[(size_t)k * TGX_SVO_BLOCK_WORDS];
-                uint16_t* v = &mats[(size_t)k * TGX_SVO_BLOCK_VOXELS];
                 s.get(b, 4ull * TGX_SVO_BLOCK_WORDS);
                 uint32_t cnt = 0;
-                for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++) cnt += popc32(b[w]);
+                for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++) {
+                    wpre[(size_t)k * TGX_SVO_BLOCK_WORDS + w] = (uint16_t)cnt;
+                    cnt += popc32(b[w]);
+                }
+                moff[l0 + k] = (uint32_t)n_solid;
                 n_solid += cnt;
                 if (s.bad || n_solid > hd.n_solid) { s.bad = true; break; }
-                packed.resize(cnt);
-                s.get(packed.data(), 2ull * cnt);
-                uint32_t j = 0;
-                for (uint32_t w = 0; w < TGX_SVO_BLOCK_WORDS; w++)
-                    for (uint32_t word = b[w]; word; word &= word - 1u) {
-                        const uint16_t mv = packed[j++];
-                        if (ctx->n_luts && (uint32_t)(mv >> 8) >= ctx->n_luts) s.bad = true;
-                        v[32u * w + (uint32_t)__builtin_ctz(word)] = mv;
-                    }
+                const size_t at = mats.size();
+                mats.resize(at + cnt);
+                s.get(mats.data() + at, 2ull * cnt);
+                for (uint32_t j = 0; j < cnt && !s.bad; j++)
+                    if (ctx->n_luts && (uint32_t)(mats[at + j] >> 8) >= ctx->n_luts) s.bad = true;
             }
             if (s.bad) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: truncated or bad leaf data", path);
             if (!upload) continue;
             TGX_HIP_CHECK(ctx, hipMemcpy((uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
                                          bits.data(), 4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyHostToDevice));
-            TGX_HIP_CHECK(ctx, hipMemcpy((uint16_t*)ctx->svo_leaf_mats.ptr + (size_t)l0 * TGX_SVO_BLOCK_VOXELS,
-                                         mats.data(), 2ull * TGX_SVO_BLOCK_VOXELS * m, hipMemcpyHostToDevice));
+            TGX_HIP_CHECK(ctx, hipMemcpy((uint16_t*)ctx->svo_leaf_wpre.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
+                                         wpre.data(), 2ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyHostToDevice));
+            if (!mats.empty())
+                TGX_HIP_CHECK(ctx, hipMemcpy((uint16_t*)ctx->svo_leaf_mats.ptr + chunk_first, mats.data(),
+                                             2ull * mats.size(), hipMemcpyHostToDevice));
         }
         if (n_solid != hd.n_solid) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s: solid voxel count mismatch", path);
+        moff[nl] = (uint32_t)n_solid;
         return TGX_OK;
     };
     int rc;
@@ -1160,7 +1266,9 @@ int tgx_svo_load(tgx_context* ctx, const char* path)
     const size_t nlr = nl ? nl : 1;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->svo_nodes, sizeof(svo_gnode) * g.size(), "svo nodes")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_bits, 4ull * TGX_SVO_BLOCK_WORDS * nlr, "svo leaf bits")) ||
-        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_mats, 2ull * TGX_SVO_BLOCK_VOXELS * nlr, "svo leaf materials")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_mats, 2ull * (hd.n_solid ? hd.n_solid : 1), "svo leaf materials")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_moff, 4ull * (nlr + 1), "svo material offsets")) ||
+        (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_wpre, 2ull * TGX_SVO_BLOCK_WORDS * nlr, "svo material prefixes")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->svo_leaf_n, 4ull * nlr, "svo leaf n")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->svo_list_off, 4ull * (nlr + 1), "svo list offsets")) ||
         (rc = tgx_buffer_reserve(ctx, &ctx->svo_list, 4ull * (list.empty() ? 1 : list.size()), "svo list")) ||
@@ -1171,6 +1279,8 @@ int tgx_svo_load(tgx_context* ctx, const char* path)
     s.h = h_leaf;
     if ((rc = leaf_pass(true))) return rc;
     if (s.h != h) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "%s changed while loading", path);
+    TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_leaf_moff.ptr, moff.data(), 4ull * (nl + 1u), hipMemcpyHostToDevice));
+    ctx->svo_n_solid = hd.n_solid;
     TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_nodes.ptr, g.data(), sizeof(svo_gnode) * g.size(), hipMemcpyHostToDevice));
     if (nl) TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_leaf_n.ptr, leaf_n.data(), 4ull * nl, hipMemcpyHostToDevice));
     TGX_HIP_CHECK(ctx, hipMemcpy(ctx->svo_list_off.ptr, off.data(), 4ull * (nl + 1u), hipMemcpyHostToDevice));
@@ -1213,8 +1323,28 @@ int tgx_svo_export(tgx_context* ctx, tgx_svo_node* nodes_out, uint32_t* leaf_bit
     }
     if (leaf_bits && in.n_leaves)
         TGX_HIP_CHECK(ctx, hipMemcpy(leaf_bits, ctx->svo_leaf_bits.ptr, 4ull * TGX_SVO_BLOCK_WORDS * in.n_leaves, hipMemcpyDeviceToHost));
-    if (leaf_materials && in.n_leaves)
-        TGX_HIP_CHECK(ctx, hipMemcpy(leaf_materials, ctx->svo_leaf_mats.ptr, 2ull * TGX_SVO_BLOCK_VOXELS * in.n_leaves, hipMemcpyDeviceToHost));
+    if (leaf_materials && in.n_leaves) {
+        /* the dense canonical grid from the solid voxels' materials */
+        const uint32_t nl = in.n_leaves;
+        std::vector<uint32_t> moff(nl + 1u), bw((size_t)k_svo_chunk * TGX_SVO_BLOCK_WORDS);
+        std::vector<uint16_t> pm;
+        TGX_HIP_CHECK(ctx, hipMemcpy(moff.data(), ctx->svo_leaf_moff.ptr, 4ull * (nl + 1u), hipMemcpyDeviceToHost));
+        for (uint32_t l0 = 0; l0 < nl; l0 += k_svo_chunk) {
+            const uint32_t m = std::min(k_svo_chunk, nl - l0);
+            TGX_HIP_CHECK(ctx, hipMemcpy(bw.data(), (const uint32_t*)ctx->svo_leaf_bits.ptr + (size_t)l0 * TGX_SVO_BLOCK_WORDS,
+                                         4ull * TGX_SVO_BLOCK_WORDS * m, hipMemcpyDeviceToHost));
+            pm.resize((size_t)moff[l0 + m] - moff[l0]);
+            if (!pm.empty())
+                TGX_HIP_CHECK(ctx, hipMemcpy(pm.data(), (const uint16_t*)ctx->svo_leaf_mats.ptr + moff[l0],
+                                             2ull * pm.size(), hipMemcpyDeviceToHost));
+            uint16_t* out = leaf_materials + (size_t)l0 * TGX_SVO_BLOCK_VOXELS;
+            memset(out, 0, 2ull * TGX_SVO_BLOCK_VOXELS * m);
+            size_t j = 0;
+            for (size_t w = 0; w < (size_t)m * TGX_SVO_BLOCK_WORDS; w++)
+                for (uint32_t word = bw[w]; word; word &= word - 1u)
+                    out[32u * w + (uint32_t)__builtin_ctz(word)] = pm[j++];
+        }
+    }
     if (leaf_n && in.n_leaves)
         TGX_HIP_CHECK(ctx, hipMemcpy(leaf_n, ctx->svo_leaf_n.ptr, 4ull * in.n_leaves, hipMemcpyDeviceToHost));
     if (list_offsets)

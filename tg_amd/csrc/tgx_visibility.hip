@@ -244,8 +244,9 @@ __device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, ui
     return true;
 }
 
-/* Object-level cull, one byte per (camera, object): 0 only when no
- * cluster of the object can get a rectangle from cluster_rect. The
+/* Object-level cull, one count per (camera, object): the object's cluster
+ * pointers when kept, 0 when no cluster of the object can get a rectangle
+ * from cluster_rect or a word with depth <= 1. The
  * object's box, grown by one voxel, goes through cluster_rect's transform
  * into the camera's corner-ray basis w (fx = w1 / w0, fy = w2 / w0). A
  * half-space test that holds at all eight corners holds on the whole box,
@@ -254,18 +255,34 @@ __device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, ui
  * 3 pixels (the rectangle's padding is 1.5). A box that reaches the cube
  * around the camera, where cluster_rect accepts clusters as full-screen, is
  * kept. The one-voxel growth covers the clusters' own growth and the
- * rounding of both computations. */
+ * rounding of both computations. An object whose bounding sphere (radius
+ * of the grown box) lies beyond the far plane, by a margin of 0.1 % and one
+ * voxel for the rounding of the pair's slab distance, holds no hit with
+ * depth <= 1 and is culled for every camera. */
 __global__ __launch_bounds__(256) void vis_objcull_k(const tgx_object* __restrict__ objects, uint32_t n_objects,
                                                      const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
-                                                     uint8_t* __restrict__ objvis)
+                                                     uint32_t* __restrict__ objcnt)
 {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ci = blockIdx.y;
     if (o >= n_objects) return;
     const vis_cam& cam = cams[ci];
-    uint8_t keep = 1u;
-    if (!cam.full_screen) {
-        const tgx_object ob = objects[o];
+    const tgx_object ob = objects[o];
+    const uint32_t n = ob.n_cluster_pointers_per_dim[0] * ob.n_cluster_pointers_per_dim[1] *
+                       ob.n_cluster_pointers_per_dim[2];
+    uint8_t keep = n ? 1u : 0u;
+    {
+        const float ex = 4.0f * (float)ob.n_cluster_pointers_per_dim[0] + 1.0f;
+        const float ey = 4.0f * (float)ob.n_cluster_pointers_per_dim[1] + 1.0f;
+        const float ez = 4.0f * (float)ob.n_cluster_pointers_per_dim[2] + 1.0f;
+        const float tcx = ob.translation[0] - cam.pos[0];
+        const float tcy = ob.translation[1] - cam.pos[1];
+        const float tcz = ob.translation[2] - cam.pos[2];
+        const float dist = sqrtf(tcx * tcx + tcy * tcy + tcz * tcz);
+        const float rad = sqrtf(ex * ex + ey * ey + ez * ez);
+        if (dist - rad > cam.far_plane * 1.001f + 1.0f) keep = 0u;
+    }
+    if (keep && !cam.full_screen) {
         const float ex = 4.0f * (float)ob.n_cluster_pointers_per_dim[0] + 1.0f;
         const float ey = 4.0f * (float)ob.n_cluster_pointers_per_dim[1] + 1.0f;
         const float ez = 4.0f * (float)ob.n_cluster_pointers_per_dim[2] + 1.0f;
@@ -300,21 +317,55 @@ __global__ __launch_bounds__(256) void vis_objcull_k(const tgx_object* __restric
         const bool behind = hi[0] < VIS_RECT_EPS - 1.0f;
         if (!near_cube && (behind || left || right || below || above)) keep = 0u;
     }
-    objvis[(size_t)ci * n_objects + o] = keep;
+    objcnt[(size_t)ci * n_objects + o] = keep ? n : 0u;
+}
+
+/* Per camera (one workgroup each): the object counts become exclusive
+ * offsets in place, and total[ci] the number of cluster pointers of kept
+ * objects. */
+__global__ __launch_bounds__(1024) void vis_objscan_k(uint32_t* __restrict__ objcnt, uint32_t n_objects,
+                                                      uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t* cnt = objcnt + (size_t)blockIdx.x * n_objects;
+    if (t == 0) carry = 0u;
+    __syncthreads();
+    for (uint32_t base = 0; base < n_objects; base += 1024u) {
+        const uint32_t i = base + t;
+        const uint32_t v = i < n_objects ? cnt[i] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t u = __shfl_up(incl, d);
+            if (lane >= d) incl += u;
+        }
+        if (lane == 63u) wsum[w] = incl;
+        __syncthreads();
+        uint32_t wofs = 0u, blk = 0u;
+        for (uint32_t k = 0; k < 16u; k++) {
+            wofs += k < w ? wsum[k] : 0u;
+            blk += wsum[k];
+        }
+        const uint32_t c0 = carry;
+        if (i < n_objects) cnt[i] = c0 + wofs + incl - v;
+        __syncthreads();
+        if (t == 0) carry = c0 + blk;
+        __syncthreads();
+    }
+    if (t == 0) total[blockIdx.x] = carry;
 }
 
 /* Builds the record of one (camera, cluster pointer); false if the cluster
  * is empty, its object is culled, or it is out of every pixel ray's reach
  * or beyond the far plane. */
 __device__ static bool setup_one(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
-                                 const uint32_t* __restrict__ c2o, const uint8_t* __restrict__ solid,
-                                 const uint8_t* __restrict__ objvis, uint32_t cptr, const vis_cam& cam, uint32_t W,
-                                 uint32_t H, vis_cluster& rec)
+                                 const uint8_t* __restrict__ solid, uint32_t oi, uint32_t cptr, const vis_cam& cam,
+                                 uint32_t W, uint32_t H, vis_cluster& rec)
 {
     const uint32_t cidx = cptrs[cptr];
     if (!solid[cidx]) return false; /* no solid voxel: the walk can never hit */
-    const uint32_t oi = c2o[cidx];
-    if (!objvis[oi]) return false;  /* no cluster of the object reaches a pixel */
 
     const tgx_object ob = objects[oi];
     const uint32_t rel = cptr - ob.first_cluster_pointer;
@@ -386,12 +437,14 @@ __global__ __launch_bounds__(256) void vis_solid_k(const uint4* __restrict__ bit
     solid[c] = any ? 1u : 0u;
 }
 
+/* One lane per cluster pointer of the camera's kept objects (the offsets
+ * of vis_objscan_k), grid-stride: the lane's object is the last one whose
+ * offset is not above the lane's index. */
 __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
                                                  const uint32_t* __restrict__ cptrs,
-                                                 const uint32_t* __restrict__ c2o,
                                                  const uint8_t* __restrict__ solid,
-                                                 const uint8_t* __restrict__ objvis, uint32_t n_objects,
-                                                 uint32_t n_cptr,
+                                                 const uint32_t* __restrict__ objoff, uint32_t n_objects,
+                                                 const uint32_t* __restrict__ kept_total,
                                                  const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                  uint32_t tiles_x, vis_cluster* __restrict__ recs,
                                                  uint32_t* __restrict__ rec_flags,
@@ -400,13 +453,25 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
                                                  uint64_t* __restrict__ global_keys,
                                                  uint32_t* __restrict__ tile_count)
 {
-    const uint32_t cptr = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ci = blockIdx.y;
     const vis_cam& cam = cams[ci];
+    const uint32_t total = kept_total[ci];
+    const uint32_t* off = objoff + (size_t)ci * n_objects;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    /* wave-uniform loop: the rectangle walk below shares tiles across lanes */
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
+    const uint32_t t = base + (threadIdx.x & 63u);
     vis_cluster rec;
     bool has = false;
-    if (cptr < n_cptr)
-        has = setup_one(objects, cptrs, c2o, solid, objvis + (size_t)ci * n_objects, cptr, cam, W, H, rec);
+    if (t < total) {
+        uint32_t lo = 0, hi = n_objects - 1u;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (off[mid] <= t) lo = mid; else hi = mid - 1u;
+        }
+        const uint32_t cptr = objects[lo].first_cluster_pointer + (t - off[lo]);
+        has = setup_one(objects, cptrs, solid, lo, cptr, cam, W, H, rec);
+    }
     bool binned = false;
     if (has) {
         const uint32_t k = atomicAdd(&rec_count[ci], 1u);
@@ -427,6 +492,7 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
     const uint32_t tb = cam.tile_base;
     for_each_rect_tile(binned, binned ? rec.rect_lo : 0u, binned ? rec.rect_hi : 0u, 0ull,
                        [&](uint64_t, uint32_t tx, uint32_t ty) { atomicAdd(&tile_count[tb + ty * tiles_x + tx], 1u); });
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -512,12 +578,12 @@ __global__ __launch_bounds__(256) void vis_scatter(const vis_cluster* __restrict
                                                    uint32_t* __restrict__ cursor,
                                                    uint64_t* __restrict__ list, uint32_t capacity)
 {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t k = gid / VIS_SCATTER_LANES, sub = gid % VIS_SCATTER_LANES;
     const uint32_t ci = blockIdx.y;
-    if (k >= rec_count[ci]) return;
+    const uint32_t n_work = rec_count[ci] * VIS_SCATTER_LANES;
+    for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n_work; gid += gridDim.x * blockDim.x) {
+    const uint32_t k = gid / VIS_SCATTER_LANES, sub = gid % VIS_SCATTER_LANES;
     const uint32_t slot = cams[ci].rec_base + k;
-    if (rec_flags[slot] == 0u) return;
+    if (rec_flags[slot] == 0u) continue;
     const uint32_t lo = recs[slot].rect_lo, hi = recs[slot].rect_hi;
     const uint64_t key = ((uint64_t)recs[slot].depth_lb24 << 32) | slot;
     const uint32_t tx0 = (lo & 0xFFFFu) / VIS_TILE, ty0 = (lo >> 16) / VIS_TILE;
@@ -527,6 +593,7 @@ __global__ __launch_bounds__(256) void vis_scatter(const vis_cluster* __restrict
     for (uint32_t j = sub; j < nt; j += VIS_SCATTER_LANES) {
         const uint32_t pos = atomicAdd(&cursor[tb + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx], 1u);
         if (pos < capacity) list[pos] = key;
+    }
     }
 }
 
@@ -692,11 +759,11 @@ int tgx_vis_solid_refresh(tgx_context* ctx)
 struct vis_bufs {
     const vis_cam*      cams;
     const uint8_t*      solid;      /* u8 per cluster index                */
-    uint8_t*            objvis;     /* u8 per (camera, object)             */
+    uint32_t*           objcnt;     /* u32 per (camera, object): kept counts, then offsets */
     vis_cluster*        recs;       /* per (camera, cluster pointer)       */
     uint32_t*           rec_flags;
     uint64_t*           gkeys;
-    uint32_t*           counters;   /* 2 n_cams + 8                        */
+    uint32_t*           counters;   /* 3 n_cams + 8                        */
     uint32_t*           tile_count;
     uint32_t*           tile_off;
     uint64_t*           list;
@@ -713,29 +780,33 @@ static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cam
     const uint32_t tiles_per_cam = tiles_x * tiles_y;
     const uint32_t n_tiles = tiles_per_cam * n_cams;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
-    /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts */
+    /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts,
+     * [2 n_cams + 8, 3 n_cams + 8) cluster pointers of kept objects */
     uint32_t* d_counters = b.counters;
     uint32_t* d_global_count = d_counters + n_cams + 8;
+    uint32_t* d_kept = d_counters + 2 * n_cams + 8;
     {
         tgx_launch_scope s(ctx, "vis_fill_u32", b.timed);
         hipLaunchKernelGGL(vis_fill_u32, dim3((n_tiles + 255) / 256), dim3(256), 0, ctx->stream, b.tile_count, n_tiles, 0u);
-        hipLaunchKernelGGL(vis_fill_u32, dim3((2 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters,
-                           2 * n_cams + 8, 0u);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((3 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters,
+                           3 * n_cams + 8, 0u);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
     {
         tgx_launch_scope s(ctx, "vis_objcull_k", b.timed);
         hipLaunchKernelGGL(vis_objcull_k, dim3((ctx->n_objects + 255u) / 256u, n_cams), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, b.cams, W, H, b.objvis);
+                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, b.cams, W, H, b.objcnt);
+        hipLaunchKernelGGL(vis_objscan_k, dim3(n_cams), dim3(1024), 0, ctx->stream, b.objcnt, ctx->n_objects, d_kept);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_objcull_k");
     {
         tgx_launch_scope s(ctx, "vis_setup", b.timed);
-        hipLaunchKernelGGL(vis_setup, dim3((n_cptr + 255) / 256, n_cams), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr,
-                           (const uint32_t*)ctx->cluster_idx_to_object.ptr, b.solid, (const uint8_t*)b.objvis,
-                           ctx->n_objects, n_cptr, b.cams, W, H, tiles_x, b.recs, b.rec_flags, d_counters,
-                           d_global_count, b.gkeys, b.tile_count);
+        const uint32_t cap = ctx->tuning.vis_setup_blocks;
+        const uint32_t blocks = (n_cptr + 255) / 256 < cap ? (n_cptr + 255) / 256 : cap;
+        hipLaunchKernelGGL(vis_setup, dim3(blocks, n_cams), dim3(256), 0, ctx->stream,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr, b.solid,
+                           (const uint32_t*)b.objcnt, ctx->n_objects, (const uint32_t*)d_kept, b.cams, W, H, tiles_x,
+                           b.recs, b.rec_flags, d_counters, d_global_count, b.gkeys, b.tile_count);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_setup");
     {
@@ -747,7 +818,10 @@ static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cam
     {
         tgx_launch_scope s(ctx, "vis_scatter", b.timed);
         const uint64_t threads = (uint64_t)n_cptr * VIS_SCATTER_LANES;
-        hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)((threads + 255) / 256), n_cams), dim3(256), 0, ctx->stream,
+        const uint64_t want = (threads + 255) / 256;
+        const uint64_t cap = ctx->tuning.vis_setup_blocks;
+        hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)(want < cap ? want : cap), n_cams),
+                           dim3(256), 0, ctx->stream,
                            b.recs, b.rec_flags, d_counters, b.cams, tiles_x, b.tile_count, b.list, b.capacity);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_scatter");
@@ -812,10 +886,10 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_clusters, sizeof(vis_cluster) * n_slots, "vis records"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_rec_flags, 4ull * n_slots, "vis record flags"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_global_keys, 8ull * n_slots, "vis global keys"))) return rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (2 * n_cams + 8), "vis counters"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (3 * n_cams + 8), "vis counters"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_count, 4ull * n_tiles, "tile counts"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_objvis, (size_t)ctx->n_objects * n_cams, "object cull flags")))
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_objcnt, 4ull * ctx->n_objects * n_cams, "object cull counts")))
         return rc;
     if (!resident)
         TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->vis_cams.ptr, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice,
@@ -823,7 +897,7 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     vis_bufs b;
     b.cams = resident ? resident : (const vis_cam*)ctx->vis_cams.ptr;
     b.solid = (const uint8_t*)ctx->vis_solid.ptr;
-    b.objvis = (uint8_t*)ctx->vis_objvis.ptr;
+    b.objcnt = (uint32_t*)ctx->vis_objcnt.ptr;
     b.recs = (vis_cluster*)ctx->vis_clusters.ptr;
     b.rec_flags = (uint32_t*)ctx->vis_rec_flags.ptr;
     b.gkeys = (uint64_t*)ctx->vis_global_keys.ptr;
@@ -1049,8 +1123,8 @@ static size_t noalloc_fixed_bytes(const tgx_context* ctx, uint32_t n_cams, uint3
 {
     const uint64_t slots = (uint64_t)ctx->n_cluster_pointers * n_cams;
     return align256(sizeof(vis_cam) * n_cams) + align256(ctx->n_clusters) +
-           align256((size_t)ctx->n_objects * n_cams) + align256(sizeof(vis_cluster) * slots) + align256(4 * slots) +
-           align256(8 * slots) + align256(4ull * (2 * n_cams + 8)) + 2 * align256(4ull * tiles_per_cam * n_cams) +
+           align256(4ull * ctx->n_objects * n_cams) + align256(sizeof(vis_cluster) * slots) + align256(4 * slots) +
+           align256(8 * slots) + align256(4ull * (3 * n_cams + 8)) + 2 * align256(4ull * tiles_per_cam * n_cams) +
            align256(64);
 }
 
@@ -1109,11 +1183,11 @@ int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint
         uint8_t* d_solid = (uint8_t*)carve(ctx->n_clusters);
         b.cams = d_cams;
         b.solid = d_solid;
-        b.objvis = (uint8_t*)carve((size_t)ctx->n_objects * n);
+        b.objcnt = (uint32_t*)carve(4ull * ctx->n_objects * n);
         b.recs = (vis_cluster*)carve(sizeof(vis_cluster) * slots);
         b.rec_flags = (uint32_t*)carve(4 * slots);
         b.gkeys = (uint64_t*)carve(8 * slots);
-        b.counters = (uint32_t*)carve(4ull * (2 * n + 8));
+        b.counters = (uint32_t*)carve(4ull * (3 * n + 8));
         b.tile_count = (uint32_t*)carve(4ull * tiles_per_cam * n);
         b.tile_off = (uint32_t*)carve(4ull * tiles_per_cam * n);
         b.stats = (unsigned long long*)carve(64);
