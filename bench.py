@@ -188,11 +188,12 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     plain = irr_frame(7)
     prm = T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=0)
 
-    def run(flags: int, threshold: float, min_samples: int, ema: float, milestones=(), frames: int = pretrain):
+    def run(flags: int, threshold: float, min_samples: int, ema: float, milestones=(), frames: int = pretrain,
+            batch: int = 1 << 12):
         """a fresh cache trained for `frames` untimed frames (errors read at
         the milestones), then `steps` timed frames"""
         ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16,
-                                    train_batch=1 << 12, train_spp=4, fallback_threshold=threshold,
+                                    train_batch=batch, train_spp=4, fallback_threshold=threshold,
                                     fallback_min_samples=min_samples, cell_size=8, flags=flags, ema_decay=ema))
         log(f"neural: flags {flags} threshold {threshold} ema {ema}: {frames} training frames")
         curve = []
@@ -224,6 +225,11 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     ms_nf, _, _, irr_nf, _ = run(T.NRC_FLAG_NO_FALLBACK, 1.0, 1, NRC_EMA_DECAY, frames=side)
     ms_ne, _, st_ne, irr_ne, _ = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD, NRC_FALLBACK_MIN_SAMPLES,
                                      0.0, frames=side)
+    # the bench's form with 4 Adam steps of 16384 samples per frame instead of
+    # 16 of 4096, at the side forms' frame count (its error against the
+    # bench form's convergence entry at the same count)
+    ms_b16, _, st_b16, irr_b16, _ = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD,
+                                        NRC_FALLBACK_MIN_SAMPLES, NRC_EMA_DECAY, frames=side, batch=1 << 14)
     marks = sorted({m for m in (200, 400, 800, 1600, 3200, 6400, 12800) if m < pretrain} | {pretrain})
     ms, prof, st, irr, curve = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD, NRC_FALLBACK_MIN_SAMPLES,
                                    NRC_EMA_DECAY, marks)
@@ -294,6 +300,8 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "no_fallback": {"frame_ms": ms_nf, **err(irr_nf)},
         "no_weight_average": {"frame_ms": ms_ne, "fallback_pixels": int(st_ne["fallback_pixels"]), **err(irr_ne)},
         "side_forms_frames": side,
+        "train_batch_16384": {"frame_ms": ms_b16, "fallback_pixels": int(st_b16["fallback_pixels"]),
+                              "adam_steps_per_frame": (1 << 16) // (1 << 14), **err(irr_b16)},
         "moving_camera": moving,
         "train_samples": int(st["train_samples"]), "adam_steps_total": int(st["adam_steps"]),
         "train_loss": float(st["train_loss"]), "pretrain_frames": pretrain,

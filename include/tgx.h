@@ -553,6 +553,9 @@ typedef struct tgx_tuning {
                                      voxel (0) */
     uint32_t vis_setup_blocks;    /* visibility: workgroups per camera of the grid-stride cull
                                      and tile-list scatter passes, 64..16384 (2048) */
+    uint32_t trace_dda;           /* lane-machine launch: 1 runs a wave's leaf DDA without the
+                                     iteration count (two steps per exit test) when no DDA lane's
+                                     direction has a component below 2^-100 (0) */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);

@@ -338,6 +338,7 @@ int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint
         {&tgx_tuning::trace_regroup, 4u, {0u, 1u, 2u, 4u}},          /* trace launch form, node steps per round */
         {&tgx_tuning::trace_skip, 2u, {0u, 1u}},                      /* exact empty-brick skip (regrouped form) */
         {&tgx_tuning::trace_blocks_per_cu, 2u, {1u, 2u}},             /* lane-machine workgroups per CU */
+        {&tgx_tuning::trace_dda, 2u, {0u, 1u}},                       /* lane-machine DDA without iteration count */
         {&tgx_tuning::trace_refill, 5u, {4u, 8u, 16u, 24u, 32u}},     /* idle lanes that trigger a queue refill */
         {&tgx_tuning::vis_setup_blocks, 4u, {256u, 1024u, 2048u, 4096u}}, /* cull / scatter workgroups per camera */
     };
@@ -364,7 +365,9 @@ int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint
         /* the skip exists in the regrouped form only, the workgroups per CU
          * in the lane-machine form only */
         if (kb.field == &tgx_tuning::trace_skip && best.trace_regroup == 0u) continue;
-        if (kb.field == &tgx_tuning::trace_blocks_per_cu && best.trace_regroup != 0u) continue;
+        if ((kb.field == &tgx_tuning::trace_blocks_per_cu || kb.field == &tgx_tuning::trace_dda) &&
+            best.trace_regroup != 0u)
+            continue;
         const uint32_t cur = best.*kb.field;
         for (uint32_t i = 0; i < kb.n && !rc; i++) {
             if (kb.values[i] == cur) continue;

@@ -347,7 +347,7 @@ __device__ static inline void brick_skip(int& x, int& y, int& z, float& tmx, flo
 /* One step of a lane: one iteration of the reference's node loop, or up to
  * DDA_BURST iterations of the leaf DDA. Returns true when the ray is done;
  * res is then its result. */
-template <bool kLds, bool kSkip = false>
+template <bool kLds, bool kSkip = false, bool kFast = false>
 __device__ static inline bool lane_step(const trace_params& P, const uint2* s_inner, float half, float maxd,
                                         trace_lane& L, svo_res& res, bool active, bool allow_node, bool allow_dda,
                                         bool defer_leave = false)
@@ -436,6 +436,45 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         /* brick skips need every moving axis' t_delta below 2^100 */
         const uint32_t skip_ok = (!kSkip || ((sx == 0 || tdx < 0x1p100f) && (sy == 0 || tdy < 0x1p100f) &&
                                              (sz == 0 || tdz < 0x1p100f))) ? 1u : 0u;
+        /* With every moving axis' t_delta below 2^100 no t_max overflows,
+         * so each step moves a coordinate and the walk leaves the 32^3
+         * block within 94 steps: the iteration cap cannot bind, and a wave
+         * whose DDA lanes all have such directions runs the walk without
+         * counting iterations, two per exit test. A lane's count is then
+         * left behind, which only lanes that never take this path read. */
+        const bool small_td = (sx == 0 || tdx < 0x1p100f) && (sy == 0 || tdy < 0x1p100f) &&
+                              (sz == 0 || tdz < 0x1p100f);
+        if (kFast && !kSkip && __all(small_td || !in_dda)) {
+#pragma unroll 1
+            for (uint32_t k = 0; k < DDA_BURST; k += 2) {
+#pragma unroll
+                for (uint32_t u = 0; u < 2u; u++) {
+                    const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
+                    const uint32_t inb = live & (uint32_t)(bm >> b);
+                    const uint32_t wi = 32u * (uint32_t)z + (uint32_t)y;
+                    if ((inb & 1u) && wi != wi_last) { wi_last = wi; wv = grid[wi]; }
+                    const uint32_t h = inb & (wv >> (uint32_t)x) & 1u;
+                    const float m = fminf(fminf(tmx, tmy), tmz);
+                    const uint32_t sz_ = tmz == m ? 1u : 0u;
+                    const uint32_t sy_ = (tmy == m ? 1u : 0u) & (sz_ ^ 1u);
+                    const uint32_t sx_ = (sz_ | sy_) ^ 1u;
+                    const uint32_t move = live & (h ^ 1u);
+                    const uint32_t mx = move & sx_, my = move & sy_, mz = move & sz_;
+                    tmx = mx ? tmx + tdx : tmx;
+                    tmy = my ? tmy + tdy : tmy;
+                    tmz = mz ? tmz + tdz : tmz;
+                    x = mx ? x + sx : x;
+                    y = my ? y + sy : y;
+                    z = mz ? z + sz : z;
+                    const uint32_t outside = (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
+                    const uint32_t lv = move & outside;
+                    hit_i |= h;
+                    leave_i |= lv;
+                    live &= (h | lv) ^ 1u;
+                }
+                if (!__any(live != 0u)) break;
+            }
+        } else
 #pragma unroll 1
         for (uint32_t k = 0; k < DDA_BURST; k++) {
             const uint32_t cap = it >> 8;
@@ -597,7 +636,7 @@ struct trace_fuse {
  * (flag bit 0) are added to *hit_counter when it is not null. Fused GI
  * (kFused true): see trace_fuse; out and hit_counter are not used. `cursor`
  * must be 0 at launch. */
-template <bool kLds, bool kFused>
+template <bool kLds, bool kFused, bool kFast>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
@@ -671,7 +710,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
          * 1 bounce hit, 2 shadow ray started, 4 shadow hit; << 3 for the
          * second list */
         uint32_t ev = 0u;
-        if (lane_step<kLds>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda) && active) {
+        if (lane_step<kLds, false, kFast>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda) && active) {
             active = false;
             if constexpr (!kFused) {
                 out[slot] = res;
@@ -1405,8 +1444,11 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
         const uint32_t stash = fused ? 16u * TRACE_BLOCK : 0u;
         const bool lds = (uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX;
         const uint32_t shmem = (lds ? 8u * ((P.n_inner + P.n_leaves + 1u) & ~1u) : 0u) + stash;
-        auto kernel = lds ? (fused ? gi_trace_k<true, true> : gi_trace_k<true, false>)
-                          : (fused ? gi_trace_k<false, true> : gi_trace_k<false, false>);
+        const bool fast = ctx->tuning.trace_dda != 0u;
+        auto kernel = lds ? (fused ? (fast ? gi_trace_k<true, true, true> : gi_trace_k<true, true, false>)
+                                   : (fast ? gi_trace_k<true, false, true> : gi_trace_k<true, false, false>))
+                          : (fused ? (fast ? gi_trace_k<false, true, true> : gi_trace_k<false, true, false>)
+                                   : (fast ? gi_trace_k<false, false, true> : gi_trace_k<false, false, false>));
         hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor, out,
                            hit_counter, refill, vote, F);
     }

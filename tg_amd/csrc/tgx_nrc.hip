@@ -503,11 +503,16 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
     }
     __syncthreads();
     if (!last) return;
-    if (loss_out && threadIdx.x == 0) {
+    if (loss_out) {
+        /* the block sums (gridDim.x <= 256) in a fixed tree */
         __threadfence();
-        float acc = 0.0f;
-        for (uint32_t b = 0; b < gridDim.x; b++) acc += ((volatile const float*)loss_part)[b];
-        *loss_out += acc;
+        s[threadIdx.x] = threadIdx.x < gridDim.x ? ((volatile const float*)loss_part)[threadIdx.x] : 0.0f;
+        __syncthreads();
+        for (uint32_t w = 128u; w > 0u; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *loss_out += s[0];
     }
     if (threadIdx.x == 0) {
         dev[NRC_DEV_STEP] = t_u;
