@@ -555,7 +555,15 @@ typedef struct tgx_tuning {
                                      and tile-list scatter passes, 64..16384 (2048) */
     uint32_t trace_dda;           /* lane-machine launch: 1 runs a wave's leaf DDA without the
                                      iteration count (two steps per exit test) when no DDA lane's
-                                     direction has a component below 2^-100 (0) */
+                                     direction has a component below 2^-100 (1) */
+    uint32_t vis_tile;            /* visibility: tile edge in pixels, 8, 16 or 32 (16); one
+                                     workgroup of vis_tile^2 lanes walks each tile's list */
+    uint32_t trace_cross;         /* uncounted DDA: 0 walks every voxel; k in 1..64 makes a
+                                     wave cross 8^3 bricks without solid voxels in one exact
+                                     step once >= k/64 of its walking lanes are in one (40) */
+    uint32_t trace_node;          /* lane-machine launch: node-loop iterations a descending lane
+                                     may take per step before its DDA burst, 1..4 (3) */
+    uint32_t trace_burst;         /* uncounted DDA: iterations per burst, even, 2..32 (8) */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);
@@ -606,10 +614,15 @@ int tgx_frame_graph_stats(tgx_context* ctx, tgx_frame_graph* g, tgx_frame_stats*
 int tgx_frame_graph_destroy(tgx_context* ctx, tgx_frame_graph* g);
 
 /* Times the frame (uncaptured, `frames` runs per candidate after one
- * untimed run) over trace workgroups per CU {1, 2} x refill thresholds
- * {4, 8, 16, 24, 32}, keeps the fastest in the context's tuning and
- * returns it in best_out (may be null). Every candidate gives the same
- * results; cache frames advance training once per run. */
+ * untimed run) by coordinate descent over the tuning knobs: trace launch
+ * form {0, 1, 2, 4}, empty-brick skip, trace workgroups per CU {1, 2},
+ * uncounted DDA, node iterations per step {1, 2, 3, 4}, empty-brick
+ * crossing {0, 24, 32, 40, 48}, refill thresholds {4, 8, 16, 24, 32}, visibility
+ * workgroups per camera {256, 1024, 2048, 4096} and visibility tile edge
+ * {8, 16, 32}. Keeps the fastest in the context's tuning and returns it in
+ * best_out (may be null). Every candidate gives the same results; cache
+ * frames advance training once per run. While a frame graph exists,
+ * candidates that would need larger buffers are skipped. */
 int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
                  const tgx_gi_params* gi, uint32_t flags, uint64_t* vis_dev, float* radiance_dev,
                  float* irradiance_dev, uint32_t frames, tgx_tuning* best_out);

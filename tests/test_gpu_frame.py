@@ -218,16 +218,19 @@ def test_autotune_keeps_results(ctx, batch_scene):
         g.launch()
         before = buf.read()
         g.destroy()
-        say("autotune (coordinate descent over trace form, skip, workgroups, refill, cull blocks), 2 cameras")
+        say("autotune (coordinate descent over trace form, skip, workgroups, DDA, refill, cull blocks, "
+            "tile edge), 2 cameras")
         best = ctx.autotune(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr, frames=1)
         assert tuple(ctx.tuning()[0]) == tuple(best[0])
         assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 8, 16, 24, 32)
         assert best["trace_regroup"][0] in (0, 1, 2, 4) and best["trace_skip"][0] in (0, 1)
-        assert best["vis_setup_blocks"][0] in (256, 1024, 2048, 4096)
+        assert best["vis_setup_blocks"][0] in (256, 1024, 2048, 4096) and best["vis_tile"][0] in (8, 16, 32)
+        assert best["trace_dda"][0] in (0, 1) and best["trace_node"][0] in (1, 2, 3, 4)
+        assert best["trace_cross"][0] in (0, 24, 32, 40, 48)
         assert best["trace_sort"][0] == 0 and best["trace_vote"][0] == 0
         # every form the tuner tries gives the same frame
-        for form in ((1, 0), (2, 1), (0, 0)):
-            ctx.set_tuning(trace_regroup=form[0], trace_skip=form[1], vis_setup_blocks=256)
+        for form in ((1, 0, 8), (2, 1, 32), (0, 0, 16)):
+            ctx.set_tuning(trace_regroup=form[0], trace_skip=form[1], vis_setup_blocks=256, vis_tile=form[2])
             g = ctx.frame_graph(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr)
             g.launch()
             got = buf.read()
@@ -243,7 +246,8 @@ def test_autotune_keeps_results(ctx, batch_scene):
             assert a.tobytes() == b.tobytes()
     finally:
         ctx.set_tuning(trace_sort=0, trace_refill=16, trace_blocks_per_cu=2, trace_vote=0, trace_regroup=0,
-                       trace_skip=0, vis_setup_blocks=2048)
+                       trace_skip=0, vis_setup_blocks=2048, trace_dda=1, vis_tile=16, trace_cross=40,
+                       trace_node=3)
         buf.free()
 
 

@@ -347,3 +347,45 @@ def test_svo_depth12_wide_world_bit_exact(ctx):
     rr, ri, _ = ref.gi(cam, 480, 270, vis, prm)
     assert (vis != T.VISIBILITY_EMPTY).mean() > 0.02
     assert rad.tobytes() == rr.tobytes() and irr.tobytes() == ri.tobytes()
+
+
+def test_svo_trace_dda_forms_bit_exact(ctx):
+    """The leaf DDA's forms (counted; uncounted; uncounted with exact
+    crossings of 8^3 bricks without solid voxels at several wave thresholds)
+    and 1 to 4 node-loop iterations per lane step give the oracle's hits on
+    rays chosen to stress the crossing: unit
+    directions, directions with equal-magnitude components (ties between
+    axes), components of 1e-6, 1e-30 and 0, and directions scaled by 1e-3
+    to 1e3."""
+    p = T.noise_objects_params(seed=1)
+    sc = T.generate_scene(p)
+    ctx.upload(sc)
+    cam = T.generate_camera(p, 0, 1920, 1080)[0]
+    mn = np.floor(cam["camera"][:3]) - 1024.0
+    ctx.svo_build(mn, 11)
+    ref_svo = O.OracleSvo(O.OracleScene(sc), mn, 11)
+    rng = np.random.default_rng(5)
+    o, d = rays_from(rng, 24000, 0, 512, centre=cam["camera"][:3])
+    d = d.astype(np.float64)
+    k = np.arange(len(d))
+    sgn = np.where(rng.uniform(size=d.shape) < 0.5, -1.0, 1.0)
+    diag = k % 6 == 1
+    d[diag] = sgn[diag] * np.abs(d[diag, :1])                      # |dx| = |dy| = |dz|
+    pair = k % 6 == 2
+    d[pair, 1] = sgn[pair, 1] * np.abs(d[pair, 0])                 # |dx| = |dy|
+    tiny = k % 6 == 3
+    d[tiny, (k[tiny] // 6) % 3] *= np.array([1e-6, 1e-30, 0.0])[(k[tiny] // 18) % 3]
+    scaled = k % 6 == 4
+    d[scaled] *= 10.0 ** rng.uniform(-3, 3, size=(int(scaled.sum()), 1))
+    d = d.astype(np.float32)
+    ref = ref_svo.trace(o, d, 1024.0)
+    try:
+        for dda, cross, node, burst in ((0, 0, 1, 8), (1, 0, 1, 8), (1, 1, 2, 2), (1, 32, 3, 8), (1, 40, 4, 16),
+                                        (1, 64, 3, 32)):
+            ctx.set_tuning(trace_dda=dda, trace_cross=cross, trace_node=node, trace_burst=burst)
+            say(f"depth-11 trace, 24000 stress rays, trace_dda {dda} trace_cross {cross} trace_node {node} "
+                f"trace_burst {burst}")
+            assert_same_hits(ctx.svo_trace(o, d, 1024.0), ref, f"dda {dda} cross {cross} node {node}")
+    finally:
+        ctx.set_tuning(trace_dda=1, trace_cross=40, trace_node=3, trace_burst=8)
+    assert 0.2 < ref["hit"].mean() < 0.95

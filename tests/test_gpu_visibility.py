@@ -234,3 +234,51 @@ def test_object_cull_at_the_far_plane(ctx):
     assert hits.size > 50
     # some hits lie in the last percent of the depth range
     assert ((hits >> np.uint64(40)) > np.uint64(int(0.97 * 16777215))).any()
+
+
+@pytest.mark.parametrize("tile", [8, 32])
+def test_tile_edges_bit_exact(ctx, noise_scene, tile):
+    """Tiles of 8x8 and 32x32 pixels (one workgroup of 64 or 1024 lanes per
+    tile; the global-list threshold and the first tile-list capacity follow
+    the tile edge) give the oracle's words: 1080p, an odd resolution, a
+    3-camera batch, a forced tile-list overflow and the allocation-free
+    entry point."""
+    p, sc = noise_scene
+    ctx.upload(sc)
+    os_ = O.OracleScene(sc)
+    try:
+        ctx.set_tuning(vis_tile=tile)
+        for w, h, ci in ((1920, 1080, 0), (333, 197, 3)):
+            say(f"tile {tile}: {w}x{h} camera {ci}")
+            cam = T.generate_camera(p, ci, w, h)
+            ref, _ = O.visibility_culled(os_, cam, w, h)
+            assert_same(ctx.visibility(cam, w, h)[0], ref, f"tile {tile} {w}x{h}")
+            assert ctx.last_stats().tile_size == tile
+        cams = np.concatenate([T.generate_camera(p, i, 640, 360) for i in range(3)])
+        refs = [O.visibility_culled(os_, cams[i:i + 1], 640, 360)[0] for i in range(3)]
+        say(f"tile {tile}: 3-camera batch 640x360")
+        batch = ctx.visibility(cams, 640, 360)
+        for i in range(3):
+            assert_same(batch[i], refs[i], f"tile {tile} camera {i}")
+        try:
+            say(f"tile {tile}: 3-camera batch with tile-list capacity 500")
+            ctx.set_list_capacity(500)
+            batch = ctx.visibility(cams, 640, 360)
+            assert ctx.last_stats().tile_entries > 500
+            for i in range(3):
+                assert_same(batch[i], refs[i], f"tile {tile} overflow camera {i}")
+        finally:
+            ctx.set_list_capacity(0)
+        dev = ctx.alloc(3 * 640 * 360 * 8)
+        try:
+            say(f"tile {tile}: allocation-free entry point, 3 cameras 640x360")
+            ctx.visibility_device_noalloc(cams, 640, 360, dev)
+            ctx.synchronize()
+            out = np.empty((3, 360, 640), np.uint64)
+            ctx.d2h(out, dev)
+            for i in range(3):
+                assert_same(out[i], refs[i], f"tile {tile} no-alloc camera {i}")
+        finally:
+            ctx.free(dev)
+    finally:
+        ctx.set_tuning(vis_tile=16)

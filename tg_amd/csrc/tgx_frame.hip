@@ -339,8 +339,11 @@ int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint
         {&tgx_tuning::trace_skip, 2u, {0u, 1u}},                      /* exact empty-brick skip (regrouped form) */
         {&tgx_tuning::trace_blocks_per_cu, 2u, {1u, 2u}},             /* lane-machine workgroups per CU */
         {&tgx_tuning::trace_dda, 2u, {0u, 1u}},                       /* lane-machine DDA without iteration count */
+        {&tgx_tuning::trace_node, 4u, {1u, 2u, 3u, 4u}},              /* node iterations per lane step */
+        {&tgx_tuning::trace_cross, 5u, {0u, 24u, 32u, 40u, 48u}},     /* empty-brick crossing threshold */
         {&tgx_tuning::trace_refill, 5u, {4u, 8u, 16u, 24u, 32u}},     /* idle lanes that trigger a queue refill */
         {&tgx_tuning::vis_setup_blocks, 4u, {256u, 1024u, 2048u, 4096u}}, /* cull / scatter workgroups per camera */
+        {&tgx_tuning::vis_tile, 3u, {8u, 16u, 32u}},                  /* visibility tile edge */
     };
     best.trace_sort = 0u;
     best.trace_vote = 0u;
@@ -365,16 +368,27 @@ int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint
         /* the skip exists in the regrouped form only, the workgroups per CU
          * in the lane-machine form only */
         if (kb.field == &tgx_tuning::trace_skip && best.trace_regroup == 0u) continue;
-        if ((kb.field == &tgx_tuning::trace_blocks_per_cu || kb.field == &tgx_tuning::trace_dda) &&
+        if ((kb.field == &tgx_tuning::trace_blocks_per_cu || kb.field == &tgx_tuning::trace_dda ||
+             kb.field == &tgx_tuning::trace_node || kb.field == &tgx_tuning::trace_cross) &&
             best.trace_regroup != 0u)
             continue;
+        /* the crossing exists in the uncounted DDA only */
+        if (kb.field == &tgx_tuning::trace_cross && best.trace_dda == 0u) continue;
         const uint32_t cur = best.*kb.field;
         for (uint32_t i = 0; i < kb.n && !rc; i++) {
             if (kb.values[i] == cur) continue;
             tgx_tuning t = best;
             t.*kb.field = kb.values[i];
             double ms = 0.0;
-            if ((rc = time_candidate(t, ms))) break;
+            if ((rc = time_candidate(t, ms))) {
+                /* a graph holds the buffers and this candidate needs larger ones */
+                if (rc == TGX_ERR_CAPACITY && ctx->n_graphs > 0) {
+                    rc = TGX_OK;
+                    ctx->last_error.clear();
+                    continue;
+                }
+                break;
+            }
             if (ms < best_ms) { best_ms = ms; best = t; }
         }
     }

@@ -27,11 +27,14 @@
  *                    order are the reference's; only loads are skipped.
  *
  * gi_trace_k is persistent: each lane runs one ray as a state machine
- * whose steps are either one node step of the reference's loop or up to
- * DDA_BURST iterations of the leaf DDA, so lanes of a wave stay busy while
- * they are in different phases. Whenever `refill` (tuning) lanes of a wave
- * have finished, the wave takes that many new rays from the queue with one
- * atomic (ballot + lane rank).
+ * whose steps are up to `node_iters` node iterations of the reference's
+ * loop while the lane descends, then up to `burst` iterations of the leaf
+ * DDA, so lanes of a wave stay busy while they are in different phases
+ * (both tuning). In the uncounted DDA, once enough of a wave's walking
+ * lanes are in 8^3 bricks without solid voxels, those lanes cross their
+ * bricks in one exact step (brick_cross). Whenever `refill` (tuning) lanes
+ * of a wave have finished, the wave takes that many new rays from the
+ * queue with one atomic (ballot + lane rank).
  *
  * GI frame (estimator documented at tgx_gi_params in tgx.h):
  *   gi_gen_k     surface of each visible pixel, spp bounce rays at
@@ -344,71 +347,137 @@ __device__ static inline void brick_skip(int& x, int& y, int& z, float& tmx, flo
     moves = kx + ky + kz - outside;
 }
 
-/* One step of a lane: one iteration of the reference's node loop, or up to
- * DDA_BURST iterations of the leaf DDA. Returns true when the ray is done;
- * res is then its result. */
-template <bool kLds, bool kSkip = false, bool kFast = false>
+/* Crossing of an 8^3 brick without solid voxels for the uncounted DDA
+ * (every moving axis' |t_delta| below 2^100), cheaper than brick_skip:
+ * the exit axis comes from the approximate exit times
+ * A_a = t_max_a + (N_a - 1) t_delta_a (one fma each). The sequential sums
+ * V_a(N_a - 1) differ from them by at most 8 ulps, so when the runner-up
+ * is later by more than 2^-19 of its value (and at least 2^-100) the axis
+ * is certain; otherwise the lane does not cross (returns 0) and walks the
+ * brick voxel by voxel. The exit axis' event time V_e(N_e - 1) is then
+ * summed exactly, and each other axis b takes the steps whose events
+ * precede it: V_b(j) < V_e, or <= where b wins the tie (z over y over x),
+ * written as < the next float above V_e. */
+__device__ static inline uint32_t brick_cross(int& x, int& y, int& z, float& tmx, float& tmy, float& tmz,
+                                              float tdx, float tdy, float tdz, int sx, int sy, int sz)
+{
+    const int nx = sx > 0 ? 8 - (x & 7) : (x & 7) + 1;
+    const int ny = sy > 0 ? 8 - (y & 7) : (y & 7) + 1;
+    const int nz = sz > 0 ? 8 - (z & 7) : (z & 7) + 1;
+    const float ax = sx ? fmaf((float)(nx - 1), tdx, tmx) : SVO_INF;
+    const float ay = sy ? fmaf((float)(ny - 1), tdy, tmy) : SVO_INF;
+    const float az = sz ? fmaf((float)(nz - 1), tdz, tmz) : SVO_INF;
+    const uint32_t ez = az <= ay && az <= ax ? 1u : 0u;
+    const uint32_t ey = (ez ^ 1u) & (ay <= ax ? 1u : 0u);
+    const float ae = ez ? az : (ey ? ay : ax);
+    const float second = ez ? fminf(ax, ay) : (ey ? fminf(ax, az) : fminf(ay, az));
+    if (!(second == SVO_INF || (second >= 0x1p-100f && second - ae > second * 0x1p-19f))) return 0u;
+    const float tde = ez ? tdz : (ey ? tdy : tdx);
+    const int ne = ez ? nz : (ey ? ny : nx);
+    float te = ez ? tmz : (ey ? tmy : tmx);
+#pragma unroll
+    for (int j = 1; j < 8; j++) te = j < ne ? te + tde : te;
+    const float up = __uint_as_float(__float_as_uint(te) + 1u);
+    /* thresholds; -1 keeps the exit axis out of the count loop */
+    const float thx = (ez | ey) ? te : -1.0f;
+    const float thy = ez ? te : (ey ? -1.0f : up);
+    const float thz = ez ? -1.0f : up;
+    float ux = tmx, uy = tmy, uz = tmz;
+    int kx = 0, ky = 0, kz = 0;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+        const bool cx = ux < thx, cy = uy < thy, cz = uz < thz;
+        ux = cx ? ux + tdx : ux;
+        uy = cy ? uy + tdy : uy;
+        uz = cz ? uz + tdz : uz;
+        kx += cx ? 1 : 0;
+        ky += cy ? 1 : 0;
+        kz += cz ? 1 : 0;
+    }
+    const uint32_t ex = (ez | ey) ^ 1u;
+    tmx = ex ? te + tdx : ux;
+    tmy = ey ? te + tdy : uy;
+    tmz = ez ? te + tdz : uz;
+    x += sx * (ex ? nx : kx);
+    y += sy * (ey ? ny : ky);
+    z += sz * (ez ? nz : kz);
+    return 1u;
+}
+
+/* One step of a lane: up to node_iters iterations of the reference's node
+ * loop while the lane descends, then up to DDA_BURST iterations of the leaf
+ * DDA. Returns true when the ray is done; res is then its result. */
+template <bool kLds, bool kSkip = false, uint32_t kDda = 0u>
 __device__ static inline bool lane_step(const trace_params& P, const uint2* s_inner, float half, float maxd,
                                         trace_lane& L, svo_res& res, bool active, bool allow_node, bool allow_dda,
-                                        bool defer_leave = false)
+                                        bool defer_leave = false, uint32_t cross = 0u, uint32_t node_iters = 1u,
+                                        uint32_t burst = DDA_BURST)
 {
     bool adv = false, done = false;
     /* a lane with a pending advance waits for the wave's next advance */
     const bool pend = active && (L.node & ND_ADV);
     allow_node = allow_node && active && !pend;
     allow_dda = allow_dda && active && !pend;
-    if (!(L.node & ND_DDA) && allow_node && (L.cur == SVO_NONE || L.steps >= TGX_SVO_MAX_STEPS || L.dist > maxd)) {
-        res = res_miss(L.steps);
-        done = true;
-        allow_node = false;
-        allow_dda = false;
-    }
-    if (!(L.node & ND_DDA) && allow_node) {
-        /* one iteration of the reference's node loop */
-        L.steps++;
-        const uint2 nd = inner_at<kLds>(P.inner, s_inner, L.cur);
-        const uint32_t n = L.node;
-        const uint32_t lvl = ND_LVL(n);
-        const float ps = (float)(1u << (P.depth - lvl));
-        const float ce = ps * 0.5f;
-        const float pmx = -half + (float)ND_CX(n) * ps, pmy = -half + (float)ND_CY(n) * ps,
-                    pmz = -half + (float)ND_CZ(n) * ps;
-        const float hx = pmx + ce, hy = pmy + ce, hz = pmz + ce;
-        uint32_t rc = 0;
-        if (hx < L.px || (L.px == hx && L.dx > 0.0f)) rc |= 1u;
-        if (hy < L.py || (L.py == hy && L.dy > 0.0f)) rc |= 2u;
-        if (hz < L.pz || (L.pz == hz && L.dz > 0.0f)) rc |= 4u;
-        L.node = (n & ~(7u << 24)) | (rc << 24);
-        adv = true;
-        const uint32_t valid = nd.y >> 24;
-        if (valid & (1u << rc)) {
-            const uint32_t ci = nd.x + (uint32_t)__popc(valid & ((1u << rc) - 1u));
-            if (lvl + 1u < P.leaf_level) {
-                L.cur = ci;
-                L.node = (2u * ND_CX(n) + (rc & 1u)) | ((2u * ND_CY(n) + ((rc >> 1) & 1u)) << 7) |
-                         ((2u * ND_CZ(n) + (rc >> 2)) << 14) | ((lvl + 1u) << 21);
-                adv = false;
-            } else {
-                const uint32_t leaf = ci - P.n_inner;
-                const unsigned long long bm = bricks_at<kLds>(P.bricks, s_inner, P.n_inner, leaf);
-                if (bm != 0ull) {
-                    /* tg_amanatides_woo setup over the 32^3 block, entry voxel clamped */
-                    const float cmx = (rc & 1u) ? hx : pmx, cmy = (rc & 2u) ? hy : pmy, cmz = (rc & 4u) ? hz : pmz;
-                    const float gx = L.px - cmx, gy = L.py - cmy, gz = L.pz - cmz;
-                    const int x = (int)fmin2(fmax2(floorf(gx), 0.0f), 31.0f);
-                    const int y = (int)fmin2(fmax2(floorf(gy), 0.0f), 31.0f);
-                    const int z = (int)fmin2(fmax2(floorf(gz), 0.0f), 31.0f);
-                    L.tmx = dda_len((float)(x + 1) - gx, gx - (float)x, L.dx);
-                    L.tmy = dda_len((float)(y + 1) - gy, gy - (float)y, L.dy);
-                    L.tmz = dda_len((float)(z + 1) - gz, gz - (float)z, L.dz);
-                    L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10);
-                    L.bm = bm;
-                    L.leaf = leaf;
-                    L.wi = 0xFFFFFFFFu;
-                    L.node |= ND_DDA;
+    /* a lane that descended to an inner node may take the next node
+     * iteration in the same step (wave-uniform loop) */
+    bool go = allow_node && !(L.node & ND_DDA);
+#pragma unroll 1
+    for (uint32_t ni = 0; ni < node_iters; ni++) {
+        if (ni != 0u && !__any(go)) break;
+        if (go && (L.cur == SVO_NONE || L.steps >= TGX_SVO_MAX_STEPS || L.dist > maxd)) {
+            res = res_miss(L.steps);
+            done = true;
+            go = false;
+            allow_dda = false;
+        }
+        if (go) {
+            /* one iteration of the reference's node loop */
+            L.steps++;
+            const uint2 nd = inner_at<kLds>(P.inner, s_inner, L.cur);
+            const uint32_t n = L.node;
+            const uint32_t lvl = ND_LVL(n);
+            const float ps = (float)(1u << (P.depth - lvl));
+            const float ce = ps * 0.5f;
+            const float pmx = -half + (float)ND_CX(n) * ps, pmy = -half + (float)ND_CY(n) * ps,
+                        pmz = -half + (float)ND_CZ(n) * ps;
+            const float hx = pmx + ce, hy = pmy + ce, hz = pmz + ce;
+            uint32_t rc = 0;
+            if (hx < L.px || (L.px == hx && L.dx > 0.0f)) rc |= 1u;
+            if (hy < L.py || (L.py == hy && L.dy > 0.0f)) rc |= 2u;
+            if (hz < L.pz || (L.pz == hz && L.dz > 0.0f)) rc |= 4u;
+            L.node = (n & ~(7u << 24)) | (rc << 24);
+            adv = true;
+            const uint32_t valid = nd.y >> 24;
+            if (valid & (1u << rc)) {
+                const uint32_t ci = nd.x + (uint32_t)__popc(valid & ((1u << rc) - 1u));
+                if (lvl + 1u < P.leaf_level) {
+                    L.cur = ci;
+                    L.node = (2u * ND_CX(n) + (rc & 1u)) | ((2u * ND_CY(n) + ((rc >> 1) & 1u)) << 7) |
+                             ((2u * ND_CZ(n) + (rc >> 2)) << 14) | ((lvl + 1u) << 21);
                     adv = false;
+                } else {
+                    const uint32_t leaf = ci - P.n_inner;
+                    const unsigned long long bm = bricks_at<kLds>(P.bricks, s_inner, P.n_inner, leaf);
+                    if (bm != 0ull) {
+                        /* tg_amanatides_woo setup over the 32^3 block, entry voxel clamped */
+                        const float cmx = (rc & 1u) ? hx : pmx, cmy = (rc & 2u) ? hy : pmy, cmz = (rc & 4u) ? hz : pmz;
+                        const float gx = L.px - cmx, gy = L.py - cmy, gz = L.pz - cmz;
+                        const int x = (int)fmin2(fmax2(floorf(gx), 0.0f), 31.0f);
+                        const int y = (int)fmin2(fmax2(floorf(gy), 0.0f), 31.0f);
+                        const int z = (int)fmin2(fmax2(floorf(gz), 0.0f), 31.0f);
+                        L.tmx = dda_len((float)(x + 1) - gx, gx - (float)x, L.dx);
+                        L.tmy = dda_len((float)(y + 1) - gy, gy - (float)y, L.dy);
+                        L.tmz = dda_len((float)(z + 1) - gz, gz - (float)z, L.dz);
+                        L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10);
+                        L.bm = bm;
+                        L.leaf = leaf;
+                        L.wi = 0xFFFFFFFFu;
+                        L.node |= ND_DDA;
+                        adv = false;
+                    }
                 }
             }
+            go = !adv && !(L.node & ND_DDA);
         }
     }
     /* Up to DDA_BURST iterations of the leaf DDA. The loop is wave-uniform
@@ -444,9 +513,26 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
          * left behind, which only lanes that never take this path read. */
         const bool small_td = (sx == 0 || tdx < 0x1p100f) && (sy == 0 || tdy < 0x1p100f) &&
                               (sz == 0 || tdz < 0x1p100f);
-        if (kFast && !kSkip && __all(small_td || !in_dda)) {
+        if (kDda != 0u && !kSkip && __all(small_td || !in_dda)) {
 #pragma unroll 1
-            for (uint32_t k = 0; k < DDA_BURST; k += 2) {
+            for (uint32_t k = 0; k < burst; k += 2) {
+                if constexpr (kDda == 2u) {
+                    /* once at least cross/64 of the wave's walking lanes
+                     * are in bricks without solid voxels, those lanes cross
+                     * their bricks (brick_cross) before the two steps */
+                    const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
+                    const uint32_t emp = live & ((uint32_t)(bm >> b) ^ 1u) & 1u;
+                    const uint32_t n_emp = (uint32_t)__popcll(__ballot(emp != 0u));
+                    const uint32_t n_live = (uint32_t)__popcll(__ballot(live != 0u));
+                    if (64u * n_emp >= cross * n_live && n_emp != 0u) {
+                        if (emp && brick_cross(x, y, z, tmx, tmy, tmz, tdx, tdy, tdz, sx, sy, sz)) {
+                            const uint32_t outside =
+                                (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
+                            leave_i |= outside;
+                            live &= outside ^ 1u;
+                        }
+                    }
+                }
 #pragma unroll
                 for (uint32_t u = 0; u < 2u; u++) {
                     const uint32_t b = ((uint32_t)x >> 3) | (((uint32_t)y >> 3) << 2) | (((uint32_t)z >> 3) << 4);
@@ -636,12 +722,13 @@ struct trace_fuse {
  * (flag bit 0) are added to *hit_counter when it is not null. Fused GI
  * (kFused true): see trace_fuse; out and hit_counter are not used. `cursor`
  * must be 0 at launch. */
-template <bool kLds, bool kFused, bool kFast>
+template <bool kLds, bool kFused, uint32_t kDda>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
                                                           uint32_t* __restrict__ hit_counter, uint32_t refill,
-                                                          uint32_t vote, const trace_fuse F)
+                                                          uint32_t vote, uint32_t cross, uint32_t node_iters,
+                                                          uint32_t burst, const trace_fuse F)
 {
     extern __shared__ uint2 s_inner[];
     if constexpr (kLds) {
@@ -710,7 +797,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
          * 1 bounce hit, 2 shadow ray started, 4 shadow hit; << 3 for the
          * second list */
         uint32_t ev = 0u;
-        if (lane_step<kLds, false, kFast>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda) && active) {
+        if (lane_step<kLds, false, kDda>(P, s_inner, half, maxd, L, res, active, allow_node, allow_dda, false,
+                                         cross, node_iters, burst) &&
+            active) {
             active = false;
             if constexpr (!kFused) {
                 out[slot] = res;
@@ -1444,13 +1533,18 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
         const uint32_t stash = fused ? 16u * TRACE_BLOCK : 0u;
         const bool lds = (uint64_t)P.n_inner + P.n_leaves <= TRACE_LDS_NODES_MAX;
         const uint32_t shmem = (lds ? 8u * ((P.n_inner + P.n_leaves + 1u) & ~1u) : 0u) + stash;
-        const bool fast = ctx->tuning.trace_dda != 0u;
-        auto kernel = lds ? (fused ? (fast ? gi_trace_k<true, true, true> : gi_trace_k<true, true, false>)
-                                   : (fast ? gi_trace_k<true, false, true> : gi_trace_k<true, false, false>))
-                          : (fused ? (fast ? gi_trace_k<false, true, true> : gi_trace_k<false, true, false>)
-                                   : (fast ? gi_trace_k<false, false, true> : gi_trace_k<false, false, false>));
+        /* DDA form: 0 counted, 1 uncounted, 2 uncounted with brick crossing */
+        const uint32_t dda = ctx->tuning.trace_dda == 0u ? 0u : (ctx->tuning.trace_cross ? 2u : 1u);
+        decltype(&gi_trace_k<true, true, 0u>) kernel;
+        if (lds)
+            kernel = fused ? (dda == 2u ? gi_trace_k<true, true, 2u> : dda ? gi_trace_k<true, true, 1u> : gi_trace_k<true, true, 0u>)
+                           : (dda == 2u ? gi_trace_k<true, false, 2u> : dda ? gi_trace_k<true, false, 1u> : gi_trace_k<true, false, 0u>);
+        else
+            kernel = fused ? (dda == 2u ? gi_trace_k<false, true, 2u> : dda ? gi_trace_k<false, true, 1u> : gi_trace_k<false, true, 0u>)
+                           : (dda == 2u ? gi_trace_k<false, false, 2u> : dda ? gi_trace_k<false, false, 1u> : gi_trace_k<false, false, 0u>);
         hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor, out,
-                           hit_counter, refill, vote, F);
+                           hit_counter, refill, vote, ctx->tuning.trace_cross, ctx->tuning.trace_node,
+                           ctx->tuning.trace_burst, F);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
     return TGX_OK;

@@ -308,7 +308,10 @@ int tgx_tuning_set(tgx_context* ctx, const tgx_tuning* t)
     if (!ctx || !t) return TGX_ERR_INVALID_ARG;
     if (t->trace_sort > 3u || t->trace_refill < 1u || t->trace_refill > 64u || t->trace_blocks_per_cu < 1u ||
         t->trace_blocks_per_cu > 2u || t->trace_vote > 64u || t->trace_regroup > 8u || t->trace_skip > 1u ||
-        t->vis_setup_blocks < 64u || t->vis_setup_blocks > 16384u || t->trace_dda > 1u)
+        t->vis_setup_blocks < 64u || t->vis_setup_blocks > 16384u || t->trace_dda > 1u ||
+        (t->vis_tile != 8u && t->vis_tile != 16u && t->vis_tile != 32u) || t->trace_cross > 64u ||
+        t->trace_node < 1u || t->trace_node > 4u || t->trace_burst < 2u || t->trace_burst > 32u ||
+        (t->trace_burst & 1u))
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "tuning out of range");
     ctx->tuning = *t;
     return TGX_OK;

@@ -34,8 +34,7 @@
 
 #include <mutex>
 
-#define VIS_TILE 16u
-#define VIS_GLOBAL_TILES 256u
+#define VIS_GLOBAL_PIXELS 65536u
 #define VIS_F32_MAX 3.402823466e+38f
 
 /* Conservative-rectangle constants: the cluster box is grown by GAMMA
@@ -406,7 +405,7 @@ __device__ static bool setup_one(const tgx_object* __restrict__ objects, const u
  * step, so one screen-sized rectangle does not serialise on one lane. */
 template <typename F>
 __device__ static inline void for_each_rect_tile(bool has, uint32_t rect_lo, uint32_t rect_hi,
-                                                 uint64_t payload, F&& f)
+                                                 uint64_t payload, uint32_t tshift, F&& f)
 {
     const uint32_t lane = threadIdx.x & 63u;
     unsigned long long mask = __ballot(has);
@@ -415,8 +414,8 @@ __device__ static inline void for_each_rect_tile(bool has, uint32_t rect_lo, uin
         mask &= mask - 1ull;
         const uint32_t lo = __shfl(rect_lo, l), hi = __shfl(rect_hi, l);
         const uint64_t pl = __shfl(payload, l);
-        const uint32_t tx0 = (lo & 0xFFFFu) / VIS_TILE, ty0 = (lo >> 16) / VIS_TILE;
-        const uint32_t tx1 = (hi & 0xFFFFu) / VIS_TILE, ty1 = (hi >> 16) / VIS_TILE;
+        const uint32_t tx0 = (lo & 0xFFFFu) >> tshift, ty0 = (lo >> 16) >> tshift;
+        const uint32_t tx1 = (hi & 0xFFFFu) >> tshift, ty1 = (hi >> 16) >> tshift;
         const uint32_t ntx = tx1 - tx0 + 1u, nt = ntx * (ty1 - ty0 + 1u);
         for (uint32_t j = lane; j < nt; j += 64u) f(pl, tx0 + j % ntx, ty0 + j / ntx);
     }
@@ -446,7 +445,7 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
                                                  const uint32_t* __restrict__ objoff, uint32_t n_objects,
                                                  const uint32_t* __restrict__ kept_total,
                                                  const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
-                                                 uint32_t tiles_x, vis_cluster* __restrict__ recs,
+                                                 uint32_t tiles_x, uint32_t tshift, vis_cluster* __restrict__ recs,
                                                  uint32_t* __restrict__ rec_flags,
                                                  uint32_t* __restrict__ rec_count,
                                                  uint32_t* __restrict__ global_count,
@@ -476,11 +475,12 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
     if (has) {
         const uint32_t k = atomicAdd(&rec_count[ci], 1u);
         recs[cam.rec_base + k] = rec;
-        /* Rectangles of more than VIS_GLOBAL_TILES tiles skip the tile
-         * lists: every tile of the camera merges them from the global list. */
-        const uint32_t ntx = (rec.rect_hi & 0xFFFFu) / VIS_TILE - (rec.rect_lo & 0xFFFFu) / VIS_TILE + 1u;
-        const uint32_t nty = (rec.rect_hi >> 16) / VIS_TILE - (rec.rect_lo >> 16) / VIS_TILE + 1u;
-        if (ntx * nty > VIS_GLOBAL_TILES) {
+        /* Rectangles whose tiles cover more than VIS_GLOBAL_PIXELS pixels
+         * skip the tile lists: every tile of the camera merges them from
+         * the global list. */
+        const uint32_t ntx = ((rec.rect_hi & 0xFFFFu) >> tshift) - ((rec.rect_lo & 0xFFFFu) >> tshift) + 1u;
+        const uint32_t nty = ((rec.rect_hi >> 16) >> tshift) - ((rec.rect_lo >> 16) >> tshift) + 1u;
+        if (((uint64_t)ntx * nty << (2u * tshift)) > VIS_GLOBAL_PIXELS) {
             const uint32_t g = atomicAdd(&global_count[ci], 1u);
             global_keys[cam.rec_base + g] = ((uint64_t)rec.depth_lb24 << 32) | (cam.rec_base + k);
         } else {
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
         if (!binned) rec_flags[cam.rec_base + k] = 0u;
     }
     const uint32_t tb = cam.tile_base;
-    for_each_rect_tile(binned, binned ? rec.rect_lo : 0u, binned ? rec.rect_hi : 0u, 0ull,
+    for_each_rect_tile(binned, binned ? rec.rect_lo : 0u, binned ? rec.rect_hi : 0u, 0ull, tshift,
                        [&](uint64_t, uint32_t tx, uint32_t ty) { atomicAdd(&tile_count[tb + ty * tiles_x + tx], 1u); });
     }
 }
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(256) void vis_scatter(const vis_cluster* __restrict
                                                    const uint32_t* __restrict__ rec_flags,
                                                    const uint32_t* __restrict__ rec_count,
                                                    const vis_cam* __restrict__ cams, uint32_t tiles_x,
-                                                   uint32_t* __restrict__ cursor,
+                                                   uint32_t tshift, uint32_t* __restrict__ cursor,
                                                    uint64_t* __restrict__ list, uint32_t capacity)
 {
     const uint32_t ci = blockIdx.y;
@@ -586,8 +586,8 @@ __global__ __launch_bounds__(256) void vis_scatter(const vis_cluster* __restrict
     if (rec_flags[slot] == 0u) continue;
     const uint32_t lo = recs[slot].rect_lo, hi = recs[slot].rect_hi;
     const uint64_t key = ((uint64_t)recs[slot].depth_lb24 << 32) | slot;
-    const uint32_t tx0 = (lo & 0xFFFFu) / VIS_TILE, ty0 = (lo >> 16) / VIS_TILE;
-    const uint32_t tx1 = (hi & 0xFFFFu) / VIS_TILE, ty1 = (hi >> 16) / VIS_TILE;
+    const uint32_t tx0 = (lo & 0xFFFFu) >> tshift, ty0 = (lo >> 16) >> tshift;
+    const uint32_t tx1 = (hi & 0xFFFFu) >> tshift, ty1 = (hi >> 16) >> tshift;
     const uint32_t ntx = tx1 - tx0 + 1u, nt = ntx * (ty1 - ty0 + 1u);
     const uint32_t tb = cams[ci].tile_base;
     for (uint32_t j = sub; j < nt; j += VIS_SCATTER_LANES) {
@@ -621,7 +621,10 @@ __device__ static inline void lds_bitonic_sort(uint64_t* keys, uint32_t n2)
     }
 }
 
-__global__ __launch_bounds__(256) void vis_traverse(const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
+/* One workgroup of T x T lanes (T = 1 << kShift) per tile, each wave an
+ * 8 x 8 pixel block. */
+template <uint32_t kShift>
+__global__ __launch_bounds__(1u << (2u * kShift)) void vis_traverse(const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                     uint32_t tiles_x, uint32_t tiles_per_cam,
                                                     const uint32_t* __restrict__ tile_off,
                                                     const uint32_t* __restrict__ tile_end,
@@ -634,14 +637,15 @@ __global__ __launch_bounds__(256) void vis_traverse(const vis_cam* __restrict__ 
                                                     uint64_t* __restrict__ out)
 {
     __shared__ uint64_t keys[VIS_SORT_CAP];
-    __shared__ uint32_t lds_bits[4][16];
+    constexpr uint32_t kT = 1u << kShift, kWx = kT / 8u;
+    __shared__ uint32_t lds_bits[kWx * kWx][16];
     const uint32_t tile = blockIdx.x;
     const uint32_t ci = tile / tiles_per_cam;
     const uint32_t t = tile - ci * tiles_per_cam;
     const uint32_t tx = t % tiles_x, ty = t / tiles_x;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t px = tx * VIS_TILE + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t py = ty * VIS_TILE + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t px = tx * kT + (wave % kWx) * 8u + (lane & 7u);
+    const uint32_t py = ty * kT + (wave / kWx) * 8u + (lane >> 3);
     const bool valid = px < W && py < H;
 
     const vis_cam& cam = cams[ci];
@@ -755,6 +759,14 @@ int tgx_vis_solid_refresh(tgx_context* ctx)
     return TGX_OK;
 }
 
+/* Tile edge of the visibility passes: 1 << shift pixels. */
+static uint32_t vis_tile_shift(const tgx_context* ctx)
+{
+    return ctx->tuning.vis_tile == 8u ? 3u : ctx->tuning.vis_tile == 32u ? 5u : 4u;
+}
+
+static uint32_t tiles_of(uint32_t n, uint32_t shift) { return (n + (1u << shift) - 1u) >> shift; }
+
 /* Device buffers of one visibility batch. */
 struct vis_bufs {
     const vis_cam*      cams;
@@ -776,7 +788,8 @@ struct vis_bufs {
 static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cams, uint32_t W, uint32_t H,
                              uint64_t* out_dev)
 {
-    const uint32_t tiles_x = (W + VIS_TILE - 1) / VIS_TILE, tiles_y = (H + VIS_TILE - 1) / VIS_TILE;
+    const uint32_t ts = vis_tile_shift(ctx);
+    const uint32_t tiles_x = tiles_of(W, ts), tiles_y = tiles_of(H, ts);
     const uint32_t tiles_per_cam = tiles_x * tiles_y;
     const uint32_t n_tiles = tiles_per_cam * n_cams;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
@@ -805,7 +818,7 @@ static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cam
         const uint32_t blocks = (n_cptr + 255) / 256 < cap ? (n_cptr + 255) / 256 : cap;
         hipLaunchKernelGGL(vis_setup, dim3(blocks, n_cams), dim3(256), 0, ctx->stream,
                            (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr, b.solid,
-                           (const uint32_t*)b.objcnt, ctx->n_objects, (const uint32_t*)d_kept, b.cams, W, H, tiles_x,
+                           (const uint32_t*)b.objcnt, ctx->n_objects, (const uint32_t*)d_kept, b.cams, W, H, tiles_x, ts,
                            b.recs, b.rec_flags, d_counters, d_global_count, b.gkeys, b.tile_count);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_setup");
@@ -822,20 +835,21 @@ static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cam
         const uint64_t cap = ctx->tuning.vis_setup_blocks;
         hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)(want < cap ? want : cap), n_cams),
                            dim3(256), 0, ctx->stream,
-                           b.recs, b.rec_flags, d_counters, b.cams, tiles_x, b.tile_count, b.list, b.capacity);
+                           b.recs, b.rec_flags, d_counters, b.cams, tiles_x, ts, b.tile_count, b.list, b.capacity);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_scatter");
     {
         /* After the scatter, tile_count[t] is the end of tile t's list. */
         tgx_launch_scope s(ctx, "vis_traverse", b.timed);
-        hipLaunchKernelGGL(vis_traverse, dim3(n_tiles), dim3(256), 0, ctx->stream, b.cams, W, H, tiles_x,
+        auto* k = ts == 3u ? vis_traverse<3> : ts == 5u ? vis_traverse<5> : vis_traverse<4>;
+        hipLaunchKernelGGL(k, dim3(n_tiles), dim3(1u << (2u * ts)), 0, ctx->stream, b.cams, W, H, tiles_x,
                            tiles_per_cam, b.tile_off, b.tile_count, b.list, b.capacity, d_counters, d_global_count,
                            b.gkeys, b.recs, (const uint32_t*)ctx->bits.ptr, out_dev);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_traverse");
     ctx->vis_stats.clusters_total += (uint64_t)n_cptr * n_cams;
     ctx->vis_stats.tiles = tiles_per_cam;
-    ctx->vis_stats.tile_size = VIS_TILE;
+    ctx->vis_stats.tile_size = 1u << ts;
     return TGX_OK;
 }
 
@@ -854,20 +868,20 @@ static int visibility_empty(tgx_context* ctx, uint32_t n_cams, uint32_t W, uint3
     return TGX_OK;
 }
 
-static bool batch_fits_grid(uint32_t n_cams, uint32_t W, uint32_t H, uint32_t n_cptr)
+static bool batch_fits_grid(uint32_t n_cams, uint32_t W, uint32_t H, uint32_t n_cptr, uint32_t ts)
 {
-    const uint64_t tiles = (uint64_t)((W + VIS_TILE - 1) / VIS_TILE) * ((H + VIS_TILE - 1) / VIS_TILE) * n_cams;
+    const uint64_t tiles = (uint64_t)tiles_of(W, ts) * tiles_of(H, ts) * n_cams;
     return tiles < (1ull << 31) && (uint64_t)n_cptr * n_cams < (1ull << 32);
 }
 
 static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t W,
                             uint32_t H, uint64_t* out_dev, const vis_cam* resident)
 {
-    const uint32_t tiles_x = (W + VIS_TILE - 1) / VIS_TILE, tiles_y = (H + VIS_TILE - 1) / VIS_TILE;
-    const uint32_t tiles_per_cam = tiles_x * tiles_y;
+    const uint32_t ts = vis_tile_shift(ctx);
+    const uint32_t tiles_per_cam = tiles_of(W, ts) * tiles_of(H, ts);
     const uint32_t n_cptr = ctx->n_cluster_pointers;
     if (n_cptr == 0) return visibility_empty(ctx, n_cams, W, H, out_dev, true);
-    if (!batch_fits_grid(n_cams, W, H, n_cptr)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
+    if (!batch_fits_grid(n_cams, W, H, n_cptr, ts)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
     const uint64_t n_slots = (uint64_t)n_cptr * n_cams;
     const uint32_t n_tiles = tiles_per_cam * n_cams;
 
@@ -935,7 +949,8 @@ static int list_capacity_update(tgx_context* ctx, uint32_t w, uint32_t h)
     if (!ctx->vis_stats_event) TGX_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->vis_stats_event, hipEventDisableTiming));
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_stats_dev, 64, "vis stats"))) return rc;
     uint64_t want = ctx->vis_list_capacity;
-    const uint64_t tiles = (uint64_t)((w + VIS_TILE - 1) / VIS_TILE) * ((h + VIS_TILE - 1) / VIS_TILE);
+    const uint32_t ts = vis_tile_shift(ctx);
+    const uint64_t tiles = (uint64_t)tiles_of(w, ts) * tiles_of(h, ts);
     if (want == 0) {
         /* first call: room for 64 entries per tile, two per cluster pointer
          * and at least 2^22, so that large scenes rarely start overflowed */
@@ -971,7 +986,8 @@ int tgx_vis_cams_upload(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cam
     int rc = check_args(ctx, cams, n_cams, w, h);
     if (rc) return rc;
     const uint32_t per = batch_cams(ctx, n_cams);
-    const uint32_t tiles_per_cam = ((w + VIS_TILE - 1) / VIS_TILE) * ((h + VIS_TILE - 1) / VIS_TILE);
+    const uint32_t ts = vis_tile_shift(ctx);
+    const uint32_t tiles_per_cam = tiles_of(w, ts) * tiles_of(h, ts);
     std::vector<vis_cam> vc(n_cams);
     for (uint32_t i = 0; i < n_cams; i++) {
         const uint32_t j = i % per; /* index within its batch */
@@ -1141,8 +1157,8 @@ int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint
     unsigned char* base = (unsigned char*)vis_scratch_addr[ctx->device];
     TGX_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     memset(&ctx->vis_stats, 0, sizeof(ctx->vis_stats));
-    const uint32_t tiles_x = (w + VIS_TILE - 1) / VIS_TILE, tiles_y = (h + VIS_TILE - 1) / VIS_TILE;
-    const uint32_t tiles_per_cam = tiles_x * tiles_y;
+    const uint32_t ts = vis_tile_shift(ctx);
+    const uint32_t tiles_per_cam = tiles_of(w, ts) * tiles_of(h, ts);
     const uint32_t n_cptr = ctx->n_cluster_pointers;
     /* Cameras per batch: as many as fit with at least 64 tile-list entries
      * per tile (a frame whose lists overflow stays exact, only slower). */
@@ -1155,7 +1171,7 @@ int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint
                         "one camera at %ux%u over %u cluster pointers needs %zu bytes of scratch; the maximum is %llu",
                         w, h, n_cptr, noalloc_fixed_bytes(ctx, 1, tiles_per_cam) + 8ull * 64u * tiles_per_cam,
                         (unsigned long long)TGX_VIS_NOALLOC_SCRATCH_BYTES);
-    if (n_cptr && !batch_fits_grid(per, w, h, n_cptr))
+    if (n_cptr && !batch_fits_grid(per, w, h, n_cptr, ts))
         return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
     for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
         const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
