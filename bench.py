@@ -87,7 +87,10 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # cells' estimates).
 NRC_FALLBACK_THRESHOLD = 16.0   # tracer rule: trace where bias^2 > 16 x the 4-spp variance
 NRC_FALLBACK_MIN_SAMPLES = 0    # no fallback for cells merely short of samples
-NRC_EMA_DECAY = 0.999       # inference weights: moving average over ~1000 Adam steps (~60 frames)
+NRC_TRAIN_BATCH = 1 << 14   # samples per Adam step: 4 steps per frame of 65536 training samples
+NRC_EMA_DECAY = 0.996       # inference weights: moving average over ~250 Adam steps (~60 frames)
+NRC_SIDE_BATCH = 1 << 12    # the side form with 16 steps of 4096 samples per frame
+NRC_SIDE_EMA_DECAY = 0.999  # its moving average over the same ~60 frames
 NRC_REF_FRAMES = 4096       # 4-spp frames in the error reference (16384 spp)
 NRC_SIDE_FRAMES = 3200      # training frames of the forms reported beside the bench's own
 NRC_PATH_DEG_PER_FRAME = 0.05   # moving camera: yaw turned per frame
@@ -115,7 +118,7 @@ def moving_camera(ctx, sun, vis_d, rad_d, irr_d) -> dict:
                              0.1, 1000.0)
 
     ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16,
-                                train_batch=1 << 12, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
+                                train_batch=NRC_TRAIN_BATCH, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
                                 fallback_min_samples=NRC_FALLBACK_MIN_SAMPLES, cell_size=8,
                                 flags=T.NRC_FLAG_FALLBACK_VS_TRACER, ema_decay=NRC_EMA_DECAY))
     prm = T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=0)
@@ -189,7 +192,7 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     prm = T.gi_params(sun, spp=SPP, max_distance=1024.0, frame=0)
 
     def run(flags: int, threshold: float, min_samples: int, ema: float, milestones=(), frames: int = pretrain,
-            batch: int = 1 << 12):
+            batch: int = NRC_TRAIN_BATCH):
         """a fresh cache trained for `frames` untimed frames (errors read at
         the milestones), then `steps` timed frames"""
         ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / 512, train_samples=1 << 16,
@@ -204,13 +207,17 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
                 ctx.d2h(out, irr_d)
                 curve.append({"frames": k, **err(out), "fallback_pixels": int(ctx.nrc_stats()["fallback_pixels"])})
         ctx.synchronize()
-        ctx.profile(True)
-        ctx.profile_reset()
         t0 = time.perf_counter()
         for _ in range(steps):
             ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
         ctx.synchronize()
         elapsed = time.perf_counter() - t0
+        # per-kernel times from event-profiled frames after the timed ones
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(steps):
+            ctx.nrc_frame_device(cam, W, H, prm, vis_d, rad_d, irr_d)
+        ctx.synchronize()
         prof = ctx.profile_stats()
         ctx.profile(False)
         st = ctx.nrc_stats()
@@ -225,11 +232,12 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     ms_nf, _, _, irr_nf, _ = run(T.NRC_FLAG_NO_FALLBACK, 1.0, 1, NRC_EMA_DECAY, frames=side)
     ms_ne, _, st_ne, irr_ne, _ = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD, NRC_FALLBACK_MIN_SAMPLES,
                                      0.0, frames=side)
-    # the bench's form with 4 Adam steps of 16384 samples per frame instead of
-    # 16 of 4096, at the side forms' frame count (its error against the
-    # bench form's convergence entry at the same count)
-    ms_b16, _, st_b16, irr_b16, _ = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD,
-                                        NRC_FALLBACK_MIN_SAMPLES, NRC_EMA_DECAY, frames=side, batch=1 << 14)
+    # the bench's form with 16 Adam steps of 4096 samples per frame instead of
+    # 4 of 16384 (the moving average over the same number of frames), at the
+    # side forms' frame count (its error against the bench form's
+    # convergence entry at the same count)
+    ms_b4, _, st_b4, irr_b4, _ = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD,
+                                     NRC_FALLBACK_MIN_SAMPLES, NRC_SIDE_EMA_DECAY, frames=side, batch=NRC_SIDE_BATCH)
     marks = sorted({m for m in (200, 400, 800, 1600, 3200, 6400, 12800) if m < pretrain} | {pretrain})
     ms, prof, st, irr, curve = run(T.NRC_FLAG_FALLBACK_VS_TRACER, NRC_FALLBACK_THRESHOLD, NRC_FALLBACK_MIN_SAMPLES,
                                    NRC_EMA_DECAY, marks)
@@ -300,8 +308,9 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "no_fallback": {"frame_ms": ms_nf, **err(irr_nf)},
         "no_weight_average": {"frame_ms": ms_ne, "fallback_pixels": int(st_ne["fallback_pixels"]), **err(irr_ne)},
         "side_forms_frames": side,
-        "train_batch_16384": {"frame_ms": ms_b16, "fallback_pixels": int(st_b16["fallback_pixels"]),
-                              "adam_steps_per_frame": (1 << 16) // (1 << 14), **err(irr_b16)},
+        "train_batch_4096": {"frame_ms": ms_b4, "fallback_pixels": int(st_b4["fallback_pixels"]),
+                             "adam_steps_per_frame": (1 << 16) // NRC_SIDE_BATCH, "ema_decay": NRC_SIDE_EMA_DECAY,
+                             **err(irr_b4)},
         "moving_camera": moving,
         "train_samples": int(st["train_samples"]), "adam_steps_total": int(st["adam_steps"]),
         "train_loss": float(st["train_loss"]), "pretrain_frames": pretrain,
@@ -313,7 +322,7 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
         "captured_graphs": captured,
         "roofline": hbm,
         "config": {"network": "96(79 used)->64x4->3 ReLU, bf16 MFMA, fp32 Adam lr 1e-3", "train_samples": 1 << 16,
-                   "train_batch": 1 << 12, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512",
+                   "train_batch": NRC_TRAIN_BATCH, "train_spp": 4, "encoding": "12-frequency sin/cos of p/512",
                    "ema_decay": NRC_EMA_DECAY},
     }
 
@@ -365,7 +374,7 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
     for mode, flags in (("plain", 0), ("neural", T.FRAME_NEURAL)):
         if flags:
             ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / BATCH_WORLD, train_samples=1 << 16,
-                                        train_batch=1 << 12, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
+                                        train_batch=NRC_TRAIN_BATCH, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
                                         fallback_min_samples=NRC_FALLBACK_MIN_SAMPLES, cell_size=8,
                                         flags=T.NRC_FLAG_FALLBACK_VS_TRACER | T.NRC_FLAG_JOINT_TRAINING,
                                         ema_decay=NRC_EMA_DECAY))
@@ -549,7 +558,7 @@ def svo_file_bench(ctx, cam, prm, vis_d, rad_d, irr_d) -> dict:
         load_ms = 1e3 * (time.perf_counter() - t0)
     info = ctx.svo_info()
     return {"bytes": size, "save_ms": save_ms, "load_ms": load_ms, "nodes": int(info["n_nodes"]),
-            "leaves": int(info["n_leaves"]), "leaf_bytes_in_hbm": (4096 + 2 * 32768) * int(info["n_leaves"]),
+            "leaves": int(info["n_leaves"]), "device_bytes": ctx.svo_device_bytes()[0],
             "frame_after_load_identical": frame() == before}
 
 

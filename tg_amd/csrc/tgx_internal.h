@@ -44,7 +44,11 @@ struct tgx_context {
     /* Visibility work buffers. */
     tgx_buffer   vis_clusters;   /* compacted per-(camera, cluster) records */
     tgx_buffer   vis_solid;      /* u8 per cluster index: has a solid voxel */
-    tgx_buffer   vis_objcnt;     /* u32 per (camera, object): kept counts   */
+    tgx_buffer   vis_objcnt;     /* 3 u32 per (camera, object): cull lists  */
+    tgx_buffer   vis_occ;        /* occupied cluster pointers: flags, scan, list (3 n_cptr u32) */
+    tgx_buffer   vis_obj_occ;    /* uint2 per object: first occupied entry, count */
+    tgx_buffer   vis_scan_sums;  /* per-workgroup sums of the device scans + grand totals */
+    uint32_t     vis_n_occ = 0;  /* occupied cluster pointers of the scene  */
     uint64_t     scene_version = 0;     /* bumped by every write of cluster bits */
     uint64_t     vis_solid_version = ~0ull; /* scene_version vis_solid was made for */
     tgx_buffer   vis_rec_flags;  /* 1 if a record is in the tile lists      */

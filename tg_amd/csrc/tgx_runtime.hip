@@ -143,6 +143,7 @@ int tgx_destroy(tgx_context* ctx)
     }
     tgx_buffer* bufs[] = {&ctx->objects, &ctx->cluster_pointers, &ctx->cluster_idx_to_object,
                           &ctx->bits, &ctx->materials, &ctx->luts, &ctx->vis_clusters, &ctx->vis_solid, &ctx->vis_objcnt,
+                          &ctx->vis_occ, &ctx->vis_obj_occ, &ctx->vis_scan_sums,
                           &ctx->vis_rec_flags, &ctx->vis_global_keys,
                           &ctx->vis_counters, &ctx->vis_tile_count, &ctx->vis_tile_off,
                           &ctx->vis_tile_list, &ctx->vis_cams, &ctx->vis_out, &ctx->vis_stats_dev,

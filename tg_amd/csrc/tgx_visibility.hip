@@ -259,6 +259,7 @@ __device__ static bool cluster_rect(const vis_cam& cam, const tgx_object& ob, ui
  * voxel for the rounding of the pair's slab distance, holds no hit with
  * depth <= 1 and is culled for every camera. */
 __global__ __launch_bounds__(256) void vis_objcull_k(const tgx_object* __restrict__ objects, uint32_t n_objects,
+                                                     const uint2* __restrict__ obj_occ,
                                                      const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                      uint32_t* __restrict__ objcnt)
 {
@@ -267,8 +268,8 @@ __global__ __launch_bounds__(256) void vis_objcull_k(const tgx_object* __restric
     if (o >= n_objects) return;
     const vis_cam& cam = cams[ci];
     const tgx_object ob = objects[o];
-    const uint32_t n = ob.n_cluster_pointers_per_dim[0] * ob.n_cluster_pointers_per_dim[1] *
-                       ob.n_cluster_pointers_per_dim[2];
+    /* the object's cluster pointers to clusters with a solid voxel */
+    const uint32_t n = obj_occ[o].y;
     uint8_t keep = n ? 1u : 0u;
     {
         const float ex = 4.0f * (float)ob.n_cluster_pointers_per_dim[0] + 1.0f;
@@ -316,55 +317,65 @@ __global__ __launch_bounds__(256) void vis_objcull_k(const tgx_object* __restric
         const bool behind = hi[0] < VIS_RECT_EPS - 1.0f;
         if (!near_cube && (behind || left || right || below || above)) keep = 0u;
     }
-    objcnt[(size_t)ci * n_objects + o] = keep ? n : 0u;
+    objcnt[(size_t)ci * 3u * n_objects + o] = keep ? n : 0u;
 }
 
-/* Per camera (one workgroup each): the object counts become exclusive
- * offsets in place, and total[ci] the number of cluster pointers of kept
- * objects. */
+/* Per camera (one workgroup each): the kept objects (nonzero counts) in
+ * object order, compacted into ids and exclusive offsets of their cluster
+ * pointers (the camera's block of objcnt holds counts, ids, offsets, n_objects
+ * u32 each); total[ci] the number of cluster pointers of kept objects and
+ * n_kept[ci] the number of kept objects. */
 __global__ __launch_bounds__(1024) void vis_objscan_k(uint32_t* __restrict__ objcnt, uint32_t n_objects,
-                                                      uint32_t* __restrict__ total)
+                                                      uint32_t* __restrict__ total, uint32_t* __restrict__ n_kept)
 {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
+    __shared__ uint32_t wsum[16], wnum[16];
+    __shared__ uint32_t carry, carry_n;
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    uint32_t* cnt = objcnt + (size_t)blockIdx.x * n_objects;
-    if (t == 0) carry = 0u;
+    const uint32_t* cnt = objcnt + (size_t)blockIdx.x * 3u * n_objects;
+    uint32_t* kid = objcnt + (size_t)blockIdx.x * 3u * n_objects + n_objects;
+    uint32_t* koff = kid + n_objects;
+    if (t == 0) { carry = 0u; carry_n = 0u; }
     __syncthreads();
     for (uint32_t base = 0; base < n_objects; base += 1024u) {
         const uint32_t i = base + t;
         const uint32_t v = i < n_objects ? cnt[i] : 0u;
-        uint32_t incl = v;
+        const uint32_t f = v ? 1u : 0u;
+        uint32_t incl = v, ninc = f;
 #pragma unroll
         for (uint32_t d = 1; d < 64u; d <<= 1) {
-            const uint32_t u = __shfl_up(incl, d);
-            if (lane >= d) incl += u;
+            const uint32_t u = __shfl_up(incl, d), un = __shfl_up(ninc, d);
+            if (lane >= d) { incl += u; ninc += un; }
         }
-        if (lane == 63u) wsum[w] = incl;
+        if (lane == 63u) { wsum[w] = incl; wnum[w] = ninc; }
         __syncthreads();
-        uint32_t wofs = 0u, blk = 0u;
+        uint32_t wofs = 0u, blk = 0u, wn = 0u, blkn = 0u;
         for (uint32_t k = 0; k < 16u; k++) {
             wofs += k < w ? wsum[k] : 0u;
             blk += wsum[k];
+            wn += k < w ? wnum[k] : 0u;
+            blkn += wnum[k];
         }
-        const uint32_t c0 = carry;
-        if (i < n_objects) cnt[i] = c0 + wofs + incl - v;
+        const uint32_t c0 = carry, n0 = carry_n;
+        if (f) {
+            const uint32_t pos = n0 + wn + ninc - 1u;
+            kid[pos] = i;
+            koff[pos] = c0 + wofs + incl - v;
+        }
         __syncthreads();
-        if (t == 0) carry = c0 + blk;
+        if (t == 0) { carry = c0 + blk; carry_n = n0 + blkn; }
         __syncthreads();
     }
-    if (t == 0) total[blockIdx.x] = carry;
+    if (t == 0) { total[blockIdx.x] = carry; n_kept[blockIdx.x] = carry_n; }
 }
 
-/* Builds the record of one (camera, cluster pointer); false if the cluster
- * is empty, its object is culled, or it is out of every pixel ray's reach
- * or beyond the far plane. */
+/* Builds the record of one (camera, cluster pointer) whose cluster holds a
+ * solid voxel; false if it is out of every pixel ray's reach or beyond the
+ * far plane. */
 __device__ static bool setup_one(const tgx_object* __restrict__ objects, const uint32_t* __restrict__ cptrs,
-                                 const uint8_t* __restrict__ solid, uint32_t oi, uint32_t cptr, const vis_cam& cam,
-                                 uint32_t W, uint32_t H, vis_cluster& rec)
+                                 uint32_t oi, uint32_t cptr, const vis_cam& cam, uint32_t W, uint32_t H,
+                                 vis_cluster& rec)
 {
     const uint32_t cidx = cptrs[cptr];
-    if (!solid[cidx]) return false; /* no solid voxel: the walk can never hit */
 
     const tgx_object ob = objects[oi];
     const uint32_t rel = cptr - ob.first_cluster_pointer;
@@ -436,13 +447,160 @@ __global__ __launch_bounds__(256) void vis_solid_k(const uint4* __restrict__ bit
     solid[c] = any ? 1u : 0u;
 }
 
-/* One lane per cluster pointer of the camera's kept objects (the offsets
- * of vis_objscan_k), grid-stride: the lane's object is the last one whose
- * offset is not above the lane's index. */
+/* Occupied cluster pointers (those whose cluster holds a solid voxel), in
+ * pointer order: flag[p], its exclusive scan pos[p], then the list and
+ * each object's (first entry, count). An object's pointers are one
+ * contiguous range, so its occupied ones are one contiguous run. */
+__global__ __launch_bounds__(256) void vis_occ_flag_k(const uint32_t* __restrict__ cptrs, const uint8_t* __restrict__ solid,
+                                                      uint32_t n_cptr, uint32_t* __restrict__ flag)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n_cptr) flag[p] = solid[cptrs[p]] ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void vis_occ_list_k(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                                      uint32_t n_cptr, uint32_t* __restrict__ list)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n_cptr && flag[p]) list[pos[p]] = p;
+}
+
+__global__ __launch_bounds__(256) void vis_occ_obj_k(const tgx_object* __restrict__ objects, uint32_t n_objects,
+                                                     const uint32_t* __restrict__ pos, const uint32_t* __restrict__ total,
+                                                     uint32_t n_cptr, uint2* __restrict__ obj_occ)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_objects) return;
+    const tgx_object ob = objects[o];
+    const uint32_t n = ob.n_cluster_pointers_per_dim[0] * ob.n_cluster_pointers_per_dim[1] *
+                       ob.n_cluster_pointers_per_dim[2];
+    const uint32_t a = ob.first_cluster_pointer, e = a + n;
+    const uint32_t pa = a < n_cptr ? pos[a] : *total, pe = e < n_cptr ? pos[e] : *total;
+    obj_occ[o] = make_uint2(pa, pe - pa);
+}
+
+/* Device-wide exclusive scan in three passes: each workgroup scans 4096
+ * elements (4 per lane) and leaves its total in sums; one workgroup scans
+ * the totals (and writes the grand total); each workgroup then adds its
+ * offset, optionally writing the result to a second array as well. */
+#define VIS_SCAN_BLOCK 4096u
+
+__device__ static inline uint32_t block_excl_scan(uint32_t s, uint32_t* wsum, uint32_t& block_total)
+{
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t x = s;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63u) wsum[w] = x;
+    __syncthreads();
+    if (w == 0u) {
+        uint32_t v = lane < 16u ? wsum[lane] : 0u;
+        for (uint32_t d = 1; d < 16u; d <<= 1) {
+            const uint32_t y = __shfl_up(v, d);
+            if (lane >= d) v += y;
+        }
+        if (lane < 16u) wsum[lane] = v;
+    }
+    __syncthreads();
+    block_total = wsum[15];
+    const uint32_t r = x - s + (w ? wsum[w - 1] : 0u);
+    __syncthreads(); /* wsum is reused by the caller's next round */
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void vis_scan_local_k(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                         uint32_t n, uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t i0 = blockIdx.x * VIS_SCAN_BLOCK + 4u * threadIdx.x;
+    uint32_t v[4], s = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++) {
+        v[k] = i0 + k < n ? in[i0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(s, wsum, tot);
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++) {
+        if (i0 + k < n) out[i0 + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+/* Scans sums[0..nb) in place. *total gets the grand total; with stats,
+ * thread 0 also accumulates the visibility frame statistics: stats[0] +=
+ * records of the n_cams cameras, stats[1] += list entries, stats[2] = max
+ * list entries. */
+__global__ __launch_bounds__(1024) void vis_scan_top_k(uint32_t* __restrict__ sums, uint32_t nb,
+                                                       uint32_t* __restrict__ total,
+                                                       const uint32_t* __restrict__ rec_count, uint32_t n_cams,
+                                                       unsigned long long* __restrict__ stats)
+{
+    __shared__ uint32_t wsum[16];
+    uint32_t carry = 0u;
+    for (uint32_t b0 = 0; b0 < nb; b0 += VIS_SCAN_BLOCK) {
+        const uint32_t i0 = b0 + 4u * threadIdx.x;
+        uint32_t v[4], s = 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4u; k++) {
+            v[k] = i0 + k < nb ? sums[i0 + k] : 0u;
+            s += v[k];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_excl_scan(s, wsum, tot);
+#pragma unroll
+        for (uint32_t k = 0; k < 4u; k++) {
+            if (i0 + k < nb) sums[i0 + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        *total = carry;
+        if (stats) {
+            unsigned long long recs = 0;
+            for (uint32_t c = 0; c < n_cams; c++) recs += rec_count[c];
+            stats[0] += recs;
+            stats[1] += carry;
+            if (carry > stats[2]) stats[2] = carry;
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void vis_scan_add_k(uint32_t* __restrict__ out, uint32_t n,
+                                                       const uint32_t* __restrict__ sums, uint32_t* __restrict__ copy)
+{
+    const uint32_t i0 = blockIdx.x * VIS_SCAN_BLOCK + 4u * threadIdx.x;
+    const uint32_t add = sums[blockIdx.x];
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++) {
+        if (i0 + k < n) {
+            const uint32_t o = out[i0 + k] + add;
+            out[i0 + k] = o;
+            if (copy) copy[i0 + k] = o;
+        }
+    }
+}
+
+/* One lane per occupied cluster pointer (one whose cluster holds a solid
+ * voxel) of the camera's kept objects (the compact lists of vis_objscan_k;
+ * the object's occupied pointers are obj_occ[o].y entries of occ_list from
+ * obj_occ[o].x), grid-stride: the lane's object is the last kept
+ * one whose offset is not above the lane's index. A wave finds the kept
+ * object of its first index with a 64-way search (one load per lane and a
+ * ballot per round), then each lane steps over the objects its own index
+ * has passed (kept objects have at least one cluster pointer). */
 __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ objects,
                                                  const uint32_t* __restrict__ cptrs,
-                                                 const uint8_t* __restrict__ solid,
-                                                 const uint32_t* __restrict__ objoff, uint32_t n_objects,
+                                                 const uint32_t* __restrict__ occ_list,
+                                                 const uint2* __restrict__ obj_occ,
+                                                 const uint32_t* __restrict__ objlists, uint32_t n_objects,
+                                                 const uint32_t* __restrict__ n_kept,
                                                  const uint32_t* __restrict__ kept_total,
                                                  const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                  uint32_t tiles_x, uint32_t tshift, vis_cluster* __restrict__ recs,
@@ -454,22 +612,32 @@ __global__ __launch_bounds__(256) void vis_setup(const tgx_object* __restrict__ 
 {
     const uint32_t ci = blockIdx.y;
     const vis_cam& cam = cams[ci];
-    const uint32_t total = kept_total[ci];
-    const uint32_t* off = objoff + (size_t)ci * n_objects;
+    const uint32_t total = kept_total[ci], nk = n_kept[ci];
+    const uint32_t* kid = objlists + (size_t)ci * 3u * n_objects + n_objects;
+    const uint32_t* koff = kid + n_objects;
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = threadIdx.x & 63u;
     /* wave-uniform loop: the rectangle walk below shares tiles across lanes */
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
-    const uint32_t t = base + (threadIdx.x & 63u);
+    const uint32_t t = base + lane;
+    /* kept object of the wave's first index: koff[lo] <= base holds
+     * throughout (koff[0] = 0), so lane 0's test always passes */
+    uint32_t lo = 0u, span = nk;
+    while (span > 1u) {
+        const uint32_t step = (span + 63u) >> 6;
+        const bool p = lane * step < span && koff[lo + lane * step] <= base;
+        const uint32_t k = (uint32_t)__popcll(__ballot(p)) - 1u;
+        lo += k * step;
+        span = min(step, span - k * step);
+    }
     vis_cluster rec;
     bool has = false;
     if (t < total) {
-        uint32_t lo = 0, hi = n_objects - 1u;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1u) >> 1;
-            if (off[mid] <= t) lo = mid; else hi = mid - 1u;
-        }
-        const uint32_t cptr = objects[lo].first_cluster_pointer + (t - off[lo]);
-        has = setup_one(objects, cptrs, solid, lo, cptr, cam, W, H, rec);
+        uint32_t k = lo;
+        while (k + 1u < nk && koff[k + 1u] <= t) k++;
+        const uint32_t oi = kid[k];
+        const uint32_t cptr = occ_list[obj_occ[oi].x + (t - koff[k])];
+        has = setup_one(objects, cptrs, oi, cptr, cam, W, H, rec);
     }
     bool binned = false;
     if (has) {
@@ -516,54 +684,6 @@ __global__ void vis_cams_k(vis_cam_pack pack, uint32_t n, vis_cam* __restrict__ 
 {
     const uint32_t i = threadIdx.x;
     if (i < n) out[i] = pack.c[i];
-}
-
-/* Single-workgroup exclusive scan: off[i] = sum(count[0..i)), and
- * count[i] becomes the scatter cursor (= off[i]); the total goes to
- * *total. The last thread also accumulates the frame statistics:
- * stats[0] += records, stats[1] += list entries, stats[2] = max entries. */
-__global__ __launch_bounds__(1024) void vis_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ off,
-                                                 uint32_t n, uint32_t* __restrict__ total,
-                                                 const uint32_t* __restrict__ rec_count, uint32_t n_cams,
-                                                 unsigned long long* __restrict__ stats)
-{
-    __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = min(n, t * per), e = min(n, b + per);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; i++) s += count[i];
-    uint32_t x = s; /* inclusive scan within the wave */
-    for (uint32_t d = 1; d < 64u; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63u) wsum[w] = x;
-    __syncthreads();
-    if (w == 0) {
-        uint32_t v = lane < 16u ? wsum[lane] : 0u;
-        for (uint32_t d = 1; d < 16u; d <<= 1) {
-            const uint32_t y = __shfl_up(v, d);
-            if (lane >= d) v += y;
-        }
-        if (lane < 16u) wsum[lane] = v;
-    }
-    __syncthreads();
-    uint32_t run = x - s + (w ? wsum[w - 1] : 0u);
-    for (uint32_t i = b; i < e; i++) {
-        const uint32_t c = count[i];
-        off[i] = run;
-        count[i] = run;
-        run += c;
-    }
-    if (t == 1023u) {
-        *total = run;
-        unsigned long long recs = 0;
-        for (uint32_t c = 0; c < n_cams; c++) recs += rec_count[c];
-        stats[0] += recs;
-        stats[1] += run;
-        if (run > stats[2]) stats[2] = run;
-    }
 }
 
 /* Each list entry is the sort key (depth_lb24 << 32) | record slot. Eight
@@ -622,7 +742,11 @@ __device__ static inline void lds_bitonic_sort(uint64_t* keys, uint32_t n2)
 }
 
 /* One workgroup of T x T lanes (T = 1 << kShift) per tile, each wave an
- * 8 x 8 pixel block. */
+ * 8 x 8 pixel block. The sorted entries are staged VIS_STAGE at a time:
+ * the workgroup loads their records, then their clusters' bit words, into
+ * LDS with all lanes at once, so the walk over the entries reads no
+ * global memory. */
+#define VIS_STAGE 64u
 template <uint32_t kShift>
 __global__ __launch_bounds__(1u << (2u * kShift)) void vis_traverse(const vis_cam* __restrict__ cams, uint32_t W, uint32_t H,
                                                     uint32_t tiles_x, uint32_t tiles_per_cam,
@@ -638,7 +762,9 @@ __global__ __launch_bounds__(1u << (2u * kShift)) void vis_traverse(const vis_ca
 {
     __shared__ uint64_t keys[VIS_SORT_CAP];
     constexpr uint32_t kT = 1u << kShift, kWx = kT / 8u;
-    __shared__ uint32_t lds_bits[kWx * kWx][16];
+    __shared__ vis_cluster s_rec[VIS_STAGE];
+    __shared__ uint32_t s_bits[VIS_STAGE][16];
+    __shared__ uint32_t s_more;
     const uint32_t tile = blockIdx.x;
     const uint32_t ci = tile / tiles_per_cam;
     const uint32_t t = tile - ci * tiles_per_cam;
@@ -666,7 +792,6 @@ __global__ __launch_bounds__(1u << (2u * kShift)) void vis_traverse(const vis_ca
     const uint32_t n_own = overflow ? rec_count[ci] : tile_end[tile] - b;
     const uint32_t n_all = n_own + (overflow ? 0u : global_count[ci]);
     const uint64_t* gk = global_keys + cam.rec_base;
-    uint32_t* my_bits = lds_bits[wave];
     for (uint32_t cb = 0; cb < n_all; cb += VIS_SORT_CAP) {
         /* Front to back: sort this chunk of the tile's list by depth bound. */
         const uint32_t n = min(VIS_SORT_CAP, n_all - cb);
@@ -685,27 +810,38 @@ __global__ __launch_bounds__(1u << (2u * kShift)) void vis_traverse(const vis_ca
         }
         __syncthreads();
         lds_bitonic_sort(keys, n2);
-        for (uint32_t j = 0; j < n; j++) {
-            const uint64_t key = keys[j];
-            const uint32_t lb = (uint32_t)(key >> 32);
-            /* Sorted bounds: once no lane can improve, none of the rest can. */
-            if (!__any(valid && (uint32_t)(best >> 40) >= lb)) break;
-            const vis_cluster& c = recs[(uint32_t)key];
-            const uint32_t lo = c.rect_lo, hi = c.rect_hi;
-            const bool act = valid && px >= (lo & 0xFFFFu) && px <= (hi & 0xFFFFu) && py >= (lo >> 16) &&
-                             py <= (hi >> 16) && (uint32_t)(best >> 40) >= lb;
-            if (!__any(act)) continue;
-            if (lane < 16u) my_bits[lane] = bits[(size_t)c.cidx * 16u + lane];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (act) {
-                const uint64_t word = pair_word(c, my_bits, dwx, dwy, dwz, far_plane);
-                best = word < best ? word : best;
+        for (uint32_t sb = 0; sb < n; sb += VIS_STAGE) {
+            const uint32_t ns = min(VIS_STAGE, n - sb);
+            for (uint32_t q = threadIdx.x; q < ns * 20u; q += blockDim.x) {
+                const uint32_t e = q / 20u, f = q - e * 20u;
+                ((uint32_t*)&s_rec[e])[f] = ((const uint32_t*)&recs[(uint32_t)keys[sb + e]])[f];
             }
-            __builtin_amdgcn_wave_barrier();
+            if (threadIdx.x == 0) s_more = 0u;
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < ns * 16u; q += blockDim.x)
+                s_bits[q >> 4][q & 15u] = bits[(size_t)s_rec[q >> 4].cidx * 16u + (q & 15u)];
+            __syncthreads();
+            bool stop = false;
+            for (uint32_t e = 0; e < ns; e++) {
+                const uint32_t lb = (uint32_t)(keys[sb + e] >> 32);
+                /* Sorted bounds: once no lane can improve, none of the rest can. */
+                if (!__any(valid && (uint32_t)(best >> 40) >= lb)) { stop = true; break; }
+                const vis_cluster& c = s_rec[e];
+                const uint32_t lo = c.rect_lo, hi = c.rect_hi;
+                const bool act = valid && px >= (lo & 0xFFFFu) && px <= (hi & 0xFFFFu) && py >= (lo >> 16) &&
+                                 py <= (hi >> 16) && (uint32_t)(best >> 40) >= lb;
+                if (act) {
+                    const uint64_t word = pair_word(c, s_bits[e], dwx, dwy, dwz, far_plane);
+                    best = word < best ? word : best;
+                }
+            }
+            /* the next stage only while some wave may still improve */
+            if (!stop && lane == 0u) s_more = 1u;
+            __syncthreads();
+            const bool more = s_more != 0u;
+            __syncthreads(); /* s_rec, s_bits and s_more are rewritten by the next stage */
+            if (!more) break;
         }
-        __syncthreads(); /* keys are rewritten by the next chunk */
     }
     if (valid) out[(size_t)ci * W * H + (size_t)py * W + px] = best;
 }
@@ -747,17 +883,87 @@ static bool make_cam(const tgx_camera& c, vis_cam& v)
     return true;
 }
 
+static uint32_t scan_groups(uint64_t n) { return (uint32_t)((n + VIS_SCAN_BLOCK - 1u) / VIS_SCAN_BLOCK); }
+
+/* Exclusive scan of in[0..n) into out (and copy, if not null); sums holds
+ * scan_groups(n) u32. The total goes to *total; with stats, the frame
+ * statistics are accumulated as well (see vis_scan_top_k). */
+static int scan_launch(tgx_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums,
+                       uint32_t* copy, uint32_t* total, const uint32_t* rec_count = nullptr, uint32_t n_cams = 0,
+                       unsigned long long* stats = nullptr)
+{
+    const uint32_t nb = scan_groups(n);
+    if (nb) hipLaunchKernelGGL(vis_scan_local_k, dim3(nb), dim3(1024), 0, ctx->stream, in, out, n, sums);
+    hipLaunchKernelGGL(vis_scan_top_k, dim3(1), dim3(1024), 0, ctx->stream, sums, nb, total, rec_count, n_cams, stats);
+    if (nb) hipLaunchKernelGGL(vis_scan_add_k, dim3(nb), dim3(1024), 0, ctx->stream, out, n, sums, copy);
+    TGX_LAUNCH_CHECK(ctx, "vis_scan");
+    return TGX_OK;
+}
+
+/* Device buffers of the occupied-pointer lists: flags, scan and list
+ * (n_cptr u32 each), per-object (first, count), scan sums
+ * (scan_groups(n_cptr) + 1 u32, the last the number of occupied pointers). */
+struct vis_occ_bufs {
+    uint32_t* flag;
+    uint32_t* pos;
+    uint32_t* list;
+    uint2*    obj_occ;
+    uint32_t* sums;
+};
+
+static int occ_launch(tgx_context* ctx, const uint8_t* solid, const vis_occ_bufs& o)
+{
+    const uint32_t n = ctx->n_cluster_pointers;
+    uint32_t* total = o.sums + scan_groups(n);
+    hipLaunchKernelGGL(vis_occ_flag_k, dim3((n + 255u) / 256u), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)ctx->cluster_pointers.ptr, solid, n, o.flag);
+    TGX_LAUNCH_CHECK(ctx, "vis_occ_flag_k");
+    int rc;
+    if ((rc = scan_launch(ctx, o.flag, o.pos, n, o.sums, nullptr, total))) return rc;
+    hipLaunchKernelGGL(vis_occ_list_k, dim3((n + 255u) / 256u), dim3(256), 0, ctx->stream, (const uint32_t*)o.flag,
+                       (const uint32_t*)o.pos, n, o.list);
+    hipLaunchKernelGGL(vis_occ_obj_k, dim3((ctx->n_objects + 255u) / 256u), dim3(256), 0, ctx->stream,
+                       (const tgx_object*)ctx->objects.ptr, ctx->n_objects, (const uint32_t*)o.pos,
+                       (const uint32_t*)total, n, o.obj_occ);
+    TGX_LAUNCH_CHECK(ctx, "vis_occ_obj_k");
+    return TGX_OK;
+}
+
+/* Solid flags per cluster and the scene's occupied cluster pointers, once
+ * per scene version; the count of occupied pointers comes back to the host
+ * (it sizes the per-camera record slots). */
 int tgx_vis_solid_refresh(tgx_context* ctx)
 {
-    if (ctx->vis_solid_version == ctx->scene_version || ctx->n_clusters == 0) return TGX_OK;
+    if (ctx->vis_solid_version == ctx->scene_version) return TGX_OK;
+    const uint32_t n = ctx->n_cluster_pointers;
+    if (ctx->n_clusters == 0 || n == 0) {
+        ctx->vis_n_occ = 0;
+        ctx->vis_solid_version = ctx->scene_version;
+        return TGX_OK;
+    }
     int rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_solid, ctx->n_clusters, "cluster solid flags"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_occ, 12ull * n, "occupied cluster pointers"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_obj_occ, 8ull * (ctx->n_objects ? ctx->n_objects : 1u),
+                                 "occupied pointers per object")))
+        return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_scan_sums, 4ull * (scan_groups(n) + 1u), "scan sums"))) return rc;
     hipLaunchKernelGGL(vis_solid_k, dim3((ctx->n_clusters + 255u) / 256u), dim3(256), 0, ctx->stream,
                        (const uint4*)ctx->bits.ptr, ctx->n_clusters, (uint8_t*)ctx->vis_solid.ptr);
     TGX_LAUNCH_CHECK(ctx, "vis_solid_k");
+    uint32_t* occ = (uint32_t*)ctx->vis_occ.ptr;
+    const vis_occ_bufs o = {occ, occ + n, occ + 2ull * n, (uint2*)ctx->vis_obj_occ.ptr, (uint32_t*)ctx->vis_scan_sums.ptr};
+    if ((rc = occ_launch(ctx, (const uint8_t*)ctx->vis_solid.ptr, o))) return rc;
+    uint32_t n_occ = 0;
+    TGX_HIP_CHECK(ctx, hipMemcpyAsync(&n_occ, o.sums + scan_groups(n), 4, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->vis_n_occ = n_occ;
     ctx->vis_solid_version = ctx->scene_version;
     return TGX_OK;
 }
+
+/* Record slots per camera: one per occupied cluster pointer (at least one). */
+static uint32_t rec_slots(const tgx_context* ctx) { return ctx->vis_n_occ ? ctx->vis_n_occ : 1u; }
 
 /* Tile edge of the visibility passes: 1 << shift pixels. */
 static uint32_t vis_tile_shift(const tgx_context* ctx)
@@ -770,12 +976,15 @@ static uint32_t tiles_of(uint32_t n, uint32_t shift) { return (n + (1u << shift)
 /* Device buffers of one visibility batch. */
 struct vis_bufs {
     const vis_cam*      cams;
-    const uint8_t*      solid;      /* u8 per cluster index                */
-    uint32_t*           objcnt;     /* u32 per (camera, object): kept counts, then offsets */
-    vis_cluster*        recs;       /* per (camera, cluster pointer)       */
+    const uint32_t*     occ_list;   /* occupied cluster pointers            */
+    const uint2*        obj_occ;    /* per object: first occupied entry, count */
+    uint32_t            slots;      /* record slots per camera              */
+    uint32_t*           scan_sums;  /* scan_groups(tiles of the batch) u32  */
+    uint32_t*           objcnt;     /* 3 u32 per (camera, object): counts, kept ids, kept offsets */
+    vis_cluster*        recs;       /* slots per camera                     */
     uint32_t*           rec_flags;
     uint64_t*           gkeys;
-    uint32_t*           counters;   /* 3 n_cams + 8                        */
+    uint32_t*           counters;   /* 4 n_cams + 8                        */
     uint32_t*           tile_count;
     uint32_t*           tile_off;
     uint64_t*           list;
@@ -794,43 +1003,50 @@ static int visibility_launch(tgx_context* ctx, const vis_bufs& b, uint32_t n_cam
     const uint32_t n_tiles = tiles_per_cam * n_cams;
     const uint32_t n_cptr = ctx->n_cluster_pointers;
     /* counters: [0, n_cams) records, [n_cams] list total, [n_cams + 8, 2 n_cams + 8) global counts,
-     * [2 n_cams + 8, 3 n_cams + 8) cluster pointers of kept objects */
+     * [2 n_cams + 8, 3 n_cams + 8) cluster pointers of kept objects, [3 n_cams + 8, 4 n_cams + 8) kept
+     * objects */
     uint32_t* d_counters = b.counters;
     uint32_t* d_global_count = d_counters + n_cams + 8;
     uint32_t* d_kept = d_counters + 2 * n_cams + 8;
+    uint32_t* d_nkept = d_counters + 3 * n_cams + 8;
     {
         tgx_launch_scope s(ctx, "vis_fill_u32", b.timed);
         hipLaunchKernelGGL(vis_fill_u32, dim3((n_tiles + 255) / 256), dim3(256), 0, ctx->stream, b.tile_count, n_tiles, 0u);
-        hipLaunchKernelGGL(vis_fill_u32, dim3((3 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters,
-                           3 * n_cams + 8, 0u);
+        hipLaunchKernelGGL(vis_fill_u32, dim3((4 * n_cams + 8 + 255) / 256), dim3(256), 0, ctx->stream, d_counters,
+                           4 * n_cams + 8, 0u);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
     {
         tgx_launch_scope s(ctx, "vis_objcull_k", b.timed);
         hipLaunchKernelGGL(vis_objcull_k, dim3((ctx->n_objects + 255u) / 256u, n_cams), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, b.cams, W, H, b.objcnt);
-        hipLaunchKernelGGL(vis_objscan_k, dim3(n_cams), dim3(1024), 0, ctx->stream, b.objcnt, ctx->n_objects, d_kept);
+                           (const tgx_object*)ctx->objects.ptr, ctx->n_objects, b.obj_occ, b.cams, W, H, b.objcnt);
+        hipLaunchKernelGGL(vis_objscan_k, dim3(n_cams), dim3(1024), 0, ctx->stream, b.objcnt, ctx->n_objects, d_kept,
+                           d_nkept);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_objcull_k");
     {
         tgx_launch_scope s(ctx, "vis_setup", b.timed);
         const uint32_t cap = ctx->tuning.vis_setup_blocks;
-        const uint32_t blocks = (n_cptr + 255) / 256 < cap ? (n_cptr + 255) / 256 : cap;
+        const uint32_t blocks = (b.slots + 255) / 256 < cap ? (b.slots + 255) / 256 : cap;
         hipLaunchKernelGGL(vis_setup, dim3(blocks, n_cams), dim3(256), 0, ctx->stream,
-                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr, b.solid,
-                           (const uint32_t*)b.objcnt, ctx->n_objects, (const uint32_t*)d_kept, b.cams, W, H, tiles_x, ts,
+                           (const tgx_object*)ctx->objects.ptr, (const uint32_t*)ctx->cluster_pointers.ptr, b.occ_list,
+                           b.obj_occ,
+                           (const uint32_t*)b.objcnt, ctx->n_objects, (const uint32_t*)d_nkept, (const uint32_t*)d_kept,
+                           b.cams, W, H, tiles_x, ts,
                            b.recs, b.rec_flags, d_counters, d_global_count, b.gkeys, b.tile_count);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_setup");
     {
+        /* list offsets; tile_count becomes the scatter cursor */
         tgx_launch_scope s(ctx, "vis_scan", b.timed);
-        hipLaunchKernelGGL(vis_scan, dim3(1), dim3(1024), 0, ctx->stream, b.tile_count, b.tile_off, n_tiles,
-                           d_counters + n_cams, d_counters, n_cams, b.stats);
+        int rc;
+        if ((rc = scan_launch(ctx, b.tile_count, b.tile_off, n_tiles, b.scan_sums, b.tile_count, d_counters + n_cams,
+                              d_counters, n_cams, b.stats)))
+            return rc;
     }
-    TGX_LAUNCH_CHECK(ctx, "vis_scan");
     {
         tgx_launch_scope s(ctx, "vis_scatter", b.timed);
-        const uint64_t threads = (uint64_t)n_cptr * VIS_SCATTER_LANES;
+        const uint64_t threads = (uint64_t)b.slots * VIS_SCATTER_LANES;
         const uint64_t want = (threads + 255) / 256;
         const uint64_t cap = ctx->tuning.vis_setup_blocks;
         hipLaunchKernelGGL(vis_scatter, dim3((uint32_t)(want < cap ? want : cap), n_cams),
@@ -868,10 +1084,10 @@ static int visibility_empty(tgx_context* ctx, uint32_t n_cams, uint32_t W, uint3
     return TGX_OK;
 }
 
-static bool batch_fits_grid(uint32_t n_cams, uint32_t W, uint32_t H, uint32_t n_cptr, uint32_t ts)
+static bool batch_fits_grid(uint32_t n_cams, uint32_t W, uint32_t H, uint32_t slots, uint32_t ts)
 {
     const uint64_t tiles = (uint64_t)tiles_of(W, ts) * tiles_of(H, ts) * n_cams;
-    return tiles < (1ull << 31) && (uint64_t)n_cptr * n_cams < (1ull << 32);
+    return tiles < (1ull << 31) && (uint64_t)slots * n_cams < (1ull << 32);
 }
 
 static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t W,
@@ -881,18 +1097,19 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     const uint32_t tiles_per_cam = tiles_of(W, ts) * tiles_of(H, ts);
     const uint32_t n_cptr = ctx->n_cluster_pointers;
     if (n_cptr == 0) return visibility_empty(ctx, n_cams, W, H, out_dev, true);
-    if (!batch_fits_grid(n_cams, W, H, n_cptr, ts)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
-    const uint64_t n_slots = (uint64_t)n_cptr * n_cams;
-    const uint32_t n_tiles = tiles_per_cam * n_cams;
-
     int rc;
     if ((rc = tgx_vis_solid_refresh(ctx))) return rc;
+    const uint32_t slots = rec_slots(ctx);
+    if (!batch_fits_grid(n_cams, W, H, slots, ts)) return tgx_fail(ctx, TGX_ERR_CAPACITY, "visibility batch too large");
+    const uint64_t n_slots = (uint64_t)slots * n_cams;
+    const uint32_t n_tiles = tiles_per_cam * n_cams;
+
     std::vector<vis_cam> vc;
     if (!resident) {
         vc.resize(n_cams);
         for (uint32_t i = 0; i < n_cams; i++) {
             make_cam(cams[i], vc[i]);
-            vc[i].rec_base = i * n_cptr;
+            vc[i].rec_base = i * slots;
             vc[i].tile_base = i * tiles_per_cam;
         }
         if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_cams, sizeof(vis_cam) * n_cams, "vis cams"))) return rc;
@@ -900,17 +1117,21 @@ static int visibility_batch(tgx_context* ctx, const tgx_camera* cams, uint32_t n
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_clusters, sizeof(vis_cluster) * n_slots, "vis records"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_rec_flags, 4ull * n_slots, "vis record flags"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_global_keys, 8ull * n_slots, "vis global keys"))) return rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (3 * n_cams + 8), "vis counters"))) return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_counters, 4ull * (4 * n_cams + 8), "vis counters"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_count, 4ull * n_tiles, "tile counts"))) return rc;
     if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_tile_off, 4ull * n_tiles, "tile offsets"))) return rc;
-    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_objcnt, 4ull * ctx->n_objects * n_cams, "object cull counts")))
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_objcnt, 12ull * ctx->n_objects * n_cams, "object cull lists")))
         return rc;
+    if ((rc = tgx_buffer_reserve(ctx, &ctx->vis_scan_sums, 4ull * (scan_groups(n_tiles) + 1u), "scan sums"))) return rc;
     if (!resident)
         TGX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->vis_cams.ptr, vc.data(), sizeof(vis_cam) * n_cams, hipMemcpyHostToDevice,
                                           ctx->stream));
     vis_bufs b;
     b.cams = resident ? resident : (const vis_cam*)ctx->vis_cams.ptr;
-    b.solid = (const uint8_t*)ctx->vis_solid.ptr;
+    b.occ_list = (const uint32_t*)ctx->vis_occ.ptr + 2ull * n_cptr;
+    b.obj_occ = (const uint2*)ctx->vis_obj_occ.ptr;
+    b.slots = slots;
+    b.scan_sums = (uint32_t*)ctx->vis_scan_sums.ptr;
     b.objcnt = (uint32_t*)ctx->vis_objcnt.ptr;
     b.recs = (vis_cluster*)ctx->vis_clusters.ptr;
     b.rec_flags = (uint32_t*)ctx->vis_rec_flags.ptr;
@@ -972,8 +1193,8 @@ static int list_capacity_update(tgx_context* ctx, uint32_t w, uint32_t h)
 /* Cameras per batch: bounded so that per-camera record slots stay modest. */
 static uint32_t batch_cams(const tgx_context* ctx, uint32_t n_cams)
 {
-    const uint64_t budget = 1ull << 28; /* 256 MiB of records */
-    uint64_t per = (uint64_t)ctx->n_cluster_pointers * sizeof(vis_cluster);
+    const uint64_t budget = 1ull << 32; /* 4 GiB of records, flags and keys */
+    uint64_t per = (uint64_t)rec_slots(ctx) * (sizeof(vis_cluster) + 12u);
     uint64_t b = per ? budget / per : n_cams;
     if (b < 1) b = 1;
     if (b > 1024) b = 1024; /* grid.y of the per-camera passes */
@@ -985,6 +1206,7 @@ int tgx_vis_cams_upload(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cam
 {
     int rc = check_args(ctx, cams, n_cams, w, h);
     if (rc) return rc;
+    if ((rc = tgx_vis_solid_refresh(ctx))) return rc;
     const uint32_t per = batch_cams(ctx, n_cams);
     const uint32_t ts = vis_tile_shift(ctx);
     const uint32_t tiles_per_cam = tiles_of(w, ts) * tiles_of(h, ts);
@@ -992,7 +1214,7 @@ int tgx_vis_cams_upload(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cam
     for (uint32_t i = 0; i < n_cams; i++) {
         const uint32_t j = i % per; /* index within its batch */
         make_cam(cams[i], vc[i]);
-        vc[i].rec_base = j * ctx->n_cluster_pointers;
+        vc[i].rec_base = j * rec_slots(ctx);
         vc[i].tile_base = j * tiles_per_cam;
     }
     void* d = nullptr;
@@ -1015,6 +1237,7 @@ int tgx_visibility_record(tgx_context* ctx, const void* dev_cams, const tgx_came
         return tgx_fail(ctx, TGX_ERR_NOT_BUILT, "record a frame only after one tgx_visibility_device call");
     hipLaunchKernelGGL(vis_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->vis_stats_dev.ptr, 8u, 0u);
     TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    if ((rc = tgx_vis_solid_refresh(ctx))) return rc;
     const uint32_t per = batch_cams(ctx, n_cams);
     const vis_cam* rc_cams = (const vis_cam*)dev_cams;
     for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
@@ -1040,6 +1263,7 @@ int tgx_visibility_device(tgx_context* ctx, const tgx_camera* cams, uint32_t n_c
         hipLaunchKernelGGL(vis_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->vis_stats_dev.ptr, 8u, 0u);
     }
     TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
+    if ((rc = tgx_vis_solid_refresh(ctx))) return rc;
     const uint32_t per = batch_cams(ctx, n_cams);
     for (uint32_t c0 = 0; c0 < n_cams; c0 += per) {
         const uint32_t n = (n_cams - c0) < per ? (n_cams - c0) : per;
@@ -1135,12 +1359,20 @@ static size_t align256(size_t x) { return (x + 255u) & ~(size_t)255u; }
 
 /* Scratch layout of one batch of n_cams cameras; returns the bytes before
  * the tile list. */
+static uint32_t noalloc_sums(const tgx_context* ctx, uint32_t n_cams, uint32_t tiles_per_cam)
+{
+    const uint32_t a = scan_groups(ctx->n_cluster_pointers) + 1u, t = scan_groups((uint64_t)tiles_per_cam * n_cams) + 1u;
+    return a > t ? a : t;
+}
+
 static size_t noalloc_fixed_bytes(const tgx_context* ctx, uint32_t n_cams, uint32_t tiles_per_cam)
 {
     const uint64_t slots = (uint64_t)ctx->n_cluster_pointers * n_cams;
     return align256(sizeof(vis_cam) * n_cams) + align256(ctx->n_clusters) +
-           align256(4ull * ctx->n_objects * n_cams) + align256(sizeof(vis_cluster) * slots) + align256(4 * slots) +
-           align256(8 * slots) + align256(4ull * (3 * n_cams + 8)) + 2 * align256(4ull * tiles_per_cam * n_cams) +
+           align256(12ull * ctx->n_cluster_pointers) + align256(8ull * ctx->n_objects) +
+           align256(4ull * noalloc_sums(ctx, n_cams, tiles_per_cam)) +
+           align256(12ull * ctx->n_objects * n_cams) + align256(sizeof(vis_cluster) * slots) + align256(4 * slots) +
+           align256(8 * slots) + align256(4ull * (4 * n_cams + 8)) + 2 * align256(4ull * tiles_per_cam * n_cams) +
            align256(64);
 }
 
@@ -1197,13 +1429,20 @@ int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint
         vis_bufs b;
         vis_cam* d_cams = (vis_cam*)carve(sizeof(vis_cam) * n);
         uint8_t* d_solid = (uint8_t*)carve(ctx->n_clusters);
+        uint32_t* d_occ = (uint32_t*)carve(12ull * n_cptr);
+        uint2* d_obj_occ = (uint2*)carve(8ull * ctx->n_objects);
+        uint32_t* d_sums = (uint32_t*)carve(4ull * noalloc_sums(ctx, n, tiles_per_cam));
+        const vis_occ_bufs ob = {d_occ, d_occ + n_cptr, d_occ + 2ull * n_cptr, d_obj_occ, d_sums};
         b.cams = d_cams;
-        b.solid = d_solid;
-        b.objcnt = (uint32_t*)carve(4ull * ctx->n_objects * n);
+        b.occ_list = d_occ + 2ull * n_cptr;
+        b.obj_occ = d_obj_occ;
+        b.slots = n_cptr;
+        b.scan_sums = d_sums;
+        b.objcnt = (uint32_t*)carve(12ull * ctx->n_objects * n);
         b.recs = (vis_cluster*)carve(sizeof(vis_cluster) * slots);
         b.rec_flags = (uint32_t*)carve(4 * slots);
         b.gkeys = (uint64_t*)carve(8 * slots);
-        b.counters = (uint32_t*)carve(4ull * (3 * n + 8));
+        b.counters = (uint32_t*)carve(4ull * (4 * n + 8));
         b.tile_count = (uint32_t*)carve(4ull * tiles_per_cam * n);
         b.tile_off = (uint32_t*)carve(4ull * tiles_per_cam * n);
         b.stats = (unsigned long long*)carve(64);
@@ -1214,11 +1453,13 @@ int tgx_visibility_device_noalloc(tgx_context* ctx, const tgx_camera* cams, uint
         b.timed = false;
         hipLaunchKernelGGL(vis_cams_k, dim3(1), dim3(64), 0, ctx->stream, pack, n, d_cams);
         TGX_LAUNCH_CHECK(ctx, "vis_cams_k");
-        /* The solid flags are derived from the scene's bits on every call:
-         * the array keeps whatever the last call on this GPU left. */
+        /* The solid flags and the occupied-pointer lists are derived from
+         * the scene on every call: the array keeps whatever the last call on
+         * this GPU left. */
         hipLaunchKernelGGL(vis_solid_k, dim3((ctx->n_clusters + 255u) / 256u), dim3(256), 0, ctx->stream,
                            (const uint4*)ctx->bits.ptr, ctx->n_clusters, d_solid);
         TGX_LAUNCH_CHECK(ctx, "vis_solid_k");
+        if ((rc = occ_launch(ctx, d_solid, ob))) return rc;
         hipLaunchKernelGGL(vis_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)b.stats, 16u, 0u);
         TGX_LAUNCH_CHECK(ctx, "vis_fill_u32");
         if ((rc = visibility_launch(ctx, b, n, w, h, out))) return rc;
