@@ -538,7 +538,7 @@ int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
 typedef struct tgx_tuning {
     uint32_t trace_sort;          /* bit 0: sort bounce rays, bit 1: sort shadow rays,
                                      by (direction octant, origin cell) before tracing (0) */
-    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (16) */
+    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (12) */
     uint32_t trace_blocks_per_cu; /* persistent 1024-lane trace workgroups per CU, 1..2 (2) */
     uint32_t trace_vote;          /* 0: every iteration runs node steps and DDA bursts for the
                                      lanes needing them; k in 1..64: a wave runs only DDA
@@ -617,12 +617,14 @@ int tgx_frame_graph_destroy(tgx_context* ctx, tgx_frame_graph* g);
  * untimed run) by coordinate descent over the tuning knobs: trace launch
  * form {0, 1, 2, 4}, empty-brick skip, trace workgroups per CU {1, 2},
  * uncounted DDA, node iterations per step {1, 2, 3, 4}, empty-brick
- * crossing {0, 24, 32, 40, 48}, refill thresholds {4, 8, 16, 24, 32}, visibility
+ * crossing {0, 24, 32, 40, 48}, refill thresholds {4, 8, 12, 16, 24, 32}, visibility
  * workgroups per camera {256, 1024, 2048, 4096} and visibility tile edge
  * {8, 16, 32}. Keeps the fastest in the context's tuning and returns it in
  * best_out (may be null). Every candidate gives the same results; cache
- * frames advance training once per run. While a frame graph exists,
- * candidates that would need larger buffers are skipped. */
+ * frames advance training once per run, so a cache that is still learning
+ * makes later candidates look faster (tune the plain form and keep its
+ * tuning for the cache form: the kernels are the same). While a frame
+ * graph exists, candidates that would need larger buffers are skipped. */
 int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint32_t w, uint32_t h,
                  const tgx_gi_params* gi, uint32_t flags, uint64_t* vis_dev, float* radiance_dev,
                  float* irradiance_dev, uint32_t frames, tgx_tuning* best_out);

@@ -41,15 +41,16 @@
  *                slots fixed by the list entry (gi_gen_tail_k then moves
  *                the queue's tail)
  *   gi_trace_k   bounce rays; a lane whose bounce hit faces the sun stores
- *                the albedo-weighted contribution and traces the shadow ray
- *                next, clearing the contribution if it is occluded
- *   gi_resolve_k per pixel, contributions summed in sample order
+ *                the sample record (weight, hit node and voxel) and traces
+ *                the shadow ray next, clearing the weight if it is occluded
+ *   gi_resolve_k per pixel, each sample's contribution (the hit voxel's
+ *                albedo times its weight) summed in sample order
  * With sorted shadow rays (tuning trace_sort bit 1) the shadow rays go
  * through a queue instead: gi_shade_k writes them, a second gi_trace_k
- * traces them and gi_lit_k stores the unshadowed contributions. Both forms
- * do the same fp32 operations per sample. Contributions are stored per
- * (pixel, sample) and summed in sample order, so the fp32 result equals a
- * sequential per-pixel loop bit for bit.
+ * traces them and gi_lit_k stores the unshadowed sample records. Both forms
+ * do the same fp32 operations per sample. Sample records are stored per
+ * (pixel, sample) and resolved and summed in sample order, so the fp32
+ * result equals a sequential per-pixel loop bit for bit.
  */
 #include "tgx_gi_dev.h"
 #include "tgx_m4.h"
@@ -681,9 +682,9 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
 
 /* Fused GI tracing (gi_trace_k<*, true>): the bounce queue traced with its
  * shadow rays in the same launch. When a lane's bounce ray hits a face that
- * faces the sun, the lane writes the sample's albedo-weighted contribution
- * and continues with the shadow ray itself; an occluded shadow ray clears
- * the contribution again. Same arithmetic as gi_shade_k + gi_lit_k. Ray
+ * faces the sun, the lane writes the sample record (weight, hit node and
+ * voxel) and continues with the shadow ray itself; an occluded shadow ray
+ * clears the weight again. Same arithmetic as gi_shade_k + gi_lit_k. Ray
  * ids at or above split count for the second counter block (stats1), the
  * others for stats0 (both may be the same block): bounce hits go to
  * stats[2], shadow rays to stats[3] and shadow hits to stats[7]. */
@@ -705,9 +706,7 @@ __device__ static inline uint16_t leaf_material(const leaf_mat_view& M, uint32_t
 }
 
 struct trace_fuse {
-    leaf_mat_view        mats;
-    const tgx_lut_entry* luts;
-    float*               contrib;
+    float*               contrib;     /* 3 floats per sample: weight, hit node, voxel */
     uint32_t*            stats0;
     uint32_t*            stats1;
     uint32_t             split;
@@ -807,11 +806,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             } else if (slot & TRACE_SHADOW) {
                 const uint32_t id = slot & ~TRACE_SHADOW;
                 if (res.flags & 1u) {
-                    /* occluded: the sample contributes nothing (three single
-                     * stores: a merged store would hold three zero registers
-                     * across the loop) */
-                    volatile float* c = F.contrib + 3ull * id;
-                    c[0] = 0.0f; c[1] = 0.0f; c[2] = 0.0f;
+                    /* occluded: the sample contributes nothing (weight 0) */
+                    F.contrib[3ull * id] = 0.0f;
                     ev = id >= F.split ? 32u : 4u;
                 }
             } else {
@@ -835,11 +831,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                     const float off = sign * GI_SHADOW_OFFSET;
                     if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
                     L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
-                    const uint16_t m = leaf_material(F.mats, res.node - P.n_inner, res.voxel);
-                    const tgx_lut_entry* e = F.luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
+                    /* the sample record: weight, hit node and voxel (the
+                     * albedo is applied when the samples are resolved) */
                     const float wgt = cosv * F.sun_radiance * GI_INV_PI;
                     float* c = F.contrib + 3ull * id;
-                    c[0] = e->albedo[0] * wgt; c[1] = e->albedo[1] * wgt; c[2] = e->albedo[2] * wgt;
+                    c[0] = wgt; c[1] = __uint_as_float(res.node); c[2] = __uint_as_float(res.voxel);
                     ev |= 2u << sh;
                     slot = id | TRACE_SHADOW;
                     pending = true;
@@ -894,9 +890,9 @@ struct rg_ctl {
 };
 
 /* Result of a lane's finished ray in the fused GI trace (same operations as
- * gi_trace_k<*, true>): a sun-facing bounce hit stores its contribution and
+ * gi_trace_k<*, true>): a sun-facing bounce hit stores its sample record and
  * turns the lane into its shadow ray (pending set-up); an occluded shadow
- * ray clears its contribution. Returns the lane's event bits (1 bounce hit,
+ * ray clears its weight. Returns the lane's event bits (1 bounce hit,
  * 2 shadow ray, 4 shadow hit; << 3 for the second list). */
 __device__ static inline uint32_t fused_result(const trace_params& P, const trace_fuse& F, trace_lane& L,
                                                const svo_res& res, uint32_t& slot, float ox, float oy, float oz,
@@ -906,8 +902,7 @@ __device__ static inline uint32_t fused_result(const trace_params& P, const trac
     if (slot & TRACE_SHADOW) {
         const uint32_t id = slot & ~TRACE_SHADOW;
         if (res.flags & 1u) {
-            volatile float* c = F.contrib + 3ull * id;
-            c[0] = 0.0f; c[1] = 0.0f; c[2] = 0.0f;
+            F.contrib[3ull * id] = 0.0f;
             ev = id >= F.split ? 32u : 4u;
         }
         return ev;
@@ -925,11 +920,9 @@ __device__ static inline uint32_t fused_result(const trace_params& P, const trac
         const float off = sign * GI_SHADOW_OFFSET;
         if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
         L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
-        const uint16_t m = leaf_material(F.mats, res.node - P.n_inner, res.voxel);
-        const tgx_lut_entry* e = F.luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
         const float wgt = cosv * F.sun_radiance * GI_INV_PI;
         float* c = F.contrib + 3ull * sid;
-        c[0] = e->albedo[0] * wgt; c[1] = e->albedo[1] * wgt; c[2] = e->albedo[2] * wgt;
+        c[0] = wgt; c[1] = __uint_as_float(res.node); c[2] = __uint_as_float(res.voxel);
         ev |= 2u << sh;
         slot = sid | TRACE_SHADOW;
         pending = true;
@@ -1287,8 +1280,8 @@ __device__ static inline float shade_cos(const svo_res& r, const tgx_gi_params& 
     return sign * sc;
 }
 
-/* Bounce hits that face the sun: albedo-weighted contribution and a shadow
- * ray (started 1/64 voxel off the hit face). SHADE_ITEMS rays per thread.
+/* Bounce hits that face the sun: the sample record (weight, hit node and
+ * voxel) and a shadow ray (started 1/64 voxel off the hit face). SHADE_ITEMS rays per thread.
  * With two lists in the queue (stats0 and stats1 not null), each list's
  * bounce hits and shadow rays are counted apart; ids below split belong to
  * the first. */
@@ -1336,17 +1329,15 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_shade_k(uint32_t n_inner, leaf_ma
         if (ax == 0u) ox = ox + off; else if (ax == 1u) oy = oy + off; else oz = oz + off;
         const svo_ray sr = {ox, oy, oz, prm.sun_dir[0], prm.sun_dir[1], prm.sun_dir[2], b.id, slot};
         shadow[slot] = sr;
-        const uint16_t m = leaf_material(mats, r.node - n_inner, r.voxel);
-        const tgx_lut_entry e = luts[(size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u)];
         const float wgt = cosv * prm.sun_radiance * GI_INV_PI;
-        weight[3ull * slot] = e.albedo[0] * wgt;
-        weight[3ull * slot + 1] = e.albedo[1] * wgt;
-        weight[3ull * slot + 2] = e.albedo[2] * wgt;
+        weight[3ull * slot] = wgt;
+        weight[3ull * slot + 1] = __uint_as_float(r.node);
+        weight[3ull * slot + 2] = __uint_as_float(r.voxel);
         slot++;
     }
 }
 
-/* Unshadowed samples store their contribution at (entry, sample). */
+/* Unshadowed samples store their sample record at (entry, sample). */
 __global__ __launch_bounds__(256) void gi_lit_k(const svo_ray* __restrict__ shadow, const uint32_t* __restrict__ n_shadow,
                                                 const svo_res* __restrict__ res, const float* __restrict__ weight,
                                                 float* __restrict__ contrib)
@@ -1358,12 +1349,28 @@ __global__ __launch_bounds__(256) void gi_lit_k(const svo_ray* __restrict__ shad
     contrib[c] = weight[w]; contrib[c + 1] = weight[w + 1]; contrib[c + 2] = weight[w + 2];
 }
 
+/* A sample's contribution from its record (weight, hit node, voxel): the
+ * albedo of the hit voxel's material times the weight, 0 for weight 0. */
+__device__ static inline void sample_value(const float* rec, uint32_t n_inner, const leaf_mat_view& mats,
+                                           const tgx_lut_entry* __restrict__ luts, float& v0, float& v1, float& v2)
+{
+    const float wgt = rec[0];
+    v0 = 0.0f; v1 = 0.0f; v2 = 0.0f;
+    if (wgt != 0.0f) {
+        const uint16_t m = leaf_material(mats, __float_as_uint(rec[1]) - n_inner, __float_as_uint(rec[2]));
+        const tgx_lut_entry* e = luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
+        v0 = e->albedo[0] * wgt; v1 = e->albedo[1] * wgt; v2 = e->albedo[2] * wgt;
+    }
+}
+
 /* Per list entry: E = (sum of its sample contributions in sample order) /
  * spp, radiance = albedo * E. */
 __global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf_rec* __restrict__ surf,
                                                     const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ count_ptr,
                                                     const float* __restrict__ contrib, uint32_t spp,
+                                                    uint32_t n_inner, leaf_mat_view mats,
+                                                    const tgx_lut_entry* __restrict__ luts,
                                                     float* __restrict__ e_entry, float* __restrict__ e_var,
                                                     float* __restrict__ radiance, float* __restrict__ irradiance)
 {
@@ -1372,14 +1379,20 @@ __global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf_rec* __restric
     const uint32_t pix = list[i];
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     const float* c = contrib + 3ull * (size_t)i * spp;
-    for (uint32_t k = 0; k < spp; k++) { a0 = a0 + c[3 * k]; a1 = a1 + c[3 * k + 1]; a2 = a2 + c[3 * k + 2]; }
+    for (uint32_t k = 0; k < spp; k++) {
+        float v0, v1, v2;
+        sample_value(c + 3 * k, n_inner, mats, luts, v0, v1, v2);
+        a0 = a0 + v0; a1 = a1 + v1; a2 = a2 + v2;
+    }
     const float inv = 1.0f / (float)spp;
     const float e0 = a0 * inv, e1 = a1 * inv, e2 = a2 * inv;
     if (e_entry) { e_entry[3ull * i] = e0; e_entry[3ull * i + 1] = e1; e_entry[3ull * i + 2] = e2; }
     if (e_var) {
         float q = 0.0f;
         for (uint32_t k = 0; k < spp; k++) {
-            const float x0 = c[3 * k] - e0, x1 = c[3 * k + 1] - e1, x2 = c[3 * k + 2] - e2;
+            float v0, v1, v2;
+            sample_value(c + 3 * k, n_inner, mats, luts, v0, v1, v2);
+            const float x0 = v0 - e0, x1 = v1 - e1, x2 = v2 - e2;
             q += x0 * x0 + x1 * x1 + x2 * x2;
         }
         e_var[i] = spp > 1u ? q / ((float)spp * (float)(spp - 1u)) : 0.0f;
@@ -1734,8 +1747,6 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
     if (!(ctx->tuning.trace_sort & 2u)) {
         /* bounce and shadow rays in one launch */
         trace_fuse F = {};
-        F.mats = leaf_mats_of(ctx);
-        F.luts = (const tgx_lut_entry*)ctx->luts.ptr;
         F.contrib = contrib;
         F.stats0 = two ? sets[0].counters : cnt;
         F.stats1 = two ? sets[1].counters : cnt;
@@ -1782,7 +1793,8 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         const gi_surf_rec* surf = t.surf ? t.surf : (const gi_surf_rec*)ctx->gi_surf.ptr;
         tgx_launch_scope s(ctx, "gi_resolve_k");
         hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(t.max_count)), dim3(256), 0, ctx->stream, surf, t.list,
-                           t.count_dev, contrib + 3ull * id_base[k], t.prm.spp, t.e_entry, t.e_var, t.radiance_dev,
+                           t.count_dev, contrib + 3ull * id_base[k], t.prm.spp, ctx->svo_n_inner, leaf_mats_of(ctx),
+                           (const tgx_lut_entry*)ctx->luts.ptr, t.e_entry, t.e_var, t.radiance_dev,
                            t.irradiance_dev);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_resolve_k");
