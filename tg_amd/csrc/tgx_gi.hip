@@ -515,6 +515,8 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
         const bool small_td = (sx == 0 || tdx < 0x1p100f) && (sy == 0 || tdy < 0x1p100f) &&
                               (sz == 0 || tdz < 0x1p100f);
         if (kDda != 0u && !kSkip && __all(small_td || !in_dda)) {
+            /* the count is not kept on this path (see above) */
+            it = 0u;
 #pragma unroll 1
             for (uint32_t k = 0; k < burst; k += 2) {
                 if constexpr (kDda == 2u) {
