@@ -182,6 +182,7 @@ struct trace_lane {
 #define ND_RC(n)  (((n) >> 24) & 7u)
 #define ND_DDA    (1u << 27)
 #define ND_ADV    (1u << 28)   /* an advance out of the child slot is pending */
+#define ND_SETUP  (1u << 29)   /* entered a leaf: its DDA is set up at the lane's next burst */
 
 __device__ static inline svo_res res_miss(uint32_t steps)
 {
@@ -460,20 +461,12 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
                     const uint32_t leaf = ci - P.n_inner;
                     const unsigned long long bm = bricks_at<kLds>(P.bricks, s_inner, P.n_inner, leaf);
                     if (bm != 0ull) {
-                        /* tg_amanatides_woo setup over the 32^3 block, entry voxel clamped */
-                        const float cmx = (rc & 1u) ? hx : pmx, cmy = (rc & 2u) ? hy : pmy, cmz = (rc & 4u) ? hz : pmz;
-                        const float gx = L.px - cmx, gy = L.py - cmy, gz = L.pz - cmz;
-                        const int x = (int)fmin2(fmax2(floorf(gx), 0.0f), 31.0f);
-                        const int y = (int)fmin2(fmax2(floorf(gy), 0.0f), 31.0f);
-                        const int z = (int)fmin2(fmax2(floorf(gz), 0.0f), 31.0f);
-                        L.tmx = dda_len((float)(x + 1) - gx, gx - (float)x, L.dx);
-                        L.tmy = dda_len((float)(y + 1) - gy, gy - (float)y, L.dy);
-                        L.tmz = dda_len((float)(z + 1) - gz, gz - (float)z, L.dz);
-                        L.vox = (uint32_t)x | ((uint32_t)y << 5) | ((uint32_t)z << 10);
+                        /* the DDA over the 32^3 block starts at the lane's
+                         * next burst (its set-up runs once per step for all
+                         * lanes that entered a leaf) */
                         L.bm = bm;
                         L.leaf = leaf;
-                        L.wi = 0xFFFFFFFFu;
-                        L.node |= ND_DDA;
+                        L.node |= ND_DDA | ND_SETUP;
                         adv = false;
                     }
                 }
@@ -485,6 +478,24 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
      * (one __any exit) and lanes that stop are predicated off, which keeps
      * the scalar unit free of per-lane exit bookkeeping. */
     const bool in_dda = (L.node & ND_DDA) && allow_dda;
+    if (__any(in_dda && (L.node & ND_SETUP))) {
+        if (in_dda && (L.node & ND_SETUP)) {
+            /* tg_amanatides_woo setup over the 32^3 block (the child slot's
+             * box), entry voxel clamped */
+            float cmx, cmy, cmz;
+            child_min(L.node, ND_RC(L.node), half, 32.0f, cmx, cmy, cmz);
+            const float gx = L.px - cmx, gy = L.py - cmy, gz = L.pz - cmz;
+            const int x0 = (int)fmin2(fmax2(floorf(gx), 0.0f), 31.0f);
+            const int y0 = (int)fmin2(fmax2(floorf(gy), 0.0f), 31.0f);
+            const int z0 = (int)fmin2(fmax2(floorf(gz), 0.0f), 31.0f);
+            L.tmx = dda_len((float)(x0 + 1) - gx, gx - (float)x0, L.dx);
+            L.tmy = dda_len((float)(y0 + 1) - gy, gy - (float)y0, L.dy);
+            L.tmz = dda_len((float)(z0 + 1) - gz, gz - (float)z0, L.dz);
+            L.vox = (uint32_t)x0 | ((uint32_t)y0 << 5) | ((uint32_t)z0 << 10);
+            L.wi = 0xFFFFFFFFu;
+            L.node &= ~ND_SETUP;
+        }
+    }
     bool hit = false, leave = false;
     int x = (int)(L.vox & 31u), y = (int)((L.vox >> 5) & 31u), z = (int)((L.vox >> 10) & 31u);
     uint32_t it = L.vox >> 15;
