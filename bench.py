@@ -373,11 +373,13 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
     plain_rays = None
     for mode, flags in (("plain", 0), ("neural", T.FRAME_NEURAL)):
         if flags:
+            # 16 Adam steps per frame: the batch trains for only `pretrain`
+            # frames before it is timed
             ctx.nrc_create(T.nrc_config(origin=(0.0, 0.0, 0.0), scale=1.0 / BATCH_WORLD, train_samples=1 << 16,
-                                        train_batch=NRC_TRAIN_BATCH, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
+                                        train_batch=NRC_SIDE_BATCH, train_spp=4, fallback_threshold=NRC_FALLBACK_THRESHOLD,
                                         fallback_min_samples=NRC_FALLBACK_MIN_SAMPLES, cell_size=8,
                                         flags=T.NRC_FLAG_FALLBACK_VS_TRACER | T.NRC_FLAG_JOINT_TRAINING,
-                                        ema_decay=NRC_EMA_DECAY))
+                                        ema_decay=NRC_SIDE_EMA_DECAY))
         if not flags:
             # the cache learns while candidates run, so their times are not
             # comparable: the cache form keeps the plain form's tuning (the
