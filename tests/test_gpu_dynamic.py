@@ -76,6 +76,18 @@ def test_create_destroy_sequence_matches_model(ctx):
             ref, _ = O.visibility_culled(O.OracleScene(model.scene()), cam, 96, 64)
             assert np.array_equal(got, ref), f"step {step}: visibility differs from the oracle"
     assert model.n_cluster_pointers > 0
+    # a destroy right after a visibility call moves the pointers the
+    # visibility pass listed: the next call must see the new layout
+    assert len(alive) >= 2
+    ctx.visibility(cam, 96, 64)
+    o = alive.pop(0)
+    say(f"destroy object {o} right after a visibility call")
+    ctx.object_destroy(o)
+    model.destroy(o)
+    got = ctx.visibility(cam, 96, 64)[0]
+    ref, _ = O.visibility_culled(O.OracleScene(model.scene()), cam, 96, 64)
+    assert np.array_equal(got, ref), "visibility after a destroy differs from the oracle"
+    assert model.n_cluster_pointers > 0
     mn = np.array([-64.0, -64.0, -128.0], np.float32)
     ctx.svo_build(mn, 8)
     ref = O.OracleSvo(O.OracleScene(model.scene()), mn, 8).export()

@@ -176,6 +176,9 @@ int tgx_object_destroy(tgx_context* ctx, uint32_t o)
     if (!ctx->dynamic) return tgx_fail(ctx, TGX_ERR_NO_SCENE, "no dynamic scene: call tgx_scene_reserve first");
     if (ctx->n_graphs > 0) return tgx_fail(ctx, TGX_ERR_CAPACITY, "a frame graph holds the scene: destroy it first");
     if (o >= ctx->dyn_objects.size()) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "object %u out of range", o);
+    /* cluster pointers and objects move: the visibility pass's
+     * occupied-pointer lists are rebuilt */
+    ctx->scene_version++;
     tgx_object& ob = ctx->dyn_objects[o];
     const uint32_t n = ob.n_cluster_pointers_per_dim[0] * ob.n_cluster_pointers_per_dim[1] * ob.n_cluster_pointers_per_dim[2];
     if (n == 0) return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "object %u is not alive", o);
