@@ -1235,11 +1235,12 @@ __global__ void gi_list_k(const uint32_t* __restrict__ flag, const uint32_t* __r
     if (i == n - 1) *count = off[i] + flag[i];
 }
 
-/* spp cosine bounce rays per listed pixel, from p + sqrt(3) n; ray id =
- * id_base + list entry * spp + sample (id_base separates the lists that
- * share a queue). Entry i's rays take the queue slots after the tail
- * counters[1] had at launch, so the launch needs no atomics;
- * gi_gen_tail_k then moves the tail and the list's ray count. */
+/* spp cosine bounce rays per listed pixel, from p + sqrt(3) n, one thread
+ * per (entry, sample); ray id = id_base + list entry * spp + sample
+ * (id_base separates the lists that share a queue). Entry i's rays take the
+ * queue slots after the tail counters[1] had at launch, so the launch needs
+ * no atomics and a wave writes 64 consecutive rays; gi_gen_tail_k then
+ * moves the tail and the list's ray count. */
 __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restrict__ surf,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count_ptr, tgx_gi_params prm,
@@ -1250,24 +1251,22 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restri
 {
     /* a device counter can advance the frame of replayed work */
     if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + salt_add + 1u);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = j / prm.spp, k = j - i * prm.spp;
     if (i >= *count_ptr) return;
     /* entry i's samples take slots [tail + i spp, + spp) after the queue's
      * tail at launch; gi_gen_tail_k advances the tail afterwards */
-    const uint32_t first = counters[1] + i * prm.spp;
     const uint32_t pix = list[i];
     const gi_surf_rec sr = surf[pix];
     const f3 n = {sr.n[0], sr.n[1], sr.n[2]};
     const f3 ob = {sr.p[0] + GI_SQRT3 * n.x, sr.p[1] + GI_SQRT3 * n.y, sr.p[2] + GI_SQRT3 * n.z};
-    for (uint32_t k = 0; k < prm.spp; k++) {
-        uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
-        if (st == 0u) st = 0x6D2B79F5u;
-        const f3 d = gi_cosine_direction(n, st);
-        const svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, id_base + i * prm.spp + k, 0u};
-        rays[first + k] = r;
-        const size_t c = 3ull * ((size_t)id_base + (size_t)i * prm.spp + k);
-        contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
-    }
+    uint32_t st = gi_fmix32(pix * prm.spp + k + 0x9E3779B9u * (prm.frame + 1u));
+    if (st == 0u) st = 0x6D2B79F5u;
+    const f3 d = gi_cosine_direction(n, st);
+    const svo_ray r = {ob.x, ob.y, ob.z, d.x, d.y, d.z, id_base + j, 0u};
+    rays[counters[1] + j] = r;
+    const size_t c = 3ull * ((size_t)id_base + (size_t)j);
+    contrib[c] = 0.0f; contrib[c + 1] = 0.0f; contrib[c + 2] = 0.0f;
 }
 
 /* Advances the bounce queue's tail (and the list's generated-ray count)
@@ -1743,7 +1742,8 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         const gi_surf_rec* surf = t.surf ? t.surf : (const gi_surf_rec*)ctx->gi_surf.ptr;
         {
             tgx_launch_scope s(ctx, "gi_gen_k");
-            hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for(t.max_count)), dim3(GI_BLOCK), 0, ctx->stream, surf, t.list,
+            hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for((uint64_t)t.max_count * t.prm.spp)), dim3(GI_BLOCK), 0,
+                               ctx->stream, surf, t.list,
                                t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k], t.salt_add);
         }
         hipLaunchKernelGGL(gi_gen_tail_k, dim3(1), dim3(1), 0, ctx->stream, t.count_dev, t.max_count, t.prm.spp, cnt,
