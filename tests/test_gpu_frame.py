@@ -327,3 +327,22 @@ def test_joint_training_learns_every_camera(ctx):
         e_nrc = N.rel_mse(irr[c][m], refs[c][m])
         say(f"camera {c}: rel-MSE plain 4 spp {e_plain:.3f}, jointly trained cache {e_nrc:.3f}")
         assert e_nrc < 0.15 * e_plain
+
+
+def test_tuning_rejects_out_of_range_values(ctx):
+    """tgx_tuning_set refuses values outside each knob's documented range
+    and leaves the tuning as it was."""
+    before = ctx.tuning().copy()
+    bad = [("trace_refill", 0), ("trace_refill", 65), ("trace_blocks_per_cu", 3), ("trace_sort", 4),
+           ("trace_regroup", 9), ("trace_skip", 2), ("vis_setup_blocks", 32), ("vis_setup_blocks", 20000),
+           ("trace_dda", 2), ("vis_tile", 12), ("vis_tile", 64), ("trace_cross", 65), ("trace_node", 0),
+           ("trace_node", 5), ("trace_burst", 3), ("trace_burst", 34), ("trace_burst", 0)]
+    for name, value in bad:
+        with pytest.raises(T.TgxError) as e:
+            ctx.set_tuning(**{name: value})
+        assert e.value.code == 1, name
+        assert ctx.tuning().tobytes() == before.tobytes(), name
+    ctx.set_tuning(vis_tile=8, trace_node=4, trace_burst=32, trace_cross=64)
+    t = ctx.tuning()
+    assert (t["vis_tile"][0], t["trace_node"][0], t["trace_burst"][0], t["trace_cross"][0]) == (8, 4, 32, 64)
+    ctx.set_tuning(**{k: int(before[k][0]) for k in before.dtype.names})
