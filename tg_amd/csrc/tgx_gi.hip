@@ -193,15 +193,17 @@ __device__ static inline svo_res res_miss(uint32_t steps)
 /* Ray setup (tg_sparse_voxel_octree.c:570-594) for the ray whose world-space
  * origin and direction the lane holds in px, py, pz and dx, dy, dz: false
  * if the root box is missed (the result is then a miss with 0 steps). */
-__device__ static inline bool lane_init_self(const trace_params& P, float half, trace_lane& L)
+__device__ static inline bool lane_init_self(const trace_params& P, float half, trace_lane& L, bool have_td = false)
 {
     const float ox = L.px - P.cx, oy = L.py - P.cy, oz = L.pz - P.cz;
     L.steps = 0;
     L.cur = 0;
     L.node = 0;
-    L.tdx = dda_len(1.0f, 1.0f, L.dx);
-    L.tdy = dda_len(1.0f, 1.0f, L.dy);
-    L.tdz = dda_len(1.0f, 1.0f, L.dz);
+    if (!have_td) {
+        L.tdx = dda_len(1.0f, 1.0f, L.dx);
+        L.tdy = dda_len(1.0f, 1.0f, L.dy);
+        L.tdz = dda_len(1.0f, 1.0f, L.dz);
+    }
     if (__all(fabsf(ox) < half && fabsf(oy) < half && fabsf(oz) < half)) {
         /* origins strictly inside the root box: every near-plane quotient is
          * <= 0 and the far one > 0, so the slab test below would keep the
@@ -724,6 +726,7 @@ struct trace_fuse {
     uint32_t*            stats1;
     uint32_t             split;
     float                sun0, sun1, sun2;
+    float                sun_td0, sun_td1, sun_td2;   /* 1 / |sun component| (F32_MAX for 0) */
     float                sun_radiance;
 };
 
@@ -787,7 +790,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
         }
         if (pending) {
             pending = false;
-            active = lane_init_self(P, half, L);
+            /* a shadow ray's step lengths are the sun's, set with its ray */
+            active = lane_init_self(P, half, L, kFused && (slot & TRACE_SHADOW));
             if (!kFused && !active) out[slot] = res_miss(0u);
         }
         if (!__any(active || pending)) {
@@ -844,6 +848,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                     const float off = sign * GI_SHADOW_OFFSET;
                     if (ax == 0u) L.px = L.px + off; else if (ax == 1u) L.py = L.py + off; else L.pz = L.pz + off;
                     L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
+                    L.tdx = F.sun_td0; L.tdy = F.sun_td1; L.tdz = F.sun_td2;
                     /* the sample record: weight, hit node and voxel (the
                      * albedo is applied when the samples are resolved) */
                     const float wgt = cosv * F.sun_radiance * GI_INV_PI;
@@ -1765,6 +1770,13 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         F.stats1 = two ? sets[1].counters : cnt;
         F.split = two ? id_base[1] : 0xFFFFFFFFu;
         F.sun0 = prm.sun_dir[0]; F.sun1 = prm.sun_dir[1]; F.sun2 = prm.sun_dir[2];
+        /* the step lengths a shadow ray's set-up would divide out, in the
+         * same IEEE single precision */
+        for (int a = 0; a < 3; a++) {
+            const float d = prm.sun_dir[a];
+            const float td = d == 0.0f ? SVO_F32_MAX : 1.0f / (d > 0.0f ? d : -d);
+            (a == 0 ? F.sun_td0 : a == 1 ? F.sun_td1 : F.sun_td2) = td;
+        }
         F.sun_radiance = prm.sun_radiance;
         if (ctx->tuning.trace_regroup) {
             if ((rc = launch_trace_rg(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, F,
