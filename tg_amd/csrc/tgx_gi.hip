@@ -699,10 +699,9 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
  * shadow rays in the same launch. When a lane's bounce ray hits a face that
  * faces the sun, the lane writes the sample record (weight, hit node and
  * voxel) and continues with the shadow ray itself; an occluded shadow ray
- * clears the weight again. Same arithmetic as gi_shade_k + gi_lit_k. Ray
- * ids at or above split count for the second counter block (stats1), the
- * others for stats0 (both may be the same block): bounce hits go to
- * stats[2], shadow rays to stats[3] and shadow hits to stats[7]. */
+ * clears the weight again. Same arithmetic as gi_shade_k + gi_lit_k. The
+ * records carry the flags gi_resolve_k counts bounce hits, shadow rays and
+ * shadow hits from. */
 /* Leaf materials are stored for solid voxels only, leaf by leaf in bit
  * order: solid voxel v of leaf l is entry mat_off[l] + (solid voxels of the
  * leaf's earlier words, wpre) + (solid voxels below v in its word). */
@@ -721,16 +720,24 @@ __device__ static inline uint16_t leaf_material(const leaf_mat_view& M, uint32_t
 }
 
 struct trace_fuse {
-    float*               contrib;     /* 3 floats per sample: weight, hit node, voxel */
-    uint32_t*            stats0;
-    uint32_t*            stats1;
-    uint32_t             split;
+    float*               contrib;     /* 3 floats per sample: weight, hit node, voxel | flags */
     float                sun0, sun1, sun2;
     float                sun_td0, sun_td1, sun_td2;   /* 1 / |sun component| (F32_MAX for 0) */
     float                sun_radiance;
 };
 
 #define TRACE_SHADOW 0x80000000u   /* slot of a lane tracing a shadow ray: id | TRACE_SHADOW */
+
+/* Sample records of the GI trace: (weight, hit node, hit voxel | flags).
+ * A bounce miss leaves (0, 0, 0); a hit sets SAMPLE_HIT; a sun-facing hit
+ * writes its weight, node and voxel with SAMPLE_HIT | SAMPLE_LIT (its
+ * shadow ray is traced); an occluded shadow ray clears the weight and sets
+ * SAMPLE_OCC. The resolve pass of the fused trace counts bounce hits,
+ * shadow rays and shadow hits from the flags. */
+#define SAMPLE_HIT 0x80000000u
+#define SAMPLE_LIT 0x40000000u
+#define SAMPLE_OCC 0x20000000u
+#define SAMPLE_VOXEL 0x1FFFFFFFu
 
 /* Persistent traversal of queue rays[0 .. *n_rays) up to distance maxd.
  * Plain tracing (kFused false): the result of ray i goes to out[i] and hits
@@ -761,8 +768,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
     const float half = P.half;
     trace_lane L = {};
     uint32_t slot = 0;
-    /* wave totals: bounce hits, shadow rays and shadow hits per list */
-    uint32_t n_hit0 = 0, n_hit1 = 0, n_sh0 = 0, n_sh1 = 0, n_shhit0 = 0, n_shhit1 = 0;
+    /* the wave's bounce hits (plain trace) */
+    uint32_t n_hit0 = 0;
     /* pending: the lane holds a ray (world origin in px..pz, direction in
      * dx..dz) that is set up at the next iteration's start */
     bool active = false, exhausted = false, pending = false;
@@ -825,14 +832,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                 if (res.flags & 1u) {
                     /* occluded: the sample contributes nothing (weight 0) */
                     F.contrib[3ull * id] = 0.0f;
-                    ev = id >= F.split ? 32u : 4u;
+                    F.contrib[3ull * id + 2] = __uint_as_float(SAMPLE_HIT | SAMPLE_LIT | SAMPLE_OCC);
                 }
             } else {
                 /* the ray's origin and id from LDS; its direction is L.dx..dz */
                 const float4 o = s_ray[lane];
                 const uint32_t id = __float_as_uint(o.w);
-                const uint32_t sh = id >= F.split ? 3u : 0u;
-                ev = (res.flags & 1u) << sh;
                 /* cosine of the hit face with the sun (gi_shade_k) */
                 const uint32_t ax = (res.flags >> 2) & 3u;
                 const float sign = (res.flags & 16u) ? 1.0f : -1.0f;
@@ -853,33 +858,20 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                      * albedo is applied when the samples are resolved) */
                     const float wgt = cosv * F.sun_radiance * GI_INV_PI;
                     float* c = F.contrib + 3ull * id;
-                    c[0] = wgt; c[1] = __uint_as_float(res.node); c[2] = __uint_as_float(res.voxel);
-                    ev |= 2u << sh;
+                    c[0] = wgt; c[1] = __uint_as_float(res.node); c[2] = __uint_as_float(res.voxel | SAMPLE_HIT | SAMPLE_LIT);
                     slot = id | TRACE_SHADOW;
                     pending = true;
+                } else if (res.flags & 1u) {
+                    F.contrib[3ull * id + 2] = __uint_as_float(SAMPLE_HIT);
                 }
             }
         }
-        if (__any(ev != 0u)) {
-            n_hit0 += (uint32_t)__popcll(__ballot(ev & 1u));
-            n_sh0 += (uint32_t)__popcll(__ballot(ev & 2u));
-            n_shhit0 += (uint32_t)__popcll(__ballot(ev & 4u));
-            n_hit1 += (uint32_t)__popcll(__ballot(ev & 8u));
-            n_sh1 += (uint32_t)__popcll(__ballot(ev & 16u));
-            n_shhit1 += (uint32_t)__popcll(__ballot(ev & 32u));
+        if constexpr (!kFused) {
+            if (__any(ev != 0u)) n_hit0 += (uint32_t)__popcll(__ballot(ev & 1u));
         }
     }
-    if (lane == 0) {
-        if constexpr (!kFused) {
-            if (hit_counter && n_hit0) atomicAdd(hit_counter, n_hit0);
-        } else {
-            if (n_hit0) atomicAdd(&F.stats0[2], n_hit0);
-            if (n_sh0) atomicAdd(&F.stats0[3], n_sh0);
-            if (n_shhit0) atomicAdd(&F.stats0[7], n_shhit0);
-            if (n_hit1) atomicAdd(&F.stats1[2], n_hit1);
-            if (n_sh1) atomicAdd(&F.stats1[3], n_sh1);
-            if (n_shhit1) atomicAdd(&F.stats1[7], n_shhit1);
-        }
+    if constexpr (!kFused) {
+        if (lane == 0 && hit_counter && n_hit0) atomicAdd(hit_counter, n_hit0);
     }
 }
 
@@ -909,24 +901,20 @@ struct rg_ctl {
 
 /* Result of a lane's finished ray in the fused GI trace (same operations as
  * gi_trace_k<*, true>): a sun-facing bounce hit stores its sample record and
- * turns the lane into its shadow ray (pending set-up); an occluded shadow
- * ray clears its weight. Returns the lane's event bits (1 bounce hit,
- * 2 shadow ray, 4 shadow hit; << 3 for the second list). */
-__device__ static inline uint32_t fused_result(const trace_params& P, const trace_fuse& F, trace_lane& L,
+ * turns the lane into its shadow ray (pending set-up); another hit stores
+ * flags SAMPLE_HIT; an occluded shadow ray clears its weight and sets SAMPLE_OCC. */
+__device__ static inline void fused_result(const trace_params& P, const trace_fuse& F, trace_lane& L,
                                                const svo_res& res, uint32_t& slot, float ox, float oy, float oz,
                                                uint32_t sid, bool& pending)
 {
-    uint32_t ev = 0u;
     if (slot & TRACE_SHADOW) {
         const uint32_t id = slot & ~TRACE_SHADOW;
         if (res.flags & 1u) {
             F.contrib[3ull * id] = 0.0f;
-            ev = id >= F.split ? 32u : 4u;
+            F.contrib[3ull * id + 2] = __uint_as_float(SAMPLE_HIT | SAMPLE_LIT | SAMPLE_OCC);
         }
-        return ev;
+        return;
     }
-    const uint32_t sh = sid >= F.split ? 3u : 0u;
-    ev = (res.flags & 1u) << sh;
     const uint32_t ax = (res.flags >> 2) & 3u;
     const float sign = (res.flags & 16u) ? 1.0f : -1.0f;
     const float sc = ax == 0u ? F.sun0 : (ax == 1u ? F.sun1 : F.sun2);
@@ -940,12 +928,12 @@ __device__ static inline uint32_t fused_result(const trace_params& P, const trac
         L.dx = F.sun0; L.dy = F.sun1; L.dz = F.sun2;
         const float wgt = cosv * F.sun_radiance * GI_INV_PI;
         float* c = F.contrib + 3ull * sid;
-        c[0] = wgt; c[1] = __uint_as_float(res.node); c[2] = __uint_as_float(res.voxel);
-        ev |= 2u << sh;
+        c[0] = wgt; c[1] = __uint_as_float(res.node); c[2] = __uint_as_float(res.voxel | SAMPLE_HIT | SAMPLE_LIT);
         slot = sid | TRACE_SHADOW;
         pending = true;
+    } else if (res.flags & 1u) {
+        F.contrib[3ull * sid + 2] = __uint_as_float(SAMPLE_HIT);
     }
-    return ev;
 }
 
 template <bool kLds, bool kSkip>
@@ -971,7 +959,6 @@ __global__ __launch_bounds__(RG_BLOCK, 1) void gi_trace_rg_k(trace_params P, con
     trace_lane L = {};
     uint32_t slot = 0, sid = 0;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-    uint32_t n_hit0 = 0, n_hit1 = 0, n_sh0 = 0, n_sh1 = 0, n_shhit0 = 0, n_shhit1 = 0;
     bool active = false, pending = false;
     while (true) {
         /* refill: idle lanes sit in the last waves after a regroup */
@@ -1001,7 +988,6 @@ __global__ __launch_bounds__(RG_BLOCK, 1) void gi_trace_rg_k(trace_params P, con
         /* one step of each lane's phase; node-walk lanes take up to
          * node_iters steps while the wave has node work */
         const bool was_dda = active && (L.node & ND_DDA);
-        uint32_t ev = 0u;
         for (uint32_t k = 0; k < node_iters; k++) {
             const bool step_dda = k == 0 && was_dda;
             const bool step_node = active && !was_dda && !(L.node & ND_DDA);
@@ -1011,16 +997,8 @@ __global__ __launch_bounds__(RG_BLOCK, 1) void gi_trace_rg_k(trace_params P, con
                                        step_dda, true) &&
                 active && (step_dda || step_node)) {
                 active = false;
-                ev |= fused_result(P, F, L, res, slot, ox, oy, oz, sid, pending);
+                fused_result(P, F, L, res, slot, ox, oy, oz, sid, pending);
             }
-        }
-        if (__any(ev != 0u)) {
-            n_hit0 += (uint32_t)__popcll(__ballot(ev & 1u));
-            n_sh0 += (uint32_t)__popcll(__ballot(ev & 2u));
-            n_shhit0 += (uint32_t)__popcll(__ballot(ev & 4u));
-            n_hit1 += (uint32_t)__popcll(__ballot(ev & 8u));
-            n_sh1 += (uint32_t)__popcll(__ballot(ev & 16u));
-            n_shhit1 += (uint32_t)__popcll(__ballot(ev & 32u));
         }
         /* regroup: DDA lanes first, then node-walk lanes (pending shadow
          * rays included), then idle lanes */
@@ -1078,14 +1056,6 @@ __global__ __launch_bounds__(RG_BLOCK, 1) void gi_trace_rg_k(trace_params P, con
         }
         /* the planes and counts are written again only after the next
          * round's first barrier */
-    }
-    if (lane == 0) {
-        if (n_hit0) atomicAdd(&F.stats0[2], n_hit0);
-        if (n_sh0) atomicAdd(&F.stats0[3], n_sh0);
-        if (n_shhit0) atomicAdd(&F.stats0[7], n_shhit0);
-        if (n_hit1) atomicAdd(&F.stats1[2], n_hit1);
-        if (n_sh1) atomicAdd(&F.stats1[3], n_sh1);
-        if (n_shhit1) atomicAdd(&F.stats1[7], n_shhit1);
     }
 }
 
@@ -1252,13 +1222,13 @@ __global__ __launch_bounds__(GI_BLOCK) void gi_gen_k(const gi_surf_rec* __restri
                                                      const uint32_t* __restrict__ frame_salt,
                                                      svo_ray* __restrict__ rays, float* __restrict__ contrib,
                                                      const uint32_t* __restrict__ counters, uint32_t id_base,
-                                                     uint32_t salt_add)
+                                                     uint32_t salt_add, uint32_t max_count)
 {
     /* a device counter can advance the frame of replayed work */
     if (frame_salt) prm.frame += 0x632BE5ABu * (*frame_salt + salt_add + 1u);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = j / prm.spp, k = j - i * prm.spp;
-    if (i >= *count_ptr) return;
+    if (i >= *count_ptr || i >= max_count) return;
     /* entry i's samples take slots [tail + i spp, + spp) after the queue's
      * tail at launch; gi_gen_tail_k advances the tail afterwards */
     const uint32_t pix = list[i];
@@ -1366,51 +1336,101 @@ __global__ __launch_bounds__(256) void gi_lit_k(const svo_ray* __restrict__ shad
     contrib[c] = weight[w]; contrib[c + 1] = weight[w + 1]; contrib[c + 2] = weight[w + 2];
 }
 
-/* A sample's contribution from its record (weight, hit node, voxel): the
- * albedo of the hit voxel's material times the weight, 0 for weight 0. */
-__device__ static inline void sample_value(const float* rec, uint32_t n_inner, const leaf_mat_view& mats,
-                                           const tgx_lut_entry* __restrict__ luts, float& v0, float& v1, float& v2)
+/* Zero stand-ins for the material look-ups of samples with weight 0, so
+ * the look-ups of a chunk of samples are issued without branches and
+ * overlap in flight. */
+static __device__ uint32_t g_zero_u32[1];
+static __device__ uint16_t g_zero_u16[1];
+static __device__ tgx_lut_entry g_zero_lut;
+
+/* The contributions of samples k0 .. k0 + RESOLVE_CHUNK - 1 of an entry
+ * whose records start at c: the albedo of the hit voxel's material times
+ * the weight, 0 for weight 0 and for samples at or past spp. Adds the
+ * record flags of the samples below spp to n_hit, n_sh and n_occ. */
+#define RESOLVE_CHUNK 4u
+__device__ static inline void sample_values(const float* __restrict__ c, uint32_t k0, uint32_t spp, uint32_t n_inner,
+                                            const leaf_mat_view& M, const tgx_lut_entry* __restrict__ luts,
+                                            float v[RESOLVE_CHUNK][3], uint32_t& n_hit, uint32_t& n_sh,
+                                            uint32_t& n_occ)
 {
-    const float wgt = rec[0];
-    v0 = 0.0f; v1 = 0.0f; v2 = 0.0f;
-    if (wgt != 0.0f) {
-        const uint16_t m = leaf_material(mats, __float_as_uint(rec[1]) - n_inner, __float_as_uint(rec[2]));
-        const tgx_lut_entry* e = luts + ((size_t)(m >> 8) * TGX_LUT_ENTRIES + (m & 255u));
-        v0 = e->albedo[0] * wgt; v1 = e->albedo[1] * wgt; v2 = e->albedo[2] * wgt;
+    float w[RESOLVE_CHUNK];
+    uint32_t leaf[RESOLVE_CHUNK], vox[RESOLVE_CHUNK];
+#pragma unroll
+    for (uint32_t j = 0; j < RESOLVE_CHUNK; j++) {
+        const bool in = k0 + j < spp;
+        const uint32_t k = in ? k0 + j : spp - 1u;
+        const float wr = c[3 * k];
+        const uint32_t f = __float_as_uint(c[3 * k + 2]);
+        w[j] = in ? wr : 0.0f;
+        leaf[j] = __float_as_uint(c[3 * k + 1]) - n_inner;
+        vox[j] = f & SAMPLE_VOXEL;
+        n_hit += in ? f >> 31 : 0u;
+        n_sh += in ? (f >> 30) & 1u : 0u;
+        n_occ += in ? (f >> 29) & 1u : 0u;
+    }
+    uint32_t bw[RESOLVE_CHUNK], off[RESOLVE_CHUNK], pre[RESOLVE_CHUNK];
+#pragma unroll
+    for (uint32_t j = 0; j < RESOLVE_CHUNK; j++) {
+        const bool live = w[j] != 0.0f;
+        const size_t wi = (size_t)leaf[j] * TGX_SVO_BLOCK_WORDS + (vox[j] >> 5);
+        bw[j] = *(live ? M.bits + wi : g_zero_u32);
+        off[j] = *(live ? M.mat_off + leaf[j] : g_zero_u32);
+        pre[j] = *(live ? M.wpre + wi : g_zero_u16);
+    }
+    uint32_t mat[RESOLVE_CHUNK];
+#pragma unroll
+    for (uint32_t j = 0; j < RESOLVE_CHUNK; j++) {
+        const bool live = w[j] != 0.0f;
+        const uint32_t below = bw[j] & ((1u << (vox[j] & 31u)) - 1u);
+        mat[j] = *(live ? M.pool + off[j] + pre[j] + (uint32_t)__popc(below) : g_zero_u16);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < RESOLVE_CHUNK; j++) {
+        const bool live = w[j] != 0.0f;
+        const tgx_lut_entry* e =
+            live ? luts + ((size_t)(mat[j] >> 8) * TGX_LUT_ENTRIES + (mat[j] & 255u)) : &g_zero_lut;
+        const float al0 = e->albedo[0], al1 = e->albedo[1], al2 = e->albedo[2];
+        v[j][0] = live ? al0 * w[j] : 0.0f;
+        v[j][1] = live ? al1 * w[j] : 0.0f;
+        v[j][2] = live ? al2 * w[j] : 0.0f;
     }
 }
 
-/* Per list entry: E = (sum of its sample contributions in sample order) /
- * spp, radiance = albedo * E. */
-__global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf_rec* __restrict__ surf,
-                                                    const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count_ptr,
-                                                    const float* __restrict__ contrib, uint32_t spp,
-                                                    uint32_t n_inner, leaf_mat_view mats,
-                                                    const tgx_lut_entry* __restrict__ luts,
-                                                    float* __restrict__ e_entry, float* __restrict__ e_var,
-                                                    float* __restrict__ radiance, float* __restrict__ irradiance)
+/* One list entry's estimate from its spp sample records, summed in sample
+ * order. */
+__device__ static inline void resolve_entry(const gi_surf_rec* __restrict__ surf, const uint32_t* __restrict__ list,
+                                            uint32_t i, const float* __restrict__ contrib, uint32_t spp,
+                                            uint32_t n_inner, const leaf_mat_view& mats,
+                                            const tgx_lut_entry* __restrict__ luts, float* __restrict__ e_entry,
+                                            float* __restrict__ e_var, float* __restrict__ radiance,
+                                            float* __restrict__ irradiance, uint32_t& n_hit, uint32_t& n_sh,
+                                            uint32_t& n_occ)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count_ptr) return;
     const uint32_t pix = list[i];
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     const float* c = contrib + 3ull * (size_t)i * spp;
-    for (uint32_t k = 0; k < spp; k++) {
-        float v0, v1, v2;
-        sample_value(c + 3 * k, n_inner, mats, luts, v0, v1, v2);
-        a0 = a0 + v0; a1 = a1 + v1; a2 = a2 + v2;
+    for (uint32_t k0 = 0; k0 < spp; k0 += RESOLVE_CHUNK) {
+        float v[RESOLVE_CHUNK][3];
+        sample_values(c, k0, spp, n_inner, mats, luts, v, n_hit, n_sh, n_occ);
+#pragma unroll
+        for (uint32_t j = 0; j < RESOLVE_CHUNK; j++)
+            if (k0 + j < spp) { a0 = a0 + v[j][0]; a1 = a1 + v[j][1]; a2 = a2 + v[j][2]; }
     }
     const float inv = 1.0f / (float)spp;
     const float e0 = a0 * inv, e1 = a1 * inv, e2 = a2 * inv;
     if (e_entry) { e_entry[3ull * i] = e0; e_entry[3ull * i + 1] = e1; e_entry[3ull * i + 2] = e2; }
     if (e_var) {
         float q = 0.0f;
-        for (uint32_t k = 0; k < spp; k++) {
-            float v0, v1, v2;
-            sample_value(c + 3 * k, n_inner, mats, luts, v0, v1, v2);
-            const float x0 = v0 - e0, x1 = v1 - e1, x2 = v2 - e2;
-            q += x0 * x0 + x1 * x1 + x2 * x2;
+        uint32_t d0 = 0, d1 = 0, d2 = 0;
+        for (uint32_t k0 = 0; k0 < spp; k0 += RESOLVE_CHUNK) {
+            float v[RESOLVE_CHUNK][3];
+            sample_values(c, k0, spp, n_inner, mats, luts, v, d0, d1, d2);
+#pragma unroll
+            for (uint32_t j = 0; j < RESOLVE_CHUNK; j++)
+                if (k0 + j < spp) {
+                    const float x0 = v[j][0] - e0, x1 = v[j][1] - e1, x2 = v[j][2] - e2;
+                    q += x0 * x0 + x1 * x1 + x2 * x2;
+                }
         }
         e_var[i] = spp > 1u ? q / ((float)spp * (float)(spp - 1u)) : 0.0f;
     }
@@ -1421,6 +1441,53 @@ __global__ __launch_bounds__(256) void gi_resolve_k(const gi_surf_rec* __restric
         radiance[3ull * pix + 2] = s.albedo[2] * e2;
     }
     if (irradiance) { irradiance[3ull * pix] = e0; irradiance[3ull * pix + 1] = e1; irradiance[3ull * pix + 2] = e2; }
+}
+
+__device__ static inline uint32_t wave_sum(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+
+/* Per list entry: E = (sum of its sample contributions in sample order) /
+ * spp, radiance = albedo * E; grid-stride over the entries. With `stats`
+ * (records of the fused trace), the list's bounce hits, shadow rays and
+ * shadow hits are added to stats[2], stats[3] and stats[7] from the record
+ * flags, one atomic per block and counter (same-address atomics from every
+ * wave would serialise). */
+#define RESOLVE_BLOCK 1024u
+#define RESOLVE_MAX_BLOCKS 256u
+__global__ __launch_bounds__(RESOLVE_BLOCK) void gi_resolve_k(const gi_surf_rec* __restrict__ surf,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count_ptr, uint32_t max_count,
+                                                    const float* __restrict__ contrib, uint32_t spp,
+                                                    uint32_t n_inner, leaf_mat_view mats,
+                                                    const tgx_lut_entry* __restrict__ luts,
+                                                    float* __restrict__ e_entry, float* __restrict__ e_var,
+                                                    float* __restrict__ radiance, float* __restrict__ irradiance,
+                                                    uint32_t* __restrict__ stats)
+{
+    const uint32_t count = *count_ptr;
+    const uint32_t n = count < max_count ? count : max_count;
+    uint32_t n_hit = 0, n_sh = 0, n_occ = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        resolve_entry(surf, list, i, contrib, spp, n_inner, mats, luts, e_entry, e_var, radiance, irradiance, n_hit,
+                      n_sh, n_occ);
+    }
+    if (stats) {
+        __shared__ uint32_t s_cnt[RESOLVE_BLOCK / 64u][3];
+        n_hit = wave_sum(n_hit);
+        n_sh = wave_sum(n_sh);
+        n_occ = wave_sum(n_occ);
+        const uint32_t w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63u) == 0u) { s_cnt[w][0] = n_hit; s_cnt[w][1] = n_sh; s_cnt[w][2] = n_occ; }
+        __syncthreads();
+        if (threadIdx.x < 3u) {
+            uint32_t t = 0;
+            for (uint32_t k = 0; k < RESOLVE_BLOCK / 64u; k++) t += s_cnt[k][threadIdx.x];
+            if (t) atomicAdd(&stats[threadIdx.x == 0u ? 2 : threadIdx.x == 1u ? 3 : 7], t);
+        }
+    }
 }
 
 /* Sort key of a queued ray: its direction octant, then the Morton code of
@@ -1749,7 +1816,7 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
             tgx_launch_scope s(ctx, "gi_gen_k");
             hipLaunchKernelGGL(gi_gen_k, dim3(blocks_for((uint64_t)t.max_count * t.prm.spp)), dim3(GI_BLOCK), 0,
                                ctx->stream, surf, t.list,
-                               t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k], t.salt_add);
+                               t.count_dev, t.prm, t.frame_salt, rays, contrib, cnt, id_base[k], t.salt_add, t.max_count);
         }
         hipLaunchKernelGGL(gi_gen_tail_k, dim3(1), dim3(1), 0, ctx->stream, t.count_dev, t.max_count, t.prm.spp, cnt,
                            two ? t.counters : nullptr);
@@ -1766,9 +1833,6 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         /* bounce and shadow rays in one launch */
         trace_fuse F = {};
         F.contrib = contrib;
-        F.stats0 = two ? sets[0].counters : cnt;
-        F.stats1 = two ? sets[1].counters : cnt;
-        F.split = two ? id_base[1] : 0xFFFFFFFFu;
         F.sun0 = prm.sun_dir[0]; F.sun1 = prm.sun_dir[1]; F.sun2 = prm.sun_dir[2];
         /* the step lengths a shadow ray's set-up would divide out, in the
          * same IEEE single precision */
@@ -1812,15 +1876,17 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
     }
     TGX_LAUNCH_CHECK(ctx, "gi_lit_k");
     }
+    /* the fused trace leaves its counts to the resolve pass */
     for (uint32_t k = 0; k < n_sets; k++) {
         const gi_trace_set& t = sets[k];
         if (t.max_count == 0) continue;
         const gi_surf_rec* surf = t.surf ? t.surf : (const gi_surf_rec*)ctx->gi_surf.ptr;
         tgx_launch_scope s(ctx, "gi_resolve_k");
-        hipLaunchKernelGGL(gi_resolve_k, dim3(blocks_for(t.max_count)), dim3(256), 0, ctx->stream, surf, t.list,
-                           t.count_dev, contrib + 3ull * id_base[k], t.prm.spp, ctx->svo_n_inner, leaf_mats_of(ctx),
-                           (const tgx_lut_entry*)ctx->luts.ptr, t.e_entry, t.e_var, t.radiance_dev,
-                           t.irradiance_dev);
+        const uint32_t rb = (uint32_t)(((uint64_t)t.max_count + RESOLVE_BLOCK - 1) / RESOLVE_BLOCK);
+        hipLaunchKernelGGL(gi_resolve_k, dim3(rb < RESOLVE_MAX_BLOCKS ? rb : RESOLVE_MAX_BLOCKS), dim3(RESOLVE_BLOCK), 0,
+                           ctx->stream, surf, t.list, t.count_dev, t.max_count, contrib + 3ull * id_base[k], t.prm.spp, ctx->svo_n_inner,
+                           leaf_mats_of(ctx), (const tgx_lut_entry*)ctx->luts.ptr, t.e_entry, t.e_var, t.radiance_dev,
+                           t.irradiance_dev, fused ? (two ? t.counters : cnt) : nullptr);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_resolve_k");
     return TGX_OK;
