@@ -365,46 +365,49 @@ __device__ static inline void brick_skip(int& x, int& y, int& z, float& tmx, flo
 __device__ static inline uint32_t brick_cross(int& x, int& y, int& z, float& tmx, float& tmy, float& tmz,
                                               float tdx, float tdy, float tdz, int sx, int sy, int sz)
 {
-    const int nx = sx > 0 ? 8 - (x & 7) : (x & 7) + 1;
-    const int ny = sy > 0 ? 8 - (y & 7) : (y & 7) + 1;
-    const int nz = sz > 0 ? 8 - (z & 7) : (z & 7) + 1;
-    const float ax = sx ? fmaf((float)(nx - 1), tdx, tmx) : SVO_INF;
-    const float ay = sy ? fmaf((float)(ny - 1), tdy, tmy) : SVO_INF;
-    const float az = sz ? fmaf((float)(nz - 1), tdz, tmz) : SVO_INF;
+    /* voxels left to the brick's far face on each axis, minus one:
+     * 7 - (x & 7) moving up, x & 7 moving down */
+    const uint32_t mx = ((uint32_t)~sx) >> 29, my = ((uint32_t)~sy) >> 29, mz = ((uint32_t)~sz) >> 29;
+    const uint32_t rx = ((uint32_t)x ^ mx) & 7u, ry = ((uint32_t)y ^ my) & 7u, rz = ((uint32_t)z ^ mz) & 7u;
+    const float ax = sx ? fmaf((float)rx, tdx, tmx) : SVO_INF;
+    const float ay = sy ? fmaf((float)ry, tdy, tmy) : SVO_INF;
+    const float az = sz ? fmaf((float)rz, tdz, tmz) : SVO_INF;
     const uint32_t ez = az <= ay && az <= ax ? 1u : 0u;
     const uint32_t ey = (ez ^ 1u) & (ay <= ax ? 1u : 0u);
+    const uint32_t ex = (ez | ey) ^ 1u;
     const float ae = ez ? az : (ey ? ay : ax);
-    const float second = ez ? fminf(ax, ay) : (ey ? fminf(ax, az) : fminf(ay, az));
-    if (!(second == SVO_INF || (second >= 0x1p-100f && second - ae > second * 0x1p-19f))) return 0u;
+    /* the runner-up is the median of the three */
+    const float second = __builtin_amdgcn_fmed3f(ax, ay, az);
+    const uint32_t sure = (second == SVO_INF ? 1u : 0u) |
+                          ((second >= 0x1p-100f ? 1u : 0u) & (second - ae > second * 0x1p-19f ? 1u : 0u));
+    if (!sure) return 0u;
     const float tde = ez ? tdz : (ey ? tdy : tdx);
-    const int ne = ez ? nz : (ey ? ny : nx);
+    const uint32_t re = ez ? rz : (ey ? ry : rx);
     float te = ez ? tmz : (ey ? tmy : tmx);
 #pragma unroll
-    for (int j = 1; j < 8; j++) te = j < ne ? te + tde : te;
+    for (uint32_t j = 0; j < 7u; j++) te = j < re ? te + tde : te;
     const float up = __uint_as_float(__float_as_uint(te) + 1u);
-    /* thresholds; -1 keeps the exit axis out of the count loop */
-    const float thx = (ez | ey) ? te : -1.0f;
-    const float thy = ez ? te : (ey ? -1.0f : up);
-    const float thz = ez ? -1.0f : up;
-    float ux = tmx, uy = tmy, uz = tmz;
-    int kx = 0, ky = 0, kz = 0;
+    /* the two other axes p < q (in x, y, z order) take the steps whose
+     * events come first: below te, or at te where they win the tie (z over
+     * y over x), written as below the next float above te */
+    const float tdp = ex ? tdy : tdx, tdq = ez ? tdy : tdz;
+    const float thp = ex ? up : te, thq = ez ? te : up;
+    float up_ = ex ? tmy : tmx, uq = ez ? tmy : tmz;
+    int kp = 0, kq = 0;
 #pragma unroll
     for (int j = 0; j < 7; j++) {
-        const bool cx = ux < thx, cy = uy < thy, cz = uz < thz;
-        ux = cx ? ux + tdx : ux;
-        uy = cy ? uy + tdy : uy;
-        uz = cz ? uz + tdz : uz;
-        kx += cx ? 1 : 0;
-        ky += cy ? 1 : 0;
-        kz += cz ? 1 : 0;
+        const bool cp = up_ < thp, cq = uq < thq;
+        up_ = cp ? up_ + tdp : up_;
+        uq = cq ? uq + tdq : uq;
+        kp += cp ? 1 : 0;
+        kq += cq ? 1 : 0;
     }
-    const uint32_t ex = (ez | ey) ^ 1u;
-    tmx = ex ? te + tdx : ux;
-    tmy = ey ? te + tdy : uy;
-    tmz = ez ? te + tdz : uz;
-    x += sx * (ex ? nx : kx);
-    y += sy * (ey ? ny : ky);
-    z += sz * (ez ? nz : kz);
+    tmx = ex ? te + tdx : up_;
+    tmy = ey ? te + tdy : (ex ? up_ : uq);
+    tmz = ez ? te + tdz : uq;
+    x += sx * (ex ? (int)rx + 1 : kp);
+    y += sy * (ey ? (int)ry + 1 : (ex ? kp : kq));
+    z += sz * (ez ? (int)rz + 1 : kq);
     return 1u;
 }
 
