@@ -538,7 +538,7 @@ int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
 typedef struct tgx_tuning {
     uint32_t trace_sort;          /* bit 0: sort bounce rays, bit 1: sort shadow rays,
                                      by (direction octant, origin cell) before tracing (0) */
-    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (12) */
+    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (10) */
     uint32_t trace_blocks_per_cu; /* persistent 1024-lane trace workgroups per CU, 1..2 (2) */
     uint32_t trace_vote;          /* 0: every iteration runs node steps and DDA bursts for the
                                      lanes needing them; k in 1..64: a wave runs only DDA
@@ -564,6 +564,10 @@ typedef struct tgx_tuning {
     uint32_t trace_node;          /* lane-machine launch: node-loop iterations a descending lane
                                      may take per step before its DDA burst, 1..4 (3) */
     uint32_t trace_burst;         /* uncounted DDA: iterations per burst, even, 2..32 (8) */
+    uint32_t trace_chunk;         /* lane-machine launch: rays a wave takes from the queue per
+                                     cursor atomic, kept for its next refills, 0..256 (32); 0
+                                     takes only the idle lanes' rays. A wave takes only what it
+                                     hands out once fewer than two chunks per wave are left */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);

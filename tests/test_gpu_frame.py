@@ -218,11 +218,12 @@ def test_autotune_keeps_results(ctx, batch_scene):
         g.launch()
         before = buf.read()
         g.destroy()
-        say("autotune (coordinate descent over trace form, skip, workgroups, DDA, refill, cull blocks, "
-            "tile edge), 2 cameras")
+        say("autotune (coordinate descent over trace form, skip, workgroups, DDA, refill, queue chunk, cull "
+            "blocks, tile edge), 2 cameras")
         best = ctx.autotune(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr, frames=1)
         assert tuple(ctx.tuning()[0]) == tuple(best[0])
-        assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 8, 12, 16, 24, 32)
+        assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 8, 10, 12, 16, 24)
+        assert best["trace_chunk"][0] in (0, 32, 64)
         assert best["trace_regroup"][0] in (0, 1, 2, 4) and best["trace_skip"][0] in (0, 1)
         assert best["vis_setup_blocks"][0] in (256, 1024, 2048, 4096) and best["vis_tile"][0] in (8, 16, 32)
         assert best["trace_dda"][0] in (0, 1) and best["trace_node"][0] in (1, 2, 3, 4)
@@ -245,9 +246,9 @@ def test_autotune_keeps_results(ctx, batch_scene):
         for a, b in zip(before, after):
             assert a.tobytes() == b.tobytes()
     finally:
-        ctx.set_tuning(trace_sort=0, trace_refill=12, trace_blocks_per_cu=2, trace_vote=0, trace_regroup=0,
+        ctx.set_tuning(trace_sort=0, trace_refill=10, trace_blocks_per_cu=2, trace_vote=0, trace_regroup=0,
                        trace_skip=0, vis_setup_blocks=2048, trace_dda=1, vis_tile=16, trace_cross=40,
-                       trace_node=3)
+                       trace_node=3, trace_chunk=32)
         buf.free()
 
 
@@ -336,13 +337,14 @@ def test_tuning_rejects_out_of_range_values(ctx):
     bad = [("trace_refill", 0), ("trace_refill", 65), ("trace_blocks_per_cu", 3), ("trace_sort", 4),
            ("trace_regroup", 9), ("trace_skip", 2), ("vis_setup_blocks", 32), ("vis_setup_blocks", 20000),
            ("trace_dda", 2), ("vis_tile", 12), ("vis_tile", 64), ("trace_cross", 65), ("trace_node", 0),
-           ("trace_node", 5), ("trace_burst", 3), ("trace_burst", 34), ("trace_burst", 0)]
+           ("trace_node", 5), ("trace_burst", 3), ("trace_burst", 34), ("trace_burst", 0), ("trace_chunk", 257)]
     for name, value in bad:
         with pytest.raises(T.TgxError) as e:
             ctx.set_tuning(**{name: value})
         assert e.value.code == 1, name
         assert ctx.tuning().tobytes() == before.tobytes(), name
-    ctx.set_tuning(vis_tile=8, trace_node=4, trace_burst=32, trace_cross=64)
+    ctx.set_tuning(vis_tile=8, trace_node=4, trace_burst=32, trace_cross=64, trace_chunk=256)
     t = ctx.tuning()
-    assert (t["vis_tile"][0], t["trace_node"][0], t["trace_burst"][0], t["trace_cross"][0]) == (8, 4, 32, 64)
+    assert (t["vis_tile"][0], t["trace_node"][0], t["trace_burst"][0], t["trace_cross"][0],
+            t["trace_chunk"][0]) == (8, 4, 32, 64, 256)
     ctx.set_tuning(**{k: int(before[k][0]) for k in before.dtype.names})

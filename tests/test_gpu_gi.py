@@ -123,7 +123,7 @@ def test_queued_shadow_rays_equal_fused_launch(ctx, noise_setup, sort):
             nst = ctx.nrc_stats()
             out.append((rad, irr, tuple(gst), nrad, nirr, tuple(nst)))
     finally:
-        ctx.set_tuning(trace_sort=0, trace_refill=12, trace_blocks_per_cu=2, trace_vote=0)
+        ctx.set_tuning(trace_sort=0, trace_refill=10, trace_blocks_per_cu=2, trace_vote=0)
     a, b = out
     for k in (0, 1, 3, 4):
         assert a[k].tobytes() == b[k].tobytes(), f"output {k} differs"
@@ -131,14 +131,15 @@ def test_queued_shadow_rays_equal_fused_launch(ctx, noise_setup, sort):
     assert a[5][1] > 0 and a[5][3] > 0 and a[5][4] > 0  # fallback pixels, training and fallback rays
 
 
-@pytest.mark.parametrize("regroup,dda,cross,node", [(1, 0, 0, 1), (3, 0, 0, 1), (0, 1, 0, 1), (0, 1, 1, 2),
-                                                    (0, 1, 40, 3), (0, 0, 0, 4)])
-def test_regrouped_trace_equals_oracle_and_lane_machine(ctx, noise_setup, regroup, dda, cross, node):
+@pytest.mark.parametrize("regroup,dda,cross,node,chunk", [(1, 0, 0, 1, 32), (3, 0, 0, 1, 32), (0, 1, 0, 1, 64),
+                                                          (0, 1, 1, 2, 32), (0, 1, 40, 3, 32), (0, 0, 0, 4, 256)])
+def test_regrouped_trace_equals_oracle_and_lane_machine(ctx, noise_setup, regroup, dda, cross, node, chunk):
     """The fused GI launch with the workgroup's rays regrouped by phase
     (tuning trace_regroup), and the lane machine's DDA without iteration
     counts (trace_dda), with and without exact crossings of bricks without
-    solid voxels (trace_cross), and with several node-loop iterations per
-    lane step (trace_node), give the oracle's bits and counts at 480x270 and on
+    solid voxels (trace_cross), with several node-loop iterations per lane
+    step (trace_node), and with several refills' rays per queue atomic
+    (trace_chunk), give the oracle's bits and counts at 480x270 and on
     every 8th row at 1920x1080, and the same cache frames (training targets
     and fallback pixels in one launch) as the lane state machine with the
     counted DDA loop."""
@@ -152,20 +153,22 @@ def test_regrouped_trace_equals_oracle_and_lane_machine(ctx, noise_setup, regrou
     cam = T.generate_camera(p, 0, w, h)
     out = []
     try:
-        for m, f, c, n in ((0, 0, 0, 1), (regroup, dda, cross, node)):
-            ctx.set_tuning(trace_regroup=m, trace_dda=f, trace_cross=c, trace_node=n)
-            say(f"trace_regroup {m} trace_dda {f} trace_cross {c} trace_node {n}: plain GI 480x270 and 3 cache frames")
+        for m, f, c, n, q in ((0, 0, 0, 1, 0), (regroup, dda, cross, node, chunk)):
+            ctx.set_tuning(trace_regroup=m, trace_dda=f, trace_cross=c, trace_node=n, trace_chunk=q)
+            say(f"trace_regroup {m} trace_dda {f} trace_cross {c} trace_node {n} trace_chunk {q}: plain GI 480x270 "
+                "and 3 cache frames")
             vis, rad, irr = ctx.frame(cam, w, h, prm)
             gst = tuple(int(x) for x in ctx.gi_stats())
             ctx.nrc_create(cfg)
             for _ in range(3):
                 _, nrad, nirr = ctx.nrc_frame(cam, w, h, prm)
             out.append((rad, irr, gst, nrad, nirr, tuple(ctx.nrc_stats())))
-        say(f"trace_regroup {regroup} trace_dda {dda} trace_cross {cross} trace_node {node}: GI 1920x1080")
+        say(f"trace_regroup {regroup} trace_dda {dda} trace_cross {cross} trace_node {node} trace_chunk {chunk}: "
+            "GI 1920x1080")
         cam2 = T.generate_camera(p, 0, 1920, 1080)
         vis2, rad2, irr2 = ctx.frame(cam2, 1920, 1080, prm)
     finally:
-        ctx.set_tuning(trace_regroup=0, trace_dda=1, trace_cross=40, trace_node=3)
+        ctx.set_tuning(trace_regroup=0, trace_dda=1, trace_cross=40, trace_node=3, trace_chunk=32)
     a, b = out
     for k in (0, 1, 3, 4):
         assert a[k].tobytes() == b[k].tobytes(), f"output {k} differs"

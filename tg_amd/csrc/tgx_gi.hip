@@ -33,8 +33,9 @@
  * (both tuning). In the uncounted DDA, once enough of a wave's walking
  * lanes are in 8^3 bricks without solid voxels, those lanes cross their
  * bricks in one exact step (brick_cross). Whenever `refill` (tuning) lanes
- * of a wave have finished, the wave takes that many new rays from the
- * queue with one atomic (ballot + lane rank).
+ * of a wave have finished, the wave hands that many new rays to them
+ * (ballot + lane rank), from a slice of the queue it took with one atomic
+ * (`chunk` rays, tuning).
  *
  * GI frame (estimator documented at tgx_gi_params in tgx.h):
  *   gi_gen_k     surface of each visible pixel, spp bounce rays at
@@ -753,7 +754,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
                                                           uint32_t* __restrict__ hit_counter, uint32_t refill,
                                                           uint32_t vote, uint32_t cross, uint32_t node_iters,
-                                                          uint32_t burst, const trace_fuse F)
+                                                          uint32_t burst, uint32_t chunk, const trace_fuse F)
 {
     extern __shared__ uint2 s_inner[];
     if constexpr (kLds) {
@@ -776,19 +777,42 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
     /* pending: the lane holds a ray (world origin in px..pz, direction in
      * dx..dz) that is set up at the next iteration's start */
     bool active = false, exhausted = false, pending = false;
+    /* The wave's slice of the queue, rays [q_next, q_end) taken from the
+     * cursor and not handed out yet: one atomic takes `chunk` rays for
+     * several refills (same-address atomics of all waves serialise). Near
+     * the queue's end (less than two chunks per wave left, as of the last
+     * atomic) a wave takes only what it hands out, so no wave holds back
+     * rays that idle waves could trace. */
+    uint32_t q_next = 0, q_end = 0, seen = 0;
+    bool drained = false;
+    const uint32_t n_waves = gridDim.x * (TRACE_BLOCK / 64u);
     while (true) {
         /* wave-uniform refill once `refill` lanes are idle */
         const unsigned long long idle = __ballot(!active && !pending);
         const uint32_t want = (uint32_t)__popcll(idle);
         if (!exhausted && want >= refill) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(cursor, want);
-            /* the whole wave runs this block, so lane 0 is the first active */
-            base = __builtin_amdgcn_readfirstlane(base);
-            if (base + want >= n_rays) exhausted = true;
+            const uint32_t avail = q_end - q_next;
+            uint32_t nb = 0, nend = 0;
+            if (avail < want && !drained) {
+                const uint32_t need = want - avail;
+                const bool tail = (uint64_t)(n_rays - (seen < n_rays ? seen : n_rays)) <
+                                  2ull * n_waves * chunk;
+                const uint32_t take = tail || chunk < need ? need : chunk;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(cursor, take);
+                /* the whole wave runs this block, so lane 0 is the first active */
+                base = __builtin_amdgcn_readfirstlane(base);
+                seen = base;
+                if ((uint64_t)base + take >= n_rays) drained = true;
+                if (base < n_rays) {
+                    nb = base;
+                    nend = (uint64_t)base + take < n_rays ? base + take : n_rays;
+                }
+            }
+            const uint32_t rk = lane_rank(idle);
             if (!active && !pending) {
-                const uint32_t r = base + lane_rank(idle);
-                if (r < n_rays) {
+                const uint32_t r = rk < avail ? q_next + rk : nb + (rk - avail);
+                if (rk < avail || rk - avail < nend - nb) {
                     slot = r;
                     const svo_ray q = rays[r];
                     L.px = q.ox; L.py = q.oy; L.pz = q.oz;
@@ -797,6 +821,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                     pending = true;
                 }
             }
+            if (avail >= want) {
+                q_next += want;
+            } else {
+                const uint32_t used = want - avail;
+                q_next = nend - nb > used ? nb + used : nend;
+                q_end = nend;
+            }
+            exhausted = drained && q_next == q_end;
         }
         if (pending) {
             pending = false;
@@ -1644,7 +1676,7 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
                            : (dda == 2u ? gi_trace_k<false, false, 2u> : dda ? gi_trace_k<false, false, 1u> : gi_trace_k<false, false, 0u>);
         hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor, out,
                            hit_counter, refill, vote, ctx->tuning.trace_cross, ctx->tuning.trace_node,
-                           ctx->tuning.trace_burst, F);
+                           ctx->tuning.trace_burst, ctx->tuning.trace_chunk, F);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
     return TGX_OK;
