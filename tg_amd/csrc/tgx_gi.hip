@@ -569,9 +569,10 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
                     tmx = mx ? tmx + tdx : tmx;
                     tmy = my ? tmy + tdy : tmy;
                     tmz = mz ? tmz + tdz : tmz;
-                    x = mx ? x + sx : x;
-                    y = my ? y + sy : y;
-                    z = mz ? z + sz : z;
+                    /* 24-bit multiply-adds (mx..mz are 0 or 1) */
+                    x = x + __mul24((int)mx, sx);
+                    y = y + __mul24((int)my, sy);
+                    z = z + __mul24((int)mz, sz);
                     const uint32_t outside = (((uint32_t)x | (uint32_t)y | (uint32_t)z) >> 5) != 0u ? 1u : 0u;
                     const uint32_t lv = move & outside;
                     hit_i |= h;
