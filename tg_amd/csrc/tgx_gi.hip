@@ -566,9 +566,13 @@ __device__ static inline bool lane_step(const trace_params& P, const uint2* s_in
                     const uint32_t sx_ = (sz_ | sy_) ^ 1u;
                     const uint32_t move = live & (h ^ 1u);
                     const uint32_t mx = move & sx_, my = move & sy_, mz = move & sz_;
-                    tmx = mx ? tmx + tdx : tmx;
-                    tmy = my ? tmy + tdy : tmy;
-                    tmz = mz ? tmz + tdz : tmz;
+                    /* t_max += t_delta on the stepping axis as fma(1, td, t)
+                     * (the same rounded sum) and fma(0, td, t) elsewhere (t
+                     * itself, or +0 for a t of -0, which compares and adds
+                     * alike) */
+                    tmx = fmaf((float)mx, tdx, tmx);
+                    tmy = fmaf((float)my, tdy, tmy);
+                    tmz = fmaf((float)mz, tdz, tmz);
                     /* 24-bit multiply-adds (mx..mz are 0 or 1) */
                     x = x + __mul24((int)mx, sx);
                     y = y + __mul24((int)my, sy);
