@@ -183,8 +183,14 @@ __device__ static inline void forward_tile(const bf16x8* __restrict__ wf, uint32
     for (uint32_t s = 0; s < 2u; s++) y = mfma16(wf[F_L4 / 8u + s * 64u + lane], h[3][s], y);
 }
 
-/* Forward pass for inference: one activation buffer, y in lanes 0..15. */
-__device__ static inline f32x4 infer_tile(const bf16x8* __restrict__ wf, uint32_t lane, const bf16x8 x[3])
+/* The 38 forward A fragments of one lane: fragment f of the forward
+ * fragment array is wf[64 f + lane] (layer 0: 4 tiles x 3 K steps, layers
+ * 1..3: 4 x 2 each, layer 4: 2). */
+#define NRC_FWD_FRAGS (NRC_FWD_ELEMS / 512u)
+
+/* Forward pass for inference with the fragments in registers: one
+ * activation buffer, y in lanes 0..15. */
+__device__ static inline f32x4 infer_tile(const bf16x8 (&w)[NRC_FWD_FRAGS], const bf16x8 x[3])
 {
     f32x4 acc[4];
     bf16x8 h[2];
@@ -192,23 +198,22 @@ __device__ static inline f32x4 infer_tile(const bf16x8* __restrict__ wf, uint32_
     for (uint32_t t = 0; t < 4u; t++) {
         acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (uint32_t s = 0; s < 3u; s++) acc[t] = mfma16(wf[F_L0 / 8u + (t * 3u + s) * 64u + lane], x[s], acc[t]);
+        for (uint32_t s = 0; s < 3u; s++) acc[t] = mfma16(w[t * 3u + s], x[s], acc[t]);
     }
     relu_pack(acc, h);
 #pragma unroll
     for (uint32_t L = 1; L < 4u; L++) {
-        const uint32_t base = (F_L1 + (L - 1u) * 4096u) / 8u;
 #pragma unroll
         for (uint32_t t = 0; t < 4u; t++) {
             acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (uint32_t s = 0; s < 2u; s++) acc[t] = mfma16(wf[base + (t * 2u + s) * 64u + lane], h[s], acc[t]);
+            for (uint32_t s = 0; s < 2u; s++) acc[t] = mfma16(w[12u + (L - 1u) * 8u + t * 2u + s], h[s], acc[t]);
         }
         relu_pack(acc, h);
     }
     f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (uint32_t s = 0; s < 2u; s++) y = mfma16(wf[F_L4 / 8u + s * 64u + lane], h[s], y);
+    for (uint32_t s = 0; s < 2u; s++) y = mfma16(w[36u + s], h[s], y);
     return y;
 }
 
@@ -521,35 +526,43 @@ __global__ __launch_bounds__(256) void nrc_adam_k(const float* __restrict__ part
 }
 
 /* Inference over list entries b < *count (persistent waves, 16 samples a
- * tile, forward fragments in LDS). With cells, an entry whose cell is
+ * tile, each wave's forward fragments in its registers, two waves per SIMD;
+ * the next tile's surface record is loaded while the current one is
+ * evaluated). With cells, an entry whose cell is
  * unreliable gets fallback flag 1 instead of an output: fewer than min_n
  * samples, or a bias^2 sum above w_ref times the targets' t^2 sum plus
  * w_var times their variance sum. */
-__global__ __launch_bounds__(NRC_BLOCK) void nrc_infer_k(const bf16x8* __restrict__ wf_g,
-                                                         const gi_surf_rec* __restrict__ surf,
-                                                         const uint32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ count_ptr, nrc_enc enc,
-                                                         float* __restrict__ out_entry, float* __restrict__ radiance,
-                                                         float* __restrict__ irradiance,
-                                                         const nrc_cell* __restrict__ cells, uint32_t cell_shift,
-                                                         float w_ref, float w_var, uint32_t min_n,
-                                                         uint32_t* __restrict__ fb_flag)
+__global__ __launch_bounds__(NRC_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void nrc_infer_k(
+    const bf16x8* __restrict__ wf_g, const gi_surf_rec* __restrict__ surf, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ count_ptr, nrc_enc enc, float* __restrict__ out_entry, float* __restrict__ radiance,
+    float* __restrict__ irradiance, const nrc_cell* __restrict__ cells, uint32_t cell_shift, float w_ref, float w_var,
+    uint32_t min_n, uint32_t* __restrict__ fb_flag)
 {
-    __shared__ bf16x8 wf[NRC_FWD_ELEMS / 8u];
-    for (uint32_t i = threadIdx.x; i < NRC_FWD_ELEMS / 8u; i += NRC_BLOCK) wf[i] = wf_g[i];
-    __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, n = lane & 15u;
+    bf16x8 w[NRC_FWD_FRAGS];
+#pragma unroll
+    for (uint32_t f = 0; f < NRC_FWD_FRAGS; f++) w[f] = wf_g[64u * f + lane];
     const uint32_t count = *count_ptr;
     const uint32_t n_tiles = (count + 15u) / 16u;
     const uint32_t waves = gridDim.x * (NRC_BLOCK / 64u);
-    for (uint32_t tile = (blockIdx.x * NRC_BLOCK + threadIdx.x) >> 6; tile < n_tiles; tile += waves) {
+    uint32_t tile = (blockIdx.x * NRC_BLOCK + threadIdx.x) >> 6;
+    gi_surf_rec rn = {};
+    {
+        const uint32_t b = tile * 16u + n;
+        if (tile < n_tiles && b < count) rn = surf[list ? list[b] : b];
+    }
+    for (; tile < n_tiles; tile += waves) {
         const uint32_t b = tile * 16u + n;
         const bool valid = b < count;
-        gi_surf_rec r = {};
-        if (valid) r = surf[list ? list[b] : b];
+        const gi_surf_rec r = rn;
+        {
+            const uint32_t t2 = tile + waves, b2 = t2 * 16u + n;
+            rn = gi_surf_rec{};
+            if (t2 < n_tiles && b2 < count) rn = surf[list ? list[b2] : b2];
+        }
         bf16x8 x[3];
         encode(r, enc, g, x);
-        const f32x4 y = infer_tile(wf, lane, x);
+        const f32x4 y = infer_tile(w, x);
         if (!valid || g != 0u) continue;
         const float e0 = fmaxf(y[0], 0.0f), e1 = fmaxf(y[1], 0.0f), e2 = fmaxf(y[2], 0.0f);
         if (out_entry) { out_entry[3ull * b] = e0; out_entry[3ull * b + 1] = e1; out_entry[3ull * b + 2] = e2; }
@@ -782,7 +795,7 @@ int launch_infer(tgx_context* ctx, const gi_surf_rec* surf, const uint32_t* list
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     const uint32_t want = nrc_blocks((max_count + 15u) / 16u, NRC_BLOCK / 64u);
-    const uint32_t cap = (uint32_t)cus * 4u;
+    const uint32_t cap = (uint32_t)cus * 2u;   /* two waves per SIMD */
     {
         tgx_launch_scope s(ctx, "nrc_infer_k");
         hipLaunchKernelGGL(nrc_infer_k, dim3(want < cap ? want : cap), dim3(NRC_BLOCK), 0, ctx->stream,
