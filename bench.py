@@ -162,7 +162,7 @@ def neural_bench(ctx, cam, sun, vis_d, rad_d, irr_d, steps: int, pretrain: int, 
     8192-spp halves: a quarter of the halves' rel-MSE is the reference's own
     variance, which every measured (raw) error carries and the debiased
     figures subtract (the cache is trained on other samples, so its error
-    and the reference's noise add). The 2 % verdict is taken on the raw
+    and the reference's noise add). The 2 % criterion is taken on the raw
     error. A moving camera (the yaw turning NRC_PATH_DEG_PER_FRAME per
     frame) gives the error a cache reaches while its view changes."""
     def irr_frame(frame: int) -> np.ndarray:
