@@ -568,6 +568,10 @@ typedef struct tgx_tuning {
                                      cursor atomic, kept for its next refills, 0..256 (32); 0
                                      takes only the idle lanes' rays. A wave takes only what it
                                      hands out once fewer than two chunks per wave are left */
+    uint32_t trace_small;         /* lane-machine launch with more than one workgroup per CU: a
+                                     queue of fewer than trace_small rays per lane of one
+                                     workgroup per CU is traced by one workgroup per CU (the
+                                     others exit at once), 0..64 (4); 0 keeps every workgroup */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);

@@ -337,7 +337,8 @@ def test_tuning_rejects_out_of_range_values(ctx):
     bad = [("trace_refill", 0), ("trace_refill", 65), ("trace_blocks_per_cu", 3), ("trace_sort", 4),
            ("trace_regroup", 9), ("trace_skip", 2), ("vis_setup_blocks", 32), ("vis_setup_blocks", 20000),
            ("trace_dda", 2), ("vis_tile", 12), ("vis_tile", 64), ("trace_cross", 65), ("trace_node", 0),
-           ("trace_node", 5), ("trace_burst", 3), ("trace_burst", 34), ("trace_burst", 0), ("trace_chunk", 257)]
+           ("trace_node", 5), ("trace_burst", 3), ("trace_burst", 34), ("trace_burst", 0), ("trace_chunk", 257),
+           ("trace_small", 65)]
     for name, value in bad:
         with pytest.raises(T.TgxError) as e:
             ctx.set_tuning(**{name: value})

@@ -353,8 +353,9 @@ def test_svo_trace_dda_forms_bit_exact(ctx):
     """The leaf DDA's forms (counted; uncounted; uncounted with exact
     crossings of 8^3 bricks without solid voxels at several wave thresholds),
     1 to 4 node-loop iterations per lane step, and queue slices of 0 to 256
-    rays per cursor atomic with refill thresholds of 1 to 64 idle lanes give
-    the oracle's hits on rays chosen to stress the crossing: unit
+    rays per cursor atomic with refill thresholds of 1 to 64 idle lanes, on
+    one or two workgroups per CU (trace_small), give the oracle's hits on
+    rays chosen to stress the crossing: unit
     directions, directions with equal-magnitude components (ties between
     axes), components of 1e-6, 1e-30 and 0, and directions scaled by 1e-3
     to 1e3."""
@@ -381,14 +382,15 @@ def test_svo_trace_dda_forms_bit_exact(ctx):
     d = d.astype(np.float32)
     ref = ref_svo.trace(o, d, 1024.0)
     try:
-        for dda, cross, node, burst, chunk, refill in ((0, 0, 1, 8, 32, 10), (1, 0, 1, 8, 0, 1), (1, 1, 2, 2, 256, 64),
-                                                       (1, 32, 3, 8, 32, 10), (1, 40, 4, 16, 64, 4),
-                                                       (1, 64, 3, 32, 256, 1)):
+        for dda, cross, node, burst, chunk, refill, small in ((0, 0, 1, 8, 32, 10, 4), (1, 0, 1, 8, 0, 1, 0),
+                                                              (1, 1, 2, 2, 256, 64, 4), (1, 32, 3, 8, 32, 10, 0),
+                                                              (1, 40, 4, 16, 64, 4, 64), (1, 64, 3, 32, 256, 1, 4)):
             ctx.set_tuning(trace_dda=dda, trace_cross=cross, trace_node=node, trace_burst=burst, trace_chunk=chunk,
-                           trace_refill=refill)
+                           trace_refill=refill, trace_small=small)
             say(f"depth-11 trace, 24000 stress rays, trace_dda {dda} trace_cross {cross} trace_node {node} "
-                f"trace_burst {burst} trace_chunk {chunk} trace_refill {refill}")
+                f"trace_burst {burst} trace_chunk {chunk} trace_refill {refill} trace_small {small}")
             assert_same_hits(ctx.svo_trace(o, d, 1024.0), ref, f"dda {dda} cross {cross} node {node} chunk {chunk}")
     finally:
-        ctx.set_tuning(trace_dda=1, trace_cross=40, trace_node=3, trace_burst=8, trace_chunk=32, trace_refill=10)
+        ctx.set_tuning(trace_dda=1, trace_cross=40, trace_node=3, trace_burst=8, trace_chunk=32, trace_refill=10,
+                       trace_small=4)
     assert 0.2 < ref["hit"].mean() < 0.95

@@ -73,7 +73,8 @@ TUNING_DTYPE = np.dtype([("trace_sort", "<u4"), ("trace_refill", "<u4"), ("trace
                          ("trace_vote", "<u4"), ("trace_regroup", "<u4"), ("trace_skip", "<u4"),
                          ("vis_setup_blocks", "<u4"), ("trace_dda", "<u4"),
                          ("vis_tile", "<u4"), ("trace_cross", "<u4"),
-                         ("trace_node", "<u4"), ("trace_burst", "<u4"), ("trace_chunk", "<u4")])
+                         ("trace_node", "<u4"), ("trace_burst", "<u4"), ("trace_chunk", "<u4"),
+                         ("trace_small", "<u4")])
 
 NRC_N_PARAMS = 18624
 NRC_CONFIG_DTYPE = np.dtype([("origin", "<f4", (3,)), ("scale", "<f4"), ("learning_rate", "<f4"),

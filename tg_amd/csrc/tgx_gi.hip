@@ -759,16 +759,23 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                                                           uint32_t* __restrict__ cursor, svo_res* __restrict__ out,
                                                           uint32_t* __restrict__ hit_counter, uint32_t refill,
                                                           uint32_t vote, uint32_t cross, uint32_t node_iters,
-                                                          uint32_t burst, uint32_t chunk, const trace_fuse F)
+                                                          uint32_t burst, uint32_t chunk, uint32_t small_rays,
+                                                          uint32_t small_blocks, const trace_fuse F)
 {
     extern __shared__ uint2 s_inner[];
+    /* a queue of fewer than small_rays rays is traced by the first
+     * small_blocks workgroups (one per CU) alone: with few rays per lane the
+     * launch is as long as its slowest rays, which run faster with fewer
+     * waves per SIMD */
+    const uint32_t n_rays = *n_rays_ptr;
+    const bool small = n_rays < small_rays;
+    if (small && blockIdx.x >= small_blocks) return;
     if constexpr (kLds) {
         for (uint32_t i = threadIdx.x; i < P.n_inner; i += blockDim.x) s_inner[i] = P.inner[i];
         const uint2* gb = (const uint2*)P.bricks;
         for (uint32_t i = threadIdx.x; i < P.n_leaves; i += blockDim.x) s_inner[P.n_inner + i] = gb[i];
         __syncthreads();
     }
-    const uint32_t n_rays = *n_rays_ptr;
     const uint32_t lane = threadIdx.x & 63u;
     /* fused GI: each lane's bounce ray origin and id, behind the nodes; the
      * wave's part of the stash is a uniform base indexed by lane */
@@ -790,7 +797,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
      * rays that idle waves could trace. */
     uint32_t q_next = 0, q_end = 0, seen = 0;
     bool drained = false;
-    const uint32_t n_waves = gridDim.x * (TRACE_BLOCK / 64u);
+    const uint32_t n_waves = (small && small_blocks < gridDim.x ? small_blocks : gridDim.x) * (TRACE_BLOCK / 64u);
     while (true) {
         /* wave-uniform refill once `refill` lanes are idle */
         const unsigned long long idle = __ballot(!active && !pending);
@@ -1662,6 +1669,10 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
     const trace_params P = params_of(ctx);
     const uint32_t refill = ctx->tuning.trace_refill;
     const uint32_t vote = ctx->tuning.trace_vote;
+    /* with more than one workgroup per CU, queues below trace_small rays
+     * per lane of one workgroup per CU run on one workgroup per CU */
+    const uint64_t small64 = grid.x > (uint32_t)cus ? (uint64_t)ctx->tuning.trace_small * TRACE_BLOCK * (uint32_t)cus : 0u;
+    const uint32_t small_rays = small64 < 0xFFFFFFFFull ? (uint32_t)small64 : 0xFFFFFFFFu;
     {
         tgx_launch_scope s(ctx, "gi_trace_k");
         /* dynamic LDS: nodes and brick masks (when they fit), then for the
@@ -1681,7 +1692,7 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
                            : (dda == 2u ? gi_trace_k<false, false, 2u> : dda ? gi_trace_k<false, false, 1u> : gi_trace_k<false, false, 0u>);
         hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor, out,
                            hit_counter, refill, vote, ctx->tuning.trace_cross, ctx->tuning.trace_node,
-                           ctx->tuning.trace_burst, ctx->tuning.trace_chunk, F);
+                           ctx->tuning.trace_burst, ctx->tuning.trace_chunk, small_rays, (uint32_t)cus, F);
     }
     TGX_LAUNCH_CHECK(ctx, "gi_trace_k");
     return TGX_OK;

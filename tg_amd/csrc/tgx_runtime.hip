@@ -312,7 +312,7 @@ int tgx_tuning_set(tgx_context* ctx, const tgx_tuning* t)
         t->vis_setup_blocks < 64u || t->vis_setup_blocks > 16384u || t->trace_dda > 1u ||
         (t->vis_tile != 8u && t->vis_tile != 16u && t->vis_tile != 32u) || t->trace_cross > 64u ||
         t->trace_node < 1u || t->trace_node > 4u || t->trace_burst < 2u || t->trace_burst > 32u ||
-        (t->trace_burst & 1u) || t->trace_chunk > 256u)
+        (t->trace_burst & 1u) || t->trace_chunk > 256u || t->trace_small > 64u)
         return tgx_fail(ctx, TGX_ERR_INVALID_ARG, "tuning out of range");
     ctx->tuning = *t;
     return TGX_OK;
