@@ -154,9 +154,10 @@ __device__ static inline void relu_pack(const f32x4 acc[4], bf16x8 out[2])
         }
 }
 
-/* Forward pass of one 16-sample tile; wf = forward fragments (LDS or
- * global). h[L] = activations after hidden layer L; y in lanes 0..15. */
-__device__ static inline void forward_tile(const bf16x8* __restrict__ wf, uint32_t lane, const bf16x8 x[3],
+/* Forward pass of one 16-sample tile with the forward fragments in
+ * registers (fragment f of the forward array is wf[64 f + lane]). h[L] =
+ * activations after hidden layer L; y in lanes 0..15. */
+__device__ static inline void forward_tile(const bf16x8 (&w)[NRC_FWD_ELEMS / 512u], const bf16x8 x[3],
                                            bf16x8 h[4][2], f32x4& y)
 {
     f32x4 acc[4];
@@ -164,23 +165,22 @@ __device__ static inline void forward_tile(const bf16x8* __restrict__ wf, uint32
     for (uint32_t t = 0; t < 4u; t++) {
         acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (uint32_t s = 0; s < 3u; s++) acc[t] = mfma16(wf[F_L0 / 8u + (t * 3u + s) * 64u + lane], x[s], acc[t]);
+        for (uint32_t s = 0; s < 3u; s++) acc[t] = mfma16(w[t * 3u + s], x[s], acc[t]);
     }
     relu_pack(acc, h[0]);
 #pragma unroll
     for (uint32_t L = 1; L < 4u; L++) {
-        const uint32_t base = (F_L1 + (L - 1u) * 4096u) / 8u;
 #pragma unroll
         for (uint32_t t = 0; t < 4u; t++) {
             acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (uint32_t s = 0; s < 2u; s++) acc[t] = mfma16(wf[base + (t * 2u + s) * 64u + lane], h[L - 1u][s], acc[t]);
+            for (uint32_t s = 0; s < 2u; s++) acc[t] = mfma16(w[12u + (L - 1u) * 8u + t * 2u + s], h[L - 1u][s], acc[t]);
         }
         relu_pack(acc, h[L]);
     }
     y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (uint32_t s = 0; s < 2u; s++) y = mfma16(wf[F_L4 / 8u + s * 64u + lane], h[3][s], y);
+    for (uint32_t s = 0; s < 2u; s++) y = mfma16(w[36u + s], h[3][s], y);
 }
 
 /* The 38 forward A fragments of one lane: fragment f of the forward
@@ -227,22 +227,24 @@ __device__ static inline void mask_pack(const f32x4 acc[4], const bf16x8 act[2],
             out[s][j] = (float)act[s][j] > 0.0f ? (__bf16)acc[2u * s + j / 4u][j & 3u] : (__bf16)0.0f;
 }
 
-/* Backward pass: dz[L - 1] = dZ_L fragments for L = 1..4 from dZ_5. */
-__device__ static inline void backward_tile(const bf16x8* __restrict__ wb, uint32_t lane, const bf16x8 h[4][2],
-                                            bf16x8 dz5, bf16x8 dz[4][2])
+/* Backward pass with the backward fragments in registers (fragment f of
+ * the backward array is wb[64 f + lane]: layers 1..3 4 x 2 each, layer 4
+ * 4 x 1): dz[L - 1] = dZ_L fragments for L = 1..4 from dZ_5. */
+#define NRC_BWD_FRAGS (NRC_BWD_ELEMS / 512u)
+__device__ static inline void backward_tile(const bf16x8 (&w)[NRC_BWD_FRAGS], const bf16x8 h[4][2], bf16x8 dz5,
+                                            bf16x8 dz[4][2])
 {
     f32x4 acc[4];
 #pragma unroll
-    for (uint32_t t = 0; t < 4u; t++) acc[t] = mfma16(wb[B_L4 / 8u + t * 64u + lane], dz5, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    for (uint32_t t = 0; t < 4u; t++) acc[t] = mfma16(w[24u + t], dz5, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
     mask_pack(acc, h[3], dz[3]);
 #pragma unroll
     for (uint32_t L = 3; L >= 1u; L--) {
-        const uint32_t base = (B_L1 + (L - 1u) * 4096u) / 8u;
 #pragma unroll
         for (uint32_t t = 0; t < 4u; t++) {
             acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (uint32_t s = 0; s < 2u; s++) acc[t] = mfma16(wb[base + (t * 2u + s) * 64u + lane], dz[L][s], acc[t]);
+            for (uint32_t s = 0; s < 2u; s++) acc[t] = mfma16(w[(L - 1u) * 8u + t * 2u + s], dz[L][s], acc[t]);
         }
         mask_pack(acc, h[L - 1u], dz[L - 1u]);
     }
@@ -275,7 +277,8 @@ __device__ static inline uint32_t to_fix(float v)
 /* Forward + backward of one 16-sample tile per wave. Sample b uses
  * surf[list ? list[b] : b] and targets[3 b]; samples b >= *count get a
  * zero loss gradient. inv_n = 1 / (3 n). */
-__global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restrict__ wf, const bf16x8* __restrict__ wb,
+__global__ __launch_bounds__(NRC_BLOCK) __attribute__((amdgpu_waves_per_eu(1, 2))) void nrc_fwdbwd_k(
+                                                          const bf16x8* __restrict__ wf, const bf16x8* __restrict__ wb,
                                                           const gi_surf_rec* __restrict__ surf,
                                                           const uint32_t* __restrict__ list,
                                                           const uint32_t* __restrict__ count_ptr, uint32_t first,
@@ -289,6 +292,12 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
     const uint32_t tile = (blockIdx.x * NRC_BLOCK + threadIdx.x) >> 6;
     const uint32_t b = tile * 16u + n;          /* sample within the batch */
     if (tile * 16u >= Bc) return;               /* whole wave */
+    /* every fragment is loaded up front, so the layers wait on no load */
+    bf16x8 wfr[NRC_FWD_FRAGS], wbr[NRC_BWD_FRAGS];
+#pragma unroll
+    for (uint32_t f = 0; f < NRC_FWD_FRAGS; f++) wfr[f] = wf[64u * f + lane];
+#pragma unroll
+    for (uint32_t f = 0; f < NRC_BWD_FRAGS; f++) wbr[f] = wb[64u * f + lane];
     /* the batch is sg.len samples from each of Bc / sg.len cameras */
     const uint32_t k = b / sg.len, bl = b - k * sg.len;
     surf += (size_t)k * sg.surf;
@@ -302,7 +311,7 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
     bf16x8 x[3], h[4][2];
     encode(r, enc, g, x);
     f32x4 y;
-    forward_tile(wf, lane, x, h, y);
+    forward_tile(wfr, x, h, y);
     /* loss gradient in lanes g == 0 (features 0..2 of the output tile) */
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
     if (valid && g == 0u) { t0 = targets[3ull * (first + bl)]; t1 = targets[3ull * (first + bl) + 1]; t2 = targets[3ull * (first + bl) + 2]; }
@@ -330,7 +339,7 @@ __global__ __launch_bounds__(NRC_BLOCK) void nrc_fwdbwd_k(const bf16x8* __restri
         loss[b] = 0.0f;
     }
     bf16x8 dz[4][2];
-    backward_tile(wb, lane, h, dz5, dz);
+    backward_tile(wbr, h, dz5, dz);
     /* store activations and gradients [feature][batch] */
 #pragma unroll
     for (uint32_t s = 0; s < 3u; s++)
