@@ -1132,12 +1132,23 @@ __device__ static inline float gi_rand(uint32_t& s, float lo, float hi)
 }
 __device__ static inline f3 gi_cosine_direction(f3 n, uint32_t& st)
 {
-    for (int attempt = 0; attempt < 32; attempt++) {
-        const float x = gi_rand(st, -1.0f, 1.0f);
-        const float y = gi_rand(st, -1.0f, 1.0f);
-        const float z = gi_rand(st, -1.0f, 1.0f);
-        const float l2 = x * x + y * y + z * z;
-        if (l2 > 1.0f || l2 < 1e-6f) continue;
+    /* Up to 32 attempts, each drawing a point of the cube and keeping it if
+     * it lies in the unit ball (and n plus its direction is not ~0). The
+     * draws run in an inner loop until a lane's point is kept, so the
+     * normalisation runs once per kept point instead of in every pass of a
+     * loop whose lanes accept at different passes; the draws and results
+     * are the same. */
+    int attempt = 0;
+    while (attempt < 32) {
+        float x, y, z, l2;
+        do {
+            x = gi_rand(st, -1.0f, 1.0f);
+            y = gi_rand(st, -1.0f, 1.0f);
+            z = gi_rand(st, -1.0f, 1.0f);
+            l2 = x * x + y * y + z * z;
+            attempt++;
+        } while ((l2 > 1.0f || l2 < 1e-6f) && attempt < 32);
+        if (l2 > 1.0f || l2 < 1e-6f) break;
         const float l = sqrtf(l2);
         const float dx = n.x + x / l, dy = n.y + y / l, dz = n.z + z / l;
         const float dl2 = dx * dx + dy * dy + dz * dz;
