@@ -457,6 +457,17 @@ def batch_bench(args, steps: int, pretrain: int) -> dict:
                          "alg_bytes_per_launch": alg, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS if achieved else None},
         }
+        if flags:
+            # joint training: every camera's 2^16 targets per frame
+            n_train = len(cams) * (1 << 16)
+            t_ms = sum(k_ms.get(k, 0.0) for k in ("nrc_fwdbwd_k", "nrc_wgrad_k", "nrc_adam_k"))
+            i_ms = k_ms.get("nrc_infer_k", 0.0)
+            t_tf = NRC_TRAIN_FLOPS * n_train / (t_ms * 1e-3) / 1e12 if t_ms else None
+            i_tf = NRC_FWD_FLOPS * int(st["visible_pixels"]) / (i_ms * 1e-3) / 1e12 if i_ms else None
+            out[mode]["mfma"] = {"train_samples_per_frame": n_train, "train_tflops": t_tf, "infer_tflops": i_tf,
+                                 "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                                 "train_frac": t_tf / MFMA_BF16_PEAK_TFLOPS if t_tf else None,
+                                 "infer_frac": i_tf / MFMA_BF16_PEAK_TFLOPS if i_tf else None}
     if plain_rays is not None:
         out["neural"]["rays_per_s_equivalent"] = plain_rays / (out["neural"]["frame_ms"] * 1e-3)
         out["neural"]["speedup_vs_plain"] = out["plain"]["frame_ms"] / out["neural"]["frame_ms"]
