@@ -53,3 +53,14 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.tgx_scene_upload(None, None) == 1
     assert lib.tgx_visibility(None, None, 0, 0, 0, None) == 1
     assert lib.tgx_last_error(None) == b"null context"
+
+
+def test_tuning_dtype_matches_the_header_struct():
+    """tg_amd's tgx_tuning record has the header struct's fields, in order
+    (all uint32_t), so tgx_tuning_get / _set exchange the same bytes."""
+    text = re.sub(r"/\*.*?\*/", "", (INCLUDE / "tgx.h").read_text(), flags=re.S)
+    body = re.search(r"typedef struct tgx_tuning \{(.*?)\} tgx_tuning;", text, flags=re.S).group(1)
+    fields = re.findall(r"uint32_t\s+(\w+)\s*;", body)
+    assert len(fields) == len(re.findall(r";", body))
+    assert list(T.TUNING_DTYPE.names) == fields
+    assert T.TUNING_DTYPE.itemsize == 4 * len(fields)
