@@ -571,7 +571,7 @@ typedef struct tgx_tuning {
     uint32_t trace_small;         /* lane-machine launch with more than one workgroup per CU: a
                                      queue of fewer than trace_small rays per lane of one
                                      workgroup per CU is traced by one workgroup per CU (the
-                                     others exit at once), 0..64 (4); 0 keeps every workgroup */
+                                     others exit at once), 0..64 (8); 0 keeps every workgroup */
 } tgx_tuning;
 
 int tgx_tuning_get(tgx_context* ctx, tgx_tuning* out);

@@ -392,5 +392,5 @@ def test_svo_trace_dda_forms_bit_exact(ctx):
             assert_same_hits(ctx.svo_trace(o, d, 1024.0), ref, f"dda {dda} cross {cross} node {node} chunk {chunk}")
     finally:
         ctx.set_tuning(trace_dda=1, trace_cross=40, trace_node=3, trace_burst=8, trace_chunk=32, trace_refill=10,
-                       trace_small=4)
+                       trace_small=8)
     assert 0.2 < ref["hit"].mean() < 0.95
