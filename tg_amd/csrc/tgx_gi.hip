@@ -736,6 +736,7 @@ struct trace_fuse {
 };
 
 #define TRACE_SHADOW 0x80000000u   /* slot of a lane tracing a shadow ray: id | TRACE_SHADOW */
+#define TRACE_SEGS 8u              /* queue segments of the lane-machine launch (one per XCD) */
 
 /* Sample records of the GI trace: (weight, hit node, hit voxel | flags).
  * A bounce miss leaves (0, 0, 0); a hit sets SAMPLE_HIT; a sun-facing hit
@@ -752,7 +753,7 @@ struct trace_fuse {
  * Plain tracing (kFused false): the result of ray i goes to out[i] and hits
  * (flag bit 0) are added to *hit_counter when it is not null. Fused GI
  * (kFused true): see trace_fuse; out and hit_counter are not used. `cursor`
- * must be 0 at launch. */
+ * holds TRACE_SEGS segment cursors 16 words apart, 0 at launch. */
 template <bool kLds, bool kFused, uint32_t kDda>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void gi_trace_k(trace_params P, const svo_ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ n_rays_ptr, float maxd,
@@ -798,6 +799,16 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
     uint32_t q_next = 0, q_end = 0, seen = 0;
     bool drained = false;
     const uint32_t n_waves = (small && small_blocks < gridDim.x ? small_blocks : gridDim.x) * (TRACE_BLOCK / 64u);
+    /* The queue is split into TRACE_SEGS segments with a cursor each (16
+     * words apart). A wave starts in segment blockIdx.x mod TRACE_SEGS
+     * (workgroups are dealt to the XCDs in turn, so an XCD's waves share a
+     * segment and its L2 holds that part of the queue's data) and moves to
+     * the next segment when its own runs dry. */
+    uint32_t seg = blockIdx.x % TRACE_SEGS, seg_left = TRACE_SEGS;
+    uint32_t seg_lo = (uint32_t)((uint64_t)n_rays * seg / TRACE_SEGS);
+    uint32_t seg_hi = (uint32_t)((uint64_t)n_rays * (seg + 1u) / TRACE_SEGS);
+    seen = seg_lo;
+    const uint32_t seg_waves = (n_waves + TRACE_SEGS - 1u) / TRACE_SEGS;
     while (true) {
         /* wave-uniform refill once `refill` lanes are idle */
         const unsigned long long idle = __ballot(!active && !pending);
@@ -807,18 +818,28 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             uint32_t nb = 0, nend = 0;
             if (avail < want && !drained) {
                 const uint32_t need = want - avail;
-                const bool tail = (uint64_t)(n_rays - (seen < n_rays ? seen : n_rays)) <
-                                  2ull * n_waves * chunk;
+                const bool tail = (uint64_t)(seg_hi - (seen < seg_hi ? seen : seg_hi)) <
+                                  2ull * seg_waves * chunk;
                 const uint32_t take = tail || chunk < need ? need : chunk;
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(cursor, take);
+                if (lane == 0) base = atomicAdd(cursor + 16u * seg, take);
                 /* the whole wave runs this block, so lane 0 is the first active */
-                base = __builtin_amdgcn_readfirstlane(base);
+                base = __builtin_amdgcn_readfirstlane(base) + seg_lo;
                 seen = base;
-                if ((uint64_t)base + take >= n_rays) drained = true;
-                if (base < n_rays) {
+                if (base < seg_hi) {
                     nb = base;
-                    nend = (uint64_t)base + take < n_rays ? base + take : n_rays;
+                    nend = (uint64_t)base + take < seg_hi ? base + take : seg_hi;
+                }
+                if ((uint64_t)base + take >= seg_hi) {
+                    /* this segment has run dry: on to the next one */
+                    if (--seg_left == 0u) {
+                        drained = true;
+                    } else {
+                        seg = seg + 1u == TRACE_SEGS ? 0u : seg + 1u;
+                        seg_lo = (uint32_t)((uint64_t)n_rays * seg / TRACE_SEGS);
+                        seg_hi = (uint32_t)((uint64_t)n_rays * (seg + 1u) / TRACE_SEGS);
+                        seen = seg_lo;
+                    }
                 }
             }
             const uint32_t rk = lane_rank(idle);
@@ -1666,9 +1687,9 @@ static trace_params params_of(tgx_context* ctx)
 
 /* Persistent trace of the queue rays[0 .. *n_dev) (n_max bounds it):
  * 1024-lane workgroups (tuning: per CU), the traversal layout in LDS when it
- * fits. */
+ * fits; the queue-segment cursors live in ctx->gi_counts. */
 static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n_dev, uint32_t n_max, float maxd,
-                        uint32_t* cursor, svo_res* out, uint32_t* hit_counter, const trace_fuse* fuse = nullptr)
+                        svo_res* out, uint32_t* hit_counter, const trace_fuse* fuse = nullptr)
 {
     const trace_fuse F = fuse ? *fuse : trace_fuse{};
     if (n_max == 0) return TGX_OK;
@@ -1701,7 +1722,11 @@ static int launch_trace(tgx_context* ctx, const svo_ray* rays, const uint32_t* n
         else
             kernel = fused ? (dda == 2u ? gi_trace_k<false, true, 2u> : dda ? gi_trace_k<false, true, 1u> : gi_trace_k<false, true, 0u>)
                            : (dda == 2u ? gi_trace_k<false, false, 2u> : dda ? gi_trace_k<false, false, 1u> : gi_trace_k<false, false, 0u>);
-        hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, cursor, out,
+        /* the segment cursors start at 0 (a fill kernel, so a captured
+         * frame clears them on every replay) */
+        uint32_t* seg_cursors = (uint32_t*)ctx->gi_counts.ptr + GI_CNT_SEG;
+        hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(128), 0, ctx->stream, seg_cursors, 16ull * TRACE_SEGS, 0u);
+        hipLaunchKernelGGL(kernel, grid, dim3(TRACE_BLOCK), shmem, ctx->stream, P, rays, n_dev, maxd, seg_cursors, out,
                            hit_counter, refill, vote, ctx->tuning.trace_cross, ctx->tuning.trace_node,
                            ctx->tuning.trace_burst, ctx->tuning.trace_chunk, small_rays, (uint32_t)cus, F);
     }
@@ -1908,12 +1933,12 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
             if ((rc = launch_trace_rg(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, F,
                                       ctx->tuning.trace_regroup)))
                 return rc;
-        } else if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, nullptr,
+        } else if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, res, nullptr,
                                       &F))) {
             return rc;
         }
     } else {
-    if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, cnt + 4, res, two ? nullptr : cnt + 2)))
+    if ((rc = launch_trace(ctx, rays, cnt + 1, (uint32_t)n_samples, prm.max_distance, res, two ? nullptr : cnt + 2)))
         return rc;
     {
         tgx_launch_scope s(ctx, "gi_shade_k");
@@ -1930,7 +1955,7 @@ int tgx_gi_trace_sets(tgx_context* ctx, const gi_trace_set* sets, uint32_t n_set
         if ((rc = sort_rays(ctx, shadow, cnt + 3, (uint32_t)n_samples, dst))) return rc;
         shadow = dst;
     }
-    if ((rc = launch_trace(ctx, shadow, cnt + 3, (uint32_t)n_samples, prm.max_distance, cnt + 5, sres, cnt + 7))) return rc;
+    if ((rc = launch_trace(ctx, shadow, cnt + 3, (uint32_t)n_samples, prm.max_distance, sres, cnt + 7))) return rc;
     {
         tgx_launch_scope s(ctx, "gi_lit_k");
         hipLaunchKernelGGL(gi_lit_k, dim3(blocks_for(n_samples)), dim3(256), 0, ctx->stream, shadow, cnt + 3, sres,
@@ -1988,7 +2013,7 @@ int tgx_svo_trace(tgx_context* ctx, const float* origins, const float* direction
     TGX_HIP_CHECK(ctx, hipMemcpyAsync(d_rays, h.data(), sizeof(svo_ray) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint64_t)GI_N_COUNTERS, 0u);
     hipLaunchKernelGGL(gi_fill_k, dim3(1), dim3(64), 0, ctx->stream, cnt + 6, 1ull, n);
-    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, max_distance, cnt + 4, d_res, nullptr))) return rc;
+    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, max_distance, d_res, nullptr))) return rc;
     hipLaunchKernelGGL(svo_hits_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (const svo_gnode*)ctx->svo_nodes.ptr,
                        d_res, n, d_hits);
     TGX_LAUNCH_CHECK(ctx, "svo_hits_k");
@@ -2029,7 +2054,7 @@ int tgx_svo_visibility_device(tgx_context* ctx, const tgx_camera* cam, uint32_t 
         hipLaunchKernelGGL(svo_primary_rays_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, gc, w, h, d_rays);
     }
     TGX_LAUNCH_CHECK(ctx, "svo_primary_rays_k");
-    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, SVO_F32_MAX, cnt + 4, d_res, nullptr))) return rc;
+    if ((rc = launch_trace(ctx, d_rays, cnt + 6, n, SVO_F32_MAX, d_res, nullptr))) return rc;
     {
         tgx_launch_scope s(ctx, "svo_vis_pack_k");
         hipLaunchKernelGGL(svo_vis_pack_k, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, d_res, n, cam->far_plane,

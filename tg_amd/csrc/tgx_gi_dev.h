@@ -28,7 +28,8 @@ static_assert(sizeof(gi_surf_rec) == 40, "gi_surf_rec layout");
 #define GI_CNT_SVO     16u  /* tgx_svo_trace */
 #define GI_CNT_VISIBLE 24u  /* visible pixels of the last surface pass */
 #define GI_CNT_QUEUE   32u  /* shared ray queues of tgx_gi_trace_sets with two sets */
-#define GI_CNT_TOTAL   40u
+#define GI_CNT_SEG     40u  /* the trace launch's queue-segment cursors, 16 u32 apart */
+#define GI_CNT_TOTAL   168u
 
 /* Reserves `count` consecutive queue slots for the calling lane (counts may
  * differ per lane) with one atomic per workgroup; returns the lane's first
