@@ -538,7 +538,7 @@ int tgx_shade_device(tgx_context* ctx, const tgx_camera* cam, uint32_t w, uint32
 typedef struct tgx_tuning {
     uint32_t trace_sort;          /* bit 0: sort bounce rays, bit 1: sort shadow rays,
                                      by (direction octant, origin cell) before tracing (0) */
-    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (10) */
+    uint32_t trace_refill;        /* idle lanes of a wave that trigger a queue refill, 1..64 (5) */
     uint32_t trace_blocks_per_cu; /* persistent 1024-lane trace workgroups per CU, 1..2 (2) */
     uint32_t trace_vote;          /* 0: every iteration runs node steps and DDA bursts for the
                                      lanes needing them; k in 1..64: a wave runs only DDA

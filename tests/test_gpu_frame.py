@@ -222,7 +222,7 @@ def test_autotune_keeps_results(ctx, batch_scene):
             "blocks, tile edge), 2 cameras")
         best = ctx.autotune(cams[:2], W, H, prm, buf.vis, buf.rad, buf.irr, frames=1)
         assert tuple(ctx.tuning()[0]) == tuple(best[0])
-        assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 8, 10, 12, 16, 24)
+        assert best["trace_blocks_per_cu"][0] in (1, 2) and best["trace_refill"][0] in (4, 5, 6, 8, 12, 16)
         assert best["trace_chunk"][0] in (0, 32, 64)
         assert best["trace_regroup"][0] in (0, 1, 2, 4) and best["trace_skip"][0] in (0, 1)
         assert best["vis_setup_blocks"][0] in (256, 1024, 2048, 4096) and best["vis_tile"][0] in (8, 16, 32)
@@ -246,7 +246,7 @@ def test_autotune_keeps_results(ctx, batch_scene):
         for a, b in zip(before, after):
             assert a.tobytes() == b.tobytes()
     finally:
-        ctx.set_tuning(trace_sort=0, trace_refill=10, trace_blocks_per_cu=2, trace_vote=0, trace_regroup=0,
+        ctx.set_tuning(trace_sort=0, trace_refill=5, trace_blocks_per_cu=2, trace_vote=0, trace_regroup=0,
                        trace_skip=0, vis_setup_blocks=2048, trace_dda=1, vis_tile=16, trace_cross=40,
                        trace_node=3, trace_chunk=32)
         buf.free()

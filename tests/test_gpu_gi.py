@@ -123,7 +123,7 @@ def test_queued_shadow_rays_equal_fused_launch(ctx, noise_setup, sort):
             nst = ctx.nrc_stats()
             out.append((rad, irr, tuple(gst), nrad, nirr, tuple(nst)))
     finally:
-        ctx.set_tuning(trace_sort=0, trace_refill=10, trace_blocks_per_cu=2, trace_vote=0)
+        ctx.set_tuning(trace_sort=0, trace_refill=5, trace_blocks_per_cu=2, trace_vote=0)
     a, b = out
     for k in (0, 1, 3, 4):
         assert a[k].tobytes() == b[k].tobytes(), f"output {k} differs"

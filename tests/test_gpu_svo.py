@@ -391,6 +391,6 @@ def test_svo_trace_dda_forms_bit_exact(ctx):
                 f"trace_burst {burst} trace_chunk {chunk} trace_refill {refill} trace_small {small}")
             assert_same_hits(ctx.svo_trace(o, d, 1024.0), ref, f"dda {dda} cross {cross} node {node} chunk {chunk}")
     finally:
-        ctx.set_tuning(trace_dda=1, trace_cross=40, trace_node=3, trace_burst=8, trace_chunk=32, trace_refill=10,
+        ctx.set_tuning(trace_dda=1, trace_cross=40, trace_node=3, trace_burst=8, trace_chunk=32, trace_refill=5,
                        trace_small=8)
     assert 0.2 < ref["hit"].mean() < 0.95

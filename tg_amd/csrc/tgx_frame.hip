@@ -341,7 +341,7 @@ int tgx_autotune(tgx_context* ctx, const tgx_camera* cams, uint32_t n_cams, uint
         {&tgx_tuning::trace_dda, 2u, {0u, 1u}},                       /* lane-machine DDA without iteration count */
         {&tgx_tuning::trace_node, 4u, {1u, 2u, 3u, 4u}},              /* node iterations per lane step */
         {&tgx_tuning::trace_cross, 5u, {0u, 24u, 32u, 40u, 48u}},     /* empty-brick crossing threshold */
-        {&tgx_tuning::trace_refill, 6u, {4u, 8u, 10u, 12u, 16u, 24u}}, /* idle lanes that trigger a queue refill */
+        {&tgx_tuning::trace_refill, 6u, {4u, 5u, 6u, 8u, 12u, 16u}}, /* idle lanes that trigger a queue refill */
         {&tgx_tuning::trace_chunk, 3u, {0u, 32u, 64u}},               /* rays per queue cursor atomic */
         {&tgx_tuning::vis_setup_blocks, 4u, {256u, 1024u, 2048u, 4096u}}, /* cull / scatter workgroups per camera */
         {&tgx_tuning::vis_tile, 3u, {8u, 16u, 32u}},                  /* visibility tile edge */

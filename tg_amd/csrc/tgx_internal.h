@@ -102,7 +102,7 @@ struct tgx_context {
     tgx_buffer   gi_sort;        /* sort keys and indices                   */
     uint32_t     gi_spp = 0;     /* spp of the last GI call                 */
     uint32_t     rg_lds_set = 0; /* regrouped trace variants whose LDS limit is raised */
-    tgx_tuning   tuning = {0u, 10u, 2u, 0u, 0u, 0u, 2048u, 1u, 16u, 40u, 3u, 8u, 32u, 8u};
+    tgx_tuning   tuning = {0u, 5u, 2u, 0u, 0u, 0u, 2048u, 1u, 16u, 40u, 3u, 8u, 32u, 8u};
 
     /* Neural irradiance cache (tgx_nrc.hip). */
     bool           nrc_ready = false;
