@@ -805,9 +805,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
      * segment and its L2 holds that part of the queue's data) and moves to
      * the next segment when its own runs dry. */
     uint32_t seg = blockIdx.x % TRACE_SEGS, seg_left = TRACE_SEGS;
-    uint32_t seg_lo = (uint32_t)((uint64_t)n_rays * seg / TRACE_SEGS);
-    uint32_t seg_hi = (uint32_t)((uint64_t)n_rays * (seg + 1u) / TRACE_SEGS);
-    seen = seg_lo;
+    /* segment k is rays [n k / TRACE_SEGS, n (k + 1) / TRACE_SEGS) */
+    auto seg_start = [n_rays](uint32_t k) { return (uint32_t)((uint64_t)n_rays * k / TRACE_SEGS); };
+    seen = seg_start(seg);
     const uint32_t seg_waves = (n_waves + TRACE_SEGS - 1u) / TRACE_SEGS;
     while (true) {
         /* wave-uniform refill once `refill` lanes are idle */
@@ -817,6 +817,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
             const uint32_t avail = q_end - q_next;
             uint32_t nb = 0, nend = 0;
             if (avail < want && !drained) {
+                const uint32_t seg_lo = seg_start(seg), seg_hi = seg_start(seg + 1u);
                 const uint32_t need = want - avail;
                 const bool tail = (uint64_t)(seg_hi - (seen < seg_hi ? seen : seg_hi)) <
                                   2ull * seg_waves * chunk;
@@ -836,9 +837,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 
                         drained = true;
                     } else {
                         seg = seg + 1u == TRACE_SEGS ? 0u : seg + 1u;
-                        seg_lo = (uint32_t)((uint64_t)n_rays * seg / TRACE_SEGS);
-                        seg_hi = (uint32_t)((uint64_t)n_rays * (seg + 1u) / TRACE_SEGS);
-                        seen = seg_lo;
+                        seen = seg_start(seg);
                     }
                 }
             }
